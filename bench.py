@@ -1,0 +1,276 @@
+"""PageRank / eigenvector benchmark on one MI355X.
+
+Workload: R-MAT scale 22 (4,194,304 vertices), edge factor 16 (67,108,864
+directed edges), a=0.57 b=0.19 c=0.19 d=0.05, SplitMix64-seeded, unweighted;
+PageRank with damping 0.85, epsilon 1e-9, uniform personalisation, cold start
+per step (a fresh rank map, as the reference's criterion bench does).
+
+One step = one full PageRank solve with the graph resident in HBM.  `value` is
+the throughput of the FP32-value / FP64-residual form in GTEPS (edges x sweeps
+/ second); the plain FP64 drop-in is timed beside it, and eigenvector
+centrality on the symmetrised graph after it.  The CPU baseline is the oracle
+(the C restatement of the reference's rayon path) on the host cores, which
+also provides the FP64 answer every GPU result's L1 error is reported against.
+
+Algorithmic bytes per sweep (pull form, 4-byte indices, value width s):
+  PageRank  B = 4m + 4(n+1) + n(4s + 4)        (s = 8: FP64 drop-in, 4: FP32 form)
+  Eigenvector (symmetrised, m_s = 2m)  B = 4 m_s + 4(n+1) + 24 n
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "graph-tool-rs_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PageRank GTEPS + HBM GB/s, FP32/batched vs FP64 plain GPU at L1≤1e-6"
+SEED = 0x243F6A8885A308D3
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pr_bytes(n: int, m: int, s: int) -> int:
+    return 4 * m + 4 * (n + 1) + n * (4 * s + 4)
+
+
+def ev_bytes(n: int, ms: int) -> int:
+    return 4 * ms + 4 * (n + 1) + 24 * n
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edgefactor", type=int, default=16)
+    ap.add_argument("--epsilon", type=float, default=1e-9)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle")
+    ap.add_argument("--no-eigen", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import gt_hip
+    from gt_hip import _lib
+
+    if not torch.cuda.is_available() or gt_hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+
+    n = 1 << args.scale
+    m = args.edgefactor * n
+    t0 = time.perf_counter()
+    src, tgt = gt_hip.gen_rmat(args.scale, m, 0.57, 0.19, 0.19, SEED)
+    log(f"[bench] R-MAT scale {args.scale}: n={n} m={m} generated in {time.perf_counter()-t0:.1f}s")
+    t0 = time.perf_counter()
+    adj = gt_hip.host_adjacency(n, src, tgt)
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gid = 1
+    dev = gt_hip.DeviceGraph.upload(gid, n, m, *adj)
+    t_up = time.perf_counter() - t0
+    log(f"[bench] host adjacency {t_adj:.1f}s, upload {t_up:.2f}s")
+    del adj
+
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    rank = torch.empty(n, dtype=torch.float64, device="cuda")
+    params = _lib.GtPowerIteration(args.epsilon, 0, _lib.GtPlan(n, -(-n // 4096)))
+    view = _lib.GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0)
+    unity = _lib.GtProp(_lib.GT_MAP_UNITY, 0, 0, 0, 1.0, None, 0, 0)
+    out = _lib.GtVertexOut(C.c_void_p(rank.data_ptr()), n, 0, _lib.GT_DEVICE, 0, gid)
+    iters = C.c_uint64()
+
+    def solve(fn: str) -> int:
+        out.had = 0  # cold start: every slot seeded with 1/n
+        _lib.call(fn, dev.handle, C.byref(view), C.byref(unity), None, 0.85, C.byref(params),
+                  C.byref(out), C.byref(iters))
+        return int(iters.value)
+
+    def timed(fn: str, kind: int):
+        for _ in range(args.warmup):
+            solve(fn)
+        dev.set_profiling(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sweeps = [solve(fn) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        ms_k, launches, _ = dev.sweep_stats(kind)
+        ms_other = {k: dev.sweep_stats(k)[:2] for k in range(4) if k != kind}
+        dev.set_profiling(False)
+        return dt, sweeps, ms_k, launches, ms_other
+
+    # ---- FP32-value / FP64-residual form (the chosen form)
+    dt32, sw32, ms32, la32, other32 = timed("gt_pagerank_mixed", _lib.GT_SWEEP_PAGERANK_F32)
+    x32 = rank.cpu().numpy().copy()
+    f64_in_mixed = other32[_lib.GT_SWEEP_PAGERANK_F64]
+    log(f"[bench] mixed: {dt32/args.steps*1e3:.3f} ms/solve, sweeps {sw32[0]}, "
+        f"f32 launches {la32} ({ms32/max(la32,1)*1e3:.1f} us avg), "
+        f"f64 refinement launches {f64_in_mixed[1]}")
+
+    # ---- plain FP64 drop-in
+    dt64, sw64, ms64, la64, _ = timed("gt_pagerank_f64", _lib.GT_SWEEP_PAGERANK_F64)
+    x64 = rank.cpu().numpy().copy()
+    log(f"[bench] fp64: {dt64/args.steps*1e3:.3f} ms/solve, sweeps {sw64[0]}, "
+        f"launches {la64} ({ms64/max(la64,1)*1e3:.1f} us avg)")
+
+    # Work per solve: every sweep of the mixed form is a full pass over m edges.
+    gteps32 = m * sw32[0] * args.steps / dt32 / 1e9
+    gteps64 = m * sw64[0] * args.steps / dt64 / 1e9
+    b32, b64 = pr_bytes(n, m, 4), pr_bytes(n, m, 8)
+    avg32 = ms32 / max(la32, 1) * 1e-3
+    avg64 = ms64 / max(la64, 1) * 1e-3
+    # Launches that found the loop finished move no bytes: spread the
+    # algorithmic bytes of the sweeps done over every launch of that kernel.
+    n_f32_sweeps = (sw32[0] - f64_in_mixed[1] // max(args.steps, 1)) * args.steps
+    ach32 = b32 * n_f32_sweeps / max(la32, 1) / avg32 / 1e9 if la32 else 0.0
+    ach64 = b64 * sw64[0] * args.steps / max(la64, 1) / avg64 / 1e9 if la64 else 0.0
+
+    result = {
+        "metric": METRIC,
+        "value": round(gteps32, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt32 / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 values/SpMV, f64 dangling/teleport/L1 + f64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pagerank_rmat{args.scale}_ef{args.edgefactor}",
+            "vertices": n, "edges": m, "rmat": [0.57, 0.19, 0.19, 0.05], "seed": SEED,
+            "damping": 0.85, "epsilon": args.epsilon, "weights": "unity",
+            "personalisation": "uniform", "start": "cold (fresh map per step)",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "pr_sweep<float> (FP32 PageRank sweep)",
+            "bytes_per_sweep": b32, "avg_launch_us": round(avg32 * 1e6, 2),
+        },
+        "fp64_dropin": {
+            "gteps": round(gteps64, 3), "ms_per_solve": round(dt64 / args.steps * 1e3, 4),
+            "sweeps": sw64[0], "achieved_gbs": round(ach64, 1),
+            "roofline_frac": round(ach64 / HBM_PEAK_GBS, 4), "bytes_per_sweep": b64,
+            "avg_launch_us": round(avg64 * 1e6, 2),
+        },
+        "mixed_vs_fp64_speedup": round(dt64 / dt32, 3),
+        "mixed_sweeps": {"total": sw32[0], "fp64_refinement": f64_in_mixed[1] // max(args.steps, 1)},
+        "host_adjacency_s": round(t_adj, 2), "upload_s": round(t_up, 3),
+    }
+
+    # ---- eigenvector on the symmetrised graph
+    if not args.no_eigen:
+        S = np.concatenate([src, tgt])
+        T = np.concatenate([tgt, src])
+        ms_ = int(S.shape[0])
+        adj_s = gt_hip.host_adjacency(n, S, T)
+        devs = gt_hip.DeviceGraph.upload(2, n, ms_, *adj_s)
+        del adj_s
+        devs.set_stream(stream.cuda_stream)
+        xs = torch.empty(n, dtype=torch.float64, device="cuda")
+        outs = _lib.GtVertexOut(C.c_void_p(xs.data_ptr()), n, 0, _lib.GT_DEVICE, 0, 2)
+        eig = C.c_double()
+
+        def esolve() -> int:
+            outs.had = 0
+            _lib.call("gt_eigenvector_f64", devs.handle, C.byref(view), C.byref(unity),
+                      C.byref(params), C.byref(outs), C.byref(eig), C.byref(iters))
+            return int(iters.value)
+
+        for _ in range(args.warmup):
+            esolve()
+        devs.set_profiling(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        swe = [esolve() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dte = time.perf_counter() - t
+        mse, lae, _ = devs.sweep_stats(_lib.GT_SWEEP_EIGENVECTOR)
+        devs.set_profiling(False)
+        be = ev_bytes(n, ms_)
+        avge = mse / max(lae, 1) * 1e-3
+        ache = be * lae / max(lae, 1) / avge / 1e9 if lae else 0.0
+        result["eigenvector"] = {
+            "gteps": round(ms_ * swe[0] * args.steps / dte / 1e9, 3),
+            "ms_per_solve": round(dte / args.steps * 1e3, 4), "sweeps": swe[0],
+            "launches_per_solve": lae // max(args.steps, 1),
+            "eigenvalue": eig.value, "achieved_gbs": round(ache, 1),
+            "roofline_frac": round(ache / HBM_PEAK_GBS, 4), "bytes_per_sweep": be,
+        }
+        x_eig = xs.cpu().numpy().copy()
+        log(f"[bench] eigenvector: {dte/args.steps*1e3:.3f} ms/solve, sweeps {swe[0]}, "
+            f"lambda {eig.value:.6f}")
+        devs.close()
+
+    # ---- CPU baseline and FP64 oracle answers
+    if not args.no_cpu:
+        import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        o = oracle.Adj(n, src, tgt)
+        log(f"[bench] oracle adjacency built in {time.perf_counter()-t0:.1f}s")
+        t0 = time.perf_counter()
+        want, it_o = o.pagerank(epsilon=args.epsilon, max_iter=0, grain=4096, threads=threads)
+        dtc = time.perf_counter() - t0
+        del o
+        cpu_gteps = m * it_o / dtc / 1e9
+        log(f"[bench] oracle pagerank: {dtc:.1f}s, {it_o} sweeps, {cpu_gteps:.3f} GTEPS "
+            f"on {threads} threads")
+        nz = want != 0
+
+        def errs(x):
+            return {"l1": float(np.abs(x - want).sum()),
+                    "max_rel": float((np.abs(x - want)[nz] / np.abs(want[nz])).max())}
+
+        result["cpu_baseline"] = {
+            "value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"one full pagerank_with solve (oracle C restatement, Plan grain 4096, "
+                      f"{it_o} sweeps) on the same R-MAT-{args.scale} graph, {dtc:.1f}s",
+        }
+        result["accuracy_vs_fp64_oracle"] = {
+            "mixed": errs(x32), "fp64_dropin": errs(x64),
+            "oracle_sweeps": it_o, "fp64_sweeps": sw64[0],
+        }
+        if not args.no_eigen:
+            o2 = oracle.Adj(n, S, T)
+            t0 = time.perf_counter()
+            xo, eo, ite = o2.eigenvector(epsilon=args.epsilon, max_iter=0, grain=4096,
+                                         threads=threads)
+            dte_c = time.perf_counter() - t0
+            del o2
+            result["eigenvector"]["vs_oracle"] = {
+                "l1": float(np.abs(x_eig - xo).sum()), "oracle_sweeps": ite,
+                "eigenvalue_rel_err": abs(result["eigenvector"]["eigenvalue"] - eo) / abs(eo),
+                "cpu_gteps": round(ms_ * ite / dte_c / 1e9, 4),
+            }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

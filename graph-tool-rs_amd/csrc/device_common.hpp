@@ -1,0 +1,72 @@
+// Device helpers shared by the sweep kernels: deterministic wave / workgroup
+// reductions (fixed shuffle tree, so a result depends only on the inputs) and
+// the last-workgroup ticket that folds per-workgroup partials in index order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gt {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Sum over a 256-thread workgroup; every thread gets the total.  `red` must
+// hold 4 doubles of LDS.  Contains two barriers.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return t;
+}
+
+// Two sums at once (one barrier pair instead of two).
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) {
+        red[wid] = a;
+        red[4 + wid] = b;
+    }
+    __syncthreads();
+    a = (red[0] + red[1]) + (red[2] + red[3]);
+    b = (red[4] + red[5]) + (red[6] + red[7]);
+    __syncthreads();
+}
+
+// After a workgroup has published its partials: returns true in exactly one
+// workgroup of the grid -- the last to arrive -- with every other
+// workgroup's writes visible to it.  `counter` must be zero at launch; the
+// caller of the true branch resets it.
+__device__ __forceinline__ bool last_to_arrive(uint32_t *counter, uint32_t *s_flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = atomicAdd(counter, 1u);
+        *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    bool last = *s_flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+}  // namespace gt

@@ -1,0 +1,307 @@
+// Eigenvector centrality by power iteration on gfx950, FP64.
+//
+// Semantics follow crates/gt-algo/src/centrality.rs:405-497: per sweep t,
+// s(i) = sum over in-entries (j, e) of w(e) * x_{t-1}(j) for active i (inactive
+// slots carried), norm_t = sqrt(sum s^2), x_t = s / (norm_t == 0 ? 1 : norm_t),
+// delta_t = sum over active i of |x_t(i) - x_{t-1}(i)|; stop when
+// delta_t < epsilon or t == max_iter; return (norm_T, T).
+//
+// One launch per sweep, normalisation folded: launch L writes the unscaled
+// y_L = A x_{L-1} (computed as (A y_{L-1}) / scale_{L-1}) and, in the same pass,
+// delta_{L-1} = sum |y_{L-1}/scale_{L-1} - y_{L-2}/scale_{L-2}|.  The scales
+// are known at launch time, so no second vertex pass is needed; the price is
+// that convergence of sweep L-1 is learnt in launch L (one extra SpMV at the
+// end of a solve).  Three y buffers rotate.  The last workgroup folds the
+// squared-norm and delta partials in index order and decides on the device.
+#include <cmath>
+
+#include "device_common.hpp"
+#include "gt_internal.hpp"
+#include "props.hpp"
+#include "driver.hpp"
+#include "sweep_common.hpp"
+
+namespace gt {
+
+// Eigenvector loop state, laid over SweepState's storage.
+// delta, iter and done sit at SweepState's offsets so the shared driver can
+// read them.
+struct EvState {
+    double sc_cur;      // scale of y_{L-1}
+    double delta;       // delta of the last completed check
+    double sc_prev;     // scale of y_{L-2}
+    double norm_last;   // norm_{L-1}
+    double eig;         // norm of the final iterate
+    uint64_t iter;      // reference sweep count reached
+    uint32_t done;
+    uint32_t counter;
+    uint32_t launches;  // launches that did work (= L)
+    uint32_t pad;
+    double final_scale; // scale of the final iterate
+    uint64_t final_iter;
+};
+
+struct EvArgs {
+    const Tile *tiles;
+    uint32_t ntiles;
+    uint32_t nlong;
+    const uint32_t *in_off;
+    const uint32_t *in_idx;
+    const double *w;          // weights in pull order
+    const uint8_t *active;
+    const double *y_cur;      // y_{L-1}
+    const double *y_prev;     // y_{L-2}
+    double *y_next;           // y_L
+    double epsilon;
+    uint64_t max_iter;
+    double *partials;         // [2 * gridDim.x]
+    double *long_part;
+    const uint32_t *long_rows;
+    const uint32_t *long_slot;
+    EvState *st;
+};
+
+template <bool MASK>
+struct EvFin {
+    const EvArgs &a;
+    double sc_cur, sc_prev;
+    bool first;
+    double sq = 0.0, dl = 0.0;
+    __device__ EvFin(const EvArgs &args, double c, double p, bool f)
+        : a(args), sc_cur(c), sc_prev(p), first(f) {}
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
+        if (MASK && !a.active[i]) {
+            a.y_next[i] = a.y_cur[i];
+            return;
+        }
+        const double s = acc / sc_cur;
+        a.y_next[i] = s;
+        sq += s * s;
+        if (!first) dl += fabs(a.y_cur[i] / sc_cur - a.y_prev[i] / sc_prev);
+    }
+};
+
+template <bool WEIGHTED, bool MASK>
+__global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
+    if (a.st->done) return;
+    __shared__ double lds[kTileNnz];
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    const bool first = a.st->launches == 0;
+    EvFin<MASK> fin(a, a.st->sc_cur, a.st->sc_prev, first);
+    spmv_tiles<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, a.y_cur, a.w, a.long_part,
+                                 lds, red, fin);
+    double sq = fin.sq, dl = fin.dl;
+    block_sum2(sq, dl, red);
+    if (tid == 0) {
+        a.partials[blockIdx.x] = sq;
+        a.partials[gridDim.x + blockIdx.x] = dl;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+
+    EvFin<MASK> lf(a, fin.sc_cur, fin.sc_prev, first);
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        lf.sq += a.partials[k];
+        lf.dl += a.partials[gridDim.x + k];
+    }
+    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+        lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
+    double s2 = lf.sq, d = lf.dl;
+    block_sum2(s2, d, red);
+    if (tid == 0) {
+        EvState *st = a.st;
+        const uint32_t L = st->launches + 1;
+        const double norm = sqrt(s2);
+        bool done = false;
+        if (L >= 2) {
+            st->delta = d;
+            if (!(d >= a.epsilon)) {
+                // Sweep L-1 converged: its iterate is the answer.
+                done = true;
+                st->final_iter = L - 1;
+                st->final_scale = st->sc_cur;
+                st->eig = st->norm_last;
+                st->iter = L - 1;
+            }
+        }
+        if (!done) {
+            st->iter = L;
+            if (a.max_iter > 0 && L >= a.max_iter) {
+                done = true;
+                st->final_iter = L;
+                st->final_scale = norm == 0.0 ? 1.0 : norm;
+                st->eig = norm;
+            }
+        }
+        st->sc_prev = st->sc_cur;
+        st->sc_cur = norm == 0.0 ? 1.0 : norm;
+        st->norm_last = norm;
+        st->launches = L;
+        st->done = done ? 1u : 0u;
+        st->counter = 0;
+        __threadfence();
+    }
+}
+
+// Seed slots [had, n) with the uniform start and reset the loop state.
+__global__ __launch_bounds__(kBlock) void ev_init(double *x, uint64_t had, uint32_t n,
+                                                  double uniform, EvState *st) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        if (i >= had) x[i] = uniform;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->sc_cur = 1.0;
+        st->sc_prev = 1.0;
+        st->norm_last = 0.0;
+        st->delta = 0.0;
+        st->eig = 0.0;
+        st->iter = 0;
+        st->done = 0;
+        st->counter = 0;
+        st->launches = 0;
+        st->final_scale = 1.0;
+        st->final_iter = 0;
+    }
+}
+
+// x(i) = y_T(i) / scale for active i; inactive slots are carried unscaled.
+template <bool MASK>
+__global__ __launch_bounds__(kBlock) void ev_emit(const double *y, const uint8_t *active,
+                                                  const EvState *st, uint32_t n, double *x) {
+    const double sc = st->final_scale;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double v = y[i];
+        x[i] = (MASK && !active[i]) ? v : v / sc;
+    }
+}
+
+static unsigned vgrid(uint32_t n) {
+    uint32_t b = (n + kBlock - 1) / kBlock;
+    if (b > 2048) b = 2048;
+    return b ? b : 1;
+}
+
+gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                           const gt_power_iteration *params, gt_vertex_out *x, double *eig,
+                           uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (!iters || !eig) return fail(GT_ERR_INVALID_ARG, "null output");
+    Orient *pull = nullptr, *push = nullptr;
+    bool reversed = false;
+    GT_TRY(resolve_view(g, view, &pull, &push, &reversed));
+    GT_TRY(check_vertex_out(g, x, "eigenvector: centrality"));
+    GT_TRY(check_params(g, params));
+    const uint32_t n = g->n;
+    hipStream_t s = g->stream;
+    Workspace &ws = g->ws;
+    const double uniform = n == 0 ? 0.0 : 1.0 / (double)n;
+    const double eps = params->epsilon;
+    const bool skip = std::isnan(eps) || !(eps + 1.0 >= eps);
+
+    bool weighted = false;
+    const double *wp = nullptr;
+    if (weight && weight->kind != GT_MAP_UNITY) {
+        weighted = true;
+        const void *src = nullptr;
+        int dt = GT_F64;
+        GT_TRY(stage_edge_source(g, weight, &src, &dt));
+        GT_TRY(stage_csc_weights<double>(g, weight, *pull, src, dt, ws.wcsc64));
+        wp = ws.wcsc64.as<double>();
+    }
+    *eig = 0.0;
+    if (n == 0) {
+        *iters = skip ? 0 : 1;
+        g->last_sweeps = *iters;
+        return GT_OK;
+    }
+    GT_TRY(ensure_schedule(g, *pull, n));
+    g->grid = sweep_grid(pull->ntiles);
+    GT_TRY(prof_prepare(g));
+    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, pull->nslots)));
+    GT_TRY(ws.state.reserve(sizeof(EvState)));
+    const size_t vb = sizeof(double) * n;
+    double *y[3];
+    if (x->loc == GT_DEVICE) {
+        y[0] = x->data;
+    } else {
+        GT_TRY(ws.y[0].reserve(vb));
+        y[0] = ws.y[0].as<double>();
+        const uint64_t had = std::min<uint64_t>(x->had, n);
+        if (had) GT_HIP(hipMemcpyAsync(y[0], x->data, sizeof(double) * had, hipMemcpyHostToDevice, s));
+    }
+    GT_TRY(ws.y[1].reserve(vb));
+    GT_TRY(ws.y[2].reserve(vb));
+    y[1] = ws.y[1].as<double>();
+    y[2] = ws.y[2].as<double>();
+    EvState *st = ws.state.as<EvState>();
+    hipLaunchKernelGGL(ev_init, dim3(vgrid(n)), dim3(kBlock), 0, s, y[0], x->had, n, uniform, st);
+    GT_HIP(hipGetLastError());
+
+    uint64_t T = 0;
+    double *out = y[0];
+    if (!skip) {
+        EvArgs a{};
+        a.tiles = pull->tiles.as<Tile>();
+        a.ntiles = pull->ntiles;
+        a.nlong = pull->nlong;
+        a.in_off = pull->off.as<uint32_t>();
+        a.in_idx = pull->idx.as<uint32_t>();
+        a.w = wp;
+        a.active = nullptr;
+        a.epsilon = eps;
+        a.max_iter = params->max_iter;
+        a.partials = ws.partials.as<double>();
+        a.long_part = ws.long_part.as<double>();
+        a.long_rows = pull->long_rows.as<uint32_t>();
+        a.long_slot = pull->long_slot.as<uint32_t>();
+        a.st = st;
+        auto launch = [&](uint64_t k) {
+            // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
+            const uint64_t L = k + 1;
+            EvArgs b = a;
+            b.y_cur = y[(L - 1) % 3];
+            b.y_prev = y[(L + 1) % 3];
+            b.y_next = y[L % 3];
+            if (weighted)
+                hipLaunchKernelGGL((ev_sweep<true, false>), dim3(g->grid), dim3(kBlock), 0, s, b);
+            else
+                hipLaunchKernelGGL((ev_sweep<false, false>), dim3(g->grid), dim3(kBlock), 0, s, b);
+        };
+        // The driver reads done / delta / iter through the SweepState layout;
+        // EvState keeps delta, iter and done at the same offsets.
+        static_assert(offsetof(EvState, delta) == offsetof(SweepState, delta), "layout");
+        static_assert(offsetof(EvState, iter) == offsetof(SweepState, iter), "layout");
+        static_assert(offsetof(EvState, done) == offsetof(SweepState, done), "layout");
+        static_assert(sizeof(EvState) >= sizeof(SweepState), "layout");
+        uint64_t launched = 0;
+        g->prof_kind = GT_SWEEP_EIGENVECTOR;
+        // One launch beyond the answer is needed to learn its delta.
+        const uint64_t cap = params->max_iter;
+        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, cap, launch, &launched));
+        EvState h;
+        GT_HIP(hipMemcpy(&h, st, sizeof(EvState), hipMemcpyDeviceToHost));
+        T = h.final_iter;
+        *eig = h.eig;
+        out = y[T % 3];
+    }
+    double *dst = x->loc == GT_DEVICE ? x->data : y[0];
+    hipLaunchKernelGGL((ev_emit<false>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, nullptr, st, n,
+                       dst);
+    GT_HIP(hipGetLastError());
+    if (x->loc == GT_HOST)
+        GT_HIP(hipMemcpyAsync(x->data, dst, vb, hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    g->last_sweeps = T;
+    *iters = T;
+    return GT_OK;
+}
+
+}  // namespace gt
+
+extern "C" gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                                        const gt_power_iteration *params, gt_vertex_out *x,
+                                        double *eig, uint64_t *iters) {
+    return gt::eigenvector_impl(g, view, weight, params, x, eig, iters);
+}

@@ -1,0 +1,300 @@
+// Device graph handle: upload of a host adjacency, download, the SpMV row
+// schedule, streams and the error channel of the C ABI.
+#include <algorithm>
+#include <cstring>
+
+#include "gt_internal.hpp"
+
+namespace gt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+gt_status fail(gt_status code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+gt_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+                  hipGetErrorString(e), what, file, line);
+    set_error(buf);
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GT_ERR_OOM
+                                                                       : GT_ERR_HIP;
+}
+
+gt_status DevBuf::reserve(size_t b) {
+    if (b <= bytes && ptr) return GT_OK;
+    release();
+    if (b == 0) b = 256;
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    bytes = b;
+    return GT_OK;
+}
+
+int sweep_grid(uint32_t ntiles) {
+    // 256 CUs x 8 resident workgroups keeps every CU busy; fewer tiles than
+    // that means one tile per workgroup.
+    const uint32_t cap = 2048;
+    uint32_t g = ntiles < cap ? ntiles : cap;
+    return g ? (int)g : 1;
+}
+
+gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
+                    gt_graph **out) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+        return fail(GT_ERR_HIP, "no HIP device is available");
+    gt_graph *g = new gt_graph();
+    g->id = graph_id;
+    g->n = n;
+    g->m = m;
+    g->edge_bound = edge_bound;
+    e = hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete g;
+        return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    g->stream = g->own_stream;
+    e = hipHostMalloc((void **)&g->h_state, sizeof(SweepState), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(g->own_stream);
+        delete g;
+        return hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    *out = g;
+    return GT_OK;
+}
+
+// Tiles of at most kTileNnz entries and kTileRows rows; a row longer than
+// kTileNnz becomes ceil(len / kTileNnz) split chunks with one partial slot each.
+gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
+    if (o.scheduled) return GT_OK;
+    const std::vector<uint32_t> &off = o.h_off;
+    std::vector<Tile> tiles;
+    std::vector<uint32_t> long_rows, long_slot;
+    uint32_t nslots = 0;
+    tiles.reserve(n / 256 + 16);
+    uint32_t r = 0;
+    while (r < n) {
+        uint32_t len = off[r + 1] - off[r];
+        if (len > kTileNnz) {
+            long_rows.push_back(r);
+            long_slot.push_back(nslots);
+            for (uint32_t b = off[r]; b < off[r + 1]; b += kTileNnz) {
+                uint32_t e = std::min<uint64_t>((uint64_t)b + kTileNnz, off[r + 1]);
+                tiles.push_back(Tile{nslots++, kSplit, b, e});
+            }
+            ++r;
+            continue;
+        }
+        uint32_t r0 = r;
+        uint32_t nnz = 0;
+        while (r < n && r - r0 < kTileRows) {
+            uint32_t l = off[r + 1] - off[r];
+            if (l > kTileNnz || nnz + l > kTileNnz) break;
+            nnz += l;
+            ++r;
+        }
+        tiles.push_back(Tile{r0, r, off[r0], off[r]});
+    }
+    long_slot.push_back(nslots);
+    o.ntiles = (uint32_t)tiles.size();
+    o.nlong = (uint32_t)long_rows.size();
+    o.nslots = nslots;
+    GT_TRY(o.tiles.reserve(sizeof(Tile) * std::max<size_t>(1, tiles.size())));
+    GT_TRY(o.long_rows.reserve(sizeof(uint32_t) * std::max<size_t>(1, long_rows.size())));
+    GT_TRY(o.long_slot.reserve(sizeof(uint32_t) * long_slot.size()));
+    if (!tiles.empty())
+        GT_HIP(hipMemcpy(o.tiles.ptr, tiles.data(), sizeof(Tile) * tiles.size(),
+                         hipMemcpyHostToDevice));
+    if (!long_rows.empty())
+        GT_HIP(hipMemcpy(o.long_rows.ptr, long_rows.data(),
+                         sizeof(uint32_t) * long_rows.size(), hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(o.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
+                     hipMemcpyHostToDevice));
+    o.scheduled = true;
+    (void)g;
+    return GT_OK;
+}
+
+static gt_status upload_orient(Orient &o, uint32_t n, uint64_t m, const uint32_t *off,
+                               const uint32_t *idx, const uint32_t *eid) {
+    GT_TRY(o.off.reserve(sizeof(uint32_t) * ((size_t)n + 1)));
+    GT_TRY(o.idx.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, m)));
+    GT_TRY(o.eid.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, m)));
+    GT_HIP(hipMemcpy(o.off.ptr, off, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice));
+    if (m) {
+        GT_HIP(hipMemcpy(o.idx.ptr, idx, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+        GT_HIP(hipMemcpy(o.eid.ptr, eid, sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    }
+    o.h_off.assign(off, off + (size_t)n + 1);
+    return GT_OK;
+}
+
+// Host-side validation of an uploaded orientation: monotone offsets that end
+// at m, endpoints < n, edge ids < edge_bound.
+static gt_status check_orient(const char *name, uint32_t n, uint64_t m, uint64_t edge_bound,
+                              const uint32_t *off, const uint32_t *idx, const uint32_t *eid) {
+    if (off[0] != 0 || off[n] != m)
+        return fail(GT_ERR_INVALID_ARG, std::string(name) + ": offsets must start at 0 and end at m");
+    for (uint32_t v = 0; v < n; ++v)
+        if (off[v + 1] < off[v])
+            return fail(GT_ERR_INVALID_ARG, std::string(name) + ": offsets must be non-decreasing");
+    for (uint64_t j = 0; j < m; ++j) {
+        if (idx[j] >= n)
+            return fail(GT_ERR_NO_SUCH_VERTEX, std::string(name) + ": endpoint out of range");
+        if (eid[j] >= edge_bound)
+            return fail(GT_ERR_INVALID_ARG, std::string(name) + ": edge id >= edge_bound");
+    }
+    return GT_OK;
+}
+
+// Profiling: HIP events around every sweep launch, on the stream it runs on.
+gt_status prof_begin(gt_graph *g, uint64_t k) {
+    if (!g->profiling) return GT_OK;
+    GT_HIP(hipEventRecord(g->ev[2 * (k % 256)], g->stream));
+    return GT_OK;
+}
+gt_status prof_end(gt_graph *g, uint64_t k) {
+    if (!g->profiling) return GT_OK;
+    GT_HIP(hipEventRecord(g->ev[2 * (k % 256) + 1], g->stream));
+    if ((k % 256) == 255) {
+        GT_HIP(hipStreamSynchronize(g->stream));
+        for (int q = 0; q < 256; ++q) {
+            float ms = 0.f;
+            GT_HIP(hipEventElapsedTime(&ms, g->ev[2 * q], g->ev[2 * q + 1]));
+            g->prof_ms[g->prof_kind] += ms;
+        }
+        g->prof_launches[g->prof_kind] += 256;
+    }
+    return GT_OK;
+}
+gt_status prof_flush(gt_graph *g, uint64_t launched) {
+    if (!g->profiling) return GT_OK;
+    const uint64_t rem = launched % 256;
+    GT_HIP(hipStreamSynchronize(g->stream));
+    for (uint64_t q = 0; q < rem; ++q) {
+        float ms = 0.f;
+        GT_HIP(hipEventElapsedTime(&ms, g->ev[2 * q], g->ev[2 * q + 1]));
+        g->prof_ms[g->prof_kind] += ms;
+    }
+    g->prof_launches[g->prof_kind] += rem;
+    return GT_OK;
+}
+gt_status prof_prepare(gt_graph *g) {
+    if (g->profiling && g->ev.empty()) {
+        g->ev.resize(512);
+        for (auto &e : g->ev) GT_HIP(hipEventCreate(&e));
+    }
+    return GT_OK;
+}
+
+}  // namespace gt
+
+using namespace gt;
+
+extern "C" {
+
+const char *gt_last_error(void) { return g_last_error.c_str(); }
+
+gt_status gt_device_count(int32_t *count) {
+    clear_error();
+    if (!count) return fail(GT_ERR_INVALID_ARG, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return GT_OK;
+}
+
+gt_status gt_graph_upload(uint64_t graph_id, uint32_t n, uint64_t edge_bound,
+                          const uint32_t *in_off, const uint32_t *in_other,
+                          const uint32_t *in_eid, const uint32_t *out_off,
+                          const uint32_t *out_other, const uint32_t *out_eid,
+                          gt_graph **out) {
+    clear_error();
+    if (!out || !in_off || !out_off) return fail(GT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (n == 0xffffffffu) return fail(GT_ERR_INVALID_ARG, "vertex count exceeds the index width");
+    uint64_t m = in_off[n];
+    if (out_off[n] != m)
+        return fail(GT_ERR_INVALID_ARG, "in-halves and out-halves hold different edge counts");
+    if (m > 0xffffffffull) return fail(GT_ERR_INVALID_ARG, "edge count exceeds the index width");
+    if (m && (!in_other || !in_eid || !out_other || !out_eid))
+        return fail(GT_ERR_INVALID_ARG, "null adjacency array");
+    GT_TRY(check_orient("in-halves", n, m, edge_bound, in_off, in_other, in_eid));
+    GT_TRY(check_orient("out-halves", n, m, edge_bound, out_off, out_other, out_eid));
+    gt_graph *g = nullptr;
+    GT_TRY(graph_new(graph_id, n, m, edge_bound, &g));
+    gt_status st = upload_orient(g->in, n, m, in_off, in_other, in_eid);
+    if (st == GT_OK) st = upload_orient(g->out, n, m, out_off, out_other, out_eid);
+    if (st != GT_OK) {
+        gt_graph_free(g);
+        return st;
+    }
+    *out = g;
+    return GT_OK;
+}
+
+gt_status gt_graph_build_edges(uint64_t graph_id, uint32_t n, uint64_t m, const uint32_t *src,
+                               const uint32_t *tgt, int32_t loc, gt_graph **out) {
+    clear_error();
+    (void)graph_id; (void)n; (void)m; (void)src; (void)tgt; (void)loc; (void)out;
+    return fail(GT_ERR_UNSUPPORTED, "device builder not available in this build");
+}
+
+gt_status gt_graph_free(gt_graph *g) {
+    if (!g) return GT_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (hipEvent_t e : g->ev) (void)hipEventDestroy(e);
+    if (g->h_state) (void)hipHostFree(g->h_state);
+    if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+    delete g;
+    return GT_OK;
+}
+
+gt_status gt_graph_info(const gt_graph *g, uint32_t *n, uint64_t *m, uint64_t *edge_bound,
+                        uint64_t *graph_id) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (edge_bound) *edge_bound = g->edge_bound;
+    if (graph_id) *graph_id = g->id;
+    return GT_OK;
+}
+
+gt_status gt_graph_download(const gt_graph *g, uint32_t *in_off, uint32_t *in_other,
+                            uint32_t *in_eid, uint32_t *out_off, uint32_t *out_other,
+                            uint32_t *out_eid) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    GT_HIP(hipStreamSynchronize(g->stream));
+    size_t nb = sizeof(uint32_t) * ((size_t)g->n + 1), mb = sizeof(uint32_t) * g->m;
+    if (in_off) GT_HIP(hipMemcpy(in_off, g->in.off.ptr, nb, hipMemcpyDeviceToHost));
+    if (out_off) GT_HIP(hipMemcpy(out_off, g->out.off.ptr, nb, hipMemcpyDeviceToHost));
+    if (g->m) {
+        if (in_other) GT_HIP(hipMemcpy(in_other, g->in.idx.ptr, mb, hipMemcpyDeviceToHost));
+        if (in_eid) GT_HIP(hipMemcpy(in_eid, g->in.eid.ptr, mb, hipMemcpyDeviceToHost));
+        if (out_other) GT_HIP(hipMemcpy(out_other, g->out.idx.ptr, mb, hipMemcpyDeviceToHost));
+        if (out_eid) GT_HIP(hipMemcpy(out_eid, g->out.eid.ptr, mb, hipMemcpyDeviceToHost));
+    }
+    return GT_OK;
+}
+
+gt_status gt_graph_set_stream(gt_graph *g, void *hip_stream) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    g->stream = hip_stream ? (hipStream_t)hip_stream : g->own_stream;
+    return GT_OK;
+}
+
+}  // extern "C"

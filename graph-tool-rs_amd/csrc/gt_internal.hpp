@@ -1,0 +1,172 @@
+// Internal declarations shared by the translation units of libgt_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "gt_hip.h"
+
+namespace gt {
+
+// ---------------------------------------------------------------- errors
+
+// Per-thread message behind gt_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+struct Status {
+    gt_status code = GT_OK;
+};
+
+// Early-return helpers.  GT_TRY propagates a gt_status; GT_HIP converts a
+// failed HIP call into GT_ERR_HIP (GT_ERR_OOM for allocation failures).
+#define GT_TRY(expr)                          \
+    do {                                      \
+        gt_status _st = (expr);               \
+        if (_st != GT_OK) return _st;         \
+    } while (0)
+
+gt_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define GT_HIP(call)                                                   \
+    do {                                                               \
+        hipError_t _e = (call);                                        \
+        if (_e != hipSuccess) return ::gt::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+gt_status fail(gt_status code, const std::string &msg);
+
+// ---------------------------------------------------------------- memory
+
+// Owning device buffer (hipMalloc / hipFree).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    // Grow to at least `b` bytes (contents are not preserved).
+    gt_status reserve(size_t b);
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ---------------------------------------------------------------- schedule
+
+// One unit of SpMV work for a workgroup.  Either a run of whole rows
+// [row_begin, row_end) whose in-entries are [nnz_begin, nnz_end), or -- when
+// row_end == kSplit -- one chunk [nnz_begin, nnz_end) of a long row whose
+// partial sum goes to slot row_begin.
+struct Tile {
+    uint32_t row_begin;
+    uint32_t row_end;
+    uint32_t nnz_begin;
+    uint32_t nnz_end;
+};
+constexpr uint32_t kSplit = 0xffffffffu;
+
+// Rows longer than kTileNnz are split into chunks of kTileNnz entries.
+constexpr uint32_t kTileNnz = 2048;
+// At most this many rows per tile (4 per thread of a 256-thread workgroup).
+constexpr uint32_t kTileRows = 1024;
+constexpr int kBlock = 256;
+
+// One orientation of the adjacency: row r lists the entries a kernel pulls
+// from (in-halves for the CSC, out-halves for the CSR), in storage order.
+struct Orient {
+    DevBuf off;   // uint32[n+1]
+    DevBuf idx;   // uint32[m]: the other endpoint
+    DevBuf eid;   // uint32[m]: edge id
+    std::vector<uint32_t> h_off;  // host copy of off, for the schedule
+    // SpMV schedule over this orientation's rows.
+    DevBuf tiles;            // Tile[ntiles]
+    DevBuf long_rows;        // uint32[nlong]: row of each split row
+    DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
+    uint32_t ntiles = 0, nlong = 0, nslots = 0;
+    bool scheduled = false;
+    // Rows too long for one thread's sequential degree sum.
+    DevBuf heavy;
+    uint32_t nheavy = 0;
+    bool heavy_ready = false;
+};
+
+struct Workspace {
+    DevBuf x[2];        // f64 iterates
+    DevBuf c[2];        // f64 contributions x/deg
+    DevBuf x32[2];      // f32 iterates
+    DevBuf c32[2];      // f32 contributions
+    DevBuf y3;          // staged f64 map / third eigenvector buffer
+    DevBuf y[3];        // eigenvector iterates
+    DevBuf deg;         // weighted out-degree, f64[n]
+    DevBuf wsrc;        // staged weight map (edge-id order)
+    DevBuf wcsc;        // weights in pull order, value type of the call
+    DevBuf wcsc64;      // weights in pull order, f64 (refinement sweeps)
+    DevBuf active;      // uint8[n]
+    DevBuf partials;    // f64[2 * grid]
+    DevBuf long_part;   // f64[nslots]
+    DevBuf state;       // device SweepState
+    DevBuf pers;        // staged personalisation
+    DevBuf masks;       // staged masks
+    DevBuf scratch_graph[6];  // filtered CSC/CSR
+};
+
+struct gt_graph_impl;
+
+}  // namespace gt
+
+// Device-side loop state, shared by every sweep launch of one solve.
+struct SweepState {
+    double p_sink;      // dangling mass entering the next sweep (PageRank)
+    double delta;       // L1 change of the last sweep
+    double norm;        // eigenvector: norm of the last unnormalised iterate
+    double norm_prev;   // eigenvector: norm of the one before
+    double aux;         // spare reduction slot
+    uint64_t iter;      // sweeps completed
+    uint32_t done;      // set once converged or capped
+    uint32_t counter;   // arrival ticket for the last-workgroup fold
+    uint32_t launches;  // launches that found work
+    uint32_t pad;
+};
+
+struct gt_graph {
+    uint64_t id = 0;
+    uint32_t n = 0;
+    uint64_t m = 0;
+    uint64_t edge_bound = 0;
+    gt::Orient in;   // CSC: in-halves
+    gt::Orient out;  // CSR: out-halves
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    gt::Workspace ws;
+    int grid = 0;           // workgroups of a sweep launch
+    bool profiling = false;
+    // Summed event time and launch count of sweep launches, per kernel kind
+    // (gt_sweep_kind); prof_kind is the kind being launched.
+    double prof_ms[4] = {0, 0, 0, 0};
+    uint64_t prof_launches[4] = {0, 0, 0, 0};
+    int prof_kind = 0;
+    uint64_t last_sweeps = 0;
+    std::vector<hipEvent_t> ev;  // profiling events
+    SweepState *h_state = nullptr;  // pinned mirror of the device state
+};
+
+namespace gt {
+
+// Build (once) the SpMV schedule of an orientation from its host offsets.
+gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n);
+// Allocate a graph shell with its stream.
+gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
+                    gt_graph **out);
+// Workgroups per sweep launch for a schedule of `ntiles` tiles.
+int sweep_grid(uint32_t ntiles);
+
+}  // namespace gt

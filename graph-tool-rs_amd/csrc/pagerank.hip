@@ -1,0 +1,575 @@
+// PageRank power iteration on gfx950.
+//
+// Semantics follow crates/gt-algo/src/centrality.rs:261-384 (pagerank_with):
+// per active vertex i, r = p_sink * pers(i) + sum over in-entries (s, e) with
+// deg(s) != 0 of prev(s) * w(e) / deg(s); next(i) = (1-d) pers(i) + d r; the
+// sweep's L1 change decides convergence; inactive slots are carried; sinks
+// (deg == 0) feed p_sink; an odd sweep count leaves the latest iterate in the
+// caller's map (:378-382).
+//
+// Data layout per sweep (pull form, one launch per sweep):
+//   in_off  u32[n+1], in_idx u32[m]   CSC in storage order, streamed once
+//   c_cur   V[n]                      gathered: c(s) = x(s)/deg(s), 0 for sinks
+//   x_cur   V[n] read, x_next V[n] written, c_next V[n] written
+//   out_off u32[n+1]                  unity out-degree (weighted: deg f64[n])
+// The tile engine (sweep_common.hpp) does the gather and row sums.  Rows longer
+// than a tile are split; their partial sums are combined, in chunk order, by
+// the last workgroup to finish, which also folds the per-workgroup delta and
+// dangling-mass partials in index order and decides convergence on the
+// device.  Launches after convergence exit at once.
+#include <cmath>
+#include <type_traits>
+
+#include "device_common.hpp"
+#include "gt_internal.hpp"
+#include "props.hpp"
+#include "driver.hpp"
+#include "sweep_common.hpp"
+
+namespace gt {
+
+struct PrArgs {
+    const Tile *tiles;
+    uint32_t ntiles;
+    uint32_t nlong;
+    const uint32_t *in_off;
+    const uint32_t *in_idx;
+    const void *w;            // weights in pull order (value type), weighted only
+    const uint32_t *out_off;  // unity out-degree source
+    const double *deg;        // weighted out-degree
+    const uint8_t *active;    // masked views only
+    const double *pers;       // dense personalisation
+    double pval;              // constant personalisation
+    double damping;
+    double eps_abs;           // stop when delta < max(eps_abs, eps_rel * mass)
+    double eps_rel;
+    uint64_t max_iter;        // cap on st->iter; 0 = none
+    const void *x_cur;
+    void *x_next;
+    const void *c_cur;
+    void *c_next;
+    double *partials;         // [2 * gridDim.x]
+    double *long_part;        // [nslots]
+    const uint32_t *long_rows;
+    const uint32_t *long_slot;
+    SweepState *st;
+};
+
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+struct PrFin {
+    const PrArgs &a;
+    double p_sink;
+    double dsum = 0.0;
+    double ssum = 0.0;
+    __device__ PrFin(const PrArgs &args, double ps) : a(args), p_sink(ps) {}
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
+        const V *xc = static_cast<const V *>(a.x_cur);
+        V *xn = static_cast<V *>(a.x_next);
+        V *cn = static_cast<V *>(a.c_next);
+        if (MASK && !a.active[i]) {
+            xn[i] = xc[i];
+            cn[i] = V(0);
+            return;
+        }
+        const double pv = DPERS ? a.pers[i] : a.pval;
+        const double nv = (1.0 - a.damping) * pv + a.damping * (p_sink * pv + acc);
+        const V nvv = static_cast<V>(nv);
+        dsum += fabs(static_cast<double>(nvv) - static_cast<double>(xc[i]));
+        xn[i] = nvv;
+        const double dg =
+            WEIGHTED ? a.deg[i] : static_cast<double>(a.out_off[i + 1] - a.out_off[i]);
+        if (dg != 0.0) {
+            cn[i] = static_cast<V>(static_cast<double>(nvv) / dg);
+        } else {
+            cn[i] = V(0);
+            ssum += static_cast<double>(nvv);
+        }
+    }
+};
+
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
+    if (a.st->done) return;
+    __shared__ V lds[kTileNnz];
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    PrFin<V, WEIGHTED, DPERS, MASK> fin(a, a.st->p_sink);
+    spmv_tiles<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, static_cast<const V *>(a.c_cur),
+                            static_cast<const V *>(a.w), a.long_part, lds, red, fin);
+
+    double dsum = fin.dsum, ssum = fin.ssum;
+    block_sum2(dsum, ssum, red);
+    if (tid == 0) {
+        a.partials[blockIdx.x] = dsum;
+        a.partials[gridDim.x + blockIdx.x] = ssum;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+
+    // The last workgroup: finish the split rows, fold everything in index order.
+    PrFin<V, WEIGHTED, DPERS, MASK> lf(a, fin.p_sink);
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        lf.dsum += a.partials[k];
+        lf.ssum += a.partials[gridDim.x + k];
+    }
+    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+        lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
+    double d = lf.dsum, s = lf.ssum;
+    block_sum2(d, s, red);
+    if (tid == 0) {
+        SweepState *st = a.st;
+        const uint64_t it = st->iter + 1;
+        const double thr = fmax(a.eps_abs, a.eps_rel * st->aux);
+        st->delta = d;
+        st->p_sink = s;
+        st->iter = it;
+        st->launches += 1;
+        st->done = (!(d >= thr) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
+        st->counter = 0;
+        __threadfence();
+    }
+}
+
+// Seed slots [had, n) of the f64 map with pers, load the initial iterate into
+// the value type, compute c0 = x0/deg, the first dangling mass and the total
+// mass; resets the loop state (iter = `iter0`).
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, uint64_t had,
+                                                  uint32_t n, uint64_t iter0) {
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    V *x0 = static_cast<V *>(a.x_next);
+    V *c0 = static_cast<V *>(a.c_next);
+    double ssum = 0.0, mass = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        double xi;
+        if (i >= had) {
+            xi = DPERS ? a.pers[i] : a.pval;
+            rank[i] = xi;
+        } else {
+            xi = rank[i];
+        }
+        const V xv = static_cast<V>(xi);
+        x0[i] = xv;
+        if (MASK && !a.active[i]) {
+            c0[i] = V(0);
+            continue;
+        }
+        mass += fabs(static_cast<double>(xv));
+        const double dg =
+            WEIGHTED ? a.deg[i] : static_cast<double>(a.out_off[i + 1] - a.out_off[i]);
+        if (dg != 0.0) {
+            c0[i] = static_cast<V>(static_cast<double>(xv) / dg);
+        } else {
+            c0[i] = V(0);
+            ssum += static_cast<double>(xv);
+        }
+    }
+    block_sum2(ssum, mass, red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = ssum;
+        a.partials[gridDim.x + blockIdx.x] = mass;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    double s = 0.0, ms = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        s += a.partials[k];
+        ms += a.partials[gridDim.x + k];
+    }
+    block_sum2(s, ms, red);
+    if (threadIdx.x == 0) {
+        SweepState *st = a.st;
+        st->p_sink = s;
+        st->aux = ms;
+        st->delta = 0.0;
+        st->iter = iter0;
+        st->done = 0;
+        st->launches = 0;
+        st->counter = 0;
+        __threadfence();
+    }
+}
+
+// x64[i] = x32[i]: hand the FP32 iterate to the FP64 refinement sweeps.
+__global__ __launch_bounds__(kBlock) void widen_iterate(const float *__restrict__ x32, uint32_t n,
+                                                        double *__restrict__ x64) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        x64[i] = static_cast<double>(x32[i]);
+}
+
+// ------------------------------------------------------------------ host
+
+namespace {
+
+struct PrPlan {
+    bool weighted = false;
+    bool dpers = false;
+    bool masked = false;
+};
+
+template <typename V, bool W, bool P, bool M>
+void launch_sweep(const PrArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+template <typename V, bool W, bool P, bool M>
+void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64_t iter0,
+                 hipStream_t s) {
+    uint32_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL((pr_init<V, W, P, M>), dim3(blocks), dim3(kBlock), 0, s, a, rank, had, n,
+                       iter0);
+}
+
+template <typename V>
+void dispatch_sweep(const PrPlan &p, const PrArgs &a, int grid, hipStream_t s) {
+    const int key = (p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0);
+    switch (key) {
+        case 0: launch_sweep<V, false, false, false>(a, grid, s); break;
+        case 1: launch_sweep<V, false, false, true>(a, grid, s); break;
+        case 2: launch_sweep<V, false, true, false>(a, grid, s); break;
+        case 3: launch_sweep<V, false, true, true>(a, grid, s); break;
+        case 4: launch_sweep<V, true, false, false>(a, grid, s); break;
+        case 5: launch_sweep<V, true, false, true>(a, grid, s); break;
+        case 6: launch_sweep<V, true, true, false>(a, grid, s); break;
+        default: launch_sweep<V, true, true, true>(a, grid, s); break;
+    }
+}
+
+template <typename V>
+void dispatch_init(const PrPlan &p, const PrArgs &a, double *rank, uint64_t had, uint32_t n,
+                   uint64_t iter0, hipStream_t s) {
+    const int key = (p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0);
+    switch (key) {
+        case 0: launch_init<V, false, false, false>(a, rank, had, n, iter0, s); break;
+        case 1: launch_init<V, false, false, true>(a, rank, had, n, iter0, s); break;
+        case 2: launch_init<V, false, true, false>(a, rank, had, n, iter0, s); break;
+        case 3: launch_init<V, false, true, true>(a, rank, had, n, iter0, s); break;
+        case 4: launch_init<V, true, false, false>(a, rank, had, n, iter0, s); break;
+        case 5: launch_init<V, true, false, true>(a, rank, had, n, iter0, s); break;
+        case 6: launch_init<V, true, true, false>(a, rank, had, n, iter0, s); break;
+        default: launch_init<V, true, true, true>(a, rank, had, n, iter0, s); break;
+    }
+}
+
+}  // namespace
+
+gt_status resolve_view(gt_graph *g, const gt_view *view, Orient **pull, Orient **push,
+                       bool *reversed) {
+    int kind = view ? view->kind : GT_VIEW_DIRECTED;
+    // The message of ViewError::NotDirected (crates/gt-py/src/dispatch.rs:176-179).
+    if (kind == GT_VIEW_UNDIRECTED || kind == GT_VIEW_UNDIRECTED_FILTERED)
+        return fail(GT_ERR_NOT_DIRECTED,
+                    std::string("view: ") +
+                        (kind == GT_VIEW_UNDIRECTED ? "Undirected" : "UndirectedFiltered") +
+                        " has no in-edges; this kernel accepts [Directed, Reversed, "
+                        "DirectedFiltered, ReversedFiltered]");
+    if (kind == GT_VIEW_DIRECTED_FILTERED || kind == GT_VIEW_REVERSED_FILTERED)
+        return fail(GT_ERR_UNSUPPORTED, "filtered views are not served by this build yet");
+    if (kind != GT_VIEW_DIRECTED && kind != GT_VIEW_REVERSED)
+        return fail(GT_ERR_INVALID_ARG, "unknown view kind");
+    *reversed = (kind == GT_VIEW_REVERSED);
+    *pull = *reversed ? &g->out : &g->in;
+    *push = *reversed ? &g->in : &g->out;
+    return GT_OK;
+}
+
+gt_status check_vertex_out(gt_graph *g, const gt_vertex_out *r, const char *who) {
+    if (!r || (!r->data && g->n)) return fail(GT_ERR_INVALID_ARG, "output map is null");
+    if (r->graph_id && r->graph_id != g->id)
+        return fail(GT_ERR_WRONG_GRAPH, std::string(who) + " map does not belong to this graph");
+    if (r->len < g->n)
+        return fail(GT_ERR_INVALID_ARG, "output map is shorter than the vertex bound");
+    if (r->loc != GT_HOST && r->loc != GT_DEVICE) return fail(GT_ERR_INVALID_ARG, "bad location");
+    return GT_OK;
+}
+
+gt_status check_params(gt_graph *g, const gt_power_iteration *p) {
+    if (!p) return fail(GT_ERR_INVALID_ARG, "params is null");
+    if (p->plan.len != g->n)
+        return fail(GT_ERR_PLAN_COVER, "sweep: the plan must cover the output exactly");
+    if (p->plan.chunks == 0) return fail(GT_ERR_INVALID_ARG, "plan has zero chunks");
+    if (p->max_iter == 0 && !(p->epsilon > 0.0) && !std::isnan(p->epsilon))
+        return fail(GT_ERR_INVALID_ARG,
+                    "epsilon must be positive when max_iter is 0: the loop would never end");
+    return GT_OK;
+}
+
+namespace {
+
+// Everything a PageRank solve needs on the device, prepared once per call.
+struct PrSolve {
+    gt_graph *g = nullptr;
+    Orient *pull = nullptr, *push = nullptr;
+    PrPlan plan;
+    PrArgs base{};
+    double *rank64 = nullptr;  // the f64 map on the device (caller's or staged)
+    const void *wsrc = nullptr;
+    int wdtype = GT_F64;
+};
+
+gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                     const gt_prop *pers, double damping, const gt_power_iteration *params,
+                     gt_vertex_out *rank, PrSolve &S) {
+    S.g = g;
+    bool reversed = false;
+    GT_TRY(resolve_view(g, view, &S.pull, &S.push, &reversed));
+    GT_TRY(check_vertex_out(g, rank, "pagerank: rank"));
+    GT_TRY(check_params(g, params));
+    const uint32_t n = g->n;
+    hipStream_t s = g->stream;
+    const double uniform = n == 0 ? 0.0 : 1.0 / (double)n;
+    PrArgs &a = S.base;
+    GT_TRY(stage_vertex_prop(g, pers, g->ws.pers, &a.pers, &a.pval, &S.plan.dpers, uniform));
+    if (weight && weight->kind != GT_MAP_UNITY) {
+        S.plan.weighted = true;
+        GT_TRY(stage_edge_source(g, weight, &S.wsrc, &S.wdtype));
+        GT_TRY(stage_degree(g, weight, *S.push, S.wsrc, S.wdtype, g->ws.deg));
+        a.deg = g->ws.deg.as<double>();
+    }
+    if (n == 0) return GT_OK;
+    GT_TRY(ensure_schedule(g, *S.pull, n));
+    g->grid = sweep_grid(S.pull->ntiles);
+    GT_TRY(prof_prepare(g));
+
+    Workspace &ws = g->ws;
+    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, S.pull->nslots)));
+    GT_TRY(ws.state.reserve(sizeof(SweepState)));
+    GT_HIP(hipMemsetAsync(ws.state.ptr, 0, sizeof(SweepState), s));
+
+    if (rank->loc == GT_DEVICE) {
+        S.rank64 = rank->data;
+    } else {
+        GT_TRY(ws.y3.reserve(sizeof(double) * n));
+        S.rank64 = ws.y3.as<double>();
+        const uint64_t had = std::min<uint64_t>(rank->had, n);
+        if (had)
+            GT_HIP(hipMemcpyAsync(S.rank64, rank->data, sizeof(double) * had,
+                                  hipMemcpyHostToDevice, s));
+    }
+
+    a.tiles = S.pull->tiles.as<Tile>();
+    a.ntiles = S.pull->ntiles;
+    a.nlong = S.pull->nlong;
+    a.in_off = S.pull->off.as<uint32_t>();
+    a.in_idx = S.pull->idx.as<uint32_t>();
+    a.out_off = S.push->off.as<uint32_t>();
+    a.damping = damping;
+    a.eps_abs = params->epsilon;
+    a.eps_rel = 0.0;
+    a.max_iter = params->max_iter;
+    a.partials = ws.partials.as<double>();
+    a.long_part = ws.long_part.as<double>();
+    a.long_rows = S.pull->long_rows.as<uint32_t>();
+    a.long_slot = S.pull->long_slot.as<uint32_t>();
+    a.st = ws.state.as<SweepState>();
+    return GT_OK;
+}
+
+// Init + sweeps in value type V on buffers x / c; returns the sweep count
+// reached (st->iter) and leaves the latest iterate in x[count & 1].
+template <typename V>
+gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_t iter0,
+                   double eps_pred, uint64_t cap, uint64_t *iter_out) {
+    gt_graph *g = S.g;
+    hipStream_t s = g->stream;
+    g->prof_kind = std::is_same<V, double>::value ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
+    a.x_next = x[0];
+    a.c_next = c[0];
+    dispatch_init<V>(S.plan, a, S.rank64, had, g->n, iter0, s);
+    GT_HIP(hipGetLastError());
+    uint64_t launched = 0;
+    auto launch = [&](uint64_t k) {
+        const int par = (int)(k & 1);
+        PrArgs b = a;
+        b.x_cur = x[par];
+        b.x_next = x[par ^ 1];
+        b.c_cur = c[par];
+        b.c_next = c[par ^ 1];
+        dispatch_sweep<V>(S.plan, b, g->grid, s);
+    };
+    GT_TRY(drive(g, a.st, eps_pred, cap, launch, &launched));
+    *iter_out = g->h_state->iter;
+    return GT_OK;
+}
+
+// Copy the f64 iterate `latest` into the caller's map and wait.
+gt_status pr_deliver(gt_graph *g, const double *latest, gt_vertex_out *rank) {
+    hipStream_t s = g->stream;
+    const size_t b = sizeof(double) * g->n;
+    if (rank->loc == GT_DEVICE) {
+        if (latest != rank->data)
+            GT_HIP(hipMemcpyAsync(rank->data, latest, b, hipMemcpyDeviceToDevice, s));
+    } else {
+        GT_HIP(hipMemcpyAsync(rank->data, latest, b, hipMemcpyDeviceToHost, s));
+    }
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
+
+bool no_sweep_for(double eps) { return std::isnan(eps) || !(eps + 1.0 >= eps); }
+
+}  // namespace
+
+gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                            const gt_prop *pers, double damping,
+                            const gt_power_iteration *params, gt_vertex_out *rank,
+                            uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (!iters) return fail(GT_ERR_INVALID_ARG, "iters is null");
+    PrSolve S;
+    GT_TRY(pr_prepare(g, view, weight, pers, damping, params, rank, S));
+    const uint32_t n = g->n;
+    const bool skip = no_sweep_for(params->epsilon);
+    if (n == 0) {
+        *iters = skip ? 0 : 1;
+        g->last_sweeps = *iters;
+        return GT_OK;
+    }
+    Workspace &ws = g->ws;
+    const size_t vb = sizeof(double) * (size_t)n;
+    GT_TRY(ws.x[1].reserve(vb));
+    GT_TRY(ws.c[0].reserve(vb));
+    GT_TRY(ws.c[1].reserve(vb));
+    if (S.plan.weighted) {
+        GT_TRY(stage_csc_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
+        S.base.w = ws.wcsc64.ptr;
+    }
+    double *x[2] = {S.rank64, ws.x[1].as<double>()};
+    double *c[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
+    uint64_t sweeps = 0;
+    if (skip) {
+        // Seed only.
+        PrArgs a = S.base;
+        a.x_next = x[0];
+        a.c_next = c[0];
+        dispatch_init<double>(S.plan, a, S.rank64, rank->had, n, 0, g->stream);
+        GT_HIP(hipGetLastError());
+    } else {
+        GT_TRY(pr_phase<double>(S, S.base, x, c, rank->had, 0, params->epsilon, params->max_iter,
+                                &sweeps));
+    }
+    g->last_sweeps = sweeps;
+    *iters = sweeps;
+    return pr_deliver(g, x[sweeps & 1], rank);
+}
+
+// FP32 sweeps (values, weights, SpMV accumulation) with FP64 dangling mass,
+// teleport update and L1 test, until delta < max(eps, 2^-22 * mass) -- below
+// that the L1 change of an FP32 iterate is rounding noise -- then `refine`
+// FP64 sweeps from the widened FP32 iterate.
+gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                              const gt_prop *pers, double damping,
+                              const gt_power_iteration *params, gt_vertex_out *rank,
+                              uint64_t refine, uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (!iters) return fail(GT_ERR_INVALID_ARG, "iters is null");
+    PrSolve S;
+    GT_TRY(pr_prepare(g, view, weight, pers, damping, params, rank, S));
+    const uint32_t n = g->n;
+    const bool skip = no_sweep_for(params->epsilon);
+    if (n == 0 || skip) {
+        if (n) {
+            // Seeding only: reuse the FP64 path.
+            return pagerank_f64_impl(g, view, weight, pers, damping, params, rank, iters);
+        }
+        *iters = skip ? 0 : 1;
+        g->last_sweeps = *iters;
+        return GT_OK;
+    }
+    Workspace &ws = g->ws;
+    hipStream_t s = g->stream;
+    for (int k = 0; k < 2; ++k) {
+        GT_TRY(ws.x32[k].reserve(sizeof(float) * n));
+        GT_TRY(ws.c32[k].reserve(sizeof(float) * n));
+    }
+    float *x32[2] = {ws.x32[0].as<float>(), ws.x32[1].as<float>()};
+    float *c32[2] = {ws.c32[0].as<float>(), ws.c32[1].as<float>()};
+    PrArgs a32 = S.base;
+    if (S.plan.weighted) {
+        GT_TRY(stage_csc_weights<float>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc));
+        a32.w = ws.wcsc.ptr;
+    }
+    a32.eps_rel = std::ldexp(1.0, -22);
+    uint64_t cap32 = params->max_iter;
+    uint64_t it32 = 0;
+    GT_TRY(pr_phase<float>(S, a32, x32, c32, rank->had, 0,
+                           std::max(params->epsilon, std::ldexp(1.0, -22)), cap32, &it32));
+    // Widen the FP32 iterate into the caller's f64 map.
+    {
+        uint32_t blocks = std::min<uint32_t>((n + kBlock - 1) / kBlock, 2048);
+        hipLaunchKernelGGL(widen_iterate, dim3(blocks), dim3(kBlock), 0, s, x32[it32 & 1], n,
+                           S.rank64);
+        GT_HIP(hipGetLastError());
+    }
+    uint64_t total = it32;
+    const double *latest = S.rank64;
+    const bool capped = params->max_iter > 0 && it32 >= params->max_iter;
+    if (refine > 0 && !capped) {
+        GT_TRY(ws.x[1].reserve(sizeof(double) * n));
+        GT_TRY(ws.c[0].reserve(sizeof(double) * n));
+        GT_TRY(ws.c[1].reserve(sizeof(double) * n));
+        PrArgs a64 = S.base;
+        if (S.plan.weighted) {
+            GT_TRY(stage_csc_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
+            a64.w = ws.wcsc64.ptr;
+        }
+        uint64_t cap64 = refine;
+        if (params->max_iter > 0) cap64 = std::min<uint64_t>(refine, params->max_iter - it32);
+        a64.max_iter = it32 + cap64;
+        double *x64[2] = {S.rank64, ws.x[1].as<double>()};
+        double *c64[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
+        uint64_t it = 0;
+        GT_TRY(pr_phase<double>(S, a64, x64, c64, n, it32, params->epsilon, cap64, &it));
+        total = it;
+        latest = x64[(it - it32) & 1];
+    }
+    g->last_sweeps = total;
+    *iters = total;
+    return pr_deliver(g, latest, rank);
+}
+
+}  // namespace gt
+
+using namespace gt;
+
+extern "C" {
+
+gt_status gt_pagerank_f64(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                          const gt_prop *pers, double damping, const gt_power_iteration *params,
+                          gt_vertex_out *rank, uint64_t *iters) {
+    return pagerank_f64_impl(g, view, weight, pers, damping, params, rank, iters);
+}
+
+gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                            const gt_prop *pers, double damping, const gt_power_iteration *params,
+                            gt_vertex_out *rank, uint64_t *iters) {
+    return pagerank_mixed_impl(g, view, weight, pers, damping, params, rank, 1, iters);
+}
+
+gt_status gt_graph_set_profiling(gt_graph *g, int32_t enable) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    g->profiling = enable != 0;
+    for (int k = 0; k < 4; ++k) {
+        g->prof_ms[k] = 0.0;
+        g->prof_launches[k] = 0;
+    }
+    return GT_OK;
+}
+
+gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms, uint64_t *launches,
+                         uint64_t *last_sweeps) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (kind < 0 || kind > 3) return fail(GT_ERR_INVALID_ARG, "unknown sweep kind");
+    if (total_ms) *total_ms = g->prof_ms[kind];
+    if (launches) *launches = g->prof_launches[kind];
+    if (last_sweeps) *last_sweeps = g->last_sweeps;
+    return GT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// Staging of the property maps a sweep reads: edge weights permuted into the
+// pull orientation's storage order, the weighted out-degree, and vertex maps
+// (personalisation) as f64 device arrays.
+//
+// The weighted out-degree restates crates/gt-algo/src/centrality.rs:281-297:
+// deg(v) = sum of w(e) over v's out-entries in storage order, starting from
+// 0.0.  Rows up to kSeqDeg entries are summed sequentially by one thread, so
+// the value is the reference's bit for bit; longer rows use a fixed
+// shuffle-tree order (deterministic, within rounding of the sequential sum).
+#include "device_common.hpp"
+#include "gt_internal.hpp"
+#include "props.hpp"
+
+namespace gt {
+
+constexpr uint32_t kSeqDeg = 4096;
+
+template <typename S>
+__device__ __forceinline__ double read_w(const void *w, uint32_t e, int kind, double value) {
+    if (kind == GT_MAP_CONSTANT) return value;
+    return static_cast<double>(static_cast<const S *>(w)[e]);
+}
+
+// One thread per row; rows longer than kSeqDeg are left for deg_long.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void deg_short(const uint32_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ eid,
+                                                    const void *w, int kind, double value,
+                                                    uint32_t n, double *__restrict__ deg) {
+    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+        const uint32_t lo = off[v], hi = off[v + 1];
+        if (hi - lo > kSeqDeg) continue;
+        double acc = 0.0;
+        for (uint32_t j = lo; j < hi; ++j) acc += read_w<S>(w, eid[j], kind, value);
+        deg[v] = acc;
+    }
+}
+
+// One workgroup per long row (list computed on the host).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void deg_long(const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ eid,
+                                                   const void *w, int kind, double value,
+                                                   const uint32_t *__restrict__ rows,
+                                                   double *__restrict__ deg) {
+    __shared__ double red[8];
+    const uint32_t v = rows[blockIdx.x];
+    const uint32_t lo = off[v], hi = off[v + 1];
+    double acc = 0.0;
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) acc += read_w<S>(w, eid[j], kind, value);
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) deg[v] = acc;
+}
+
+// wcsc[j] = w(eid[j]) in the value type of the sweep.
+template <typename S, typename V>
+__global__ __launch_bounds__(kBlock) void permute_w(const uint32_t *__restrict__ eid,
+                                                    const void *w, int kind, double value,
+                                                    uint64_t m, V *__restrict__ out) {
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < m;
+         j += (uint64_t)gridDim.x * kBlock)
+        out[j] = static_cast<V>(read_w<S>(w, eid[j], kind, value));
+}
+
+__global__ __launch_bounds__(kBlock) void f32_to_f64(const float *__restrict__ a, uint64_t n,
+                                                     double *__restrict__ b) {
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n;
+         j += (uint64_t)gridDim.x * kBlock)
+        b[j] = static_cast<double>(a[j]);
+}
+
+static unsigned grid_for(uint64_t n) {
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b > 4096) b = 4096;
+    return b ? (unsigned)b : 1u;
+}
+
+// Long rows of an orientation (> kSeqDeg entries), cached.
+static gt_status heavy_rows(Orient &o, uint32_t n) {
+    if (o.heavy_ready) return GT_OK;
+    std::vector<uint32_t> rows;
+    for (uint32_t v = 0; v < n; ++v)
+        if (o.h_off[v + 1] - o.h_off[v] > kSeqDeg) rows.push_back(v);
+    o.nheavy = (uint32_t)rows.size();
+    GT_TRY(o.heavy.reserve(sizeof(uint32_t) * std::max<size_t>(1, rows.size())));
+    if (!rows.empty())
+        GT_HIP(hipMemcpy(o.heavy.ptr, rows.data(), sizeof(uint32_t) * rows.size(),
+                         hipMemcpyHostToDevice));
+    o.heavy_ready = true;
+    return GT_OK;
+}
+
+gt_status stage_edge_source(gt_graph *g, const gt_prop *w, const void **src, int *dtype) {
+    if (w->kind == GT_MAP_CONSTANT) {
+        *src = nullptr;
+        *dtype = GT_F64;
+        return GT_OK;
+    }
+    if (w->kind != GT_MAP_DENSE) return fail(GT_ERR_INVALID_ARG, "unknown weight map kind");
+    if (!w->data && g->edge_bound) return fail(GT_ERR_INVALID_ARG, "weight map data is null");
+    if (w->len < g->edge_bound)
+        return fail(GT_ERR_INVALID_ARG, "weight map is shorter than the edge bound");
+    if (w->graph_id && w->graph_id != g->id)
+        return fail(GT_ERR_WRONG_GRAPH, "weight map does not belong to this graph");
+    if (w->dtype != GT_F64 && w->dtype != GT_F32)
+        return fail(GT_ERR_INVALID_ARG, "weight map dtype must be f64 or f32");
+    *dtype = w->dtype;
+    if (w->loc == GT_DEVICE) {
+        *src = w->data;
+        return GT_OK;
+    }
+    const size_t bytes = (w->dtype == GT_F64 ? 8 : 4) * (size_t)g->edge_bound;
+    GT_TRY(g->ws.wsrc.reserve(bytes));
+    if (bytes) GT_HIP(hipMemcpyAsync(g->ws.wsrc.ptr, w->data, bytes, hipMemcpyHostToDevice, g->stream));
+    *src = g->ws.wsrc.ptr;
+    return GT_OK;
+}
+
+template <typename S>
+static gt_status launch_deg(gt_graph *g, Orient &push, const void *src, int kind, double value,
+                            double *deg) {
+    hipStream_t s = g->stream;
+    const uint32_t n = g->n;
+    hipLaunchKernelGGL((deg_short<S>), dim3(grid_for(n)), dim3(kBlock), 0, s,
+                       push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value, n, deg);
+    GT_TRY(heavy_rows(push, n));
+    if (push.nheavy)
+        hipLaunchKernelGGL((deg_long<S>), dim3(push.nheavy), dim3(kBlock), 0, s,
+                           push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value,
+                           push.heavy.as<uint32_t>(), deg);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+template <typename S, typename V>
+static gt_status launch_permute(gt_graph *g, Orient &pull, const void *src, int kind, double value,
+                                V *out) {
+    hipLaunchKernelGGL((permute_w<S, V>), dim3(grid_for(g->m)), dim3(kBlock), 0, g->stream,
+                       pull.eid.as<uint32_t>(), src, kind, value, (uint64_t)g->m, out);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
+gt_status stage_degree(gt_graph *g, const gt_prop *w, Orient &push, const void *src, int dtype,
+                       DevBuf &out) {
+    GT_TRY(out.reserve(sizeof(double) * std::max<uint32_t>(1, g->n)));
+    if (dtype == GT_F32)
+        return launch_deg<float>(g, push, src, w->kind, w->value, out.as<double>());
+    return launch_deg<double>(g, push, src, w->kind, w->value, out.as<double>());
+}
+
+template <typename V>
+gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
+                            int dtype, DevBuf &out) {
+    GT_TRY(out.reserve(sizeof(V) * std::max<uint64_t>(1, g->m)));
+    if (dtype == GT_F32)
+        return launch_permute<float, V>(g, pull, src, w->kind, w->value, out.as<V>());
+    return launch_permute<double, V>(g, pull, src, w->kind, w->value, out.as<V>());
+}
+template gt_status stage_csc_weights<double>(gt_graph *, const gt_prop *, Orient &, const void *,
+                                             int, DevBuf &);
+template gt_status stage_csc_weights<float>(gt_graph *, const gt_prop *, Orient &, const void *,
+                                            int, DevBuf &);
+
+gt_status stage_vertex_prop(gt_graph *g, const gt_prop *pm, DevBuf &buf, const double **dev,
+                            double *scalar, bool *dense, double default_value) {
+    *dev = nullptr;
+    *dense = false;
+    if (!pm) {
+        *scalar = default_value;
+        return GT_OK;
+    }
+    if (pm->kind == GT_MAP_UNITY) {
+        *scalar = 1.0;
+        return GT_OK;
+    }
+    if (pm->kind == GT_MAP_CONSTANT) {
+        *scalar = pm->value;
+        return GT_OK;
+    }
+    if (pm->kind != GT_MAP_DENSE) return fail(GT_ERR_INVALID_ARG, "unknown vertex map kind");
+    const uint32_t n = g->n;
+    if (pm->len < n) return fail(GT_ERR_INVALID_ARG, "vertex map is shorter than the vertex bound");
+    if (!pm->data && n) return fail(GT_ERR_INVALID_ARG, "vertex map data is null");
+    if (pm->graph_id && pm->graph_id != g->id)
+        return fail(GT_ERR_WRONG_GRAPH, "vertex map does not belong to this graph");
+    *dense = true;
+    if (pm->dtype == GT_F64 && pm->loc == GT_DEVICE) {
+        *dev = static_cast<const double *>(pm->data);
+        return GT_OK;
+    }
+    GT_TRY(buf.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
+    if (pm->dtype == GT_F64) {
+        if (n) GT_HIP(hipMemcpyAsync(buf.ptr, pm->data, sizeof(double) * n, hipMemcpyHostToDevice,
+                                     g->stream));
+    } else if (pm->dtype == GT_F32) {
+        const float *f = static_cast<const float *>(pm->data);
+        if (pm->loc == GT_HOST) {
+            std::vector<double> tmp(f, f + n);
+            GT_HIP(hipMemcpyAsync(buf.ptr, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                                  g->stream));
+            GT_HIP(hipStreamSynchronize(g->stream));
+        } else {
+            hipLaunchKernelGGL(f32_to_f64, dim3(grid_for(n)), dim3(kBlock), 0, g->stream, f,
+                               (uint64_t)n, buf.as<double>());
+            GT_HIP(hipGetLastError());
+        }
+    } else {
+        return fail(GT_ERR_INVALID_ARG, "vertex map dtype must be f64 or f32");
+    }
+    *dev = buf.as<double>();
+    return GT_OK;
+}
+
+}  // namespace gt

@@ -1,0 +1,32 @@
+// Property-map staging used by the centrality entry points.
+#pragma once
+
+#include "gt_internal.hpp"
+
+namespace gt {
+
+// Resolve a weight map to a device source indexed by edge id.  CONSTANT maps
+// yield src == nullptr (the kernels read w->value instead).
+gt_status stage_edge_source(gt_graph *g, const gt_prop *w, const void **src, int *dtype);
+
+// deg(v) = sum of w over the out-entries of `push` row v, in storage order.
+gt_status stage_degree(gt_graph *g, const gt_prop *w, Orient &push, const void *src, int dtype,
+                       DevBuf &out);
+
+// Weights permuted into the storage order of `pull`, converted to V.
+template <typename V>
+gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
+                            int dtype, DevBuf &out);
+
+// A vertex map as a scalar (UNITY / CONSTANT / absent -> default_value) or
+// an f64 device array.
+gt_status stage_vertex_prop(gt_graph *g, const gt_prop *pm, DevBuf &buf, const double **dev,
+                            double *scalar, bool *dense, double default_value);
+
+// Shared validation of the centrality entry points.
+gt_status resolve_view(gt_graph *g, const gt_view *view, Orient **pull, Orient **push,
+                       bool *reversed);
+gt_status check_vertex_out(gt_graph *g, const gt_vertex_out *r, const char *who);
+gt_status check_params(gt_graph *g, const gt_power_iteration *p);
+
+}  // namespace gt

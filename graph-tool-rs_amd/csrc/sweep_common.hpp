@@ -1,0 +1,100 @@
+// The pull-SpMV tile engine shared by the PageRank and eigenvector sweeps.
+//
+// A workgroup of 256 threads walks tiles blockIdx.x, blockIdx.x + gridDim.x,
+// ...  For each tile:
+//   phase 1  the tile's <= kTileNnz in-entries are streamed (coalesced u32
+//            index loads, eight per thread in flight) and the gathered source
+//            values -- times the CSC-ordered weight when weighted -- land in LDS;
+//   phase 2  rows are summed out of LDS: one thread per row when the tile's
+//            rows average fewer than 16 entries, one wavefront per row with a
+//            shuffle-tree reduction otherwise.  A split-row chunk is reduced by
+//            the whole workgroup into its partial slot.
+// `fin(row, acc)` is called exactly once per completed row, by one thread.
+#pragma once
+
+#include "device_common.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+
+template <typename V, bool WEIGHTED>
+__device__ __forceinline__ void gather_tile(const uint32_t *__restrict__ idx,
+                                            const V *__restrict__ src,
+                                            const V *__restrict__ w, uint32_t nb,
+                                            uint32_t cnt, V *lds) {
+    constexpr int K = kTileNnz / kBlock;
+    uint32_t s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t j = threadIdx.x + k * kBlock;
+        s[k] = j < cnt ? __builtin_nontemporal_load(idx + nb + j) : 0u;
+    }
+    V v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t j = threadIdx.x + k * kBlock;
+        v[k] = j < cnt ? src[s[k]] : V(0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t j = threadIdx.x + k * kBlock;
+        if (j < cnt) {
+            V x = v[k];
+            if (WEIGHTED) x *= __builtin_nontemporal_load(w + nb + j);
+            lds[j] = x;
+        }
+    }
+}
+
+template <typename V, bool WEIGHTED, typename Fin>
+__device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint32_t ntiles,
+                                           const uint32_t *__restrict__ off,
+                                           const uint32_t *__restrict__ idx,
+                                           const V *__restrict__ src, const V *__restrict__ w,
+                                           double *__restrict__ long_part, V *lds, double *red,
+                                           Fin &fin) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    for (uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile t = tiles[ti];
+        const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
+        gather_tile<V, WEIGHTED>(idx, src, w, nb, cnt, lds);
+        __syncthreads();
+        if (t.row_end == kSplit) {
+            double v = 0.0;
+            for (uint32_t k = tid; k < cnt; k += kBlock) v += static_cast<double>(lds[k]);
+            v = block_sum(v, red);  // ends with a barrier: lds is free again
+            if (tid == 0) long_part[t.row_begin] = v;
+            continue;
+        }
+        const uint32_t rows = t.row_end - t.row_begin;
+        if (cnt < 16u * rows) {
+            for (uint32_t r = t.row_begin + tid; r < t.row_end; r += kBlock) {
+                const uint32_t lo = off[r] - nb, hi = off[r + 1] - nb;
+                V acc = V(0);
+                for (uint32_t k = lo; k < hi; ++k) acc += lds[k];
+                fin(r, static_cast<double>(acc));
+            }
+        } else {
+            for (uint32_t r = t.row_begin + wid; r < t.row_end; r += kBlock / kWave) {
+                const uint32_t lo = off[r] - nb, hi = off[r + 1] - nb;
+                V acc = V(0);
+                for (uint32_t k = lo + lane; k < hi; k += kWave) acc += lds[k];
+                acc = wave_sum(acc);
+                if (lane == 0) fin(r, static_cast<double>(acc));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sum of a split row's partial slots, in chunk order.
+__device__ __forceinline__ double long_row_sum(const double *long_part, const uint32_t *long_slot,
+                                               uint32_t k) {
+    double acc = 0.0;
+    for (uint32_t q = long_slot[k]; q < long_slot[k + 1]; ++q) acc += long_part[q];
+    return acc;
+}
+
+}  // namespace gt

@@ -1,0 +1,553 @@
+"""gt_hip -- the PageRank / eigenvector-centrality path of graph-tool-rs on MI355X.
+
+The call surface mirrors gt-algo's centrality module
+(crates/gt-algo/src/centrality.rs:79-99, 234-497) and the gt-core types it is
+written against:
+
+* :class:`Plan` -- ``Plan::new(n, grain)`` (crates/gt-core/src/par/plan.rs:55-101);
+* :class:`PowerIteration` -- ``{epsilon, max_iter, plan}`` (centrality.rs:79-99);
+* :class:`Unity`, :class:`Constant`, :class:`DenseProp` -- the property maps the
+  path reads (crates/gt-core/src/prop/dense.rs:88-127, 373-460);
+* :class:`AdjList` -- ``with_vertices`` / ``add_edge`` (crates/gt-core/src/adj/list.rs:59-97,
+  258-280) with the four directed views ``dispatch_bidi`` accepts
+  (crates/gt-py/src/dispatch.rs:55-60, 315-336);
+* :func:`pagerank`, :func:`pagerank_with`, :func:`eigenvector` with the
+  reference's signatures and return values.
+
+Every computation runs in libgt_hip.so on the GPU; this module only sizes maps,
+resolves views and translates errors (ValueError for caller errors, as
+crates/gt-py/src/module.rs:78-101 maps them; RuntimeError for device failures).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import math
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import (GT_DEVICE, GT_F32, GT_F64, GT_HOST, GT_MAP_CONSTANT, GT_MAP_DENSE,
+                   GT_MAP_UNITY, GtError, GtProp, GtPowerIteration, GtValueError,
+                   GtVertexOut, GtView, call, lib)
+
+__all__ = [
+    "Plan", "PowerIteration", "Unity", "Constant", "DenseProp", "AdjList", "View",
+    "DeviceGraph", "pagerank", "pagerank_with", "pagerank_mixed", "eigenvector",
+    "gen_er", "gen_rmat", "host_adjacency", "GtError", "GtValueError", "device_count",
+]
+
+_u32p = C.POINTER(C.c_uint32)
+
+
+def _ptr(a: np.ndarray, t=_u32p):
+    return a.ctypes.data_as(t)
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    call("gt_device_count", C.byref(n))
+    return int(n.value)
+
+
+# --------------------------------------------------------------------- Plan
+
+class Plan:
+    """``Plan::new(n, grain)``: ``max(1, ceil(n / grain))`` balanced chunks."""
+
+    def __init__(self, n: int, grain: int):
+        if grain <= 0:
+            raise ValueError("Plan::new: grain must be non-zero")
+        self._n = int(n)
+        self._chunks = max(1, -(-self._n // int(grain)))
+
+    @classmethod
+    def from_parts(cls, n: int, chunks: int) -> "Plan":
+        p = cls.__new__(cls)
+        p._n, p._chunks = int(n), int(chunks)
+        return p
+
+    def len(self) -> int:
+        return self._n
+
+    def chunks(self) -> int:
+        return self._chunks
+
+    def range(self, k: int) -> range:
+        if not 0 <= k < self._chunks:
+            raise IndexError(f"Plan::range: chunk {k} out of range (chunks = {self._chunks})")
+        base, rem = divmod(self._n, self._chunks)
+        start = k * base + min(k, rem)
+        return range(start, start + base + (1 if k < rem else 0))
+
+    def __eq__(self, o) -> bool:
+        return isinstance(o, Plan) and (self._n, self._chunks) == (o._n, o._chunks)
+
+    def __repr__(self) -> str:
+        return f"Plan(n={self._n}, chunks={self._chunks})"
+
+
+@dataclass
+class PowerIteration:
+    """Convergence threshold on the per-sweep L1 change, sweep cap (0 = none)
+    and the plan that must cover the vertex index space."""
+    epsilon: float
+    max_iter: int
+    plan: Plan
+
+    def _c(self) -> GtPowerIteration:
+        return GtPowerIteration(float(self.epsilon), int(self.max_iter),
+                                _lib.GtPlan(self.plan.len(), self.plan.chunks()))
+
+
+# ---------------------------------------------------------- property maps
+
+class Unity:
+    """The all-ones edge map (``Unity``; ``IS_UNITY`` selects the integer
+    out-degree)."""
+    NEW: "Unity"
+
+
+Unity.NEW = Unity()
+
+
+class Constant:
+    """A map with one value everywhere (``Constant::new``)."""
+
+    def __init__(self, value: float):
+        self.value = float(value)
+
+
+def _is_torch_cuda(x) -> bool:
+    return hasattr(x, "is_cuda") and bool(getattr(x, "is_cuda"))
+
+
+class DenseProp:
+    """A dense f64 (or f32) map owned by one graph.
+
+    ``data`` is a numpy array (host memory) or a CUDA torch tensor (device
+    memory).  ``sized_for(bound)`` grows it to exactly ``bound`` and never
+    shrinks it (crates/gt-core/src/prop/dense.rs:88-127).
+    """
+
+    def __init__(self, graph_id: int, data=None, dtype=np.float64):
+        self.graph_id = int(graph_id)
+        self.data = np.zeros(0, dtype=dtype) if data is None else data
+
+    @classmethod
+    def new(cls, graph_id: int) -> "DenseProp":
+        return cls(graph_id)
+
+    @classmethod
+    def from_vec(cls, graph_id: int, data) -> "DenseProp":
+        if not _is_torch_cuda(data):
+            data = np.ascontiguousarray(data)
+            if data.dtype not in (np.float64, np.float32):
+                data = data.astype(np.float64)
+        return cls(graph_id, data)
+
+    def __len__(self) -> int:
+        return int(self.data.shape[0])
+
+    def as_slice(self):
+        return self.data
+
+    def sized_for(self, graph_id: int, bound: int) -> int:
+        """Grow to ``bound`` (zero-filled); returns the length it had."""
+        if graph_id != self.graph_id:
+            raise GtValueError(_lib.GT_ERR_WRONG_GRAPH,
+                               f"map belongs to graph {self.graph_id}, not {graph_id}")
+        had = len(self)
+        if had < bound:
+            if _is_torch_cuda(self.data):
+                import torch
+                grown = torch.zeros(bound, dtype=self.data.dtype, device=self.data.device)
+                grown[:had] = self.data
+            else:
+                grown = np.zeros(bound, dtype=self.data.dtype)
+                grown[:had] = self.data
+            self.data = grown
+        return had
+
+
+def _prop(m, graph_id: int, keep=None) -> Optional[GtProp]:
+    """A read-only map as a gt_prop (``keep`` collects arrays to keep alive)."""
+    if m is None:
+        return None
+    if isinstance(m, Unity):
+        return GtProp(GT_MAP_UNITY, GT_F64, GT_HOST, 0, 1.0, None, 0, 0)
+    if isinstance(m, Constant):
+        return GtProp(GT_MAP_CONSTANT, GT_F64, GT_HOST, 0, m.value, None, 0, 0)
+    gid = 0
+    if isinstance(m, DenseProp):
+        gid, m = m.graph_id, m.data
+    if _is_torch_cuda(m):
+        import torch
+        if not m.is_contiguous():
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "device map must be contiguous")
+        dt = {torch.float64: GT_F64, torch.float32: GT_F32}.get(m.dtype)
+        if dt is None:
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "map dtype must be float64 or float32")
+        return GtProp(GT_MAP_DENSE, dt, GT_DEVICE, 0, 0.0, C.c_void_p(m.data_ptr()),
+                      int(m.numel()), gid)
+    a = np.ascontiguousarray(m)
+    if a.dtype not in (np.float64, np.float32):
+        a = a.astype(np.float64)
+    if keep is not None:
+        keep.append(a)
+    dt = GT_F64 if a.dtype == np.float64 else GT_F32
+    return GtProp(GT_MAP_DENSE, dt, GT_HOST, 0, 0.0, a.ctypes.data_as(C.c_void_p),
+                  int(a.shape[0]), gid)
+
+
+# ------------------------------------------------------------------ graphs
+
+_graph_ids = itertools.count(1)
+
+
+def host_adjacency(n: int, src: np.ndarray, tgt: np.ndarray):
+    """The flattened in-halves / out-halves ``add_edge(src[i], tgt[i])`` builds."""
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    tgt = np.ascontiguousarray(tgt, dtype=np.uint32)
+    m = int(src.shape[0])
+    in_off = np.zeros(n + 1, np.uint32)
+    out_off = np.zeros(n + 1, np.uint32)
+    arrs = [np.zeros(max(m, 1), np.uint32) for _ in range(4)]
+    call("gt_host_adjacency", n, m, _ptr(src), _ptr(tgt), _ptr(in_off), _ptr(arrs[0]),
+         _ptr(arrs[1]), _ptr(out_off), _ptr(arrs[2]), _ptr(arrs[3]))
+    return in_off, arrs[0][:m], arrs[1][:m], out_off, arrs[2][:m], arrs[3][:m]
+
+
+class DeviceGraph:
+    """A device-resident graph handle (``gt_graph``)."""
+
+    def __init__(self, handle: int, graph_id: int):
+        self._h = C.c_void_p(handle)
+        self.graph_id = graph_id
+
+    @classmethod
+    def upload(cls, graph_id: int, n: int, edge_bound: int, in_off, in_other, in_eid, out_off,
+               out_other, out_eid) -> "DeviceGraph":
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in
+                (in_off, in_other, in_eid, out_off, out_other, out_eid)]
+        h = _lib.GraphP()
+        call("gt_graph_upload", graph_id, n, edge_bound, *[_ptr(a) for a in arrs], C.byref(h))
+        return cls(h.value, graph_id)
+
+    @classmethod
+    def from_edges(cls, n: int, src, tgt, graph_id: Optional[int] = None) -> "DeviceGraph":
+        gid = next(_graph_ids) if graph_id is None else graph_id
+        adj = host_adjacency(n, src, tgt)
+        return cls.upload(gid, n, int(np.asarray(src).shape[0]), *adj)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def info(self) -> Tuple[int, int, int, int]:
+        n, m, eb, gid = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        call("gt_graph_info", self._h, C.byref(n), C.byref(m), C.byref(eb), C.byref(gid))
+        return n.value, m.value, eb.value, gid.value
+
+    def download(self):
+        n, m, _, _ = self.info()
+        out = [np.zeros(n + 1, np.uint32), np.zeros(max(m, 1), np.uint32),
+               np.zeros(max(m, 1), np.uint32), np.zeros(n + 1, np.uint32),
+               np.zeros(max(m, 1), np.uint32), np.zeros(max(m, 1), np.uint32)]
+        call("gt_graph_download", self._h, *[_ptr(a) for a in out])
+        return out[0], out[1][:m], out[2][:m], out[3], out[4][:m], out[5][:m]
+
+    def set_stream(self, stream_ptr: Optional[int]) -> None:
+        call("gt_graph_set_stream", self._h, C.c_void_p(stream_ptr or 0))
+
+    def set_profiling(self, enable: bool) -> None:
+        call("gt_graph_set_profiling", self._h, 1 if enable else 0)
+
+    def sweep_stats(self, kind: int = _lib.GT_SWEEP_PAGERANK_F64) -> Tuple[float, int, int]:
+        """(summed event ms, launches) of one sweep kernel kind, and the
+        sweep count of the last call."""
+        ms, la, sw = C.c_double(), C.c_uint64(), C.c_uint64()
+        call("gt_sweep_stats", self._h, kind, C.byref(ms), C.byref(la), C.byref(sw))
+        return ms.value, la.value, sw.value
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().gt_graph_free(self._h)
+        self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AdjList:
+    """An adjacency built by ``add_edge`` calls; edge ``i`` is the ``i``-th added.
+
+    The device copy is built on first use and rebuilt after any mutation.
+    """
+
+    def __init__(self, n: int = 0):
+        self.graph_id = next(_graph_ids)
+        self._n = int(n)
+        self._src = np.zeros(16, np.uint32)
+        self._tgt = np.zeros(16, np.uint32)
+        self._m = 0
+        self._dev: Optional[DeviceGraph] = None
+        self._dev_epoch = -1
+        self._epoch = 0
+        self._host = None
+
+    @classmethod
+    def with_vertices(cls, n: int) -> "AdjList":
+        return cls(n)
+
+    def add_vertex(self) -> int:
+        self._n += 1
+        self._epoch += 1
+        return self._n - 1
+
+    def _grow(self, need: int) -> None:
+        cap = self._src.shape[0]
+        if need <= cap:
+            return
+        cap = max(need, 2 * cap)
+        for name in ("_src", "_tgt"):
+            a = np.zeros(cap, np.uint32)
+            a[: self._m] = getattr(self, name)[: self._m]
+            setattr(self, name, a)
+
+    def add_edge(self, s: int, t: int) -> int:
+        if not 0 <= s < self._n:
+            raise GtValueError(_lib.GT_ERR_NO_SUCH_VERTEX, f"no such vertex: {s}")
+        if not 0 <= t < self._n:
+            raise GtValueError(_lib.GT_ERR_NO_SUCH_VERTEX, f"no such vertex: {t}")
+        self._grow(self._m + 1)
+        self._src[self._m] = s
+        self._tgt[self._m] = t
+        self._m += 1
+        self._epoch += 1
+        return self._m - 1
+
+    def add_edges(self, src, tgt) -> None:
+        """``add_edge`` for every pair, in order (ParBuilder's result)."""
+        src = np.asarray(src, dtype=np.int64)
+        tgt = np.asarray(tgt, dtype=np.int64)
+        if src.shape != tgt.shape:
+            raise ValueError("source and target arrays differ in length")
+        if src.size and (src.min() < 0 or tgt.min() < 0 or src.max() >= self._n
+                         or tgt.max() >= self._n):
+            raise GtValueError(_lib.GT_ERR_NO_SUCH_VERTEX, "no such vertex")
+        k = int(src.shape[0])
+        self._grow(self._m + k)
+        self._src[self._m:self._m + k] = src
+        self._tgt[self._m:self._m + k] = tgt
+        self._m += k
+        self._epoch += 1
+
+    def num_vertices(self) -> int:
+        return self._n
+
+    def num_edges(self) -> int:
+        return self._m
+
+    def vertex_bound(self) -> int:
+        return self._n
+
+    def edge_bound(self) -> int:
+        return self._m
+
+    def edges(self) -> Tuple[np.ndarray, np.ndarray]:
+        return self._src[: self._m].copy(), self._tgt[: self._m].copy()
+
+    def _adjacency(self):
+        if self._host is None or self._host[0] != self._epoch:
+            self._host = (self._epoch, host_adjacency(self._n, self._src[: self._m],
+                                                      self._tgt[: self._m]))
+        return self._host[1]
+
+    def in_edges(self, v: int):
+        """``(other, edge id)`` pairs in storage order."""
+        in_off, in_o, in_e, _, _, _ = self._adjacency()
+        lo, hi = int(in_off[v]), int(in_off[v + 1])
+        return list(zip(in_o[lo:hi].tolist(), in_e[lo:hi].tolist()))
+
+    def out_edges(self, v: int):
+        _, _, _, out_off, out_o, out_e = self._adjacency()
+        lo, hi = int(out_off[v]), int(out_off[v + 1])
+        return list(zip(out_o[lo:hi].tolist(), out_e[lo:hi].tolist()))
+
+    def device(self) -> DeviceGraph:
+        if self._dev is None or self._dev_epoch != self._epoch:
+            if self._dev is not None:
+                self._dev.close()
+            self._dev = DeviceGraph.upload(self.graph_id, self._n, self._m, *self._adjacency())
+            self._dev_epoch = self._epoch
+        return self._dev
+
+    # views -------------------------------------------------------------
+    def reversed(self) -> "View":
+        return View(self, _lib.GT_VIEW_REVERSED)
+
+    def undirect(self) -> "View":
+        return View(self, _lib.GT_VIEW_UNDIRECTED)
+
+    def masked(self, vmask, emask, reversed: bool = False, undirected: bool = False) -> "View":
+        kind = (_lib.GT_VIEW_UNDIRECTED_FILTERED if undirected else
+                _lib.GT_VIEW_REVERSED_FILTERED if reversed else _lib.GT_VIEW_DIRECTED_FILTERED)
+        vmask = np.ascontiguousarray(vmask, dtype=np.uint8)
+        emask = np.ascontiguousarray(emask, dtype=np.uint8)
+        if vmask.shape[0] < self._n or emask.shape[0] < self._m:
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "filter: mask shorter than the bound")
+        return View(self, kind, vmask, emask)
+
+
+class View:
+    """One of the six view kinds over an :class:`AdjList`."""
+
+    def __init__(self, base: AdjList, kind: int, vmask=None, emask=None):
+        self.base, self.kind, self.vmask, self.emask = base, kind, vmask, emask
+
+    @property
+    def graph_id(self) -> int:
+        return self.base.graph_id
+
+    def vertex_bound(self) -> int:
+        return self.base.vertex_bound()
+
+    def num_vertices(self) -> int:
+        if self.vmask is None:
+            return self.base.num_vertices()
+        return int(np.count_nonzero(self.vmask[: self.base.vertex_bound()]))
+
+    def _c(self) -> GtView:
+        v = GtView(self.kind, 0, None, 0, None, 0)
+        if self.vmask is not None:
+            v.vmask = self.vmask.ctypes.data_as(C.c_void_p)
+            v.vmask_len = self.vmask.shape[0]
+            v.emask = self.emask.ctypes.data_as(C.c_void_p)
+            v.emask_len = self.emask.shape[0]
+        return v
+
+
+def _resolve(g):
+    """(device graph, gt_view, graph_id, vertex_bound, num_vertices)."""
+    if isinstance(g, View):
+        return g.base.device(), g._c(), g.graph_id, g.vertex_bound(), g.num_vertices()
+    if isinstance(g, AdjList):
+        return (g.device(), GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0), g.graph_id,
+                g.vertex_bound(), g.num_vertices())
+    if isinstance(g, DeviceGraph):
+        n, _, _, gid = g.info()
+        return g, GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0), gid, n, n
+    raise TypeError("expected an AdjList, a View or a DeviceGraph")
+
+
+def _vertex_out(m: DenseProp, graph_id: int, bound: int) -> GtVertexOut:
+    if m.graph_id != graph_id:
+        raise GtValueError(_lib.GT_ERR_WRONG_GRAPH, "map does not belong to this graph")
+    had = m.sized_for(graph_id, bound)
+    d = m.data
+    if _is_torch_cuda(d):
+        import torch
+        if d.dtype != torch.float64 or not d.is_contiguous():
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "rank map must be contiguous float64")
+        return GtVertexOut(C.c_void_p(d.data_ptr()), int(d.numel()), had, GT_DEVICE, 0, graph_id)
+    if d.dtype != np.float64 or not d.flags.c_contiguous:
+        m.data = d = np.ascontiguousarray(d, dtype=np.float64)
+    return GtVertexOut(d.ctypes.data_as(C.c_void_p), int(d.shape[0]), had, GT_HOST, 0, graph_id)
+
+
+# -------------------------------------------------------------- centrality
+
+def _check_map_owner(m: DenseProp, gid: int, what: str) -> None:
+    if isinstance(m, DenseProp) and m.graph_id != gid:
+        raise GtValueError(_lib.GT_ERR_WRONG_GRAPH,
+                           f"{what}: rank map does not belong to this graph")
+
+
+def pagerank(g, weight, damping: float, params: PowerIteration, rank: DenseProp) -> int:
+    """PageRank with the uniform personalisation ``1 / num_vertices``
+    (centrality.rs:234-248).  Returns the number of sweeps."""
+    return _pagerank("gt_pagerank_f64", g, weight, None, damping, params, rank)
+
+
+def pagerank_with(g, weight, pers, damping: float, params: PowerIteration,
+                  rank: DenseProp) -> int:
+    """PageRank with a personalisation map, not normalised (centrality.rs:261-384)."""
+    if pers is None:
+        raise ValueError("pagerank_with needs a personalisation map")
+    return _pagerank("gt_pagerank_f64", g, weight, pers, damping, params, rank)
+
+
+def pagerank_mixed(g, weight, pers, damping: float, params: PowerIteration,
+                   rank: DenseProp) -> int:
+    """FP32-value / FP64-residual PageRank (extension); ``pers=None`` is uniform."""
+    return _pagerank("gt_pagerank_mixed", g, weight, pers, damping, params, rank)
+
+
+def _bind_stream(dev: DeviceGraph, *maps) -> None:
+    """Run on torch's current stream when any map is a CUDA tensor, so the
+    call is ordered after the work that produced it."""
+    for m in maps:
+        d = m.data if isinstance(m, DenseProp) else m
+        if _is_torch_cuda(d):
+            import torch
+            dev.set_stream(torch.cuda.current_stream(d.device).cuda_stream)
+            return
+    dev.set_stream(None)
+
+
+def _pagerank(fn, g, weight, pers, damping, params, rank) -> int:
+    dev, view, gid, bound, nv = _resolve(g)
+    _bind_stream(dev, rank, weight, pers)
+    if rank.graph_id != gid:
+        raise GtValueError(_lib.GT_ERR_WRONG_GRAPH,
+                           "pagerank: rank map does not belong to this graph")
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, gid, keep)
+    if pers is None and nv != bound:
+        pers = Constant(0.0 if nv == 0 else 1.0 / nv)
+    pp = _prop(pers, gid, keep)
+    out = _vertex_out(rank, gid, bound)
+    iters = C.c_uint64()
+    call(fn, dev.handle, C.byref(view), C.byref(wp), C.byref(pp) if pp else None,
+         float(damping), C.byref(params._c()), C.byref(out), C.byref(iters))
+    return int(iters.value)
+
+
+def eigenvector(g, weight, params: PowerIteration, x: DenseProp) -> Tuple[float, int]:
+    """Eigenvector centrality (centrality.rs:405-497): ``(eigenvalue, sweeps)``."""
+    dev, view, gid, bound, nv = _resolve(g)
+    _bind_stream(dev, x, weight)
+    if x.graph_id != gid:
+        raise GtValueError(_lib.GT_ERR_WRONG_GRAPH,
+                           "eigenvector: centrality map does not belong to this graph")
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, gid, keep)
+    out = _vertex_out(x, gid, bound)
+    eig = C.c_double()
+    iters = C.c_uint64()
+    call("gt_eigenvector_f64", dev.handle, C.byref(view), C.byref(wp), C.byref(params._c()),
+         C.byref(out), C.byref(eig), C.byref(iters))
+    return float(eig.value), int(iters.value)
+
+
+# ------------------------------------------------------------- generators
+
+def gen_er(n: int, m: int, seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    src = np.zeros(max(m, 1), np.uint32)
+    tgt = np.zeros(max(m, 1), np.uint32)
+    call("gt_gen_er", n, m, seed & (2**64 - 1), _ptr(src), _ptr(tgt))
+    return src[:m], tgt[:m]
+
+
+def gen_rmat(scale: int, m: int, a: float, b: float, c: float,
+             seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    src = np.zeros(max(m, 1), np.uint32)
+    tgt = np.zeros(max(m, 1), np.uint32)
+    call("gt_gen_rmat", scale, m, a, b, c, seed & (2**64 - 1), _ptr(src), _ptr(tgt))
+    return src[:m], tgt[:m]

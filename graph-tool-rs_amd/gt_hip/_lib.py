@@ -1,0 +1,144 @@
+"""ctypes binding of libgt_hip.so (the C ABI declared in include/gt_hip.h).
+
+Loading fails loudly: there is no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(os.environ.get("GT_HIP_LIB", Path(__file__).resolve().parent / "libgt_hip.so"))
+
+u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
+f64p = C.POINTER(C.c_double)
+
+GT_OK = 0
+GT_ERR_INVALID_ARG = 1
+GT_ERR_PLAN_COVER = 2
+GT_ERR_WRONG_GRAPH = 3
+GT_ERR_NOT_DIRECTED = 4
+GT_ERR_NO_SUCH_VERTEX = 5
+GT_ERR_HIP = 6
+GT_ERR_OOM = 7
+GT_ERR_UNSUPPORTED = 8
+
+GT_HOST, GT_DEVICE = 0, 1
+GT_F64, GT_F32 = 0, 1
+GT_MAP_UNITY, GT_MAP_CONSTANT, GT_MAP_DENSE = 0, 1, 2
+(GT_VIEW_DIRECTED, GT_VIEW_REVERSED, GT_VIEW_DIRECTED_FILTERED, GT_VIEW_REVERSED_FILTERED,
+ GT_VIEW_UNDIRECTED, GT_VIEW_UNDIRECTED_FILTERED) = range(6)
+
+
+class GtProp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("loc", C.c_int32),
+                ("reserved", C.c_int32), ("value", C.c_double), ("data", C.c_void_p),
+                ("len", C.c_uint64), ("graph_id", C.c_uint64)]
+
+
+class GtPlan(C.Structure):
+    _fields_ = [("len", C.c_uint64), ("chunks", C.c_uint64)]
+
+
+class GtPowerIteration(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("max_iter", C.c_uint64), ("plan", GtPlan)]
+
+
+class GtVertexOut(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("len", C.c_uint64), ("had", C.c_uint64),
+                ("loc", C.c_int32), ("reserved", C.c_int32), ("graph_id", C.c_uint64)]
+
+
+class GtView(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("vmask", C.c_void_p),
+                ("vmask_len", C.c_uint64), ("emask", C.c_void_p), ("emask_len", C.c_uint64)]
+
+
+GT_SWEEP_PAGERANK_F64, GT_SWEEP_PAGERANK_F32, GT_SWEEP_EIGENVECTOR, GT_SWEEP_OTHER = range(4)
+
+GraphP = C.c_void_p
+
+# name -> (argtypes); every function returns gt_status (c_int).
+_PROTOS = {
+    "gt_device_count": [C.POINTER(C.c_int32)],
+    "gt_graph_upload": [C.c_uint64, C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p,
+                        C.POINTER(GraphP)],
+    "gt_graph_build_edges": [C.c_uint64, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                             C.c_int32, C.POINTER(GraphP)],
+    "gt_graph_free": [GraphP],
+    "gt_graph_info": [GraphP, u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                      C.POINTER(C.c_uint64)],
+    "gt_graph_download": [GraphP, u32p, u32p, u32p, u32p, u32p, u32p],
+    "gt_graph_set_stream": [GraphP, C.c_void_p],
+    "gt_pagerank_f64": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
+                        C.c_double, C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
+                        C.POINTER(C.c_uint64)],
+    "gt_pagerank_mixed": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
+                          C.c_double, C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
+                          C.POINTER(C.c_uint64)],
+    "gt_eigenvector_f64": [GraphP, C.POINTER(GtView), C.POINTER(GtProp),
+                           C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
+                           C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "gt_graph_set_profiling": [GraphP, C.c_int32],
+    "gt_sweep_stats": [GraphP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                       C.POINTER(C.c_uint64)],
+    "gt_gen_er": [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p],
+    "gt_gen_rmat": [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                    u32p, u32p],
+    "gt_host_adjacency": [C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p, u32p,
+                          u32p],
+}
+
+EXPORTED = tuple(_PROTOS) + ("gt_last_error",)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(
+                f"{_LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the HIP engine has no CPU fallback)")
+        L = C.CDLL(str(_LIB_PATH))
+        for name, args in _PROTOS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.gt_last_error.argtypes = []
+        L.gt_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class GtError(RuntimeError):
+    """A device-side failure (HIP error, out of memory, unsupported request)."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+class GtValueError(ValueError):
+    """A caller error (the reference raises ValueError / panics for these)."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+def check(status: int) -> None:
+    if status == GT_OK:
+        return
+    msg = lib().gt_last_error().decode(errors="replace")
+    if status in (GT_ERR_INVALID_ARG, GT_ERR_PLAN_COVER, GT_ERR_WRONG_GRAPH,
+                  GT_ERR_NOT_DIRECTED, GT_ERR_NO_SUCH_VERTEX):
+        raise GtValueError(status, msg)
+    raise GtError(status, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,233 @@
+/*
+ * gt_hip.h -- C ABI of the MI355X (gfx950) PageRank / eigenvector-centrality
+ * engine.  Plain pointers and sizes only; every entry point returns a
+ * gt_status and leaves a message for gt_last_error() on failure.
+ *
+ * Reference interface replaced (graph-tool-rs, paths relative to its tree):
+ *   gt_pagerank_f64 ....... crates/gt-algo/src/centrality.rs:234-248 (pagerank)
+ *                           and :261-384 (pagerank_with)
+ *   gt_eigenvector_f64 .... crates/gt-algo/src/centrality.rs:405-497 (eigenvector)
+ *   gt_power_iteration .... crates/gt-algo/src/centrality.rs:79-99 (PowerIteration)
+ *   gt_plan ............... crates/gt-core/src/par/plan.rs:38-101 (Plan{n, chunks})
+ *   gt_prop ............... crates/gt-core/src/prop/map.rs:28-56 (ReadProp with
+ *                           IS_UNITY / IS_CONSTANT), prop/dense.rs:373-460
+ *                           (Unity, Constant), prop/dense.rs:88-127 (DenseProp)
+ *   gt_view ............... crates/gt-py/src/dispatch.rs:17-60 (ViewKind; the four
+ *                           directed arms dispatch_bidi accepts, :315-336)
+ *   gt_graph_upload ....... the in-halves / out-halves an &AdjList yields through
+ *                           Bidirectional::in_edges / GraphRef::out_edges
+ *                           (crates/gt-core/src/graph.rs:71-107, adj/list.rs:156-161)
+ *   gt_graph_build_edges .. AdjList::add_edge in edge-id order
+ *                           (crates/gt-core/src/adj/list.rs:258-280, 471-500) and
+ *                           ParBuilder::build (adj/builder.rs:178-197)
+ *   error codes ........... the panics of centrality.rs:145-149 (plan cover),
+ *                           :300-302 / :422-424 (wrong graph), dispatch.rs:176-179
+ *                           (NotDirected)
+ * Entry points marked "extension" have no counterpart in the reference.
+ */
+#ifndef GT_HIP_H
+#define GT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gt_status {
+    GT_OK = 0,
+    GT_ERR_INVALID_ARG = 1,   /* malformed argument (null pointer, short map, ...) */
+    GT_ERR_PLAN_COVER = 2,    /* "sweep: the plan must cover the output exactly" */
+    GT_ERR_WRONG_GRAPH = 3,   /* "...map does not belong to this graph" */
+    GT_ERR_NOT_DIRECTED = 4,  /* undirected view passed to a Bidirectional kernel */
+    GT_ERR_NO_SUCH_VERTEX = 5,/* edge endpoint >= vertex count (AdjList::add_edge) */
+    GT_ERR_HIP = 6,           /* a HIP runtime call failed */
+    GT_ERR_OOM = 7,           /* device allocation failed */
+    GT_ERR_UNSUPPORTED = 8    /* valid request this build does not serve */
+} gt_status;
+
+/* Where a buffer argument lives. */
+typedef enum gt_loc { GT_HOST = 0, GT_DEVICE = 1 } gt_loc;
+/* Element type of a dense buffer. */
+typedef enum gt_dtype { GT_F64 = 0, GT_F32 = 1 } gt_dtype;
+/* Property-map kinds the path distinguishes (ReadProp::IS_UNITY / IS_CONSTANT). */
+typedef enum gt_map_kind { GT_MAP_UNITY = 0, GT_MAP_CONSTANT = 1, GT_MAP_DENSE = 2 } gt_map_kind;
+
+/* A read-only property map: edge weights (indexed by edge id, len >= edge_bound)
+ * or a vertex map such as the personalisation vector (len >= vertex_bound). */
+typedef struct gt_prop {
+    int32_t kind;        /* gt_map_kind */
+    int32_t dtype;       /* gt_dtype of data (DENSE only) */
+    int32_t loc;         /* gt_loc of data (DENSE only) */
+    int32_t reserved;
+    double value;        /* CONSTANT */
+    const void *data;    /* DENSE */
+    uint64_t len;        /* DENSE: element count */
+    uint64_t graph_id;   /* owner graph of a DENSE map; 0 = not checked */
+} gt_prop;
+
+/* Plan{n, chunks}: fixes the CPU fold order in the reference.  The device
+ * path checks the cover (len == vertex_bound) and otherwise ignores it. */
+typedef struct gt_plan {
+    uint64_t len;
+    uint64_t chunks;
+} gt_plan;
+
+typedef struct gt_power_iteration {
+    double epsilon;      /* L1 change per sweep that stops the loop */
+    uint64_t max_iter;   /* 0 = no cap */
+    gt_plan plan;
+} gt_power_iteration;
+
+/* A writable vertex map that is both the initial iterate and the result
+ * (DenseProp<f64, VertexTag> after sized_for): data[0..vertex_bound) is used,
+ * slots [had, vertex_bound) are seeded by the kernel, len >= vertex_bound. */
+typedef struct gt_vertex_out {
+    double *data;
+    uint64_t len;
+    uint64_t had;
+    int32_t loc;         /* gt_loc */
+    int32_t reserved;
+    uint64_t graph_id;   /* owner graph; 0 = not checked */
+} gt_vertex_out;
+
+typedef enum gt_view_kind {
+    GT_VIEW_DIRECTED = 0,
+    GT_VIEW_REVERSED = 1,
+    GT_VIEW_DIRECTED_FILTERED = 2,
+    GT_VIEW_REVERSED_FILTERED = 3,
+    GT_VIEW_UNDIRECTED = 4,
+    GT_VIEW_UNDIRECTED_FILTERED = 5
+} gt_view_kind;
+
+/* The view a call runs on.  Masks are uint8 (non-zero keeps), indexed by the
+ * unfiltered vertex / edge index, host memory, len >= the bound
+ * (Filtered::masked, crates/gt-core/src/view/filtered.rs:170-200). */
+typedef struct gt_view {
+    int32_t kind;        /* gt_view_kind */
+    int32_t reserved;
+    const uint8_t *vmask;
+    uint64_t vmask_len;
+    const uint8_t *emask;
+    uint64_t emask_len;
+} gt_view;
+
+/* Opaque device-resident graph: CSC of the in-halves and CSR of the
+ * out-halves, both in AdjList storage order, plus the SpMV row schedule. */
+typedef struct gt_graph gt_graph;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *gt_last_error(void);
+
+/* Number of visible devices; 0 when no GPU is present. */
+gt_status gt_device_count(int32_t *count);
+
+/* ---------------------------------------------------------------- graphs */
+
+/* Upload an adjacency that already exists on the host, in storage order:
+ * in_off[n+1] / in_other[m] / in_eid[m] are the in-halves (sources and edge ids
+ * as in_edges(v) yields them), out_off / out_other / out_eid the out-halves.
+ * Edge ids must be < edge_bound.  All pointers are host memory. */
+gt_status gt_graph_upload(uint64_t graph_id, uint32_t n, uint64_t edge_bound,
+                          const uint32_t *in_off, const uint32_t *in_other,
+                          const uint32_t *in_eid, const uint32_t *out_off,
+                          const uint32_t *out_other, const uint32_t *out_eid,
+                          gt_graph **out);
+
+/* Build on the device from an edge list, exactly as add_edge(src[i], tgt[i])
+ * for i = 0..m would: edge i gets id i, out-halves in id order, in-halves in
+ * the push-swap order of the reference's Block::insert_out.  src / tgt are in
+ * `loc` memory.  Integer-only; the result equals gt_graph_upload of the
+ * host-built adjacency word for word. */
+gt_status gt_graph_build_edges(uint64_t graph_id, uint32_t n, uint64_t m,
+                               const uint32_t *src, const uint32_t *tgt,
+                               int32_t loc, gt_graph **out);
+
+gt_status gt_graph_free(gt_graph *g);
+
+/* Sizes: vertex_bound, live edge count, edge_bound. */
+gt_status gt_graph_info(const gt_graph *g, uint32_t *n, uint64_t *m,
+                        uint64_t *edge_bound, uint64_t *graph_id);
+
+/* Copy the adjacency back to host buffers (same layout as gt_graph_upload);
+ * any pointer may be NULL to skip that array. */
+gt_status gt_graph_download(const gt_graph *g, uint32_t *in_off,
+                            uint32_t *in_other, uint32_t *in_eid,
+                            uint32_t *out_off, uint32_t *out_other,
+                            uint32_t *out_eid);
+
+/* Use this HIP stream (a hipStream_t, e.g. torch's current stream) for every
+ * later call on g; NULL restores the graph's own stream. */
+gt_status gt_graph_set_stream(gt_graph *g, void *hip_stream);
+
+/* ------------------------------------------------------------- centrality */
+
+/* PageRank, FP64 throughout: the drop-in for pagerank / pagerank_with.
+ * pers == NULL means the uniform 1/num_vertices of `pagerank`.
+ * On return *iters is the sweep count the reference returns. */
+gt_status gt_pagerank_f64(gt_graph *g, const gt_view *view,
+                          const gt_prop *weight, const gt_prop *pers,
+                          double damping, const gt_power_iteration *params,
+                          gt_vertex_out *rank, uint64_t *iters);
+
+/* Extension.  Same call, computed with FP32 values, weights and SpMV
+ * accumulation while the dangling-mass sum, the teleport update and the L1
+ * convergence test stay FP64, followed by FP64 refinement sweeps; rank is FP64.
+ * *iters counts every sweep performed (FP32 and FP64). */
+gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view,
+                            const gt_prop *weight, const gt_prop *pers,
+                            double damping, const gt_power_iteration *params,
+                            gt_vertex_out *rank, uint64_t *iters);
+
+/* Eigenvector centrality, FP64: the drop-in for eigenvector.  *eig is the
+ * last norm (the eigenvalue estimate), *iters the sweep count. */
+gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,
+                             const gt_prop *weight,
+                             const gt_power_iteration *params,
+                             gt_vertex_out *x, double *eig, uint64_t *iters);
+
+/* ------------------------------------------------------------- profiling */
+
+/* Sweep kernels, for gt_sweep_stats. */
+typedef enum gt_sweep_kind {
+    GT_SWEEP_PAGERANK_F64 = 0,
+    GT_SWEEP_PAGERANK_F32 = 1,
+    GT_SWEEP_EIGENVECTOR = 2,
+    GT_SWEEP_OTHER = 3
+} gt_sweep_kind;
+
+/* Extension.  While enabled, every sweep launch on g is bracketed by HIP
+ * events recorded on the stream it runs on; gt_sweep_stats reports, for one
+ * kernel kind, the summed device time and the launch count (launches that
+ * found the loop finished included), and the sweep count of the last call.
+ * Enabling resets the sums. */
+gt_status gt_graph_set_profiling(gt_graph *g, int32_t enable);
+gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms,
+                         uint64_t *launches, uint64_t *last_sweeps);
+
+/* ------------------------------------------------------------- host side */
+
+/* Extension: seeded workload generators (counter-based SplitMix64, the
+ * generator of crates/gt-algo/tests/u16_centrality.rs:58-72).
+ * gt_gen_er: per edge source = next % n, then target = next % n
+ * (crates/gt-algo/benches/kernels.rs:68-70).
+ * gt_gen_rmat: one 53-bit uniform per bit level, most significant first,
+ * quadrants (a, b, c, 1-a-b-c) over (source bit, target bit). */
+gt_status gt_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *src, uint32_t *tgt);
+gt_status gt_gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c,
+                      uint64_t seed, uint32_t *src, uint32_t *tgt);
+
+/* The host adjacency AdjList::add_edge(src[i], tgt[i]) for i = 0..m builds
+ * (crates/gt-core/src/adj/list.rs:258-280, 471-500), flattened into the
+ * arrays gt_graph_upload takes (in_off / out_off hold n+1 entries, the rest m). */
+gt_status gt_host_adjacency(uint32_t n, uint64_t m, const uint32_t *src,
+                            const uint32_t *tgt, uint32_t *in_off, uint32_t *in_other,
+                            uint32_t *in_eid, uint32_t *out_off, uint32_t *out_other,
+                            uint32_t *out_eid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GT_HIP_H */

@@ -1,0 +1,552 @@
+/*
+ * gt_oracle.c -- CPU restatement of graph-tool-rs' PageRank and eigenvector
+ * power iteration, used ONLY as test infrastructure: the checker that the HIP
+ * library is compared against, and the timed CPU baseline.  Nothing in the
+ * product links or calls this file.
+ *
+ * Compiled with -O2 -ffp-contract=off and no fast-math: Rust never contracts
+ * a*b+c into an FMA, so restating the reference's operations in the same
+ * order gives the same IEEE results.
+ *
+ * What each function follows (all paths relative to the reference tree):
+ *   sm_next ............ crates/gt-algo/tests/u16_centrality.rs:58-72 (SplitMix64),
+ *                        crates/gt-algo/benches/kernels.rs:49-73
+ *   adj_insert_out ..... crates/gt-core/src/adj/block.rs:171-199 (O(1) push-swap)
+ *   adj_insert_in ...... crates/gt-core/src/adj/block.rs:205-209
+ *   oracle_adj_build ... crates/gt-core/src/adj/list.rs:258-280, 471-500
+ *                        (out-half insert first, then in-half, edge id = call order)
+ *   view_* ............. crates/gt-core/src/view/reversed.rs:109-140 (halves swap),
+ *                        crates/gt-core/src/view/filtered.rs:116-124, 270-320
+ *                        (keeps_incident = keep_edge(e) && keep_vertex(other))
+ *   plan_range ......... crates/gt-core/src/par/plan.rs:55-101
+ *   run_chunks ......... crates/gt-algo/src/centrality.rs:141-167 (sweep) and
+ *                        crates/gt-core/src/par/reduce.rs:40-55 (det_reduce):
+ *                        per-chunk partial, folded left to right from 0.0
+ *   oracle_pagerank .... crates/gt-algo/src/centrality.rs:234-384
+ *   oracle_eigenvector . crates/gt-algo/src/centrality.rs:405-497
+ *
+ * Pinning: every function above is checked by tests/test_oracle.py against the
+ * reference's own known answers (u16_centrality.rs, the LAYOUT fixture of
+ * u34_diff_adjacency.rs), against closed forms (directed cycles, the scipy
+ * linear solve of the PageRank fixed point, dense eigenpairs) and against brute
+ * force (a dense n x n power iteration).  oracle_pagerank_adjoint is pinned by
+ * central finite differences of oracle_pagerank and by the row-scale
+ * invariance identity sum_v w_uv dL/dw_uv = 0.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GT_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* SplitMix64 (u16_centrality.rs:58-72)                                      */
+/* ------------------------------------------------------------------------ */
+
+static inline uint64_t sm_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t sm_next(uint64_t *s) {
+    *s += 0x9e3779b97f4a7c15ULL;
+    return sm_mix(*s);
+}
+
+/* 53-bit uniform in [0,1) from one SplitMix64 draw. */
+static inline double sm_unit(uint64_t *s) {
+    return (double)(sm_next(s) >> 11) * 0x1.0p-53;
+}
+
+/* Directed G(n, m): per edge src = next % n, then tgt = next % n
+ * (kernels.rs:68-70 draws source before target). */
+GT_EXPORT void oracle_gen_er(uint32_t n, uint64_t m, uint64_t seed,
+                             uint32_t *src, uint32_t *tgt) {
+    uint64_t s = seed;
+    for (uint64_t i = 0; i < m; ++i) {
+        src[i] = (uint32_t)(sm_next(&s) % n);
+        tgt[i] = (uint32_t)(sm_next(&s) % n);
+    }
+}
+
+/* R-MAT: edge i draws `scale` uniforms, one per bit level, most significant
+ * level first; quadrant thresholds a, a+b, a+b+c; (row,col) = (src,tgt);
+ * no label permutation, no noise. */
+GT_EXPORT void oracle_gen_rmat(uint32_t scale, uint64_t m, double a, double b,
+                               double c, uint64_t seed, uint32_t *src,
+                               uint32_t *tgt) {
+    uint64_t s = seed;
+    const double ab = a + b, abc = a + b + c;
+    for (uint64_t i = 0; i < m; ++i) {
+        uint32_t u = 0, v = 0;
+        for (uint32_t l = 0; l < scale; ++l) {
+            double r = sm_unit(&s);
+            uint32_t bit = 1u << (scale - 1 - l);
+            if (r < a) {
+            } else if (r < ab) {
+                v |= bit;
+            } else if (r < abc) {
+                u |= bit;
+            } else {
+                u |= bit;
+                v |= bit;
+            }
+        }
+        src[i] = u;
+        tgt[i] = v;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* AdjList: one block per vertex, out-half [0,out_len) then in-half           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t other;
+    uint32_t idx;
+} gt_entry; /* AdjEntry, 8 bytes (entry.rs:39-48) */
+
+typedef struct {
+    uint32_t n;
+    uint64_t edge_bound;
+    uint64_t *start;   /* n+1: block v occupies e[start[v] .. start[v+1]) */
+    uint32_t *out_len; /* n */
+    uint32_t *len;     /* n */
+    gt_entry *e;
+} gt_adj;
+
+static void adj_insert_out(gt_adj *g, uint32_t v, gt_entry x) {
+    uint64_t at = g->start[v] + g->out_len[v];
+    uint64_t back = g->start[v] + g->len[v];
+    if (g->out_len[v] < g->len[v]) {
+        /* in-half non-empty: its first entry moves to the back */
+        g->e[back] = g->e[at];
+    }
+    g->e[at] = x;
+    g->len[v] += 1;
+    g->out_len[v] += 1;
+}
+
+static void adj_insert_in(gt_adj *g, uint32_t v, gt_entry x) {
+    g->e[g->start[v] + g->len[v]] = x;
+    g->len[v] += 1;
+}
+
+GT_EXPORT void oracle_adj_free(gt_adj *g) {
+    if (!g) return;
+    free(g->start);
+    free(g->out_len);
+    free(g->len);
+    free(g->e);
+    free(g);
+}
+
+/* Build by calling add_edge(src[i], tgt[i]) for i = 0..m in order; edge i
+ * gets id i.  Returns NULL if an endpoint is out of range. */
+GT_EXPORT gt_adj *oracle_adj_build(uint32_t n, uint64_t m, const uint32_t *src,
+                                   const uint32_t *tgt) {
+    for (uint64_t i = 0; i < m; ++i)
+        if (src[i] >= n || tgt[i] >= n) return NULL;
+    gt_adj *g = (gt_adj *)calloc(1, sizeof(gt_adj));
+    g->n = n;
+    g->edge_bound = m;
+    g->start = (uint64_t *)calloc((size_t)n + 1, sizeof(uint64_t));
+    g->out_len = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    g->len = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+    for (uint64_t i = 0; i < m; ++i) {
+        g->start[src[i] + 1] += 1;
+        g->start[tgt[i] + 1] += 1;
+    }
+    for (uint32_t v = 0; v < n; ++v) g->start[v + 1] += g->start[v];
+    g->e = (gt_entry *)malloc((g->start[n] ? g->start[n] : 1) * sizeof(gt_entry));
+    for (uint64_t i = 0; i < m; ++i) {
+        gt_entry o = {tgt[i], (uint32_t)i};
+        gt_entry in = {src[i], (uint32_t)i};
+        adj_insert_out(g, src[i], o);
+        adj_insert_in(g, tgt[i], in);
+    }
+    return g;
+}
+
+/* Flatten to CSC (in-halves) and CSR (out-halves), each in storage order. */
+GT_EXPORT void oracle_adj_export(const gt_adj *g, uint32_t *in_off,
+                                 uint32_t *in_other, uint32_t *in_eid,
+                                 uint32_t *out_off, uint32_t *out_other,
+                                 uint32_t *out_eid) {
+    uint64_t pi = 0, po = 0;
+    for (uint32_t v = 0; v < g->n; ++v) {
+        in_off[v] = (uint32_t)pi;
+        out_off[v] = (uint32_t)po;
+        const gt_entry *b = g->e + g->start[v];
+        for (uint32_t k = 0; k < g->out_len[v]; ++k, ++po) {
+            out_other[po] = b[k].other;
+            out_eid[po] = b[k].idx;
+        }
+        for (uint32_t k = g->out_len[v]; k < g->len[v]; ++k, ++pi) {
+            in_other[pi] = b[k].other;
+            in_eid[pi] = b[k].idx;
+        }
+    }
+    in_off[g->n] = (uint32_t)pi;
+    out_off[g->n] = (uint32_t)po;
+}
+
+/* Whole block of v (out-half then in-half), for layout fixtures. */
+GT_EXPORT uint32_t oracle_adj_block(const gt_adj *g, uint32_t v,
+                                    uint32_t *other, uint32_t *idx,
+                                    uint32_t *out_len) {
+    const gt_entry *b = g->e + g->start[v];
+    for (uint32_t k = 0; k < g->len[v]; ++k) {
+        other[k] = b[k].other;
+        idx[k] = b[k].idx;
+    }
+    *out_len = g->out_len[v];
+    return g->len[v];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Views: reversed swaps the halves; a mask filter keeps an incidence iff     */
+/* keep_edge(e) && keep_vertex(other).                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const gt_adj *g;
+    int reversed;
+    const uint8_t *vmask; /* NULL: all vertices kept */
+    const uint8_t *emask; /* NULL: all edges kept */
+} gt_view;
+
+static inline const gt_entry *view_in(const gt_view *w, uint32_t v, uint32_t *cnt) {
+    const gt_adj *g = w->g;
+    const gt_entry *b = g->e + g->start[v];
+    if (w->reversed) {
+        *cnt = g->out_len[v];
+        return b;
+    }
+    *cnt = g->len[v] - g->out_len[v];
+    return b + g->out_len[v];
+}
+
+static inline const gt_entry *view_out(const gt_view *w, uint32_t v, uint32_t *cnt) {
+    const gt_adj *g = w->g;
+    const gt_entry *b = g->e + g->start[v];
+    if (!w->reversed) {
+        *cnt = g->out_len[v];
+        return b;
+    }
+    *cnt = g->len[v] - g->out_len[v];
+    return b + g->out_len[v];
+}
+
+static inline int view_keeps(const gt_view *w, gt_entry x) {
+    if (w->emask && !w->emask[x.idx]) return 0;
+    if (w->vmask && !w->vmask[x.other]) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plan and the ordered fold                                                 */
+/* ------------------------------------------------------------------------ */
+
+static inline void plan_range(uint64_t n, uint64_t chunks, uint64_t k,
+                              uint64_t *lo, uint64_t *hi) {
+    uint64_t base = n / chunks, rem = n % chunks;
+    uint64_t start = k * base + (k < rem ? k : rem);
+    *lo = start;
+    *hi = start + base + (k < rem ? 1 : 0);
+}
+
+static inline uint64_t plan_chunks_for(uint64_t n, uint64_t grain) {
+    uint64_t c = (n + grain - 1) / grain;
+    return c ? c : 1;
+}
+
+typedef double (*chunk_fn)(void *ctx, uint64_t lo, uint64_t hi);
+
+typedef struct {
+    chunk_fn f;
+    void *ctx;
+    uint64_t n, chunks;
+    double *partials;
+    atomic_ulong next;
+} chunk_job;
+
+static void *chunk_worker(void *arg) {
+    chunk_job *j = (chunk_job *)arg;
+    for (;;) {
+        uint64_t k = atomic_fetch_add(&j->next, 1);
+        if (k >= j->chunks) break;
+        uint64_t lo, hi;
+        plan_range(j->n, j->chunks, k, &lo, &hi);
+        j->partials[k] = j->f(j->ctx, lo, hi);
+    }
+    return NULL;
+}
+
+/* Map every chunk of Plan{n, chunks} (any thread, any order), then fold the
+ * partials left to right from 0.0: the result does not depend on `threads`. */
+static double run_chunks(uint64_t n, uint64_t chunks, int threads, chunk_fn f,
+                         void *ctx) {
+    double *partials = (double *)malloc(chunks * sizeof(double));
+    chunk_job j;
+    j.f = f;
+    j.ctx = ctx;
+    j.n = n;
+    j.chunks = chunks;
+    j.partials = partials;
+    atomic_init(&j.next, 0);
+    if (threads <= 1 || chunks == 1) {
+        chunk_worker(&j);
+    } else {
+        int t = threads;
+        if ((uint64_t)t > chunks) t = (int)chunks;
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)t);
+        for (int i = 1; i < t; ++i) pthread_create(&th[i], NULL, chunk_worker, &j);
+        chunk_worker(&j);
+        for (int i = 1; i < t; ++i) pthread_join(th[i], NULL);
+        free(th);
+    }
+    double acc = 0.0;
+    for (uint64_t k = 0; k < chunks; ++k) acc = acc + partials[k];
+    free(partials);
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Property maps as the path sees them                                       */
+/* ------------------------------------------------------------------------ */
+
+enum { MAP_UNITY = 0, MAP_CONSTANT = 1, MAP_DENSE = 2 };
+
+typedef struct {
+    int kind;
+    double value;
+    const double *data;
+} gt_map;
+
+static inline double map_get(const gt_map *m, uint64_t k) {
+    if (m->kind == MAP_UNITY) return 1.0;
+    if (m->kind == MAP_CONSTANT) return m->value;
+    return m->data[k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* PageRank (centrality.rs:261-384)                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const gt_view *w;
+    const uint8_t *active;
+    const double *deg;
+    const gt_map *weight;
+    const gt_map *pers;
+    const double *prev;
+    double *next;
+    double p_sink;
+    double d;
+    const uint32_t *sinks;
+} pr_ctx;
+
+static double pr_sink_chunk(void *c, uint64_t lo, uint64_t hi) {
+    pr_ctx *x = (pr_ctx *)c;
+    double acc = 0.0;
+    for (uint64_t k = lo; k < hi; ++k) acc += x->prev[x->sinks[k]];
+    return acc;
+}
+
+static double pr_sweep_chunk(void *c, uint64_t lo, uint64_t hi) {
+    pr_ctx *x = (pr_ctx *)c;
+    double local = 0.0;
+    const double d = x->d;
+    for (uint64_t i = lo; i < hi; ++i) {
+        if (!x->active[i]) {
+            x->next[i] = x->prev[i];
+            continue;
+        }
+        double pv = map_get(x->pers, i);
+        double r = x->p_sink * pv;
+        uint32_t cnt;
+        const gt_entry *in = view_in(x->w, (uint32_t)i, &cnt);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            if (!view_keeps(x->w, in[k])) continue;
+            uint32_t s = in[k].other;
+            double ds = x->deg[s];
+            if (ds != 0.0) r += (x->prev[s] * map_get(x->weight, in[k].idx)) / ds;
+        }
+        double nv = (1.0 - d) * pv + d * r;
+        local += fabs(nv - x->prev[i]);
+        x->next[i] = nv;
+    }
+    return local;
+}
+
+/* Returns the sweep count, or -1 if the plan does not cover the index space
+ * (the reference panics "sweep: the plan must cover the output exactly").
+ * rank[0..n) is the in/out iterate; slots [had, n) are seeded with pers. */
+GT_EXPORT int64_t oracle_pagerank(const gt_adj *g, int reversed,
+                                  const uint8_t *vmask, const uint8_t *emask,
+                                  int wkind, double wval, const double *w,
+                                  int pkind, double pval, const double *pers,
+                                  double damping, double epsilon,
+                                  uint64_t max_iter, uint64_t plan_len,
+                                  uint64_t plan_chunks, double *rank,
+                                  uint64_t had, int threads) {
+    gt_view view = {g, reversed, vmask, emask};
+    const uint32_t n = g->n;
+    gt_map wm = {wkind, wval, w};
+    gt_map pm = {pkind, pval, pers};
+
+    uint8_t *active = (uint8_t *)malloc(n ? n : 1);
+    for (uint32_t v = 0; v < n; ++v) active[v] = vmask ? (vmask[v] != 0) : 1;
+
+    double *deg = (double *)calloc(n ? n : 1, sizeof(double));
+    uint32_t *sinks = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint64_t nsinks = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (!active[v]) continue;
+        uint32_t cnt;
+        const gt_entry *out = view_out(&view, v, &cnt);
+        double k;
+        if (wkind == MAP_UNITY) {
+            uint64_t c = 0;
+            for (uint32_t j = 0; j < cnt; ++j) c += view_keeps(&view, out[j]);
+            k = (double)c;
+        } else {
+            double acc = 0.0;
+            for (uint32_t j = 0; j < cnt; ++j)
+                if (view_keeps(&view, out[j])) acc += map_get(&wm, out[j].idx);
+            k = acc;
+        }
+        deg[v] = k;
+        if (k == 0.0) sinks[nsinks++] = v;
+    }
+
+    for (uint64_t i = had; i < n; ++i) rank[i] = map_get(&pm, i);
+
+    if (plan_len != n) {
+        free(active);
+        free(deg);
+        free(sinks);
+        return -1;
+    }
+    double *scratch = (double *)calloc(n ? n : 1, sizeof(double));
+    double *cur = rank, *next = scratch;
+    uint64_t grain = (plan_len + plan_chunks - 1) / plan_chunks;
+    if (grain < 1) grain = 1;
+    uint64_t sink_chunks = plan_chunks_for(nsinks, grain);
+
+    double delta = epsilon + 1.0;
+    uint64_t iter = 0;
+    while (delta >= epsilon) {
+        pr_ctx c = {&view, active, deg, &wm, &pm, cur, next, 0.0, damping, sinks};
+        c.p_sink = run_chunks(nsinks, sink_chunks, threads, pr_sink_chunk, &c);
+        delta = run_chunks(n, plan_chunks, threads, pr_sweep_chunk, &c);
+        double *t = cur;
+        cur = next;
+        next = t;
+        iter += 1;
+        if (max_iter > 0 && iter == max_iter) break;
+    }
+    if (iter % 2 == 1) memcpy(next, cur, sizeof(double) * n);
+    free(scratch);
+    free(active);
+    free(deg);
+    free(sinks);
+    return (int64_t)iter;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Eigenvector (centrality.rs:405-497)                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const gt_view *w;
+    const uint8_t *active;
+    const gt_map *weight;
+    const double *prev;
+    double *next;
+    double scale;
+} ev_ctx;
+
+static double ev_sweep_chunk(void *c, uint64_t lo, uint64_t hi) {
+    ev_ctx *x = (ev_ctx *)c;
+    double acc = 0.0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        if (!x->active[i]) {
+            x->next[i] = x->prev[i];
+            continue;
+        }
+        double s = 0.0;
+        uint32_t cnt;
+        const gt_entry *in = view_in(x->w, (uint32_t)i, &cnt);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            if (!view_keeps(x->w, in[k])) continue;
+            s += map_get(x->weight, in[k].idx) * x->prev[in[k].other];
+        }
+        x->next[i] = s;
+        acc += s * s;
+    }
+    return acc;
+}
+
+static double ev_norm_chunk(void *c, uint64_t lo, uint64_t hi) {
+    ev_ctx *x = (ev_ctx *)c;
+    double acc = 0.0;
+    for (uint64_t i = lo; i < hi; ++i) {
+        if (!x->active[i]) continue;
+        x->next[i] /= x->scale;
+        acc += fabs(x->next[i] - x->prev[i]);
+    }
+    return acc;
+}
+
+/* Returns the sweep count (or -1 on a plan mismatch); *eig gets the last norm. */
+GT_EXPORT int64_t oracle_eigenvector(const gt_adj *g, int reversed,
+                                     const uint8_t *vmask, const uint8_t *emask,
+                                     int wkind, double wval, const double *w,
+                                     double epsilon, uint64_t max_iter,
+                                     uint64_t plan_len, uint64_t plan_chunks,
+                                     double *x, uint64_t had, int threads,
+                                     double *eig) {
+    gt_view view = {g, reversed, vmask, emask};
+    const uint32_t n = g->n;
+    gt_map wm = {wkind, wval, w};
+    uint8_t *active = (uint8_t *)malloc(n ? n : 1);
+    uint64_t nv = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        active[v] = vmask ? (vmask[v] != 0) : 1;
+        nv += active[v];
+    }
+    double uniform = nv == 0 ? 0.0 : 1.0 / (double)nv;
+    for (uint64_t i = had; i < n; ++i) x[i] = uniform;
+    *eig = 0.0;
+    if (plan_len != n) {
+        free(active);
+        return -1;
+    }
+    double *scratch = (double *)calloc(n ? n : 1, sizeof(double));
+    double *cur = x, *next = scratch;
+    double norm = 0.0, delta = epsilon + 1.0;
+    uint64_t iter = 0;
+    while (delta >= epsilon) {
+        ev_ctx c = {&view, active, &wm, cur, next, 1.0};
+        double sq = run_chunks(n, plan_chunks, threads, ev_sweep_chunk, &c);
+        norm = sqrt(sq);
+        c.scale = norm == 0.0 ? 1.0 : norm;
+        delta = run_chunks(n, plan_chunks, threads, ev_norm_chunk, &c);
+        double *t = cur;
+        cur = next;
+        next = t;
+        iter += 1;
+        if (max_iter > 0 && iter == max_iter) break;
+    }
+    if (iter % 2 == 1) memcpy(next, cur, sizeof(double) * n);
+    free(scratch);
+    free(active);
+    *eig = norm;
+    return (int64_t)iter;
+}

@@ -1,0 +1,113 @@
+"""CPU-side checks of the product: the C ABI library loads and exports every
+declared symbol; host-side generators and the host adjacency equal the
+oracle's bit for bit; the Python mirror of Plan / maps behaves like the
+reference.  No device compute is called here.
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gt_hip
+import oracle
+from graphs import EDGES, LAYOUT, Rng, thousand
+from gt_hip import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "gt_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:gt_status|const char \*)\s*(gt_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert "gt_pagerank_f64" in names and "gt_eigenvector_f64" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding declares exactly what the header declares
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_er_generator_matches_oracle():
+    for n, m, seed in [(10_000, 100_000, 0x243F6A8885A308D3), (7, 5, 1), (100, 200_000, 42)]:
+        a = gt_hip.gen_er(n, m, seed)
+        b = oracle.gen_er(n, m, seed)
+        assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+
+
+def test_rmat_generator_matches_oracle():
+    a = gt_hip.gen_rmat(16, 300_000, 0.57, 0.19, 0.19, 7)
+    b = oracle.gen_rmat(16, 300_000, 0.57, 0.19, 0.19, 7)
+    assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+
+
+def _adj_equal(n, s, t):
+    got = gt_hip.host_adjacency(n, s, t)
+    want = oracle.Adj(n, s, t).export()
+    for x, y in zip(got, want):
+        assert x.shape == y.shape and (x == y).all()
+
+
+def test_host_adjacency_layout_fixture():
+    s = np.array([e[0] for e in EDGES], np.uint32)
+    t = np.array([e[1] for e in EDGES], np.uint32)
+    _adj_equal(4, s, t)
+    g = gt_hip.AdjList.with_vertices(4)
+    for a, b in EDGES:
+        g.add_edge(a, b)
+    for v, (entries, pos) in enumerate(LAYOUT):
+        assert g.out_edges(v) == entries[:pos]
+        assert g.in_edges(v) == entries[pos:]
+
+
+def test_host_adjacency_matches_oracle_on_random_graphs():
+    r = Rng(3)
+    for n, m in [(1, 10), (5, 100), (50, 3000), (1000, 20_000)]:
+        s = np.array([r.below(n) for _ in range(m)], np.uint32)
+        t = np.array([r.below(n) for _ in range(m)], np.uint32)
+        _adj_equal(n, s, t)
+    _adj_equal(*thousand())
+
+
+def test_host_adjacency_hub_heavy():
+    # Many out-inserts on a hub with a non-empty in-half: long rotation runs.
+    s, t = oracle.gen_rmat(12, 60_000, 0.57, 0.19, 0.19, 9)
+    _adj_equal(4096, s, t)
+
+
+def test_no_such_vertex():
+    with pytest.raises(gt_hip.GtValueError):
+        gt_hip.host_adjacency(3, np.array([0, 3], np.uint32), np.array([1, 1], np.uint32))
+    g = gt_hip.AdjList.with_vertices(2)
+    with pytest.raises(ValueError):
+        g.add_edge(0, 2)
+
+
+def test_plan_matches_reference_partition():
+    p = gt_hip.Plan(1_000_001, 1000)
+    assert p.chunks() == 1001
+    cover = [x for k in range(p.chunks()) for x in p.range(k)]
+    assert cover == list(range(1_000_001))
+    lens = {len(p.range(k)) for k in range(p.chunks())}
+    assert lens <= {999, 1000}
+    assert gt_hip.Plan(0, 64).chunks() == 1
+    with pytest.raises(ValueError):
+        gt_hip.Plan(10, 0)
+
+
+def test_dense_prop_grows_and_never_shrinks():
+    m = gt_hip.DenseProp.from_vec(5, np.array([1.0, 2.0]))
+    assert m.sized_for(5, 4) == 2
+    assert m.data.tolist() == [1.0, 2.0, 0.0, 0.0]
+    assert m.sized_for(5, 3) == 4
+    assert len(m) == 4
+    with pytest.raises(ValueError):
+        m.sized_for(6, 4)
+
+
+def test_device_count_without_gpu_is_not_an_error():
+    assert gt_hip.device_count() >= 0
