@@ -189,13 +189,17 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     if (!iters || !eig) return fail(GT_ERR_INVALID_ARG, "null output");
     Orient *pull = nullptr, *push = nullptr;
     bool reversed = false;
-    GT_TRY(resolve_view(g, view, &pull, &push, &reversed));
     GT_TRY(check_vertex_out(g, x, "eigenvector: centrality"));
     GT_TRY(check_params(g, params));
+    const uint8_t *active = nullptr;
+    uint32_t nactive = 0;
+    GT_TRY(resolve_view(g, view, &pull, &push, &reversed, &active, &nactive));
+    const bool masked = active != nullptr;
     const uint32_t n = g->n;
     hipStream_t s = g->stream;
     Workspace &ws = g->ws;
-    const double uniform = n == 0 ? 0.0 : 1.0 / (double)n;
+    // New slots get 1 / num_vertices of the view (centrality.rs:418-429).
+    const double uniform = nactive == 0 ? 0.0 : 1.0 / (double)nactive;
     const double eps = params->epsilon;
     const bool skip = std::isnan(eps) || !(eps + 1.0 >= eps);
 
@@ -249,7 +253,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.in_off = pull->off.as<uint32_t>();
         a.in_idx = pull->idx.as<uint32_t>();
         a.w = wp;
-        a.active = nullptr;
+        a.active = active;
         a.epsilon = eps;
         a.max_iter = params->max_iter;
         a.partials = ws.partials.as<double>();
@@ -264,10 +268,15 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            if (weighted)
-                hipLaunchKernelGGL((ev_sweep<true, false>), dim3(g->grid), dim3(kBlock), 0, s, b);
+            const dim3 gd(g->grid), bd(kBlock);
+            if (weighted && masked)
+                hipLaunchKernelGGL((ev_sweep<true, true>), gd, bd, 0, s, b);
+            else if (weighted)
+                hipLaunchKernelGGL((ev_sweep<true, false>), gd, bd, 0, s, b);
+            else if (masked)
+                hipLaunchKernelGGL((ev_sweep<false, true>), gd, bd, 0, s, b);
             else
-                hipLaunchKernelGGL((ev_sweep<false, false>), dim3(g->grid), dim3(kBlock), 0, s, b);
+                hipLaunchKernelGGL((ev_sweep<false, false>), gd, bd, 0, s, b);
         };
         // The driver reads done / delta / iter through the SweepState layout;
         // EvState keeps delta, iter and done at the same offsets.
@@ -287,8 +296,12 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         out = y[T % 3];
     }
     double *dst = x->loc == GT_DEVICE ? x->data : y[0];
-    hipLaunchKernelGGL((ev_emit<false>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, nullptr, st, n,
-                       dst);
+    if (masked)
+        hipLaunchKernelGGL((ev_emit<true>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, active, st,
+                           n, dst);
+    else
+        hipLaunchKernelGGL((ev_emit<false>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, nullptr,
+                           st, n, dst);
     GT_HIP(hipGetLastError());
     if (x->loc == GT_HOST)
         GT_HIP(hipMemcpyAsync(x->data, dst, vb, hipMemcpyDeviceToHost, s));

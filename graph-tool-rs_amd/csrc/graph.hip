@@ -244,13 +244,6 @@ gt_status gt_graph_upload(uint64_t graph_id, uint32_t n, uint64_t edge_bound,
     return GT_OK;
 }
 
-gt_status gt_graph_build_edges(uint64_t graph_id, uint32_t n, uint64_t m, const uint32_t *src,
-                               const uint32_t *tgt, int32_t loc, gt_graph **out) {
-    clear_error();
-    (void)graph_id; (void)n; (void)m; (void)src; (void)tgt; (void)loc; (void)out;
-    return fail(GT_ERR_UNSUPPORTED, "device builder not available in this build");
-}
-
 gt_status gt_graph_free(gt_graph *g) {
     if (!g) return GT_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);

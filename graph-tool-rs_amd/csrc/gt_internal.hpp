@@ -115,8 +115,9 @@ struct Workspace {
     DevBuf long_part;   // f64[nslots]
     DevBuf state;       // device SweepState
     DevBuf pers;        // staged personalisation
-    DevBuf masks;       // staged masks
-    DevBuf scratch_graph[6];  // filtered CSC/CSR
+    DevBuf masks;       // staged vertex + edge masks
+    Orient fpull;       // filtered view: compacted pull orientation
+    Orient fpush;       // filtered view: compacted push orientation
 };
 
 struct gt_graph_impl;

@@ -134,18 +134,20 @@ gt_status gt_host_adjacency(uint32_t n, uint64_t m, const uint32_t *src, const u
         if (size[s] > 0) {
             const uint32_t cap = ind[s], base = in_off[s];
             const uint32_t front = base + head[s];
-            const uint32_t back = base + (head[s] + size[s]) % cap;
             if (size[s] < cap) {
-                in_other[back] = in_other[front];
-                in_eid[back] = in_eid[front];
+                uint32_t b = head[s] + size[s];
+                if (b >= cap) b -= cap;
+                in_other[base + b] = in_other[front];
+                in_eid[base + b] = in_eid[front];
             }
-            head[s] = (head[s] + 1) % cap;
+            head[s] = (head[s] + 1 == cap) ? 0 : head[s] + 1;
         }
         {
             const uint32_t cap = ind[t], base = in_off[t];
-            const uint32_t back = base + (head[t] + size[t]) % cap;
-            in_other[back] = s;
-            in_eid[back] = (uint32_t)i;
+            uint32_t b = head[t] + size[t];
+            if (b >= cap) b -= cap;
+            in_other[base + b] = s;
+            in_eid[base + b] = (uint32_t)i;
             ++size[t];
         }
     }

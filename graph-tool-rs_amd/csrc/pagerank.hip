@@ -255,8 +255,10 @@ void dispatch_init(const PrPlan &p, const PrArgs &a, double *rank, uint64_t had,
 }  // namespace
 
 gt_status resolve_view(gt_graph *g, const gt_view *view, Orient **pull, Orient **push,
-                       bool *reversed) {
+                       bool *reversed, const uint8_t **active, uint32_t *nactive) {
     int kind = view ? view->kind : GT_VIEW_DIRECTED;
+    *active = nullptr;
+    *nactive = g->n;
     // The message of ViewError::NotDirected (crates/gt-py/src/dispatch.rs:176-179).
     if (kind == GT_VIEW_UNDIRECTED || kind == GT_VIEW_UNDIRECTED_FILTERED)
         return fail(GT_ERR_NOT_DIRECTED,
@@ -264,8 +266,10 @@ gt_status resolve_view(gt_graph *g, const gt_view *view, Orient **pull, Orient *
                         (kind == GT_VIEW_UNDIRECTED ? "Undirected" : "UndirectedFiltered") +
                         " has no in-edges; this kernel accepts [Directed, Reversed, "
                         "DirectedFiltered, ReversedFiltered]");
-    if (kind == GT_VIEW_DIRECTED_FILTERED || kind == GT_VIEW_REVERSED_FILTERED)
-        return fail(GT_ERR_UNSUPPORTED, "filtered views are not served by this build yet");
+    if (kind == GT_VIEW_DIRECTED_FILTERED || kind == GT_VIEW_REVERSED_FILTERED) {
+        *reversed = (kind == GT_VIEW_REVERSED_FILTERED);
+        return filtered_view(g, view, *reversed, pull, push, active, nactive);
+    }
     if (kind != GT_VIEW_DIRECTED && kind != GT_VIEW_REVERSED)
         return fail(GT_ERR_INVALID_ARG, "unknown view kind");
     *reversed = (kind == GT_VIEW_REVERSED);
@@ -313,13 +317,19 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
                      gt_vertex_out *rank, PrSolve &S) {
     S.g = g;
     bool reversed = false;
-    GT_TRY(resolve_view(g, view, &S.pull, &S.push, &reversed));
     GT_TRY(check_vertex_out(g, rank, "pagerank: rank"));
     GT_TRY(check_params(g, params));
+    const uint8_t *active = nullptr;
+    uint32_t nactive = 0;
+    GT_TRY(resolve_view(g, view, &S.pull, &S.push, &reversed, &active, &nactive));
     const uint32_t n = g->n;
     hipStream_t s = g->stream;
-    const double uniform = n == 0 ? 0.0 : 1.0 / (double)n;
+    // pagerank's uniform personalisation is 1 / num_vertices of the view
+    // (centrality.rs:245-247), the filtered count on a filtered view.
+    const double uniform = nactive == 0 ? 0.0 : 1.0 / (double)nactive;
     PrArgs &a = S.base;
+    S.plan.masked = active != nullptr;
+    a.active = active;
     GT_TRY(stage_vertex_prop(g, pers, g->ws.pers, &a.pers, &a.pval, &S.plan.dpers, uniform));
     if (weight && weight->kind != GT_MAP_UNITY) {
         S.plan.weighted = true;

@@ -24,8 +24,12 @@ gt_status stage_vertex_prop(gt_graph *g, const gt_prop *pm, DevBuf &buf, const d
                             double *scalar, bool *dense, double default_value);
 
 // Shared validation of the centrality entry points.
+// The four directed views: pull / push orientations, and for a filtered view
+// the device vertex mask (`active`, else null) and the kept-vertex count.
 gt_status resolve_view(gt_graph *g, const gt_view *view, Orient **pull, Orient **push,
-                       bool *reversed);
+                       bool *reversed, const uint8_t **active, uint32_t *nactive);
+gt_status filtered_view(gt_graph *g, const gt_view *view, bool reversed, Orient **pull,
+                        Orient **push, const uint8_t **active, uint32_t *nactive);
 gt_status check_vertex_out(gt_graph *g, const gt_vertex_out *r, const char *who);
 gt_status check_params(gt_graph *g, const gt_power_iteration *p);
 

@@ -238,9 +238,32 @@ class DeviceGraph:
 
     @classmethod
     def from_edges(cls, n: int, src, tgt, graph_id: Optional[int] = None) -> "DeviceGraph":
+        """Host adjacency (push-swap simulated on the CPU), then uploaded."""
         gid = next(_graph_ids) if graph_id is None else graph_id
         adj = host_adjacency(n, src, tgt)
         return cls.upload(gid, n, int(np.asarray(src).shape[0]), *adj)
+
+    @classmethod
+    def build(cls, n: int, src, tgt, graph_id: Optional[int] = None) -> "DeviceGraph":
+        """Device builder: histogram, scan, stable scatter and push-swap replay
+        on the GPU.  ``src`` / ``tgt`` are uint32 numpy arrays or CUDA tensors."""
+        gid = next(_graph_ids) if graph_id is None else graph_id
+        h = _lib.GraphP()
+        if _is_torch_cuda(src):
+            import torch
+            s = src.to(torch.int32).contiguous()
+            t = tgt.to(torch.int32).contiguous()
+            torch.cuda.current_stream(s.device).synchronize()
+            call("gt_graph_build_edges", gid, n, int(s.numel()), C.c_void_p(s.data_ptr()),
+                 C.c_void_p(t.data_ptr()), GT_DEVICE, C.byref(h))
+        else:
+            s = np.ascontiguousarray(src, dtype=np.uint32)
+            t = np.ascontiguousarray(tgt, dtype=np.uint32)
+            if s.shape != t.shape:
+                raise ValueError("source and target arrays differ in length")
+            call("gt_graph_build_edges", gid, n, int(s.shape[0]), s.ctypes.data_as(C.c_void_p),
+                 t.ctypes.data_as(C.c_void_p), GT_HOST, C.byref(h))
+        return cls(h.value, gid)
 
     @property
     def handle(self) -> C.c_void_p:

@@ -301,3 +301,48 @@ def test_eigenvector_matches_oracle_on_symmetrised_rmat(gpu):
     co, eo, ito = oracle.Adj(n, S, T).eigenvector(epsilon=1e-9, max_iter=1000, threads=8)
     parity(c, it, co, ito, 1e-9)
     assert abs(eig - eo) <= 1e-9 * eo
+
+
+# ------------------------------------------------------------ filtered views
+
+def _masks(n, m, seed):
+    r = Rng(seed)
+    vmask = np.array([0 if r.below(7) == 3 else 1 for _ in range(n)], np.uint8)
+    emask = np.array([0 if r.below(5) == 1 else 1 for _ in range(m)], np.uint8)
+    return vmask, emask
+
+
+def test_filtered_views(gpu):
+    n, s, t = thousand()
+    m = len(s)
+    vmask, emask = _masks(n, m, 31)
+    g = adj(n, s, t)
+    o = oracle.Adj(n, s, t)
+    w = np.array([0.25 + (i % 11) / 5.0 for i in range(m)])
+    start = np.linspace(0.1, 0.9, n) / n
+    for rev in (False, True):
+        view = g.masked(vmask, emask, reversed=rev)
+        got, it = pr(view, 1e-12, 1000)
+        want, it_o = o.pagerank(vmask=vmask, emask=emask, reversed=rev, epsilon=1e-12,
+                                max_iter=1000)
+        parity(got, it, want, it_o, 1e-12)
+        rank = gt_hip.DenseProp.from_vec(g.graph_id, start.copy())
+        got, it = pr(view, 1e-12, 1000, weight=gt_hip.DenseProp.from_vec(g.graph_id, w),
+                     rank=rank)
+        want, it_o = o.pagerank(weight=w, vmask=vmask, emask=emask, reversed=rev,
+                                epsilon=1e-12, max_iter=1000, rank=start)
+        parity(got, it, want, it_o, 1e-12)
+        dropped = np.flatnonzero(vmask == 0)
+        assert (got[dropped] == start[dropped]).all()
+        x = gt_hip.DenseProp.new(g.graph_id)
+        eig, it = gt_hip.eigenvector(view, gt_hip.Unity.NEW, params(n, 1e-12, 1000), x)
+        xo, eo, ito = o.eigenvector(vmask=vmask, emask=emask, reversed=rev, epsilon=1e-12,
+                                    max_iter=1000)
+        parity(x.data, it, xo, ito, 1e-12)
+        assert abs(eig - eo) <= 1e-12 * max(1.0, abs(eo))
+
+
+def test_short_masks_are_rejected(gpu):
+    g = adj(4, [(0, 1), (1, 2)])
+    with pytest.raises(ValueError, match="filter"):
+        g.masked(np.ones(3, np.uint8), np.ones(2, np.uint8))
