@@ -1,0 +1,446 @@
+// Adjoint of the PageRank fixed point (extension: no counterpart in the
+// reference).  For x = F(x) = (1-d) p + d (M x + (q.x) p), M[t,s] = w_e/deg_s
+// over edges e: s -> t, q the sink indicator, and a loss with dL/dx = c:
+//
+//   lambda = c + d M^T lambda + d (p.lambda) q        (transposed system)
+//   dL/dw_e = d * x_s / deg_s * (lambda_t - mu_s),     mu = M^T lambda
+//   dL/dp   = ((1-d) + d (q.x)) lambda
+//   dL/dd   = lambda . (x - p) / d
+//
+// The transposed system is solved by the same power iteration, pulling over
+// the opposite orientation: row s of the out-halves gathers lambda at its
+// targets (times the out-ordered weight), so the tile engine and the
+// last-workgroup folds of the forward sweep are reused unchanged.  No
+// iterate of the forward solve is stored: only x* is read.  The fused edge
+// kernel then computes mu_s and every out-edge's gradient of row s in one
+// wavefront, writing by edge id.
+#include <cmath>
+#include <type_traits>
+
+#include "device_common.hpp"
+#include "driver.hpp"
+#include "gt_internal.hpp"
+#include "props.hpp"
+#include "sweep_common.hpp"
+
+namespace gt {
+
+struct AdjArgs {
+    const Tile *tiles;
+    uint32_t ntiles;
+    uint32_t nlong;
+    const uint32_t *row_off;   // push orientation (rows = sources)
+    const uint32_t *row_idx;   // targets
+    const void *w;             // weights in push order (value type)
+    const double *deg;         // weighted out-degree (weighted)
+    const double *cvec;        // dL/dx, f64
+    const double *pers;        // dense personalisation
+    double pval;
+    double damping;
+    double eps;
+    uint64_t max_iter;
+    const void *l_cur;
+    void *l_next;
+    double *partials;
+    double *long_part;
+    const uint32_t *long_rows;
+    const uint32_t *long_slot;
+    SweepState *st;            // p_sink holds p . lambda of the current iterate
+};
+
+template <typename V, bool WEIGHTED, bool DPERS>
+struct AdjFin {
+    const AdjArgs &a;
+    double pl;
+    double dsum = 0.0;
+    double psum = 0.0;
+    __device__ AdjFin(const AdjArgs &args, double p) : a(args), pl(p) {}
+    __device__ __forceinline__ void operator()(uint32_t s, double acc) {
+        const V *lc = static_cast<const V *>(a.l_cur);
+        V *ln = static_cast<V *>(a.l_next);
+        const double dg =
+            WEIGHTED ? a.deg[s] : static_cast<double>(a.row_off[s + 1] - a.row_off[s]);
+        const double mu = dg != 0.0 ? acc / dg : 0.0;
+        const double v = a.cvec[s] + a.damping * mu + (dg == 0.0 ? a.damping * pl : 0.0);
+        const V vv = static_cast<V>(v);
+        dsum += fabs(static_cast<double>(vv) - static_cast<double>(lc[s]));
+        ln[s] = vv;
+        psum += (DPERS ? a.pers[s] : a.pval) * static_cast<double>(vv);
+    }
+};
+
+template <typename V, bool WEIGHTED, bool DPERS>
+__global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
+    if (a.st->done) return;
+    __shared__ TileLds<V> L;
+    __shared__ uint32_t flag;
+    double *red = L.red;
+    const int tid = threadIdx.x;
+    AdjFin<V, WEIGHTED, DPERS> fin(a, a.st->p_sink);
+    spmv_tiles<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.row_idx,
+                            static_cast<const V *>(a.l_cur), static_cast<const V *>(a.w),
+                            a.long_part, L, fin);
+    double d = fin.dsum, p = fin.psum;
+    block_sum2(d, p, red);
+    if (tid == 0) {
+        a.partials[blockIdx.x] = d;
+        a.partials[gridDim.x + blockIdx.x] = p;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    AdjFin<V, WEIGHTED, DPERS> lf(a, fin.pl);
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        lf.dsum += a.partials[k];
+        lf.psum += a.partials[gridDim.x + k];
+    }
+    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+        lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
+    d = lf.dsum;
+    p = lf.psum;
+    block_sum2(d, p, red);
+    if (tid == 0) {
+        SweepState *st = a.st;
+        const uint64_t it = st->iter + 1;
+        st->delta = d;
+        st->p_sink = p;
+        st->iter = it;
+        st->launches += 1;
+        st->done = (!(d >= a.eps) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
+        st->counter = 0;
+        __threadfence();
+    }
+}
+
+// lambda_0 = c; p . lambda_0 into the state.
+template <typename V, bool DPERS>
+__global__ __launch_bounds__(kBlock) void adj_init(const AdjArgs a, uint32_t n) {
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    V *l0 = static_cast<V *>(a.l_next);
+    double p = 0.0, z = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const V v = static_cast<V>(a.cvec[i]);
+        l0[i] = v;
+        p += (DPERS ? a.pers[i] : a.pval) * static_cast<double>(v);
+    }
+    block_sum2(p, z, red);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    double t = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += a.partials[k];
+    t = block_sum(t, red);
+    if (threadIdx.x == 0) {
+        SweepState *st = a.st;
+        *st = SweepState{};
+        st->p_sink = t;
+        __threadfence();
+    }
+}
+
+// One wavefront per source row s: mu_s = (sum w_e lambda_t) / deg_s, then
+// dL/dw_e = d * x_s / deg_s * (lambda_t - mu_s) for each out-entry, written
+// at the entry's edge id.  Rows with deg_s == 0 write 0.
+template <typename V, typename G, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void grad_edges(
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ idx,
+    const uint32_t *__restrict__ eid, const V *__restrict__ w, const double *__restrict__ deg,
+    const V *__restrict__ lam, const double *__restrict__ x, double damping, uint32_t n,
+    G *__restrict__ grad) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < n; s += waves) {
+        const uint32_t lo = off[s], hi = off[s + 1];
+        if (lo == hi) continue;
+        const double dg = WEIGHTED ? deg[s] : (double)(hi - lo);
+        double acc = 0.0;
+        for (uint32_t j = lo + lane; j < hi; j += kWave)
+            acc += (WEIGHTED ? (double)w[j] : 1.0) * (double)lam[idx[j]];
+        acc = wave_sum(acc);
+        if (dg == 0.0) {
+            for (uint32_t j = lo + lane; j < hi; j += kWave) grad[eid[j]] = G(0);
+            continue;
+        }
+        const double mu = acc / dg;
+        const double cs = damping * x[s] / dg;
+        for (uint32_t j = lo + lane; j < hi; j += kWave)
+            grad[eid[j]] = static_cast<G>(cs * ((double)lam[idx[j]] - mu));
+    }
+}
+
+// Sums for the vertex gradients: p_sink* = sum over sinks of x, and
+// lambda . (x - p).  Folded by the last workgroup into st->aux / st->norm.
+template <typename V, bool WEIGHTED, bool DPERS>
+__global__ __launch_bounds__(kBlock) void grad_sums(const AdjArgs a, const V *lam,
+                                                    const double *x, uint32_t n) {
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    double sk = 0.0, ld = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double dg =
+            WEIGHTED ? a.deg[i] : static_cast<double>(a.row_off[i + 1] - a.row_off[i]);
+        if (dg == 0.0) sk += x[i];
+        ld += static_cast<double>(lam[i]) * (x[i] - (DPERS ? a.pers[i] : a.pval));
+    }
+    block_sum2(sk, ld, red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = sk;
+        a.partials[gridDim.x + blockIdx.x] = ld;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    double s = 0.0, l = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        s += a.partials[k];
+        l += a.partials[gridDim.x + k];
+    }
+    block_sum2(s, l, red);
+    if (threadIdx.x == 0) {
+        a.st->aux = s;
+        a.st->norm = l;
+        a.st->counter = 0;
+        __threadfence();
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void grad_pers_k(const V *lam, const SweepState *st,
+                                                      double damping, uint32_t n, double *gp) {
+    const double f = (1.0 - damping) + damping * st->aux;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        gp[i] = f * static_cast<double>(lam[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void f32_to_f64_v(const float *a, uint32_t n, double *b) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        b[i] = static_cast<double>(a[i]);
+}
+
+template <typename V, bool W, bool P>
+static int adj_grid(uint32_t ntiles) {
+    static const int cap = occupancy_cap((const void *)adj_sweep<V, W, P>);
+    return fit_grid(cap, ntiles);
+}
+
+static unsigned vgrid(uint64_t n) {
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b > 2048) b = 2048;
+    return b ? (unsigned)b : 1u;
+}
+static unsigned wave_grid(uint32_t n) {
+    uint64_t b = ((uint64_t)n + 3) / 4;
+    if (b > 16384) b = 16384;
+    return b ? (unsigned)b : 1u;
+}
+
+// A vertex map (f64/f32, host/device) as a device f64 array.
+static gt_status stage_vec(gt_graph *g, const gt_prop *p, DevBuf &buf, const double **out,
+                           const char *what) {
+    const uint32_t n = g->n;
+    if (!p || p->kind != GT_MAP_DENSE || (!p->data && n))
+        return fail(GT_ERR_INVALID_ARG, std::string(what) + " must be a dense vertex map");
+    if (p->len < n) return fail(GT_ERR_INVALID_ARG, std::string(what) + " is shorter than the vertex bound");
+    if (p->dtype == GT_F64 && p->loc == GT_DEVICE) {
+        *out = static_cast<const double *>(p->data);
+        return GT_OK;
+    }
+    GT_TRY(buf.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
+    if (p->dtype == GT_F64) {
+        GT_HIP(hipMemcpyAsync(buf.ptr, p->data, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+    } else if (p->dtype == GT_F32) {
+        const float *f = static_cast<const float *>(p->data);
+        if (p->loc == GT_HOST) {
+            std::vector<double> tmp(f, f + n);
+            GT_HIP(hipMemcpyAsync(buf.ptr, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                                  g->stream));
+            GT_HIP(hipStreamSynchronize(g->stream));
+        } else {
+            hipLaunchKernelGGL(f32_to_f64_v, dim3(vgrid(n)), dim3(kBlock), 0, g->stream, f, n,
+                               buf.as<double>());
+        }
+    } else {
+        return fail(GT_ERR_INVALID_ARG, std::string(what) + " dtype must be f64 or f32");
+    }
+    *out = buf.as<double>();
+    return GT_OK;
+}
+
+template <typename V>
+static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                               const gt_prop *pers, double damping,
+                               const gt_power_iteration *params, const gt_prop *xm,
+                               const gt_prop *cm, void *grad_w, int32_t gw_dtype,
+                               int32_t grad_loc, double *grad_pers, double *grad_damping,
+                               uint64_t *iters) {
+    GT_TRY(check_params(g, params));
+    Orient *pull = nullptr, *push = nullptr;
+    bool reversed = false;
+    const uint8_t *active = nullptr;
+    uint32_t nactive = 0;
+    GT_TRY(resolve_view(g, view, &pull, &push, &reversed, &active, &nactive));
+    if (active) return fail(GT_ERR_UNSUPPORTED, "the adjoint on a filtered view");
+    if (grad_w && gw_dtype != GT_F64 && gw_dtype != GT_F32)
+        return fail(GT_ERR_INVALID_ARG, "grad_w dtype must be f64 or f32");
+    if (grad_loc != GT_HOST && grad_loc != GT_DEVICE)
+        return fail(GT_ERR_INVALID_ARG, "bad location");
+    const uint32_t n = g->n;
+    hipStream_t s = g->stream;
+    Workspace &ws = g->ws;
+    AdjArgs a{};
+    const double uniform = n == 0 ? 0.0 : 1.0 / (double)n;
+    bool dpers = false;
+    GT_TRY(stage_vertex_prop(g, pers, ws.pers, &a.pers, &a.pval, &dpers, uniform));
+    const double *x = nullptr, *c = nullptr;
+    GT_TRY(stage_vec(g, xm, ws.ax, &x, "x"));
+    GT_TRY(stage_vec(g, cm, ws.ac, &c, "dL/dx"));
+    const bool weighted = weight && weight->kind != GT_MAP_UNITY;
+    if (weighted) {
+        const void *src = nullptr;
+        int dt = GT_F64;
+        GT_TRY(stage_edge_source(g, weight, &src, &dt));
+        GT_TRY(stage_degree(g, weight, *push, src, dt, ws.deg));
+        GT_TRY(stage_csc_weights<V>(g, weight, *push, src, dt, ws.wcsr));
+        a.deg = ws.deg.as<double>();
+        a.w = ws.wcsr.ptr;
+    }
+    if (n == 0) {
+        if (grad_damping) *grad_damping = 0.0;
+        if (iters) *iters = 0;
+        return GT_OK;
+    }
+    GT_TRY(ensure_schedule(g, *push, n));
+    GT_TRY(prof_prepare(g));
+    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, push->nslots)));
+    GT_TRY(ws.state.reserve(sizeof(SweepState) * 2));
+    GT_TRY(ws.lam[0].reserve(sizeof(V) * n));
+    GT_TRY(ws.lam[1].reserve(sizeof(V) * n));
+    V *lam[2] = {ws.lam[0].as<V>(), ws.lam[1].as<V>()};
+    a.tiles = push->tiles.as<Tile>();
+    a.ntiles = push->ntiles;
+    a.nlong = push->nlong;
+    a.row_off = push->off.as<uint32_t>();
+    a.row_idx = push->idx.as<uint32_t>();
+    a.cvec = c;
+    a.damping = damping;
+    a.eps = params->epsilon;
+    a.max_iter = params->max_iter;
+    a.partials = ws.partials.as<double>();
+    a.long_part = ws.long_part.as<double>();
+    a.long_rows = push->long_rows.as<uint32_t>();
+    a.long_slot = push->long_slot.as<uint32_t>();
+    a.st = ws.state.as<SweepState>();
+    GT_HIP(hipMemsetAsync(a.st, 0, sizeof(SweepState), s));
+    {
+        AdjArgs b = a;
+        b.l_next = lam[0];
+        if (dpers)
+            hipLaunchKernelGGL((adj_init<V, true>), dim3(vgrid(n)), dim3(kBlock), 0, s, b, n);
+        else
+            hipLaunchKernelGGL((adj_init<V, false>), dim3(vgrid(n)), dim3(kBlock), 0, s, b, n);
+        GT_HIP(hipGetLastError());
+    }
+    g->prof_kind = GT_SWEEP_OTHER;
+    auto launch = [&](uint64_t q) {
+        const int par = (int)(q & 1);
+        AdjArgs b = a;
+        b.l_cur = lam[par];
+        b.l_next = lam[par ^ 1];
+        const dim3 bd(kBlock);
+        if (weighted && dpers)
+            hipLaunchKernelGGL((adj_sweep<V, true, true>), dim3(adj_grid<V, true, true>(b.ntiles)),
+                               bd, 0, s, b);
+        else if (weighted)
+            hipLaunchKernelGGL((adj_sweep<V, true, false>),
+                               dim3(adj_grid<V, true, false>(b.ntiles)), bd, 0, s, b);
+        else if (dpers)
+            hipLaunchKernelGGL((adj_sweep<V, false, true>),
+                               dim3(adj_grid<V, false, true>(b.ntiles)), bd, 0, s, b);
+        else
+            hipLaunchKernelGGL((adj_sweep<V, false, false>),
+                               dim3(adj_grid<V, false, false>(b.ntiles)), bd, 0, s, b);
+    };
+    uint64_t launched = 0;
+    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, launch, &launched));
+    const uint64_t sweeps = g->h_state->iter;
+    const V *lf = lam[sweeps & 1];
+
+    // Edge gradients.
+    if (grad_w) {
+        void *gdst = grad_w;
+        const size_t gb = (gw_dtype == GT_F64 ? 8 : 4) * (size_t)g->edge_bound;
+        if (grad_loc == GT_HOST) {
+            GT_TRY(ws.gw.reserve(std::max<size_t>(gb, 8)));
+            gdst = ws.gw.ptr;
+        }
+        // Edges outside the view keep 0.
+        if (gb) GT_HIP(hipMemsetAsync(gdst, 0, gb, s));
+        const V *wv = static_cast<const V *>(a.w);
+        const dim3 gd(wave_grid(n)), bd(kBlock);
+        const uint32_t *po = push->off.as<uint32_t>(), *pi = push->idx.as<uint32_t>(),
+                       *pe = push->eid.as<uint32_t>();
+        if (gw_dtype == GT_F64) {
+            if (weighted)
+                hipLaunchKernelGGL((grad_edges<V, double, true>), gd, bd, 0, s, po, pi, pe, wv,
+                                   a.deg, lf, x, damping, n, static_cast<double *>(gdst));
+            else
+                hipLaunchKernelGGL((grad_edges<V, double, false>), gd, bd, 0, s, po, pi, pe, wv,
+                                   a.deg, lf, x, damping, n, static_cast<double *>(gdst));
+        } else {
+            if (weighted)
+                hipLaunchKernelGGL((grad_edges<V, float, true>), gd, bd, 0, s, po, pi, pe, wv,
+                                   a.deg, lf, x, damping, n, static_cast<float *>(gdst));
+            else
+                hipLaunchKernelGGL((grad_edges<V, float, false>), gd, bd, 0, s, po, pi, pe, wv,
+                                   a.deg, lf, x, damping, n, static_cast<float *>(gdst));
+        }
+        GT_HIP(hipGetLastError());
+        if (grad_loc == GT_HOST && gb)
+            GT_HIP(hipMemcpyAsync(grad_w, gdst, gb, hipMemcpyDeviceToHost, s));
+    }
+    // Vertex gradients.
+    {
+        const dim3 gd(vgrid(n)), bd(kBlock);
+        if (weighted && dpers) hipLaunchKernelGGL((grad_sums<V, true, true>), gd, bd, 0, s, a, lf, x, n);
+        else if (weighted) hipLaunchKernelGGL((grad_sums<V, true, false>), gd, bd, 0, s, a, lf, x, n);
+        else if (dpers) hipLaunchKernelGGL((grad_sums<V, false, true>), gd, bd, 0, s, a, lf, x, n);
+        else hipLaunchKernelGGL((grad_sums<V, false, false>), gd, bd, 0, s, a, lf, x, n);
+        GT_HIP(hipGetLastError());
+        if (grad_pers) {
+            double *gp = grad_pers;
+            if (grad_loc == GT_HOST) {
+                GT_TRY(ws.gp.reserve(sizeof(double) * n));
+                gp = ws.gp.as<double>();
+            }
+            hipLaunchKernelGGL((grad_pers_k<V>), gd, bd, 0, s, lf, a.st, damping, n, gp);
+            GT_HIP(hipGetLastError());
+            if (grad_loc == GT_HOST)
+                GT_HIP(hipMemcpyAsync(grad_pers, gp, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        }
+    }
+    GT_HIP(hipMemcpyAsync(g->h_state, a.st, sizeof(SweepState), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    if (grad_damping) *grad_damping = g->h_state->norm / damping;
+    if (iters) *iters = sweeps;
+    g->last_sweeps = sweeps;
+    return GT_OK;
+}
+
+}  // namespace gt
+
+using namespace gt;
+
+extern "C" gt_status gt_pagerank_backward(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                                          const gt_prop *pers, double damping,
+                                          const gt_power_iteration *params, const gt_prop *x,
+                                          const gt_prop *dl_dx, int32_t precision, void *grad_w,
+                                          int32_t grad_w_dtype, int32_t grad_loc,
+                                          double *grad_pers, double *grad_damping,
+                                          uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (precision == GT_F64)
+        return backward_impl<double>(g, view, weight, pers, damping, params, x, dl_dx, grad_w,
+                                     grad_w_dtype, grad_loc, grad_pers, grad_damping, iters);
+    if (precision == GT_F32)
+        return backward_impl<float>(g, view, weight, pers, damping, params, x, dl_dx, grad_w,
+                                    grad_w_dtype, grad_loc, grad_pers, grad_damping, iters);
+    return fail(GT_ERR_INVALID_ARG, "precision must be GT_F64 or GT_F32");
+}

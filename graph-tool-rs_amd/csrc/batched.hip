@@ -1,0 +1,387 @@
+// Batched personalised PageRank: k one-hot personalisation vectors solved
+// together as one SpMM per sweep.  Column c is pagerank_with
+// (crates/gt-algo/src/centrality.rs:261-384) with pers = e_{seeds[c]},
+// started from its personalisation (a fresh map, seed_new_slots :198-205),
+// stopped on its own L1 change and frozen from then on, so its sweep count is
+// the one its single-vector run would report.
+//
+// Layout: the state is [n][64] FP32, one 256-byte row per vertex; a
+// wavefront owns a row and lane c owns column c, so every gathered source row
+// is one coalesced 256-byte load.  Values, weights and the SpMM accumulation
+// are FP32; the teleport update, the per-column dangling mass and the
+// per-column L1 change are FP64.  Tiles of <= 1024 in-entries; longer rows
+// are split into chunks whose [64] partial sums the last workgroup combines
+// in chunk order, as it folds the per-workgroup, per-column partials in index
+// order.  k > 64 runs as consecutive blocks of 64 columns.
+#include <cmath>
+
+#include "device_common.hpp"
+#include "driver.hpp"
+#include "gt_internal.hpp"
+#include "props.hpp"
+
+namespace gt {
+
+constexpr int kCols = 64;
+constexpr uint32_t kNoSeed = 0xffffffffu;
+
+struct BatchState {
+    SweepState h;            // delta = max over active columns, iter = sweeps
+    double p_sink[kCols];
+    double delta[kCols];
+    uint64_t iters[kCols];
+    uint64_t active;         // bit c: column c still iterating
+    uint64_t pad;
+};
+
+struct BArgs {
+    const Tile *tiles;
+    uint32_t ntiles;
+    uint32_t nlong;
+    const uint32_t *in_off;
+    const uint32_t *in_idx;
+    const float *w;           // pull-order weights (weighted only)
+    const uint32_t *out_off;  // unity out-degree
+    const double *deg;        // weighted out-degree
+    const uint32_t *seeds;    // [64], kNoSeed for an unused column
+    double damping;
+    double eps;
+    uint64_t max_iter;
+    const float *x_cur;
+    float *x_next;
+    const float *c_cur;
+    float *c_next;
+    double *partials;         // [2][gridDim.x][64]
+    float *long_part;         // [nslots][64]
+    const uint32_t *long_rows;
+    const uint32_t *long_slot;
+    BatchState *st;
+};
+
+// Sum over entries lo, lo+stride, ... < hi of w(j) * c[src(j)][lane], in
+// entry order.  Eight entries are in flight per wavefront.
+template <bool WEIGHTED>
+__device__ __forceinline__ float row_accumulate(const BArgs &a, uint32_t lo, uint32_t hi,
+                                                uint32_t stride, int lane) {
+    float acc = 0.f;
+    for (uint32_t j0 = lo; j0 < hi; j0 += 8 * stride) {
+        const uint32_t mj = j0 + (uint32_t)lane * stride;
+        const bool mine = lane < 8 && mj < hi;
+        const uint32_t my_idx = mine ? a.in_idx[mj] : 0u;
+        const float my_w = (WEIGHTED && mine) ? a.w[mj] : 1.f;
+        float v[8], wv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t s = __shfl(my_idx, u, kWave);
+            wv[u] = WEIGHTED ? __shfl(my_w, u, kWave) : 1.f;
+            v[u] = (j0 + u * stride < hi) ? a.c_cur[(size_t)s * kCols + lane] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j0 + u * stride < hi) acc += wv[u] * v[u];
+    }
+    return acc;
+}
+
+template <bool WEIGHTED>
+__device__ __forceinline__ void finish_row(const BArgs &a, uint32_t r, float acc, int lane,
+                                           bool col_on, uint32_t seed, double psink,
+                                           double &dsum, double &ssum) {
+    const size_t o = (size_t)r * kCols + lane;
+    const float xo = a.x_cur[o];
+    if (!col_on) {
+        a.x_next[o] = xo;
+        a.c_next[o] = a.c_cur[o];
+        return;
+    }
+    const double pv = (r == seed) ? 1.0 : 0.0;
+    const double nv = (1.0 - a.damping) * pv + a.damping * (psink * pv + (double)acc);
+    const float nf = (float)nv;
+    dsum += fabs((double)nf - (double)xo);
+    a.x_next[o] = nf;
+    const double dg = WEIGHTED ? a.deg[r] : (double)(a.out_off[r + 1] - a.out_off[r]);
+    if (dg != 0.0) {
+        a.c_next[o] = (float)((double)nf / dg);
+    } else {
+        a.c_next[o] = 0.f;
+        ssum += (double)nf;
+    }
+}
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
+    if (a.st->h.done) return;
+    __shared__ float s_part[kBlock / kWave][kCols];
+    __shared__ double s_red[2][kBlock / kWave][kCols];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    const uint64_t active = a.st->active;
+    const bool col_on = (active >> lane) & 1ull;
+    const uint32_t seed = a.seeds[lane];
+    const double psink = a.st->p_sink[lane];
+    double dsum = 0.0, ssum = 0.0;
+
+    for (uint32_t ti = blockIdx.x; ti < a.ntiles; ti += gridDim.x) {
+        const Tile t = a.tiles[ti];
+        if (t.row_end == kSplit) {
+            // All four wavefronts share the chunk, entries strided by 4.
+            s_part[wid][lane] =
+                row_accumulate<WEIGHTED>(a, t.nnz_begin + wid, t.nnz_end, kBlock / kWave, lane);
+            __syncthreads();
+            if (wid == 0)
+                a.long_part[(size_t)t.row_begin * kCols + lane] =
+                    ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t r = t.row_begin + wid; r < t.row_end; r += kBlock / kWave) {
+            const float acc = row_accumulate<WEIGHTED>(a, a.in_off[r], a.in_off[r + 1], 1, lane);
+            finish_row<WEIGHTED>(a, r, acc, lane, col_on, seed, psink, dsum, ssum);
+        }
+    }
+
+    s_red[0][wid][lane] = dsum;
+    s_red[1][wid][lane] = ssum;
+    __syncthreads();
+    if (wid == 0) {
+        const double d = ((s_red[0][0][lane] + s_red[0][1][lane]) + s_red[0][2][lane]) +
+                         s_red[0][3][lane];
+        const double s = ((s_red[1][0][lane] + s_red[1][1][lane]) + s_red[1][2][lane]) +
+                         s_red[1][3][lane];
+        a.partials[((size_t)blockIdx.x) * kCols + lane] = d;
+        a.partials[((size_t)gridDim.x + blockIdx.x) * kCols + lane] = s;
+    }
+    if (!last_to_arrive(&a.st->h.counter, &flag)) return;
+
+    // Last workgroup: split rows, then the per-column folds in index order.
+    double d = 0.0, s = 0.0;
+    for (uint32_t b = wid; b < gridDim.x; b += kBlock / kWave) {
+        d += a.partials[(size_t)b * kCols + lane];
+        s += a.partials[((size_t)gridDim.x + b) * kCols + lane];
+    }
+    for (uint32_t k = wid; k < a.nlong; k += kBlock / kWave) {
+        float acc = 0.f;
+        for (uint32_t q = a.long_slot[k]; q < a.long_slot[k + 1]; ++q)
+            acc += a.long_part[(size_t)q * kCols + lane];
+        finish_row<WEIGHTED>(a, a.long_rows[k], acc, lane, col_on, seed, psink, d, s);
+    }
+    __syncthreads();
+    s_red[0][wid][lane] = d;
+    s_red[1][wid][lane] = s;
+    __syncthreads();
+    if (wid == 0) {
+        const double dt = ((s_red[0][0][lane] + s_red[0][1][lane]) + s_red[0][2][lane]) +
+                          s_red[0][3][lane];
+        const double stt = ((s_red[1][0][lane] + s_red[1][1][lane]) + s_red[1][2][lane]) +
+                           s_red[1][3][lane];
+        BatchState *st = a.st;
+        bool keep = false;
+        if (col_on) {
+            const uint64_t it = st->iters[lane] + 1;
+            st->iters[lane] = it;
+            st->delta[lane] = dt;
+            st->p_sink[lane] = stt;
+            keep = (dt >= a.eps) && !(a.max_iter > 0 && it >= a.max_iter);
+        }
+        const uint64_t mask = __ballot(keep);
+        double dmax = col_on ? dt : 0.0;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, kWave));
+        if (lane == 0) {
+            st->active = mask;
+            st->h.delta = dmax;
+            st->h.iter += 1;
+            st->h.launches += 1;
+            st->h.done = mask == 0 ? 1u : 0u;
+            st->h.counter = 0;
+        }
+        __threadfence();
+    }
+}
+
+// x0 = e_seed per column, c0 = x0 / deg; p_sink for the first sweep is 1
+// for a column whose seed is a sink.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void ppr_init(const BArgs a, uint32_t n, uint64_t active) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t seed = a.seeds[lane];
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves) {
+        const size_t o = (size_t)r * kCols + lane;
+        const float x = (r == seed) ? 1.f : 0.f;
+        a.x_next[o] = x;
+        const double dg = WEIGHTED ? a.deg[r] : (double)(a.out_off[r + 1] - a.out_off[r]);
+        a.c_next[o] = dg != 0.0 ? (float)(1.0 / dg) * x : 0.f;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kWave) {
+        BatchState *st = a.st;
+        double ps = 0.0;
+        if (seed != kNoSeed) {
+            const double dg =
+                WEIGHTED ? a.deg[seed] : (double)(a.out_off[seed + 1] - a.out_off[seed]);
+            ps = dg == 0.0 ? 1.0 : 0.0;
+        }
+        st->p_sink[lane] = ps;
+        st->delta[lane] = 0.0;
+        st->iters[lane] = 0;
+        if (lane == 0) {
+            st->active = active;
+            st->h = SweepState{};
+        }
+    }
+}
+
+// Copy column block [n][64] into the caller's [n][ldx] layout at column c0.
+__global__ __launch_bounds__(kBlock) void ppr_emit(const float *x, uint32_t n, uint32_t kb,
+                                                   uint64_t ldx, uint32_t c0, float *out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves)
+        if ((uint32_t)lane < kb) out[(size_t)r * ldx + c0 + lane] = x[(size_t)r * kCols + lane];
+}
+
+static unsigned row_waves_grid(uint32_t n) {
+    uint64_t b = ((uint64_t)n + 3) / 4;
+    if (b > 8192) b = 8192;
+    return b ? (unsigned)b : 1u;
+}
+
+gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                                const uint32_t *seeds, uint32_t k, double damping,
+                                const gt_power_iteration *params, float *x, uint64_t ldx,
+                                int32_t x_loc, uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (!params) return fail(GT_ERR_INVALID_ARG, "params is null");
+    if (params->plan.len != g->n)
+        return fail(GT_ERR_PLAN_COVER, "sweep: the plan must cover the output exactly");
+    if (k == 0) return GT_OK;
+    if (!seeds || !x || !iters) return fail(GT_ERR_INVALID_ARG, "null argument");
+    if (ldx < k) return fail(GT_ERR_INVALID_ARG, "ldx is smaller than k");
+    if (x_loc != GT_HOST && x_loc != GT_DEVICE) return fail(GT_ERR_INVALID_ARG, "bad location");
+    if (params->max_iter == 0 && !(params->epsilon > 0.0))
+        return fail(GT_ERR_INVALID_ARG,
+                    "epsilon must be positive when max_iter is 0: the loop would never end");
+    const uint32_t n = g->n;
+    for (uint32_t c = 0; c < k; ++c)
+        if (seeds[c] >= n) return fail(GT_ERR_NO_SUCH_VERTEX, "seed vertex out of range");
+    Orient *pull = nullptr, *push = nullptr;
+    bool reversed = false;
+    const uint8_t *active = nullptr;
+    uint32_t nactive = 0;
+    GT_TRY(resolve_view(g, view, &pull, &push, &reversed, &active, &nactive));
+    if (active) return fail(GT_ERR_UNSUPPORTED, "batched personalisation on a filtered view");
+    hipStream_t s = g->stream;
+    Workspace &ws = g->ws;
+    BArgs a{};
+    bool weighted = false;
+    if (weight && weight->kind != GT_MAP_UNITY) {
+        weighted = true;
+        const void *src = nullptr;
+        int dt = GT_F64;
+        GT_TRY(stage_edge_source(g, weight, &src, &dt));
+        GT_TRY(stage_degree(g, weight, *push, src, dt, ws.deg));
+        GT_TRY(stage_csc_weights<float>(g, weight, *pull, src, dt, ws.wcsc));
+        a.deg = ws.deg.as<double>();
+        a.w = ws.wcsc.as<float>();
+    }
+    GT_TRY(ensure_schedule_mm(*pull, n));
+    const Schedule &S = pull->mm;
+    static const int cap_w = occupancy_cap((const void *)ppr_sweep<true>);
+    static const int cap_u = occupancy_cap((const void *)ppr_sweep<false>);
+    const int grid_w = fit_grid(cap_w, S.ntiles), grid_u = fit_grid(cap_u, S.ntiles);
+    GT_TRY(prof_prepare(g));
+    const size_t vb = sizeof(float) * kCols * (size_t)n;
+    GT_TRY(ws.bx[0].reserve(vb));
+    GT_TRY(ws.bx[1].reserve(vb));
+    GT_TRY(ws.bc[0].reserve(vb));
+    GT_TRY(ws.bc[1].reserve(vb));
+    GT_TRY(ws.bpart.reserve(sizeof(double) * 2 * kCols * 2048));
+    GT_TRY(ws.blong.reserve(sizeof(float) * kCols * std::max<uint32_t>(1, S.nslots)));
+    GT_TRY(ws.bstate.reserve(sizeof(BatchState) + sizeof(uint32_t) * kCols));
+    BatchState *st = ws.bstate.as<BatchState>();
+    uint32_t *dseeds = reinterpret_cast<uint32_t *>(st + 1);
+    float *xb[2] = {ws.bx[0].as<float>(), ws.bx[1].as<float>()};
+    float *cb[2] = {ws.bc[0].as<float>(), ws.bc[1].as<float>()};
+    DevBuf hostx;
+    float *xout = x;
+    if (x_loc == GT_HOST) {
+        GT_TRY(hostx.reserve(sizeof(float) * ldx * (size_t)n));
+        xout = hostx.as<float>();
+    }
+    a.tiles = S.tiles.as<Tile>();
+    a.ntiles = S.ntiles;
+    a.nlong = S.nlong;
+    a.in_off = pull->off.as<uint32_t>();
+    a.in_idx = pull->idx.as<uint32_t>();
+    a.out_off = push->off.as<uint32_t>();
+    a.seeds = dseeds;
+    a.damping = damping;
+    a.eps = params->epsilon;
+    a.max_iter = params->max_iter;
+    a.partials = ws.bpart.as<double>();
+    a.long_part = ws.blong.as<float>();
+    a.long_rows = S.long_rows.as<uint32_t>();
+    a.long_slot = S.long_slot.as<uint32_t>();
+    a.st = st;
+    g->prof_kind = GT_SWEEP_OTHER;
+    uint64_t total = 0;
+    for (uint32_t c0 = 0; c0 < k; c0 += kCols) {
+        const uint32_t kb = std::min<uint32_t>(kCols, k - c0);
+        uint32_t hs[kCols];
+        for (uint32_t c = 0; c < (uint32_t)kCols; ++c) hs[c] = c < kb ? seeds[c0 + c] : kNoSeed;
+        GT_HIP(hipMemcpyAsync(dseeds, hs, sizeof(hs), hipMemcpyHostToDevice, s));
+        const uint64_t mask = kb == 64 ? ~0ull : ((1ull << kb) - 1);
+        BArgs ai = a;
+        ai.x_next = xb[0];
+        ai.c_next = cb[0];
+        if (weighted)
+            hipLaunchKernelGGL(ppr_init<true>, dim3(row_waves_grid(n)), dim3(kBlock), 0, s, ai, n,
+                               mask);
+        else
+            hipLaunchKernelGGL(ppr_init<false>, dim3(row_waves_grid(n)), dim3(kBlock), 0, s, ai,
+                               n, mask);
+        GT_HIP(hipGetLastError());
+        auto launch = [&](uint64_t q) {
+            const int par = (int)(q & 1);
+            BArgs b = a;
+            b.x_cur = xb[par];
+            b.x_next = xb[par ^ 1];
+            b.c_cur = cb[par];
+            b.c_next = cb[par ^ 1];
+            if (weighted)
+                hipLaunchKernelGGL(ppr_sweep<true>, dim3(grid_w), dim3(kBlock), 0, s, b);
+            else
+                hipLaunchKernelGGL(ppr_sweep<false>, dim3(grid_u), dim3(kBlock), 0, s, b);
+        };
+        uint64_t launched = 0;
+        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), params->epsilon, params->max_iter,
+                     launch, &launched));
+        const uint64_t sweeps = g->h_state->iter;
+        total += sweeps;
+        uint64_t hit[kCols];
+        GT_HIP(hipMemcpyAsync(hit, st->iters, sizeof(hit), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(ppr_emit, dim3(row_waves_grid(n)), dim3(kBlock), 0, s, xb[sweeps & 1],
+                           n, kb, ldx, c0, xout);
+        GT_HIP(hipGetLastError());
+        GT_HIP(hipStreamSynchronize(s));
+        for (uint32_t c = 0; c < kb; ++c) iters[c0 + c] = hit[c];
+    }
+    if (x_loc == GT_HOST)
+        GT_HIP(hipMemcpyAsync(x, xout, sizeof(float) * ldx * (size_t)n, hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    g->last_sweeps = total;
+    return GT_OK;
+}
+
+}  // namespace gt
+
+extern "C" gt_status gt_pagerank_batched(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                                         const uint32_t *seeds, uint32_t k, double damping,
+                                         const gt_power_iteration *params, float *x,
+                                         uint64_t ldx, int32_t x_loc, uint64_t *iters) {
+    return gt::pagerank_batched_impl(g, view, weight, seeds, k, damping, params, x, ldx, x_loc,
+                                     iters);
+}

@@ -84,14 +84,14 @@ struct EvFin {
 template <bool WEIGHTED, bool MASK>
 __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     if (a.st->done) return;
-    __shared__ double lds[kTileNnz];
-    __shared__ double red[8];
+    __shared__ TileLds<double> L;
     __shared__ uint32_t flag;
+    double *red = L.red;
     const int tid = threadIdx.x;
     const bool first = a.st->launches == 0;
     EvFin<MASK> fin(a, a.st->sc_cur, a.st->sc_prev, first);
     spmv_tiles<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, a.y_cur, a.w, a.long_part,
-                                 lds, red, fin);
+                                 L, fin);
     double sq = fin.sq, dl = fin.dl;
     block_sum2(sq, dl, red);
     if (tid == 0) {
@@ -173,6 +173,12 @@ __global__ __launch_bounds__(kBlock) void ev_emit(const double *y, const uint8_t
         const double v = y[i];
         x[i] = (MASK && !active[i]) ? v : v / sc;
     }
+}
+
+template <bool W, bool M>
+static int ev_grid(uint32_t ntiles) {
+    static const int cap = occupancy_cap((const void *)ev_sweep<W, M>);
+    return fit_grid(cap, ntiles);
 }
 
 static unsigned vgrid(uint32_t n) {
@@ -268,15 +274,19 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            const dim3 gd(g->grid), bd(kBlock);
+            const dim3 bd(kBlock);
             if (weighted && masked)
-                hipLaunchKernelGGL((ev_sweep<true, true>), gd, bd, 0, s, b);
+                hipLaunchKernelGGL((ev_sweep<true, true>), dim3(ev_grid<true, true>(b.ntiles)), bd,
+                                   0, s, b);
             else if (weighted)
-                hipLaunchKernelGGL((ev_sweep<true, false>), gd, bd, 0, s, b);
+                hipLaunchKernelGGL((ev_sweep<true, false>), dim3(ev_grid<true, false>(b.ntiles)),
+                                   bd, 0, s, b);
             else if (masked)
-                hipLaunchKernelGGL((ev_sweep<false, true>), gd, bd, 0, s, b);
+                hipLaunchKernelGGL((ev_sweep<false, true>), dim3(ev_grid<false, true>(b.ntiles)),
+                                   bd, 0, s, b);
             else
-                hipLaunchKernelGGL((ev_sweep<false, false>), gd, bd, 0, s, b);
+                hipLaunchKernelGGL((ev_sweep<false, false>),
+                                   dim3(ev_grid<false, false>(b.ntiles)), bd, 0, s, b);
         };
         // The driver reads done / delta / iter through the SweepState layout;
         // EvState keeps delta, iter and done at the same offsets.

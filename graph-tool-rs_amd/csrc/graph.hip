@@ -39,6 +39,22 @@ gt_status DevBuf::reserve(size_t b) {
     return GT_OK;
 }
 
+int occupancy_cap(const void *kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 2048;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess ||
+        cus <= 0 || per <= 0)
+        return 2048;
+    const int g = per * cus;
+    return g < 2048 ? g : 2048;
+}
+
+int fit_grid(int cap, uint32_t units) {
+    const int g = (uint32_t)cap < units ? cap : (int)units;
+    return g > 0 ? g : 1;
+}
+
 int sweep_grid(uint32_t ntiles) {
     // 256 CUs x 8 resident workgroups keeps every CU busy; fewer tiles than
     // that means one tile per workgroup.
@@ -123,6 +139,54 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
                      hipMemcpyHostToDevice));
     o.scheduled = true;
     (void)g;
+    return GT_OK;
+}
+
+gt_status ensure_schedule_mm(Orient &o, uint32_t n) {
+    Schedule &S = o.mm;
+    if (S.ready) return GT_OK;
+    const std::vector<uint32_t> &off = o.h_off;
+    std::vector<Tile> tiles;
+    std::vector<uint32_t> long_rows, long_slot;
+    uint32_t nslots = 0, r = 0;
+    while (r < n) {
+        const uint32_t len = off[r + 1] - off[r];
+        if (len > kMmTileNnz) {
+            long_rows.push_back(r);
+            long_slot.push_back(nslots);
+            for (uint32_t b = off[r]; b < off[r + 1]; b += kMmTileNnz) {
+                const uint32_t e = std::min<uint64_t>((uint64_t)b + kMmTileNnz, off[r + 1]);
+                tiles.push_back(Tile{nslots++, kSplit, b, e});
+            }
+            ++r;
+            continue;
+        }
+        const uint32_t r0 = r;
+        uint32_t nnz = 0;
+        while (r < n && r - r0 < kMmTileRows) {
+            const uint32_t l = off[r + 1] - off[r];
+            if (l > kMmTileNnz || nnz + l > kMmTileNnz) break;
+            nnz += l;
+            ++r;
+        }
+        tiles.push_back(Tile{r0, r, off[r0], off[r]});
+    }
+    long_slot.push_back(nslots);
+    S.ntiles = (uint32_t)tiles.size();
+    S.nlong = (uint32_t)long_rows.size();
+    S.nslots = nslots;
+    GT_TRY(S.tiles.reserve(sizeof(Tile) * std::max<size_t>(1, tiles.size())));
+    GT_TRY(S.long_rows.reserve(sizeof(uint32_t) * std::max<size_t>(1, long_rows.size())));
+    GT_TRY(S.long_slot.reserve(sizeof(uint32_t) * long_slot.size()));
+    if (!tiles.empty())
+        GT_HIP(hipMemcpy(S.tiles.ptr, tiles.data(), sizeof(Tile) * tiles.size(),
+                         hipMemcpyHostToDevice));
+    if (!long_rows.empty())
+        GT_HIP(hipMemcpy(S.long_rows.ptr, long_rows.data(), sizeof(uint32_t) * long_rows.size(),
+                         hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(S.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
+                     hipMemcpyHostToDevice));
+    S.ready = true;
     return GT_OK;
 }
 

@@ -76,9 +76,23 @@ constexpr uint32_t kSplit = 0xffffffffu;
 
 // Rows longer than kTileNnz are split into chunks of kTileNnz entries.
 constexpr uint32_t kTileNnz = 2048;
-// At most this many rows per tile (4 per thread of a 256-thread workgroup).
-constexpr uint32_t kTileRows = 1024;
+// At most this many rows per tile (2 per thread of a 256-thread workgroup).
+constexpr uint32_t kTileRows = 512;
 constexpr int kBlock = 256;
+
+// A tiling of an orientation's rows for the pull kernels.
+struct Schedule {
+    DevBuf tiles;            // Tile[ntiles]
+    DevBuf long_rows;        // uint32[nlong]: row of each split row
+    DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
+    uint32_t ntiles = 0, nlong = 0, nslots = 0;
+    bool ready = false;
+};
+
+// SpMM tiles: each in-entry gathers a 64-column row (256 bytes), so tiles are
+// smaller and long rows split sooner.
+constexpr uint32_t kMmTileNnz = 1024;
+constexpr uint32_t kMmTileRows = 256;
 
 // One orientation of the adjacency: row r lists the entries a kernel pulls
 // from (in-halves for the CSC, out-halves for the CSR), in storage order.
@@ -93,6 +107,7 @@ struct Orient {
     DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
     bool scheduled = false;
+    Schedule mm;             // SpMM schedule (batched personalisation)
     // Rows too long for one thread's sequential degree sum.
     DevBuf heavy;
     uint32_t nheavy = 0;
@@ -115,6 +130,15 @@ struct Workspace {
     DevBuf long_part;   // f64[nslots]
     DevBuf state;       // device SweepState
     DevBuf pers;        // staged personalisation
+    DevBuf lam[2];      // adjoint iterates
+    DevBuf ax, ac;      // adjoint: staged x* and dL/dx (f64)
+    DevBuf wcsr;        // adjoint: weights in push order
+    DevBuf gw, gp;      // adjoint: staged gradients for host outputs
+    DevBuf bx[2];       // batched: [n][64] f32 iterates
+    DevBuf bc[2];       // batched: [n][64] f32 contributions
+    DevBuf bpart;       // batched: per-workgroup, per-column partials
+    DevBuf blong;       // batched: split-row [64] partials
+    DevBuf bstate;      // batched: loop state + seeds
     DevBuf masks;       // staged vertex + edge masks
     Orient fpull;       // filtered view: compacted pull orientation
     Orient fpush;       // filtered view: compacted push orientation
@@ -164,10 +188,18 @@ namespace gt {
 
 // Build (once) the SpMV schedule of an orientation from its host offsets.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n);
+// Build (once) the SpMM schedule of an orientation.
+gt_status ensure_schedule_mm(Orient &o, uint32_t n);
 // Allocate a graph shell with its stream.
 gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
                     gt_graph **out);
 // Workgroups per sweep launch for a schedule of `ntiles` tiles.
 int sweep_grid(uint32_t ntiles);
+// Workgroups of `kernel` (256 threads) that are resident at once on the
+// device, capped at 2048: a persistent tile loop launched with this many
+// workgroups has no second wave of workgroups.
+int occupancy_cap(const void *kernel);
+// min(cap, units), at least 1.
+int fit_grid(int cap, uint32_t units);
 
 }  // namespace gt

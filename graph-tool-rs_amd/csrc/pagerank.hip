@@ -90,13 +90,13 @@ struct PrFin {
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
 __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     if (a.st->done) return;
-    __shared__ V lds[kTileNnz];
-    __shared__ double red[8];
+    __shared__ TileLds<V> L;
     __shared__ uint32_t flag;
+    double *red = L.red;
     const int tid = threadIdx.x;
     PrFin<V, WEIGHTED, DPERS, MASK> fin(a, a.st->p_sink);
     spmv_tiles<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, static_cast<const V *>(a.c_cur),
-                            static_cast<const V *>(a.w), a.long_part, lds, red, fin);
+                            static_cast<const V *>(a.w), a.long_part, L, fin);
 
     double dsum = fin.dsum, ssum = fin.ssum;
     block_sum2(dsum, ssum, red);
@@ -208,14 +208,16 @@ struct PrPlan {
 };
 
 template <typename V, bool W, bool P, bool M>
-void launch_sweep(const PrArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(grid), dim3(kBlock), 0, s, a);
+void launch_sweep(const PrArgs &a, int /*grid*/, hipStream_t s) {
+    static const int cap = occupancy_cap((const void *)pr_sweep<V, W, P, M>);
+    hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(fit_grid(cap, a.ntiles)), dim3(kBlock), 0, s,
+                       a);
 }
 template <typename V, bool W, bool P, bool M>
 void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64_t iter0,
                  hipStream_t s) {
     uint32_t blocks = (n + kBlock - 1) / kBlock;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 2048) blocks = 2048;  // partials hold 2 x 2048 slots
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL((pr_init<V, W, P, M>), dim3(blocks), dim3(kBlock), 0, s, a, rank, had, n,
                        iter0);

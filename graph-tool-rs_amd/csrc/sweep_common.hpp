@@ -1,15 +1,23 @@
-// The pull-SpMV tile engine shared by the PageRank and eigenvector sweeps.
+// The pull-SpMV tile engine shared by the PageRank, eigenvector and adjoint
+// sweeps.
 //
 // A workgroup of 256 threads walks tiles blockIdx.x, blockIdx.x + gridDim.x,
 // ...  For each tile:
 //   phase 1  the tile's <= kTileNnz in-entries are streamed (coalesced u32
 //            index loads, eight per thread in flight) and the gathered source
-//            values -- times the CSC-ordered weight when weighted -- land in LDS;
-//   phase 2  rows are summed out of LDS: one thread per row when the tile's
-//            rows average fewer than 16 entries, one wavefront per row with a
-//            shuffle-tree reduction otherwise.  A split-row chunk is reduced by
-//            the whole workgroup into its partial slot.
-// `fin(row, acc)` is called exactly once per completed row, by one thread.
+//            values -- times the pull-ordered weight when weighted -- land in
+//            LDS;
+//   phase 2  row sums out of LDS into an LDS row buffer: one thread per row
+//            when the tile's rows average fewer than 16 entries, one
+//            wavefront per row with a shuffle-tree reduction otherwise; a
+//            split-row chunk is reduced by the whole workgroup into its
+//            partial slot instead;
+//   phase 3  `fin(row, sum)` for every row of the tile, one thread per row,
+//            so the rows' global reads and writes go out together and
+//            coalesced.
+// Two barriers per tile: phase 1 of the next tile may overwrite the LDS tile
+// once phase 2 is done, and phase 2 of the next tile may overwrite the row
+// buffer once the barrier after its phase 1 has been passed.
 #pragma once
 
 #include "device_common.hpp"
@@ -46,12 +54,20 @@ __device__ __forceinline__ void gather_tile(const uint32_t *__restrict__ idx,
     }
 }
 
+// LDS a kernel using spmv_tiles must provide.
+template <typename V>
+struct TileLds {
+    V vals[kTileNnz];
+    double rowsum[kTileRows];
+    double red[8];
+};
+
 template <typename V, bool WEIGHTED, typename Fin>
 __device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ idx,
                                            const V *__restrict__ src, const V *__restrict__ w,
-                                           double *__restrict__ long_part, V *lds, double *red,
+                                           double *__restrict__ long_part, TileLds<V> &L,
                                            Fin &fin) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -59,34 +75,36 @@ __device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint3
     for (uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile t = tiles[ti];
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
-        gather_tile<V, WEIGHTED>(idx, src, w, nb, cnt, lds);
+        gather_tile<V, WEIGHTED>(idx, src, w, nb, cnt, L.vals);
         __syncthreads();
         if (t.row_end == kSplit) {
             double v = 0.0;
-            for (uint32_t k = tid; k < cnt; k += kBlock) v += static_cast<double>(lds[k]);
-            v = block_sum(v, red);  // ends with a barrier: lds is free again
+            for (uint32_t k = tid; k < cnt; k += kBlock) v += static_cast<double>(L.vals[k]);
+            v = block_sum(v, L.red);  // ends with a barrier: LDS is free again
             if (tid == 0) long_part[t.row_begin] = v;
             continue;
         }
-        const uint32_t rows = t.row_end - t.row_begin;
+        const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
         if (cnt < 16u * rows) {
-            for (uint32_t r = t.row_begin + tid; r < t.row_end; r += kBlock) {
-                const uint32_t lo = off[r] - nb, hi = off[r + 1] - nb;
+            for (uint32_t q = tid; q < rows; q += kBlock) {
+                const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
                 V acc = V(0);
-                for (uint32_t k = lo; k < hi; ++k) acc += lds[k];
-                fin(r, static_cast<double>(acc));
+                for (uint32_t k = lo; k < hi; ++k) acc += L.vals[k];
+                L.rowsum[q] = static_cast<double>(acc);
             }
         } else {
-            for (uint32_t r = t.row_begin + wid; r < t.row_end; r += kBlock / kWave) {
-                const uint32_t lo = off[r] - nb, hi = off[r + 1] - nb;
+            for (uint32_t q = wid; q < rows; q += kBlock / kWave) {
+                const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
                 V acc = V(0);
-                for (uint32_t k = lo + lane; k < hi; k += kWave) acc += lds[k];
+                for (uint32_t k = lo + lane; k < hi; k += kWave) acc += L.vals[k];
                 acc = wave_sum(acc);
-                if (lane == 0) fin(r, static_cast<double>(acc));
+                if (lane == 0) L.rowsum[q] = static_cast<double>(acc);
             }
         }
         __syncthreads();
+        for (uint32_t q = tid; q < rows; q += kBlock) fin(rb + q, L.rowsum[q]);
     }
+    __syncthreads();
 }
 
 // Sum of a split row's partial slots, in chunk order.

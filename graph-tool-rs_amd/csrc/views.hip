@@ -87,6 +87,7 @@ static gt_status compact(gt_graph *g, const Orient &base, Orient &out, const uin
                        out.eid.as<uint32_t>());
     GT_HIP(hipGetLastError());
     out.scheduled = false;
+    out.mm.ready = false;
     out.heavy_ready = false;
     return GT_OK;
 }

@@ -542,6 +542,73 @@ def _pagerank(fn, g, weight, pers, damping, params, rank) -> int:
     return int(iters.value)
 
 
+def pagerank_batched(g, weight, seeds, damping: float, params: PowerIteration, out=None):
+    """k personalised PageRank vectors in one SpMM per sweep (extension).
+
+    Column c equals ``pagerank_with`` with the one-hot personalisation on
+    ``seeds[c]``, started from it and stopped on its own L1 change.  Returns
+    ``(X, iters)``: X is [n, k] float32 (a CUDA tensor if ``out`` is one, else
+    numpy), iters the per-column sweep counts."""
+    dev, view, gid, bound, nv = _resolve(g)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    k = int(seeds.shape[0])
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, gid, keep)
+    iters = np.zeros(max(k, 1), np.uint64)
+    if out is not None and _is_torch_cuda(out):
+        import torch
+        if out.dtype != torch.float32 or out.dim() != 2 or out.shape[0] != bound \
+                or out.shape[1] < k or out.stride(1) != 1:
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "out must be [n, >=k] float32, row-major")
+        _bind_stream(dev, out)
+        ptr, ldx, loc, X = C.c_void_p(out.data_ptr()), out.stride(0), GT_DEVICE, out
+    else:
+        X = np.zeros((bound, max(k, 1)), np.float32)
+        dev.set_stream(None)
+        ptr, ldx, loc = X.ctypes.data_as(C.c_void_p), X.shape[1], GT_HOST
+    call("gt_pagerank_batched", dev.handle, C.byref(view), C.byref(wp), _ptr(seeds), k,
+         float(damping), C.byref(params._c()), ptr, ldx, loc, _ptr(iters, C.POINTER(C.c_uint64)))
+    return X, iters[:k].astype(np.int64)
+
+
+def pagerank_grad(g, weight, pers, damping: float, params: PowerIteration, x, dl_dx,
+                  precision: str = "f64", grad_w_dtype=None):
+    """Gradient of a loss through the PageRank fixed point (extension).
+
+    ``x`` is the forward solution, ``dl_dx`` = dL/dx.  Returns ``(grad_w,
+    grad_pers, grad_damping, sweeps)``: grad_w indexed by edge id, grad_pers by
+    vertex.  CUDA tensors in, CUDA tensors out; otherwise numpy."""
+    dev, view, gid, bound, nv = _resolve(g)
+    _, _, eb, _ = dev.info()
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, gid, keep)
+    pp = _prop(pers, gid, keep)
+    xp = _prop(x, gid, keep)
+    cp = _prop(dl_dx, gid, keep)
+    prec = {"f64": GT_F64, "f32": GT_F32}[precision]
+    on_dev = _is_torch_cuda(x.data if isinstance(x, DenseProp) else x)
+    if on_dev:
+        import torch
+        _bind_stream(dev, x)
+        tdt = torch.float64 if (grad_w_dtype or "f64") == "f64" else torch.float32
+        gw = torch.empty(max(eb, 1), dtype=tdt, device="cuda")
+        gpv = torch.empty(max(bound, 1), dtype=torch.float64, device="cuda")
+        gw_ptr, gp_ptr, loc = C.c_void_p(gw.data_ptr()), C.c_void_p(gpv.data_ptr()), GT_DEVICE
+    else:
+        dev.set_stream(None)
+        ndt = np.float64 if (grad_w_dtype or "f64") == "f64" else np.float32
+        gw = np.zeros(max(eb, 1), ndt)
+        gpv = np.zeros(max(bound, 1), np.float64)
+        gw_ptr, gp_ptr, loc = gw.ctypes.data_as(C.c_void_p), gpv.ctypes.data_as(C.c_void_p), GT_HOST
+    wdt = GT_F64 if (grad_w_dtype or "f64") == "f64" else GT_F32
+    gd = C.c_double()
+    it = C.c_uint64()
+    call("gt_pagerank_backward", dev.handle, C.byref(view), C.byref(wp),
+         C.byref(pp) if pp else None, float(damping), C.byref(params._c()), C.byref(xp),
+         C.byref(cp), prec, gw_ptr, wdt, loc, gp_ptr, C.byref(gd), C.byref(it))
+    return gw[:eb], gpv[:bound], float(gd.value), int(it.value)
+
+
 def eigenvector(g, weight, params: PowerIteration, x: DenseProp) -> Tuple[float, int]:
     """Eigenvector centrality (centrality.rs:405-497): ``(eigenvalue, sweeps)``."""
     dev, view, gid, bound, nv = _resolve(g)

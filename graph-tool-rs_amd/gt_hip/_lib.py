@@ -77,6 +77,13 @@ _PROTOS = {
     "gt_pagerank_mixed": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
                           C.c_double, C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
                           C.POINTER(C.c_uint64)],
+    "gt_pagerank_batched": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), u32p, C.c_uint32,
+                            C.c_double, C.POINTER(GtPowerIteration), C.c_void_p, C.c_uint64,
+                            C.c_int32, C.POINTER(C.c_uint64)],
+    "gt_pagerank_backward": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
+                             C.c_double, C.POINTER(GtPowerIteration), C.POINTER(GtProp),
+                             C.POINTER(GtProp), C.c_int32, C.c_void_p, C.c_int32, C.c_int32,
+                             C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
     "gt_eigenvector_f64": [GraphP, C.POINTER(GtView), C.POINTER(GtProp),
                            C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
                            C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
@@ -103,6 +110,13 @@ def lib() -> C.CDLL:
             raise RuntimeError(
                 f"{_LIB_PATH} is missing: build it with __graft_entry__.build() "
                 "(the HIP engine has no CPU fallback)")
+        # torch ships its own HIP runtime under the same soname; loading it
+        # first makes this library bind to it, so one runtime serves both
+        # (two runtimes in one process leave the second unable to see the GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(_LIB_PATH))
         for name, args in _PROTOS.items():
             f = getattr(L, name)

@@ -180,6 +180,37 @@ gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view,
                             double damping, const gt_power_iteration *params,
                             gt_vertex_out *rank, uint64_t *iters);
 
+/* Extension.  k personalised PageRank vectors solved together as one SpMM per
+ * sweep: column c is pagerank_with with the one-hot personalisation
+ * e_{seeds[c]}, started from it, stopped on its own L1 change (< epsilon) and
+ * frozen after, so iters[c] is the sweep count of that single-vector run.
+ * FP32 state and SpMM accumulation, FP64 teleport update, per-column dangling
+ * mass and L1 test.  x is [n][ldx] row-major FP32 in x_loc memory; columns
+ * [0, k) are written.  Directed and reversed views. */
+gt_status gt_pagerank_batched(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                              const uint32_t *seeds, uint32_t k, double damping,
+                              const gt_power_iteration *params, float *x, uint64_t ldx,
+                              int32_t x_loc, uint64_t *iters);
+
+/* Extension.  Gradient of a loss through the PageRank fixed point by implicit
+ * differentiation: solves lambda = dl_dx + d M^T lambda + d (pers . lambda) q
+ * (M[t,s] = w_e / deg_s, q = sink indicator) with the same power iteration on
+ * the opposite orientation, stores no forward iterate, then
+ *   grad_w[e]     = d * x_s / deg_s * (lambda_t - (M^T lambda)_s)   (0 if deg_s = 0)
+ *   grad_pers[v]  = ((1 - d) + d * sum_{sinks} x) * lambda_v
+ *   grad_damping  = lambda . (x - pers) / d.
+ * x is the forward solution and dl_dx = dL/dx, both DENSE vertex maps.
+ * precision GT_F64 or GT_F32 (FP32 values with FP64 L1 test and sums).
+ * grad_w: [edge_bound] of grad_w_dtype, grad_pers: [n] f64, both in grad_loc
+ * memory and optional (NULL); grad_damping is a host scalar (optional).
+ * Directed and reversed views. */
+gt_status gt_pagerank_backward(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                               const gt_prop *pers, double damping,
+                               const gt_power_iteration *params, const gt_prop *x,
+                               const gt_prop *dl_dx, int32_t precision, void *grad_w,
+                               int32_t grad_w_dtype, int32_t grad_loc, double *grad_pers,
+                               double *grad_damping, uint64_t *iters);
+
 /* Eigenvector centrality, FP64: the drop-in for eigenvector.  *eig is the
  * last norm (the eigenvalue estimate), *iters the sweep count. */
 gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,

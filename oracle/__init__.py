@@ -57,6 +57,10 @@ def lib():
                                          _f64p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
                                          _f64p, C.c_uint64, C.c_int, _f64p]
         L.oracle_eigenvector.restype = C.c_int64
+        L.oracle_pagerank_grad.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _f64p,
+                                           C.c_int, C.c_double, _f64p, C.c_double, _f64p, _f64p,
+                                           C.c_double, C.c_uint64, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_pagerank_grad.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -173,6 +177,28 @@ class Adj:
         if it < 0:
             raise ValueError("sweep: the plan must cover the output exactly")
         return r, int(it)
+
+    def pagerank_grad(self, x, cvec, weight=None, pers=None, damping=0.85, epsilon=1e-14,
+                      max_iter=0, reversed: bool = False):
+        """Gradient of L = cvec . x through the PageRank fixed point x.
+        Returns (grad_w [edge_bound], grad_pers [n], grad_damping, lambda, sweeps)."""
+        n = self.n
+        wk, wv, wa = _map(weight)
+        if pers is None:
+            pk, pv, pa = CONSTANT, (0.0 if n == 0 else 1.0 / n), None
+        else:
+            pk, pv, pa = _map(pers)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        c = np.ascontiguousarray(cvec, dtype=np.float64)
+        lam = np.zeros(max(n, 1))
+        gw = np.zeros(max(self.m, 1))
+        gp = np.zeros(max(n, 1))
+        gd = np.zeros(1)
+        it = lib().oracle_pagerank_grad(self._h, int(reversed), wk, wv, _p(wa, _f64p), pk, pv,
+                                        _p(pa, _f64p), damping, _p(x, _f64p), _p(c, _f64p),
+                                        epsilon, max_iter, _p(lam, _f64p), _p(gw, _f64p),
+                                        _p(gp, _f64p), _p(gd, _f64p))
+        return gw[: self.m], gp[:n], float(gd[0]), lam[:n], int(it)
 
     def eigenvector(self, weight=None, epsilon=1e-6, max_iter=0, grain=64,
                     x: Optional[np.ndarray] = None, reversed: bool = False, vmask=None,

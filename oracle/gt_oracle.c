@@ -29,9 +29,10 @@
  * reference's own known answers (u16_centrality.rs, the LAYOUT fixture of
  * u34_diff_adjacency.rs), against closed forms (directed cycles, the scipy
  * linear solve of the PageRank fixed point, dense eigenpairs) and against brute
- * force (a dense n x n power iteration).  oracle_pagerank_adjoint is pinned by
- * central finite differences of oracle_pagerank and by the row-scale
- * invariance identity sum_v w_uv dL/dw_uv = 0.
+ * force (a dense n x n power iteration).  oracle_pagerank_grad (a build
+ * extension: implicit differentiation of the fixed point) is pinned by central
+ * finite differences of oracle_pagerank and by the row-scale identity
+ * sum_{e in out(s)} w_e dL/dw_e = 0.
  */
 #include <math.h>
 #include <pthread.h>
@@ -548,5 +549,90 @@ GT_EXPORT int64_t oracle_eigenvector(const gt_adj *g, int reversed,
     free(scratch);
     free(active);
     *eig = norm;
+    return (int64_t)iter;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the PageRank fixed point (build extension; no reference       */
+/* counterpart).  With M[t,s] = w_e / deg_s over edges e: s -> t, sink      */
+/* indicator q (deg == 0) and x = (1-d) p + d (M x + (q.x) p), the gradient */
+/* of L = c.x through the fixed point is lambda^T dF, where                 */
+/*   lambda = c + d M^T lambda + d (p.lambda) q                             */
+/* (the transposed system, solved by the same power iteration), and        */
+/*   dL/dw_e = d * x_s / deg_s * (lambda_t - mu_s),  mu = M^T lambda        */
+/*   dL/dp   = ((1-d) + d (q.x)) lambda                                     */
+/*   dL/dd   = lambda . (x - p) / d                                         */
+/* An edge whose source has zero weighted out-degree gets gradient 0.       */
+/* Pinned by central finite differences of oracle_pagerank and by the       */
+/* row-scale identity sum_{e in out(s)} w_e dL/dw_e = 0.                    */
+/* ------------------------------------------------------------------------ */
+
+GT_EXPORT int64_t oracle_pagerank_grad(const gt_adj *g, int reversed, int wkind, double wval,
+                                       const double *w, int pkind, double pval,
+                                       const double *pers, double damping, const double *x,
+                                       const double *cvec, double epsilon, uint64_t max_iter,
+                                       double *lambda, double *grad_w, double *grad_p,
+                                       double *grad_d) {
+    gt_view view = {g, reversed, NULL, NULL};
+    const uint32_t n = g->n;
+    const double d = damping;
+    gt_map wm = {wkind, wval, w};
+    gt_map pm = {pkind, pval, pers};
+    double *deg = (double *)calloc(n ? n : 1, sizeof(double));
+    double *nl = (double *)calloc(n ? n : 1, sizeof(double));
+    double *mu = (double *)calloc(n ? n : 1, sizeof(double));
+    for (uint32_t v = 0; v < n; ++v) {
+        uint32_t cnt;
+        const gt_entry *out = view_out(&view, v, &cnt);
+        double k = 0.0;
+        for (uint32_t j = 0; j < cnt; ++j) k += map_get(&wm, out[j].idx);
+        deg[v] = wkind == MAP_UNITY ? (double)cnt : k;
+    }
+    for (uint32_t v = 0; v < n; ++v) lambda[v] = cvec[v];
+    double delta = epsilon + 1.0;
+    uint64_t iter = 0;
+    while (delta >= epsilon) {
+        double pl = 0.0;
+        for (uint32_t v = 0; v < n; ++v) pl += map_get(&pm, v) * lambda[v];
+        delta = 0.0;
+        for (uint32_t s = 0; s < n; ++s) {
+            uint32_t cnt;
+            const gt_entry *out = view_out(&view, s, &cnt);
+            double acc = 0.0;
+            for (uint32_t j = 0; j < cnt; ++j) acc += map_get(&wm, out[j].idx) * lambda[out[j].other];
+            double m = deg[s] != 0.0 ? acc / deg[s] : 0.0;
+            double v = cvec[s] + d * m + (deg[s] == 0.0 ? d * pl : 0.0);
+            delta += fabs(v - lambda[s]);
+            nl[s] = v;
+        }
+        memcpy(lambda, nl, sizeof(double) * n);
+        iter += 1;
+        if (max_iter > 0 && iter == max_iter) break;
+    }
+    double p_sink = 0.0, gd = 0.0;
+    for (uint32_t s = 0; s < n; ++s) {
+        uint32_t cnt;
+        const gt_entry *out = view_out(&view, s, &cnt);
+        double acc = 0.0;
+        for (uint32_t j = 0; j < cnt; ++j) acc += map_get(&wm, out[j].idx) * lambda[out[j].other];
+        mu[s] = deg[s] != 0.0 ? acc / deg[s] : 0.0;
+        if (deg[s] == 0.0) p_sink += x[s];
+    }
+    for (uint64_t e = 0; e < g->edge_bound; ++e) grad_w[e] = 0.0;
+    for (uint32_t s = 0; s < n; ++s) {
+        if (deg[s] == 0.0) continue;
+        uint32_t cnt;
+        const gt_entry *out = view_out(&view, s, &cnt);
+        const double cs = d * x[s] / deg[s];
+        for (uint32_t j = 0; j < cnt; ++j) grad_w[out[j].idx] = cs * (lambda[out[j].other] - mu[s]);
+    }
+    for (uint32_t v = 0; v < n; ++v) {
+        grad_p[v] = ((1.0 - d) + d * p_sink) * lambda[v];
+        gd += lambda[v] * (x[v] - map_get(&pm, v));
+    }
+    *grad_d = gd / d;
+    free(deg);
+    free(nl);
+    free(mu);
     return (int64_t)iter;
 }
