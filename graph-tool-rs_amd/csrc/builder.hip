@@ -227,9 +227,10 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const uint32_t *__restri
     }
 }
 
-// Edge ids sorted stably by key[id] (bits below `bits` significant).
-static gt_status stable_sort_ids(const uint32_t *key, uint64_t m, int bits, DevBuf &ids_out,
-                                 hipStream_t s, std::deque<DevBuf> &tmp) {
+// Items 0..m-1 sorted stably by key[i] (bits below `bits` significant);
+// ids_out receives vals[i] (or i when vals is null) in sorted order.
+gt_status stable_sort_ids(const uint32_t *key, const uint32_t *vals, uint64_t m, int bits,
+                          DevBuf &ids_out, hipStream_t s, std::deque<DevBuf> &tmp) {
     const uint32_t ntiles = (uint32_t)((m + kScanTile - 1) / kScanTile);
     DevBuf kA, kB, vA;
     GT_TRY(kA.reserve(sizeof(uint32_t) * m));
@@ -241,7 +242,7 @@ static gt_status stable_sort_ids(const uint32_t *key, uint64_t m, int bits, DevB
     GT_TRY(scanned.reserve(sizeof(uint32_t) * 256 * (size_t)ntiles));
     const int passes = std::max(1, (bits + 7) / 8);
     const uint32_t *kin = key;
-    const uint32_t *vin = nullptr;  // first pass: value = own index
+    const uint32_t *vin = vals;  // null: the first pass takes the item's own index
     // Ping-pong so the last pass lands in ids_out.
     uint32_t *kbuf[2] = {kA.as<uint32_t>(), kB.as<uint32_t>()};
     uint32_t *vbuf[2];
@@ -406,9 +407,9 @@ gt_status build_edges_impl(uint64_t graph_id, uint32_t n, uint64_t m, const uint
     if (m) {
         // 3: stable grouping by source (final out-halves) and by target.
         const int bits = bits_for(n);
-        GT_TRY(stable_sort_ids(S, m, bits, O.eid, s, tmp));
+        GT_TRY(stable_sort_ids(S, nullptr, m, bits, O.eid, s, tmp));
         DevBuf pre;
-        GT_TRY(stable_sort_ids(T, m, bits, pre, s, tmp));
+        GT_TRY(stable_sort_ids(T, nullptr, m, bits, pre, s, tmp));
         // 4: push-swap replay into the in-halves.
         DevBuf ring;
         GT_TRY(ring.reserve(sizeof(uint32_t) * m));

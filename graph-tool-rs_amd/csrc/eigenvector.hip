@@ -144,11 +144,15 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     }
 }
 
-// Seed slots [had, n) with the uniform start and reset the loop state.
-__global__ __launch_bounds__(kBlock) void ev_init(double *x, uint64_t had, uint32_t n,
-                                                  double uniform, EvState *st) {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        if (i >= had) x[i] = uniform;
+// y0[k] = the caller's slot iperm[k] (identity when null), with slots
+// [had, n) seeded with the uniform start; resets the loop state.
+__global__ __launch_bounds__(kBlock) void ev_init(const double *x, const uint32_t *iperm,
+                                                  uint64_t had, uint32_t n, double uniform,
+                                                  double *y0, EvState *st) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        const uint32_t o = iperm ? iperm[k] : k;
+        y0[k] = o >= had ? uniform : x[o];
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->sc_cur = 1.0;
         st->sc_prev = 1.0;
@@ -164,13 +168,15 @@ __global__ __launch_bounds__(kBlock) void ev_init(double *x, uint64_t had, uint3
     }
 }
 
-// x(i) = y_T(i) / scale for active i; inactive slots are carried unscaled.
+// x(o) = y_T(perm[o]) / scale for active o; inactive slots are carried
+// unscaled.  perm null: identity.
 template <bool MASK>
 __global__ __launch_bounds__(kBlock) void ev_emit(const double *y, const uint8_t *active,
-                                                  const EvState *st, uint32_t n, double *x) {
+                                                  const uint32_t *perm, const EvState *st,
+                                                  uint32_t n, double *x) {
     const double sc = st->final_scale;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const double v = y[i];
+        const double v = y[perm ? perm[i] : i];
         x[i] = (MASK && !active[i]) ? v : v / sc;
     }
 }
@@ -208,6 +214,14 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     const double uniform = nactive == 0 ? 0.0 : 1.0 / (double)nactive;
     const double eps = params->epsilon;
     const bool skip = std::isnan(eps) || !(eps + 1.0 >= eps);
+    const uint32_t *perm = nullptr, *iperm = nullptr;
+    if (!masked && n > 0) {
+        Relabeled *R = nullptr;
+        GT_TRY(ensure_relabeled(g, reversed ? 1 : 0, &R));
+        pull = &R->pull;
+        perm = R->perm.as<uint32_t>();
+        iperm = R->iperm.as<uint32_t>();
+    }
 
     bool weighted = false;
     const double *wp = nullptr;
@@ -232,21 +246,26 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, pull->nslots)));
     GT_TRY(ws.state.reserve(sizeof(EvState)));
     const size_t vb = sizeof(double) * n;
-    double *y[3];
+    // xio: the caller's map on the device (their buffer, or a staged copy).
+    double *xio = nullptr;
     if (x->loc == GT_DEVICE) {
-        y[0] = x->data;
+        xio = x->data;
     } else {
-        GT_TRY(ws.y[0].reserve(vb));
-        y[0] = ws.y[0].as<double>();
+        GT_TRY(ws.y3.reserve(vb));
+        xio = ws.y3.as<double>();
         const uint64_t had = std::min<uint64_t>(x->had, n);
-        if (had) GT_HIP(hipMemcpyAsync(y[0], x->data, sizeof(double) * had, hipMemcpyHostToDevice, s));
+        if (had) GT_HIP(hipMemcpyAsync(xio, x->data, sizeof(double) * had, hipMemcpyHostToDevice, s));
     }
+    double *y[3];
+    GT_TRY(ws.y[0].reserve(vb));
     GT_TRY(ws.y[1].reserve(vb));
     GT_TRY(ws.y[2].reserve(vb));
+    y[0] = ws.y[0].as<double>();
     y[1] = ws.y[1].as<double>();
     y[2] = ws.y[2].as<double>();
     EvState *st = ws.state.as<EvState>();
-    hipLaunchKernelGGL(ev_init, dim3(vgrid(n)), dim3(kBlock), 0, s, y[0], x->had, n, uniform, st);
+    hipLaunchKernelGGL(ev_init, dim3(vgrid(n)), dim3(kBlock), 0, s, xio, iperm, x->had, n, uniform,
+                       y[0], st);
     GT_HIP(hipGetLastError());
 
     uint64_t T = 0;
@@ -305,13 +324,14 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         *eig = h.eig;
         out = y[T % 3];
     }
-    double *dst = x->loc == GT_DEVICE ? x->data : y[0];
+    // xio never aliases a y buffer, so the permuting copy has no hazard.
+    double *dst = xio;
     if (masked)
-        hipLaunchKernelGGL((ev_emit<true>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, active, st,
-                           n, dst);
+        hipLaunchKernelGGL((ev_emit<true>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, active,
+                           nullptr, st, n, dst);
     else
         hipLaunchKernelGGL((ev_emit<false>), dim3(vgrid(n)), dim3(kBlock), 0, s, out, nullptr,
-                           st, n, dst);
+                           perm, st, n, dst);
     GT_HIP(hipGetLastError());
     if (x->loc == GT_HOST)
         GT_HIP(hipMemcpyAsync(x->data, dst, vb, hipMemcpyDeviceToHost, s));

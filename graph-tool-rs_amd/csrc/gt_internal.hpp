@@ -114,6 +114,19 @@ struct Orient {
     bool heavy_ready = false;
 };
 
+// A pull orientation in a locality-friendly labelling: vertices renumbered by
+// descending gather frequency (the row length of the opposite orientation:
+// how often a vertex appears as an entry), rows in the new order and each
+// row's entries sorted by their new label, edge ids kept.  Sweeps run in the
+// new labels; vectors cross the boundary through perm / iperm.
+struct Relabeled {
+    bool ready = false;
+    DevBuf perm;       // uint32[n]: old label -> new
+    DevBuf iperm;      // uint32[n]: new label -> old
+    Orient pull;       // relabelled pull orientation (eid = original edge ids)
+    DevBuf push_off;   // uint32[n+1]: scanned push-row lengths in new order
+};
+
 struct Workspace {
     DevBuf x[2];        // f64 iterates
     DevBuf c[2];        // f64 contributions x/deg
@@ -122,6 +135,8 @@ struct Workspace {
     DevBuf y3;          // staged f64 map / third eigenvector buffer
     DevBuf y[3];        // eigenvector iterates
     DevBuf deg;         // weighted out-degree, f64[n]
+    DevBuf deg2;        // weighted out-degree in sweep labels
+    DevBuf pers2;       // personalisation in sweep labels
     DevBuf wsrc;        // staged weight map (edge-id order)
     DevBuf wcsc;        // weights in pull order, value type of the call
     DevBuf wcsc64;      // weights in pull order, f64 (refinement sweeps)
@@ -169,6 +184,7 @@ struct gt_graph {
     uint64_t edge_bound = 0;
     gt::Orient in;   // CSC: in-halves
     gt::Orient out;  // CSR: out-halves
+    gt::Relabeled rel[2];  // [0]: pull = in-halves, [1]: pull = out-halves
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     gt::Workspace ws;
@@ -201,5 +217,7 @@ int sweep_grid(uint32_t ntiles);
 int occupancy_cap(const void *kernel);
 // min(cap, units), at least 1.
 int fit_grid(int cap, uint32_t units);
+// The relabelled pull orientation (0: in-halves, 1: out-halves), built once.
+gt_status ensure_relabeled(gt_graph *g, int which, Relabeled **out);
 
 }  // namespace gt

@@ -38,7 +38,8 @@ struct PrArgs {
     const uint32_t *out_off;  // unity out-degree source
     const double *deg;        // weighted out-degree
     const uint8_t *active;    // masked views only
-    const double *pers;       // dense personalisation
+    const uint32_t *iperm;    // sweep label -> caller's label (null: identity)
+    const double *pers;       // dense personalisation, in sweep labels
     double pval;              // constant personalisation
     double damping;
     double eps_abs;           // stop when delta < max(eps_abs, eps_rel * mass)
@@ -142,12 +143,14 @@ __global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, 
     V *c0 = static_cast<V *>(a.c_next);
     double ssum = 0.0, mass = 0.0;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        // Row i of the sweep is slot o of the caller's map.
+        const uint32_t o = a.iperm ? a.iperm[i] : i;
         double xi;
-        if (i >= had) {
+        if (o >= had) {
             xi = DPERS ? a.pers[i] : a.pval;
-            rank[i] = xi;
+            rank[o] = xi;
         } else {
-            xi = rank[i];
+            xi = rank[o];
         }
         const V xv = static_cast<V>(xi);
         x0[i] = xv;
@@ -190,11 +193,22 @@ __global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, 
     }
 }
 
-// x64[i] = x32[i]: hand the FP32 iterate to the FP64 refinement sweeps.
-__global__ __launch_bounds__(kBlock) void widen_iterate(const float *__restrict__ x32, uint32_t n,
-                                                        double *__restrict__ x64) {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        x64[i] = static_cast<double>(x32[i]);
+// out[o] = x[perm[o]] (perm null: identity), widened to f64: an iterate in
+// sweep labels back to the caller's labels.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void unpermute(const V *__restrict__ x,
+                                                    const uint32_t *__restrict__ perm, uint32_t n,
+                                                    double *__restrict__ out) {
+    for (uint32_t o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock)
+        out[o] = static_cast<double>(x[perm ? perm[o] : o]);
+}
+
+// out[k] = in[iperm[k]]: a vertex map into sweep labels.
+__global__ __launch_bounds__(kBlock) void permute_f64(const double *__restrict__ in,
+                                                      const uint32_t *__restrict__ iperm,
+                                                      uint32_t n, double *__restrict__ out) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        out[k] = in[iperm[k]];
 }
 
 // ------------------------------------------------------------------ host
@@ -312,7 +326,27 @@ struct PrSolve {
     double *rank64 = nullptr;  // the f64 map on the device (caller's or staged)
     const void *wsrc = nullptr;
     int wdtype = GT_F64;
+    // Set when the sweeps run in the relabelled orientation: sweep vectors are
+    // in new labels, rank64 stays in the caller's.
+    const uint32_t *perm = nullptr;
+    const uint32_t *iperm = nullptr;
 };
+
+unsigned vblocks(uint32_t n) {
+    uint32_t b = (n + kBlock - 1) / kBlock;
+    if (b > 2048) b = 2048;
+    return b ? b : 1u;
+}
+
+// rank64 <- latest (sweep labels), widened; no-op when it is rank64 itself.
+template <typename V>
+gt_status pr_unpermute(PrSolve &S, const V *latest) {
+    if ((const void *)latest == (const void *)S.rank64) return GT_OK;
+    hipLaunchKernelGGL((unpermute<V>), dim3(vblocks(S.g->n)), dim3(kBlock), 0, S.g->stream, latest,
+                       S.perm, S.g->n, S.rank64);
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
 
 gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
                      const gt_prop *pers, double damping, const gt_power_iteration *params,
@@ -332,12 +366,37 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     PrArgs &a = S.base;
     S.plan.masked = active != nullptr;
     a.active = active;
+    a.out_off = S.push->off.as<uint32_t>();
+    Orient *push_caller = S.push;  // caller's labels (degree staging)
+    if (!active && n > 0) {
+        Relabeled *R = nullptr;
+        GT_TRY(ensure_relabeled(g, reversed ? 1 : 0, &R));
+        S.pull = &R->pull;
+        S.perm = R->perm.as<uint32_t>();
+        S.iperm = R->iperm.as<uint32_t>();
+        a.out_off = R->push_off.as<uint32_t>();
+    }
+    a.iperm = S.iperm;
     GT_TRY(stage_vertex_prop(g, pers, g->ws.pers, &a.pers, &a.pval, &S.plan.dpers, uniform));
+    if (S.plan.dpers && S.iperm) {
+        GT_TRY(g->ws.pers2.reserve(sizeof(double) * n));
+        hipLaunchKernelGGL(permute_f64, dim3(vblocks(n)), dim3(kBlock), 0, s, a.pers, S.iperm, n,
+                           g->ws.pers2.as<double>());
+        GT_HIP(hipGetLastError());
+        a.pers = g->ws.pers2.as<double>();
+    }
     if (weight && weight->kind != GT_MAP_UNITY) {
         S.plan.weighted = true;
         GT_TRY(stage_edge_source(g, weight, &S.wsrc, &S.wdtype));
-        GT_TRY(stage_degree(g, weight, *S.push, S.wsrc, S.wdtype, g->ws.deg));
+        GT_TRY(stage_degree(g, weight, *push_caller, S.wsrc, S.wdtype, g->ws.deg));
         a.deg = g->ws.deg.as<double>();
+        if (S.iperm) {
+            GT_TRY(g->ws.deg2.reserve(sizeof(double) * n));
+            hipLaunchKernelGGL(permute_f64, dim3(vblocks(n)), dim3(kBlock), 0, s, a.deg, S.iperm,
+                               n, g->ws.deg2.as<double>());
+            GT_HIP(hipGetLastError());
+            a.deg = g->ws.deg2.as<double>();
+        }
     }
     if (n == 0) return GT_OK;
     GT_TRY(ensure_schedule(g, *S.pull, n));
@@ -366,7 +425,6 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     a.nlong = S.pull->nlong;
     a.in_off = S.pull->off.as<uint32_t>();
     a.in_idx = S.pull->idx.as<uint32_t>();
-    a.out_off = S.push->off.as<uint32_t>();
     a.damping = damping;
     a.eps_abs = params->epsilon;
     a.eps_rel = 0.0;
@@ -442,6 +500,7 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
     }
     Workspace &ws = g->ws;
     const size_t vb = sizeof(double) * (size_t)n;
+    GT_TRY(ws.x[0].reserve(vb));
     GT_TRY(ws.x[1].reserve(vb));
     GT_TRY(ws.c[0].reserve(vb));
     GT_TRY(ws.c[1].reserve(vb));
@@ -449,7 +508,8 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
         GT_TRY(stage_csc_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
         S.base.w = ws.wcsc64.ptr;
     }
-    double *x[2] = {S.rank64, ws.x[1].as<double>()};
+    // Without relabelling the caller's map is the first iterate itself.
+    double *x[2] = {S.iperm ? ws.x[0].as<double>() : S.rank64, ws.x[1].as<double>()};
     double *c[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
     uint64_t sweeps = 0;
     if (skip) {
@@ -465,6 +525,10 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
     }
     g->last_sweeps = sweeps;
     *iters = sweeps;
+    if (S.iperm) {
+        GT_TRY(pr_unpermute<double>(S, x[sweeps & 1]));
+        return pr_deliver(g, S.rank64, rank);
+    }
     return pr_deliver(g, x[sweeps & 1], rank);
 }
 
@@ -510,13 +574,9 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     uint64_t it32 = 0;
     GT_TRY(pr_phase<float>(S, a32, x32, c32, rank->had, 0,
                            std::max(params->epsilon, std::ldexp(1.0, -22)), cap32, &it32));
-    // Widen the FP32 iterate into the caller's f64 map.
-    {
-        uint32_t blocks = std::min<uint32_t>((n + kBlock - 1) / kBlock, 2048);
-        hipLaunchKernelGGL(widen_iterate, dim3(blocks), dim3(kBlock), 0, s, x32[it32 & 1], n,
-                           S.rank64);
-        GT_HIP(hipGetLastError());
-    }
+    // Widen the FP32 iterate into the caller's f64 map (and labels).
+    GT_TRY(pr_unpermute<float>(S, x32[it32 & 1]));
+    (void)s;
     uint64_t total = it32;
     const double *latest = S.rank64;
     const bool capped = params->max_iter > 0 && it32 >= params->max_iter;
@@ -532,12 +592,17 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         uint64_t cap64 = refine;
         if (params->max_iter > 0) cap64 = std::min<uint64_t>(refine, params->max_iter - it32);
         a64.max_iter = it32 + cap64;
-        double *x64[2] = {S.rank64, ws.x[1].as<double>()};
+        GT_TRY(ws.x[0].reserve(sizeof(double) * n));
+        double *x64[2] = {S.iperm ? ws.x[0].as<double>() : S.rank64, ws.x[1].as<double>()};
         double *c64[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
         uint64_t it = 0;
         GT_TRY(pr_phase<double>(S, a64, x64, c64, n, it32, params->epsilon, cap64, &it));
         total = it;
         latest = x64[(it - it32) & 1];
+        if (S.iperm) {
+            GT_TRY(pr_unpermute<double>(S, latest));
+            latest = S.rank64;
+        }
     }
     g->last_sweeps = total;
     *iters = total;

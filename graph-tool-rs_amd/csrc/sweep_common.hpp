@@ -15,9 +15,10 @@
 //   phase 3  `fin(row, sum)` for every row of the tile, one thread per row,
 //            so the rows' global reads and writes go out together and
 //            coalesced.
-// Two barriers per tile: phase 1 of the next tile may overwrite the LDS tile
-// once phase 2 is done, and phase 2 of the next tile may overwrite the row
-// buffer once the barrier after its phase 1 has been passed.
+// Three barriers per tile: after phase 1; after the short rows of phase 2
+// (the long-row queue is complete); after the long rows (the LDS tile is free
+// for the next tile's phase 1, and the row buffer is not written again before
+// the next tile's first barrier).
 #pragma once
 
 #include "device_common.hpp"
@@ -54,12 +55,18 @@ __device__ __forceinline__ void gather_tile(const uint32_t *__restrict__ idx,
     }
 }
 
-// LDS a kernel using spmv_tiles must provide.
+// Rows of at least this many entries are summed by a wavefront.
+constexpr uint32_t kShortRow = 32;
+
+// LDS a kernel using spmv_tiles must provide.  nlongq must be zero when the
+// tile loop starts (spmv_tiles resets it after each tile).
 template <typename V>
 struct TileLds {
     V vals[kTileNnz];
     double rowsum[kTileRows];
     double red[8];
+    uint16_t longq[kTileNnz / kShortRow];
+    uint32_t nlongq;
 };
 
 template <typename V, bool WEIGHTED, typename Fin>
@@ -72,6 +79,7 @@ __device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint3
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid / kWave;
+    if (tid == 0) L.nlongq = 0;  // published by the first tile's barrier
     for (uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile t = tiles[ti];
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
@@ -84,24 +92,32 @@ __device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint3
             if (tid == 0) long_part[t.row_begin] = v;
             continue;
         }
+        // Rows shorter than kShortRow: one thread each.  Longer rows (at most
+        // kTileNnz / kShortRow of them) are queued and summed by a wavefront
+        // each, so no thread walks a long row alone whatever the tile's mix.
         const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
-        if (cnt < 16u * rows) {
-            for (uint32_t q = tid; q < rows; q += kBlock) {
-                const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
+        for (uint32_t q = tid; q < rows; q += kBlock) {
+            const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
+            if (hi - lo < kShortRow) {
                 V acc = V(0);
                 for (uint32_t k = lo; k < hi; ++k) acc += L.vals[k];
                 L.rowsum[q] = static_cast<double>(acc);
-            }
-        } else {
-            for (uint32_t q = wid; q < rows; q += kBlock / kWave) {
-                const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
-                V acc = V(0);
-                for (uint32_t k = lo + lane; k < hi; k += kWave) acc += L.vals[k];
-                acc = wave_sum(acc);
-                if (lane == 0) L.rowsum[q] = static_cast<double>(acc);
+            } else {
+                L.longq[atomicAdd(&L.nlongq, 1u)] = (uint16_t)q;
             }
         }
         __syncthreads();
+        const uint32_t nq = L.nlongq;
+        for (uint32_t i = wid; i < nq; i += kBlock / kWave) {
+            const uint32_t q = L.longq[i];
+            const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
+            V acc = V(0);
+            for (uint32_t k = lo + lane; k < hi; k += kWave) acc += L.vals[k];
+            acc = wave_sum(acc);
+            if (lane == 0) L.rowsum[q] = static_cast<double>(acc);
+        }
+        __syncthreads();
+        if (tid == 0) L.nlongq = 0;
         for (uint32_t q = tid; q < rows; q += kBlock) fin(rb + q, L.rowsum[q]);
     }
     __syncthreads();

@@ -40,6 +40,11 @@ def test_backward_matches_oracle_adjoint(gpu, precision, tol):
     assert np.abs(gp - gp_o).max() <= tol * np.abs(gp_o).max()
     assert abs(gd - gd_o) <= tol * max(1.0, abs(gd_o))
     if precision == "f64":
+        # At epsilon 1e-13 the last deltas are rounding noise; compare sweep
+        # counts where the L1 change is still well above it.  (An FP32 adjoint
+        # reaches its own rounding floor first, so its count is not compared.)
+        _, _, _, it = gt_hip.pagerank_grad(g, wm, None, 0.85, _params(n, 1e-8), x, c)
+        _, _, _, _, it_o = o.pagerank_grad(x, c, weight=w, epsilon=1e-8)
         assert abs(it - it_o) <= 1
 
 
