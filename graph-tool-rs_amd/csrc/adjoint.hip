@@ -9,8 +9,8 @@
 //
 // The transposed system is solved by the same power iteration, pulling over
 // the opposite orientation: row s of the out-halves gathers lambda at its
-// targets (times the out-ordered weight), so the tile engine and the
-// last-workgroup folds of the forward sweep are reused unchanged.  No
+// targets (times the out-ordered weight), so the wave engine and the fold
+// kernel of the forward sweep are reused unchanged.  No
 // iterate of the forward solve is stored: only x* is read.  The fused edge
 // kernel then computes mu_s and every out-edge's gradient of row s in one
 // wavefront, writing by edge id.
@@ -30,8 +30,9 @@ struct AdjArgs {
     uint32_t ntiles;
     uint32_t nlong;
     const uint32_t *row_off;   // push orientation (rows = sources)
-    const uint32_t *row_idx;   // targets
-    const void *w;             // weights in push order (value type)
+    const uint32_t *packed;    // packed targets of the push orientation's wave tiles
+    uint32_t bits;             // slot bits of `packed`
+    const void *w;             // weights in packed order (value type)
     const double *deg;         // weighted out-degree (weighted)
     const double *cvec;        // dL/dx, f64
     const double *pers;        // dense personalisation
@@ -69,33 +70,53 @@ struct AdjFin {
     }
 };
 
+// The SpMV of one transposed sweep (split rows aside), with per-workgroup
+// delta and p.lambda partials; adj_fold completes the sweep.
 template <typename V, bool WEIGHTED, bool DPERS>
 __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
     if (a.st->done) return;
-    __shared__ TileLds<V> L;
-    __shared__ uint32_t flag;
-    double *red = L.red;
-    const int tid = threadIdx.x;
+    __shared__ WaveLds<V> L;
     AdjFin<V, WEIGHTED, DPERS> fin(a, a.st->p_sink);
-    spmv_tiles<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.row_idx,
+    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits,
                             static_cast<const V *>(a.l_cur), static_cast<const V *>(a.w),
                             a.long_part, L, fin);
     double d = fin.dsum, p = fin.psum;
-    block_sum2(d, p, red);
-    if (tid == 0) {
+    block_sum2(d, p, L.red);
+    if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = d;
         a.partials[gridDim.x + blockIdx.x] = p;
     }
-    if (!last_to_arrive(&a.st->counter, &flag)) return;
-    AdjFin<V, WEIGHTED, DPERS> lf(a, fin.pl);
-    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
-        lf.dsum += a.partials[k];
-        lf.psum += a.partials[gridDim.x + k];
-    }
-    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+}
+
+// Completes the preceding adj_sweep launch (`sgrid` workgroups): split rows,
+// then in the last workgroup the index-order fold and the decision.
+template <typename V, bool WEIGHTED, bool DPERS>
+__global__ __launch_bounds__(kBlock) void adj_fold(const AdjArgs a, uint32_t sgrid) {
+    if (a.st->done) return;
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    AdjFin<V, WEIGHTED, DPERS> lf(a, a.st->p_sink);
+    for (uint32_t k = blockIdx.x * kBlock + tid; k < a.nlong; k += gridDim.x * kBlock)
         lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
-    d = lf.dsum;
-    p = lf.psum;
+    double d = lf.dsum, p = lf.psum;
+    block_sum2(d, p, red);
+    double *fp = a.partials + 2 * kMaxSweepGrid;
+    if (tid == 0) {
+        fp[blockIdx.x] = d;
+        fp[gridDim.x + blockIdx.x] = p;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    d = 0.0;
+    p = 0.0;
+    for (uint32_t k = tid; k < sgrid; k += kBlock) {
+        d += a.partials[k];
+        p += a.partials[sgrid + k];
+    }
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        d += fp[k];
+        p += fp[gridDim.x + k];
+    }
     block_sum2(d, p, red);
     if (tid == 0) {
         SweepState *st = a.st;
@@ -106,7 +127,6 @@ __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
         st->launches += 1;
         st->done = (!(d >= a.eps) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
         st->counter = 0;
-        __threadfence();
     }
 }
 
@@ -214,9 +234,16 @@ __global__ __launch_bounds__(kBlock) void f32_to_f64_v(const float *a, uint32_t 
 }
 
 template <typename V, bool W, bool P>
-static int adj_grid(uint32_t ntiles) {
-    static const int cap = occupancy_cap((const void *)adj_sweep<V, W, P>);
-    return fit_grid(cap, ntiles);
+static int adj_launch(const AdjArgs &a, hipStream_t s) {
+    static const int cap = occupancy_cap((const void *)adj_sweep<V, W, P>, kSweepBlocksPerCU);
+    const int sg = wave_grid(cap, a.ntiles);
+    hipLaunchKernelGGL((adj_sweep<V, W, P>), dim3(sg), dim3(kBlock), 0, s, a);
+    return sg;
+}
+template <typename V, bool W, bool P>
+static void adj_launch_fold(const AdjArgs &a, int sg, hipStream_t s) {
+    hipLaunchKernelGGL((adj_fold<V, W, P>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s, a,
+                       (uint32_t)sg);
 }
 
 static unsigned vgrid(uint64_t n) {
@@ -224,7 +251,8 @@ static unsigned vgrid(uint64_t n) {
     if (b > 2048) b = 2048;
     return b ? (unsigned)b : 1u;
 }
-static unsigned wave_grid(uint32_t n) {
+// Workgroups for one wavefront per row.
+static unsigned row_wave_grid(uint32_t n) {
     uint64_t b = ((uint64_t)n + 3) / 4;
     if (b > 16384) b = 16384;
     return b ? (unsigned)b : 1u;
@@ -291,14 +319,12 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     GT_TRY(stage_vec(g, xm, ws.ax, &x, "x"));
     GT_TRY(stage_vec(g, cm, ws.ac, &c, "dL/dx"));
     const bool weighted = weight && weight->kind != GT_MAP_UNITY;
+    const void *wsrc = nullptr;
+    int wdt = GT_F64;
     if (weighted) {
-        const void *src = nullptr;
-        int dt = GT_F64;
-        GT_TRY(stage_edge_source(g, weight, &src, &dt));
-        GT_TRY(stage_degree(g, weight, *push, src, dt, ws.deg));
-        GT_TRY(stage_csc_weights<V>(g, weight, *push, src, dt, ws.wcsr));
+        GT_TRY(stage_edge_source(g, weight, &wsrc, &wdt));
+        GT_TRY(stage_degree(g, weight, *push, wsrc, wdt, ws.deg));
         a.deg = ws.deg.as<double>();
-        a.w = ws.wcsr.ptr;
     }
     if (n == 0) {
         if (grad_damping) *grad_damping = 0.0;
@@ -306,8 +332,18 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         return GT_OK;
     }
     GT_TRY(ensure_schedule(g, *push, n));
+    // The sweeps read weights in packed order, the edge gradients in row order.
+    const V *wrow = nullptr;
+    if (weighted) {
+        GT_TRY(stage_pull_weights<V>(g, weight, *push, wsrc, wdt, ws.wcsr));
+        a.w = ws.wcsr.ptr;
+        if (grad_w) {
+            GT_TRY(stage_csc_weights<V>(g, weight, *push, wsrc, wdt, ws.wrow));
+            wrow = ws.wrow.as<V>();
+        }
+    }
     GT_TRY(prof_prepare(g));
-    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, push->nslots)));
     GT_TRY(ws.state.reserve(sizeof(SweepState) * 2));
     GT_TRY(ws.lam[0].reserve(sizeof(V) * n));
@@ -317,7 +353,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     a.ntiles = push->ntiles;
     a.nlong = push->nlong;
     a.row_off = push->off.as<uint32_t>();
-    a.row_idx = push->idx.as<uint32_t>();
+    a.packed = push->packed.as<uint32_t>();
+    a.bits = push->bits;
     a.cvec = c;
     a.damping = damping;
     a.eps = params->epsilon;
@@ -338,27 +375,25 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         GT_HIP(hipGetLastError());
     }
     g->prof_kind = GT_SWEEP_OTHER;
-    auto launch = [&](uint64_t q) {
+    AdjArgs b = a;
+    int sg = 0;
+    auto sweep = [&](uint64_t q) {
         const int par = (int)(q & 1);
-        AdjArgs b = a;
         b.l_cur = lam[par];
         b.l_next = lam[par ^ 1];
-        const dim3 bd(kBlock);
-        if (weighted && dpers)
-            hipLaunchKernelGGL((adj_sweep<V, true, true>), dim3(adj_grid<V, true, true>(b.ntiles)),
-                               bd, 0, s, b);
-        else if (weighted)
-            hipLaunchKernelGGL((adj_sweep<V, true, false>),
-                               dim3(adj_grid<V, true, false>(b.ntiles)), bd, 0, s, b);
-        else if (dpers)
-            hipLaunchKernelGGL((adj_sweep<V, false, true>),
-                               dim3(adj_grid<V, false, true>(b.ntiles)), bd, 0, s, b);
-        else
-            hipLaunchKernelGGL((adj_sweep<V, false, false>),
-                               dim3(adj_grid<V, false, false>(b.ntiles)), bd, 0, s, b);
+        if (weighted && dpers) sg = adj_launch<V, true, true>(b, s);
+        else if (weighted) sg = adj_launch<V, true, false>(b, s);
+        else if (dpers) sg = adj_launch<V, false, true>(b, s);
+        else sg = adj_launch<V, false, false>(b, s);
+    };
+    auto fold = [&](uint64_t) {
+        if (weighted && dpers) adj_launch_fold<V, true, true>(b, sg, s);
+        else if (weighted) adj_launch_fold<V, true, false>(b, sg, s);
+        else if (dpers) adj_launch_fold<V, false, true>(b, sg, s);
+        else adj_launch_fold<V, false, false>(b, sg, s);
     };
     uint64_t launched = 0;
-    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, launch, &launched));
+    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched));
     const uint64_t sweeps = g->h_state->iter;
     const V *lf = lam[sweeps & 1];
 
@@ -372,8 +407,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         }
         // Edges outside the view keep 0.
         if (gb) GT_HIP(hipMemsetAsync(gdst, 0, gb, s));
-        const V *wv = static_cast<const V *>(a.w);
-        const dim3 gd(wave_grid(n)), bd(kBlock);
+        const V *wv = wrow;
+        const dim3 gd(row_wave_grid(n)), bd(kBlock);
         const uint32_t *po = push->off.as<uint32_t>(), *pi = push->idx.as<uint32_t>(),
                        *pe = push->eid.as<uint32_t>();
         if (gw_dtype == GT_F64) {

@@ -56,17 +56,26 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
 // workgroup of the grid -- the last to arrive -- with every other
 // workgroup's writes visible to it.  `counter` must be zero at launch; the
 // caller of the true branch resets it.
+//
+// One thread per workgroup fences: an agent-scope release (L2 write-back, no
+// invalidate) before taking the ticket, and an acquire (which invalidates
+// the non-coherent caches) only in the last workgroup.  The workgroup barrier
+// before the ticket makes the other threads' stores part of what the release
+// publishes; the one after it extends the acquire to the whole workgroup.  A
+// fence in every thread, or an acquire in every workgroup, would invalidate
+// the L2 under the gathers of the workgroups still running.
 __device__ __forceinline__ bool last_to_arrive(uint32_t *counter, uint32_t *s_flag) {
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t t = atomicAdd(counter, 1u);
-        *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        const bool last = t == gridDim.x - 1;
+        if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_flag = last ? 1u : 0u;
     }
     __syncthreads();
-    bool last = *s_flag != 0;
-    if (last) __threadfence();
-    return last;
+    return *s_flag != 0;
 }
 
 }  // namespace gt

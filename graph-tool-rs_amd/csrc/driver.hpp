@@ -16,10 +16,12 @@ gt_status prof_flush(gt_graph *g, uint64_t launched);
 // Launch sweeps until the device reports done.  Batches are sized from the
 // observed geometric decay of delta, so the host waits on the device a few
 // times per solve and few launches find the loop already finished.
-// `launch(k)` issues sweep k of this run (k = 0, 1, ...).
-template <typename Launch>
-gt_status drive(gt_graph *g, SweepState *dst, double eps_pred, uint64_t cap, Launch launch,
-                uint64_t *launched_out) {
+// `sweep(k)` issues the SpMV kernel of sweep k of this run (k = 0, 1, ...),
+// timed when profiling; `fold(k)` then issues what completes the sweep (the
+// split rows and the convergence decision).
+template <typename Sweep, typename Fold>
+gt_status drive(gt_graph *g, SweepState *dst, double eps_pred, uint64_t cap, Sweep sweep,
+                Fold fold, uint64_t *launched_out) {
     hipStream_t s = g->stream;
     uint64_t launched = 0;
     uint64_t batch = 2;
@@ -29,8 +31,9 @@ gt_status drive(gt_graph *g, SweepState *dst, double eps_pred, uint64_t cap, Lau
         if (cap > 0 && batch > cap - launched) batch = cap - launched;
         for (uint64_t k = 0; k < batch; ++k, ++launched) {
             GT_TRY(prof_begin(g, launched));
-            launch(launched);
+            sweep(launched);
             GT_TRY(prof_end(g, launched));
+            fold(launched);
         }
         GT_HIP(hipGetLastError());
         GT_HIP(hipMemcpyAsync(g->h_state, dst, sizeof(SweepState), hipMemcpyDeviceToHost, s));

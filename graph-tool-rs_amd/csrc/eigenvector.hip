@@ -11,8 +11,9 @@
 // delta_{L-1} = sum |y_{L-1}/scale_{L-1} - y_{L-2}/scale_{L-2}|.  The scales
 // are known at launch time, so no second vertex pass is needed; the price is
 // that convergence of sweep L-1 is learnt in launch L (one extra SpMV at the
-// end of a solve).  Three y buffers rotate.  The last workgroup folds the
-// squared-norm and delta partials in index order and decides on the device.
+// end of a solve).  Three y buffers rotate.  A small second kernel per sweep
+// (ev_fold) finishes the split rows, folds the squared-norm and delta
+// partials in index order and decides on the device.
 #include <cmath>
 
 #include "device_common.hpp"
@@ -46,15 +47,16 @@ struct EvArgs {
     uint32_t ntiles;
     uint32_t nlong;
     const uint32_t *in_off;
-    const uint32_t *in_idx;
-    const double *w;          // weights in pull order
+    const uint32_t *packed;   // packed entry stream of the wave tiles
+    uint32_t bits;            // slot bits of `packed`
+    const double *w;          // weights in packed order
     const uint8_t *active;
     const double *y_cur;      // y_{L-1}
     const double *y_prev;     // y_{L-2}
     double *y_next;           // y_L
     double epsilon;
     uint64_t max_iter;
-    double *partials;         // [2 * gridDim.x]
+    double *partials;         // kPartials: sweep partials, then fold partials
     double *long_part;
     const uint32_t *long_rows;
     const uint32_t *long_slot;
@@ -81,33 +83,54 @@ struct EvFin {
     }
 };
 
+// The SpMV of one sweep (split rows aside), with per-workgroup squared-norm
+// and delta partials; ev_fold completes the sweep.
 template <bool WEIGHTED, bool MASK>
 __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     if (a.st->done) return;
-    __shared__ TileLds<double> L;
-    __shared__ uint32_t flag;
-    double *red = L.red;
-    const int tid = threadIdx.x;
+    __shared__ WaveLds<double> L;
     const bool first = a.st->launches == 0;
     EvFin<MASK> fin(a, a.st->sc_cur, a.st->sc_prev, first);
-    spmv_tiles<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, a.y_cur, a.w, a.long_part,
-                                 L, fin);
+    spmv_waves<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur, a.w,
+                                 a.long_part, L, fin);
     double sq = fin.sq, dl = fin.dl;
-    block_sum2(sq, dl, red);
-    if (tid == 0) {
+    block_sum2(sq, dl, L.red);
+    if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = sq;
         a.partials[gridDim.x + blockIdx.x] = dl;
     }
-    if (!last_to_arrive(&a.st->counter, &flag)) return;
+}
 
-    EvFin<MASK> lf(a, fin.sc_cur, fin.sc_prev, first);
-    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
-        lf.sq += a.partials[k];
-        lf.dl += a.partials[gridDim.x + k];
-    }
-    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+// Completes the preceding ev_sweep launch (`sgrid` workgroups): split rows,
+// then in the last workgroup the index-order fold and the decision.
+template <bool WEIGHTED, bool MASK>
+__global__ __launch_bounds__(kBlock) void ev_fold(const EvArgs a, uint32_t sgrid) {
+    if (a.st->done) return;
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    const bool first = a.st->launches == 0;
+    EvFin<MASK> lf(a, a.st->sc_cur, a.st->sc_prev, first);
+    for (uint32_t k = blockIdx.x * kBlock + tid; k < a.nlong; k += gridDim.x * kBlock)
         lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
     double s2 = lf.sq, d = lf.dl;
+    block_sum2(s2, d, red);
+    double *fp = a.partials + 2 * kMaxSweepGrid;
+    if (tid == 0) {
+        fp[blockIdx.x] = s2;
+        fp[gridDim.x + blockIdx.x] = d;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    s2 = 0.0;
+    d = 0.0;
+    for (uint32_t k = tid; k < sgrid; k += kBlock) {
+        s2 += a.partials[k];
+        d += a.partials[sgrid + k];
+    }
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        s2 += fp[k];
+        d += fp[gridDim.x + k];
+    }
     block_sum2(s2, d, red);
     if (tid == 0) {
         EvState *st = a.st;
@@ -140,7 +163,6 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
         st->launches = L;
         st->done = done ? 1u : 0u;
         st->counter = 0;
-        __threadfence();
     }
 }
 
@@ -182,9 +204,16 @@ __global__ __launch_bounds__(kBlock) void ev_emit(const double *y, const uint8_t
 }
 
 template <bool W, bool M>
-static int ev_grid(uint32_t ntiles) {
-    static const int cap = occupancy_cap((const void *)ev_sweep<W, M>);
-    return fit_grid(cap, ntiles);
+static int ev_launch(const EvArgs &a, int grid, hipStream_t s) {
+    static const int cap = occupancy_cap((const void *)ev_sweep<W, M>, kSweepBlocksPerCU);
+    const int sg = wave_grid(grid > 0 ? grid : cap, a.ntiles);
+    hipLaunchKernelGGL((ev_sweep<W, M>), dim3(sg), dim3(kBlock), 0, s, a);
+    return sg;
+}
+template <bool W, bool M>
+static void ev_launch_fold(const EvArgs &a, int sg, hipStream_t s) {
+    hipLaunchKernelGGL((ev_fold<W, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s, a,
+                       (uint32_t)sg);
 }
 
 static unsigned vgrid(uint32_t n) {
@@ -225,13 +254,11 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
 
     bool weighted = false;
     const double *wp = nullptr;
+    const void *wsrc = nullptr;
+    int wdt = GT_F64;
     if (weight && weight->kind != GT_MAP_UNITY) {
         weighted = true;
-        const void *src = nullptr;
-        int dt = GT_F64;
-        GT_TRY(stage_edge_source(g, weight, &src, &dt));
-        GT_TRY(stage_csc_weights<double>(g, weight, *pull, src, dt, ws.wcsc64));
-        wp = ws.wcsc64.as<double>();
+        GT_TRY(stage_edge_source(g, weight, &wsrc, &wdt));
     }
     *eig = 0.0;
     if (n == 0) {
@@ -240,9 +267,12 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         return GT_OK;
     }
     GT_TRY(ensure_schedule(g, *pull, n));
-    g->grid = sweep_grid(pull->ntiles);
+    if (weighted) {
+        GT_TRY(stage_pull_weights<double>(g, weight, *pull, wsrc, wdt, ws.wcsc64));
+        wp = ws.wcsc64.as<double>();
+    }
     GT_TRY(prof_prepare(g));
-    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, pull->nslots)));
     GT_TRY(ws.state.reserve(sizeof(EvState)));
     const size_t vb = sizeof(double) * n;
@@ -276,7 +306,8 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.ntiles = pull->ntiles;
         a.nlong = pull->nlong;
         a.in_off = pull->off.as<uint32_t>();
-        a.in_idx = pull->idx.as<uint32_t>();
+        a.packed = pull->packed.as<uint32_t>();
+        a.bits = pull->bits;
         a.w = wp;
         a.active = active;
         a.epsilon = eps;
@@ -286,26 +317,24 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.long_rows = pull->long_rows.as<uint32_t>();
         a.long_slot = pull->long_slot.as<uint32_t>();
         a.st = st;
-        auto launch = [&](uint64_t k) {
+        EvArgs b = a;
+        int sg = 0;
+        auto sweep = [&](uint64_t k) {
             // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
             const uint64_t L = k + 1;
-            EvArgs b = a;
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            const dim3 bd(kBlock);
-            if (weighted && masked)
-                hipLaunchKernelGGL((ev_sweep<true, true>), dim3(ev_grid<true, true>(b.ntiles)), bd,
-                                   0, s, b);
-            else if (weighted)
-                hipLaunchKernelGGL((ev_sweep<true, false>), dim3(ev_grid<true, false>(b.ntiles)),
-                                   bd, 0, s, b);
-            else if (masked)
-                hipLaunchKernelGGL((ev_sweep<false, true>), dim3(ev_grid<false, true>(b.ntiles)),
-                                   bd, 0, s, b);
-            else
-                hipLaunchKernelGGL((ev_sweep<false, false>),
-                                   dim3(ev_grid<false, false>(b.ntiles)), bd, 0, s, b);
+            if (weighted && masked) sg = ev_launch<true, true>(b, g->grid, s);
+            else if (weighted) sg = ev_launch<true, false>(b, g->grid, s);
+            else if (masked) sg = ev_launch<false, true>(b, g->grid, s);
+            else sg = ev_launch<false, false>(b, g->grid, s);
+        };
+        auto fold = [&](uint64_t) {
+            if (weighted && masked) ev_launch_fold<true, true>(b, sg, s);
+            else if (weighted) ev_launch_fold<true, false>(b, sg, s);
+            else if (masked) ev_launch_fold<false, true>(b, sg, s);
+            else ev_launch_fold<false, false>(b, sg, s);
         };
         // The driver reads done / delta / iter through the SweepState layout;
         // EvState keeps delta, iter and done at the same offsets.
@@ -317,7 +346,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         g->prof_kind = GT_SWEEP_EIGENVECTOR;
         // One launch beyond the answer is needed to learn its delta.
         const uint64_t cap = params->max_iter;
-        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, cap, launch, &launched));
+        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, cap, sweep, fold, &launched));
         EvState h;
         GT_HIP(hipMemcpy(&h, st, sizeof(EvState), hipMemcpyDeviceToHost));
         T = h.final_iter;

@@ -39,13 +39,14 @@ gt_status DevBuf::reserve(size_t b) {
     return GT_OK;
 }
 
-int occupancy_cap(const void *kernel) {
+int occupancy_cap(const void *kernel, int per_cu) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 2048;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess ||
         cus <= 0 || per <= 0)
         return 2048;
+    if (per > per_cu) per = per_cu;
     const int g = per * cus;
     return g < 2048 ? g : 2048;
 }
@@ -55,12 +56,12 @@ int fit_grid(int cap, uint32_t units) {
     return g > 0 ? g : 1;
 }
 
-int sweep_grid(uint32_t ntiles) {
-    // 256 CUs x 8 resident workgroups keeps every CU busy; fewer tiles than
-    // that means one tile per workgroup.
-    const uint32_t cap = 2048;
-    uint32_t g = ntiles < cap ? ntiles : cap;
-    return g ? (int)g : 1;
+int fold_grid(uint32_t nlong) {
+    return fit_grid(kMaxSweepGrid, (nlong + kBlock - 1) / kBlock);
+}
+
+int wave_grid(int cap, uint32_t ntiles) {
+    return fit_grid(cap, (ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
 }
 
 gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
@@ -90,33 +91,80 @@ gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bou
     return GT_OK;
 }
 
-// Tiles of at most kTileNnz entries and kTileRows rows; a row longer than
-// kTileNnz becomes ceil(len / kTileNnz) split chunks with one partial slot each.
+// The packed entry stream of one wave tile (one workgroup per tile).  A tile
+// of whole rows is sorted by packed word -- by source, ties by slot -- with a
+// bitonic network over kWaveTileNnz LDS keys (padding keys are all ones and
+// sort last); split chunks and unpacked orientations keep row order.
+__global__ __launch_bounds__(kBlock) void pack_tiles(const Tile *__restrict__ tiles,
+                                                     const uint32_t *__restrict__ idx,
+                                                     const uint32_t *__restrict__ eid,
+                                                     uint32_t bits, uint32_t *__restrict__ packed,
+                                                     uint32_t *__restrict__ peid) {
+    __shared__ uint32_t key[kWaveTileNnz];
+    const Tile t = tiles[blockIdx.x];
+    const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
+    const int tid = threadIdx.x;
+    if (bits == 0 || t.row_end == kSplit) {
+        for (uint32_t j = tid; j < cnt; j += kBlock) {
+            packed[nb + j] = bits ? (idx[nb + j] << bits) | j : idx[nb + j];
+            peid[nb + j] = eid[nb + j];
+        }
+        return;
+    }
+    for (uint32_t j = tid; j < kWaveTileNnz; j += kBlock)
+        key[j] = j < cnt ? (idx[nb + j] << bits) | j : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= kWaveTileNnz; k <<= 1) {
+        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+            for (uint32_t i = tid; i < kWaveTileNnz; i += kBlock) {
+                const uint32_t o = i ^ h;
+                if (o > i) {
+                    const uint32_t x = key[i], y = key[o];
+                    if ((x > y) == ((i & k) == 0)) {
+                        key[i] = y;
+                        key[o] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t mask = (1u << bits) - 1u;
+    for (uint32_t j = tid; j < cnt; j += kBlock) {
+        const uint32_t p = key[j];
+        packed[nb + j] = p;
+        peid[nb + j] = eid[nb + (p & mask)];
+    }
+}
+
+// Wave tiles of at most kWaveTileNnz entries and kWaveTileRows rows; a row
+// longer than kWaveTileNnz becomes ceil(len / kWaveTileNnz) split chunks with
+// one partial slot each.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
     if (o.scheduled) return GT_OK;
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;
     uint32_t nslots = 0;
-    tiles.reserve(n / 256 + 16);
+    tiles.reserve(n / 64 + 16);
     uint32_t r = 0;
     while (r < n) {
-        uint32_t len = off[r + 1] - off[r];
-        if (len > kTileNnz) {
+        const uint32_t len = off[r + 1] - off[r];
+        if (len > kWaveTileNnz) {
             long_rows.push_back(r);
             long_slot.push_back(nslots);
-            for (uint32_t b = off[r]; b < off[r + 1]; b += kTileNnz) {
-                uint32_t e = std::min<uint64_t>((uint64_t)b + kTileNnz, off[r + 1]);
+            for (uint32_t b = off[r]; b < off[r + 1]; b += kWaveTileNnz) {
+                const uint32_t e = std::min<uint64_t>((uint64_t)b + kWaveTileNnz, off[r + 1]);
                 tiles.push_back(Tile{nslots++, kSplit, b, e});
             }
             ++r;
             continue;
         }
-        uint32_t r0 = r;
+        const uint32_t r0 = r;
         uint32_t nnz = 0;
-        while (r < n && r - r0 < kTileRows) {
-            uint32_t l = off[r + 1] - off[r];
-            if (l > kTileNnz || nnz + l > kTileNnz) break;
+        while (r < n && r - r0 < kWaveTileRows) {
+            const uint32_t l = off[r + 1] - off[r];
+            if (l > kWaveTileNnz || nnz + l > kWaveTileNnz) break;
             nnz += l;
             ++r;
         }
@@ -126,9 +174,15 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
     o.ntiles = (uint32_t)tiles.size();
     o.nlong = (uint32_t)long_rows.size();
     o.nslots = nslots;
+    // Sources and a kWaveSlotBits-bit slot share one 32-bit word when every
+    // vertex id fits in the remaining bits.
+    o.bits = (uint64_t)n <= (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
+    const uint64_t m = off[n];
     GT_TRY(o.tiles.reserve(sizeof(Tile) * std::max<size_t>(1, tiles.size())));
     GT_TRY(o.long_rows.reserve(sizeof(uint32_t) * std::max<size_t>(1, long_rows.size())));
     GT_TRY(o.long_slot.reserve(sizeof(uint32_t) * long_slot.size()));
+    GT_TRY(o.packed.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, m)));
+    GT_TRY(o.peid.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, m)));
     if (!tiles.empty())
         GT_HIP(hipMemcpy(o.tiles.ptr, tiles.data(), sizeof(Tile) * tiles.size(),
                          hipMemcpyHostToDevice));
@@ -137,8 +191,16 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
                          sizeof(uint32_t) * long_rows.size(), hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(o.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
                      hipMemcpyHostToDevice));
+    // Tiles go out in launches of at most 65535 workgroups.
+    for (uint32_t t0 = 0; t0 < o.ntiles; t0 += 65535) {
+        const uint32_t nt = std::min<uint32_t>(65535, o.ntiles - t0);
+        hipLaunchKernelGGL(pack_tiles, dim3(nt), dim3(kBlock), 0, g->stream,
+                           o.tiles.as<Tile>() + t0, o.idx.as<uint32_t>(), o.eid.as<uint32_t>(),
+                           o.bits, o.packed.as<uint32_t>(), o.peid.as<uint32_t>());
+        GT_HIP(hipGetLastError());
+    }
+    GT_HIP(hipStreamSynchronize(g->stream));
     o.scheduled = true;
-    (void)g;
     return GT_OK;
 }
 

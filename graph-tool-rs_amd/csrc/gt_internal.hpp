@@ -62,10 +62,11 @@ struct DevBuf {
 
 // ---------------------------------------------------------------- schedule
 
-// One unit of SpMV work for a workgroup.  Either a run of whole rows
-// [row_begin, row_end) whose in-entries are [nnz_begin, nnz_end), or -- when
-// row_end == kSplit -- one chunk [nnz_begin, nnz_end) of a long row whose
-// partial sum goes to slot row_begin.
+// One unit of SpMV work for a wavefront (pull sweeps) or a workgroup (SpMM).
+// Either a run of whole rows [row_begin, row_end) whose in-entries are
+// [nnz_begin, nnz_end), or -- when row_end == kSplit -- one chunk
+// [nnz_begin, nnz_end) of a long row whose partial sum goes to slot
+// row_begin.
 struct Tile {
     uint32_t row_begin;
     uint32_t row_end;
@@ -74,11 +75,15 @@ struct Tile {
 };
 constexpr uint32_t kSplit = 0xffffffffu;
 
-// Rows longer than kTileNnz are split into chunks of kTileNnz entries.
-constexpr uint32_t kTileNnz = 2048;
-// At most this many rows per tile (2 per thread of a 256-thread workgroup).
-constexpr uint32_t kTileRows = 512;
+// Pull-sweep wave tiles: at most kWaveTileNnz entries (16 per lane) and
+// kWaveTileRows rows (4 per lane); longer rows are split into chunks of
+// kWaveTileNnz entries.  An entry is packed as (source << kWaveSlotBits) |
+// slot when vertex ids fit in 32 - kWaveSlotBits bits.
+constexpr uint32_t kWaveTileNnz = 1024;
+constexpr uint32_t kWaveTileRows = 256;
+constexpr uint32_t kWaveSlotBits = 10;
 constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;  // wavefronts of 64 lanes
 
 // A tiling of an orientation's rows for the pull kernels.
 struct Schedule {
@@ -101,11 +106,14 @@ struct Orient {
     DevBuf idx;   // uint32[m]: the other endpoint
     DevBuf eid;   // uint32[m]: edge id
     std::vector<uint32_t> h_off;  // host copy of off, for the schedule
-    // SpMV schedule over this orientation's rows.
+    // Pull-sweep schedule over this orientation's rows (wave tiles).
     DevBuf tiles;            // Tile[ntiles]
     DevBuf long_rows;        // uint32[nlong]: row of each split row
     DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
+    DevBuf packed;           // uint32[m]: entries in tile-sorted packed order
+    DevBuf peid;             // uint32[m]: edge id of each packed entry
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
+    uint32_t bits = 0;       // slot bits of `packed` (0: plain sources, row order)
     bool scheduled = false;
     Schedule mm;             // SpMM schedule (batched personalisation)
     // Rows too long for one thread's sequential degree sum.
@@ -147,7 +155,8 @@ struct Workspace {
     DevBuf pers;        // staged personalisation
     DevBuf lam[2];      // adjoint iterates
     DevBuf ax, ac;      // adjoint: staged x* and dL/dx (f64)
-    DevBuf wcsr;        // adjoint: weights in push order
+    DevBuf wcsr;        // adjoint: weights in the push orientation's packed order
+    DevBuf wrow;        // adjoint: weights in push row order (edge gradients)
     DevBuf gw, gp;      // adjoint: staged gradients for host outputs
     DevBuf bx[2];       // batched: [n][64] f32 iterates
     DevBuf bc[2];       // batched: [n][64] f32 contributions
@@ -188,7 +197,7 @@ struct gt_graph {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     gt::Workspace ws;
-    int grid = 0;           // workgroups of a sweep launch
+    int grid = 0;           // sweep workgroups (0: resident count), at most 2048
     bool profiling = false;
     // Summed event time and launch count of sweep launches, per kernel kind
     // (gt_sweep_kind); prof_kind is the kind being launched.
@@ -202,21 +211,35 @@ struct gt_graph {
 
 namespace gt {
 
-// Build (once) the SpMV schedule of an orientation from its host offsets.
+// Build (once) the pull-sweep schedule of an orientation: wave tiles from the
+// host offsets, then the packed entry stream and its edge ids on the device.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n);
 // Build (once) the SpMM schedule of an orientation.
 gt_status ensure_schedule_mm(Orient &o, uint32_t n);
 // Allocate a graph shell with its stream.
 gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
                     gt_graph **out);
-// Workgroups per sweep launch for a schedule of `ntiles` tiles.
-int sweep_grid(uint32_t ntiles);
 // Workgroups of `kernel` (256 threads) that are resident at once on the
-// device, capped at 2048: a persistent tile loop launched with this many
-// workgroups has no second wave of workgroups.
-int occupancy_cap(const void *kernel);
+// device, at most `per_cu` per compute unit and 2048 in all: a persistent
+// tile loop launched with this many workgroups has no second wave of
+// workgroups.
+int occupancy_cap(const void *kernel, int per_cu = 8);
+// Resident workgroups per compute unit for the wave-engine sweeps.  More
+// wavefronts than 12 per CU only add contention for the L2 the gathers live
+// in (R-MAT scale 22: 3 per CU is the fastest f32 and f64 sweep, 8 per CU is
+// 40% slower).
+constexpr int kSweepBlocksPerCU = 3;
 // min(cap, units), at least 1.
 int fit_grid(int cap, uint32_t units);
+// Most workgroups of a sweep launch; the partials buffer holds two f64
+// partials per sweep workgroup and two per fold workgroup.
+constexpr int kMaxSweepGrid = 2048;
+constexpr size_t kPartials = 4 * (size_t)kMaxSweepGrid;
+// Workgroups of a fold kernel over `nlong` split rows (one thread each).
+int fold_grid(uint32_t nlong);
+// Workgroups that give each wavefront at least one of `ntiles` wave tiles,
+// at most `cap`.
+int wave_grid(int cap, uint32_t ntiles);
 // The relabelled pull orientation (0: in-halves, 1: out-halves), built once.
 gt_status ensure_relabeled(gt_graph *g, int which, Relabeled **out);
 

@@ -8,13 +8,14 @@
 // caller's map (:378-382).
 //
 // Data layout per sweep (pull form, one launch per sweep):
-//   in_off  u32[n+1], in_idx u32[m]   CSC in storage order, streamed once
+//   in_off  u32[n+1]                  row offsets (LDS positions in a tile)
+//   packed  u32[m]                    tile-sorted (source << bits | slot), streamed once
 //   c_cur   V[n]                      gathered: c(s) = x(s)/deg(s), 0 for sinks
 //   x_cur   V[n] read, x_next V[n] written, c_next V[n] written
 //   out_off u32[n+1]                  unity out-degree (weighted: deg f64[n])
-// The tile engine (sweep_common.hpp) does the gather and row sums.  Rows longer
-// than a tile are split; their partial sums are combined, in chunk order, by
-// the last workgroup to finish, which also folds the per-workgroup delta and
+// The wave engine (sweep_common.hpp) does the gather and row sums.  Rows longer
+// than a tile are split; a second, small kernel (pr_fold) combines their
+// partial sums in chunk order, folds the per-workgroup delta and
 // dangling-mass partials in index order and decides convergence on the
 // device.  Launches after convergence exit at once.
 #include <cmath>
@@ -33,8 +34,9 @@ struct PrArgs {
     uint32_t ntiles;
     uint32_t nlong;
     const uint32_t *in_off;
-    const uint32_t *in_idx;
-    const void *w;            // weights in pull order (value type), weighted only
+    const uint32_t *packed;   // packed entry stream of the wave tiles
+    uint32_t bits;            // slot bits of `packed`
+    const void *w;            // weights in packed order (value type), weighted only
     const uint32_t *out_off;  // unity out-degree source
     const double *deg;        // weighted out-degree
     const uint8_t *active;    // masked views only
@@ -49,7 +51,7 @@ struct PrArgs {
     void *x_next;
     const void *c_cur;
     void *c_next;
-    double *partials;         // [2 * gridDim.x]
+    double *partials;         // kPartials: sweep partials, then fold partials
     double *long_part;        // [nslots]
     const uint32_t *long_rows;
     const uint32_t *long_slot;
@@ -88,34 +90,56 @@ struct PrFin {
     }
 };
 
+// The SpMV of one sweep: every row but the split ones, with per-workgroup
+// delta and dangling-mass partials.  No fence and no ticket: pr_fold, the
+// next kernel on the stream, completes the sweep.
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
 __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     if (a.st->done) return;
-    __shared__ TileLds<V> L;
-    __shared__ uint32_t flag;
-    double *red = L.red;
-    const int tid = threadIdx.x;
+    __shared__ WaveLds<V> L;
     PrFin<V, WEIGHTED, DPERS, MASK> fin(a, a.st->p_sink);
-    spmv_tiles<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.in_idx, static_cast<const V *>(a.c_cur),
-                            static_cast<const V *>(a.w), a.long_part, L, fin);
-
+    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits,
+                            static_cast<const V *>(a.c_cur), static_cast<const V *>(a.w),
+                            a.long_part, L, fin);
     double dsum = fin.dsum, ssum = fin.ssum;
-    block_sum2(dsum, ssum, red);
-    if (tid == 0) {
+    block_sum2(dsum, ssum, L.red);
+    if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = dsum;
         a.partials[gridDim.x + blockIdx.x] = ssum;
     }
-    if (!last_to_arrive(&a.st->counter, &flag)) return;
+}
 
-    // The last workgroup: finish the split rows, fold everything in index order.
-    PrFin<V, WEIGHTED, DPERS, MASK> lf(a, fin.p_sink);
-    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
-        lf.dsum += a.partials[k];
-        lf.ssum += a.partials[gridDim.x + k];
-    }
-    for (uint32_t k = tid; k < a.nlong; k += kBlock)
+// Completes the sweep of the preceding pr_sweep launch (`sgrid` workgroups):
+// the split rows, one thread each, then -- in the last workgroup to arrive --
+// the fold of the sweep's and this kernel's partials in index order and the
+// convergence decision.
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid) {
+    if (a.st->done) return;
+    __shared__ double red[8];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    PrFin<V, WEIGHTED, DPERS, MASK> lf(a, a.st->p_sink);
+    for (uint32_t k = blockIdx.x * kBlock + tid; k < a.nlong; k += gridDim.x * kBlock)
         lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
     double d = lf.dsum, s = lf.ssum;
+    block_sum2(d, s, red);
+    double *fp = a.partials + 2 * kMaxSweepGrid;
+    if (tid == 0) {
+        fp[blockIdx.x] = d;
+        fp[gridDim.x + blockIdx.x] = s;
+    }
+    if (!last_to_arrive(&a.st->counter, &flag)) return;
+    d = 0.0;
+    s = 0.0;
+    for (uint32_t k = tid; k < sgrid; k += kBlock) {
+        d += a.partials[k];
+        s += a.partials[sgrid + k];
+    }
+    for (uint32_t k = tid; k < gridDim.x; k += kBlock) {
+        d += fp[k];
+        s += fp[gridDim.x + k];
+    }
     block_sum2(d, s, red);
     if (tid == 0) {
         SweepState *st = a.st;
@@ -127,7 +151,6 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
         st->launches += 1;
         st->done = (!(d >= thr) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
         st->counter = 0;
-        __threadfence();
     }
 }
 
@@ -221,11 +244,20 @@ struct PrPlan {
     bool masked = false;
 };
 
+// The sweep kernel; returns its workgroup count.  `grid` > 0 overrides the
+// resident-workgroup count (at most kMaxSweepGrid).
 template <typename V, bool W, bool P, bool M>
-void launch_sweep(const PrArgs &a, int /*grid*/, hipStream_t s) {
-    static const int cap = occupancy_cap((const void *)pr_sweep<V, W, P, M>);
-    hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(fit_grid(cap, a.ntiles)), dim3(kBlock), 0, s,
-                       a);
+int launch_sweep(const PrArgs &a, int grid, hipStream_t s) {
+    static const int cap = occupancy_cap((const void *)pr_sweep<V, W, P, M>, kSweepBlocksPerCU);
+    const int sg = wave_grid(grid > 0 ? grid : cap, a.ntiles);
+    hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(sg), dim3(kBlock), 0, s, a);
+    return sg;
+}
+template <typename V, bool W, bool P, bool M>
+int launch_fold(const PrArgs &a, int sg, hipStream_t s) {
+    hipLaunchKernelGGL((pr_fold<V, W, P, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s, a,
+                       (uint32_t)sg);
+    return sg;
 }
 template <typename V, bool W, bool P, bool M>
 void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64_t iter0,
@@ -237,20 +269,23 @@ void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64
                        iter0);
 }
 
-template <typename V>
-void dispatch_sweep(const PrPlan &p, const PrArgs &a, int grid, hipStream_t s) {
-    const int key = (p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0);
-    switch (key) {
-        case 0: launch_sweep<V, false, false, false>(a, grid, s); break;
-        case 1: launch_sweep<V, false, false, true>(a, grid, s); break;
-        case 2: launch_sweep<V, false, true, false>(a, grid, s); break;
-        case 3: launch_sweep<V, false, true, true>(a, grid, s); break;
-        case 4: launch_sweep<V, true, false, false>(a, grid, s); break;
-        case 5: launch_sweep<V, true, false, true>(a, grid, s); break;
-        case 6: launch_sweep<V, true, true, false>(a, grid, s); break;
-        default: launch_sweep<V, true, true, true>(a, grid, s); break;
+#define GT_PR_DISPATCH(FN)                                                      \
+    template <typename V>                                                       \
+    int dispatch_##FN(const PrPlan &p, const PrArgs &a, int grid, hipStream_t s) { \
+        switch ((p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0)) { \
+            case 0: return launch_##FN<V, false, false, false>(a, grid, s);      \
+            case 1: return launch_##FN<V, false, false, true>(a, grid, s);       \
+            case 2: return launch_##FN<V, false, true, false>(a, grid, s);       \
+            case 3: return launch_##FN<V, false, true, true>(a, grid, s);        \
+            case 4: return launch_##FN<V, true, false, false>(a, grid, s);       \
+            case 5: return launch_##FN<V, true, false, true>(a, grid, s);        \
+            case 6: return launch_##FN<V, true, true, false>(a, grid, s);        \
+            default: return launch_##FN<V, true, true, true>(a, grid, s);        \
+        }                                                                       \
     }
-}
+GT_PR_DISPATCH(sweep)
+GT_PR_DISPATCH(fold)
+#undef GT_PR_DISPATCH
 
 template <typename V>
 void dispatch_init(const PrPlan &p, const PrArgs &a, double *rank, uint64_t had, uint32_t n,
@@ -400,11 +435,10 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     }
     if (n == 0) return GT_OK;
     GT_TRY(ensure_schedule(g, *S.pull, n));
-    g->grid = sweep_grid(S.pull->ntiles);
     GT_TRY(prof_prepare(g));
 
     Workspace &ws = g->ws;
-    GT_TRY(ws.partials.reserve(sizeof(double) * 2 * 2048));
+    GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, S.pull->nslots)));
     GT_TRY(ws.state.reserve(sizeof(SweepState)));
     GT_HIP(hipMemsetAsync(ws.state.ptr, 0, sizeof(SweepState), s));
@@ -424,7 +458,8 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     a.ntiles = S.pull->ntiles;
     a.nlong = S.pull->nlong;
     a.in_off = S.pull->off.as<uint32_t>();
-    a.in_idx = S.pull->idx.as<uint32_t>();
+    a.packed = S.pull->packed.as<uint32_t>();
+    a.bits = S.pull->bits;
     a.damping = damping;
     a.eps_abs = params->epsilon;
     a.eps_rel = 0.0;
@@ -450,16 +485,18 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     dispatch_init<V>(S.plan, a, S.rank64, had, g->n, iter0, s);
     GT_HIP(hipGetLastError());
     uint64_t launched = 0;
-    auto launch = [&](uint64_t k) {
+    PrArgs b = a;
+    int sg = 0;
+    auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
-        PrArgs b = a;
         b.x_cur = x[par];
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        dispatch_sweep<V>(S.plan, b, g->grid, s);
+        sg = dispatch_sweep<V>(S.plan, b, g->grid, s);
     };
-    GT_TRY(drive(g, a.st, eps_pred, cap, launch, &launched));
+    auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };
+    GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched));
     *iter_out = g->h_state->iter;
     return GT_OK;
 }
@@ -505,7 +542,7 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
     GT_TRY(ws.c[0].reserve(vb));
     GT_TRY(ws.c[1].reserve(vb));
     if (S.plan.weighted) {
-        GT_TRY(stage_csc_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
+        GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
         S.base.w = ws.wcsc64.ptr;
     }
     // Without relabelling the caller's map is the first iterate itself.
@@ -566,7 +603,7 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     float *c32[2] = {ws.c32[0].as<float>(), ws.c32[1].as<float>()};
     PrArgs a32 = S.base;
     if (S.plan.weighted) {
-        GT_TRY(stage_csc_weights<float>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc));
+        GT_TRY(stage_pull_weights<float>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc));
         a32.w = ws.wcsc.ptr;
     }
     a32.eps_rel = std::ldexp(1.0, -22);
@@ -586,7 +623,7 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         GT_TRY(ws.c[1].reserve(sizeof(double) * n));
         PrArgs a64 = S.base;
         if (S.plan.weighted) {
-            GT_TRY(stage_csc_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
+            GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
             a64.w = ws.wcsc64.ptr;
         }
         uint64_t cap64 = refine;

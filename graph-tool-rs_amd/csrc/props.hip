@@ -132,11 +132,13 @@ static gt_status launch_deg(gt_graph *g, Orient &push, const void *src, int kind
     return GT_OK;
 }
 
+// out[j] = w(eid[j]) for the `m` entries of an orientation.
 template <typename S, typename V>
-static gt_status launch_permute(gt_graph *g, Orient &pull, const void *src, int kind, double value,
-                                V *out) {
-    hipLaunchKernelGGL((permute_w<S, V>), dim3(grid_for(g->m)), dim3(kBlock), 0, g->stream,
-                       pull.eid.as<uint32_t>(), src, kind, value, (uint64_t)g->m, out);
+static gt_status launch_permute(gt_graph *g, const uint32_t *eid, uint64_t m, const void *src,
+                                int kind, double value, V *out) {
+    if (m == 0) return GT_OK;
+    hipLaunchKernelGGL((permute_w<S, V>), dim3(grid_for(m)), dim3(kBlock), 0, g->stream, eid, src,
+                       kind, value, m, out);
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -153,14 +155,32 @@ template <typename V>
 gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
                             int dtype, DevBuf &out) {
     GT_TRY(out.reserve(sizeof(V) * std::max<uint64_t>(1, g->m)));
+    const uint32_t *eid = pull.eid.as<uint32_t>();
+    const uint64_t m = pull.h_off.empty() ? 0 : pull.h_off.back();
     if (dtype == GT_F32)
-        return launch_permute<float, V>(g, pull, src, w->kind, w->value, out.as<V>());
-    return launch_permute<double, V>(g, pull, src, w->kind, w->value, out.as<V>());
+        return launch_permute<float, V>(g, eid, m, src, w->kind, w->value, out.as<V>());
+    return launch_permute<double, V>(g, eid, m, src, w->kind, w->value, out.as<V>());
 }
 template gt_status stage_csc_weights<double>(gt_graph *, const gt_prop *, Orient &, const void *,
                                              int, DevBuf &);
 template gt_status stage_csc_weights<float>(gt_graph *, const gt_prop *, Orient &, const void *,
                                             int, DevBuf &);
+
+template <typename V>
+gt_status stage_pull_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
+                             int dtype, DevBuf &out) {
+    if (!pull.scheduled) return fail(GT_ERR_INVALID_ARG, "internal: pull weights before schedule");
+    GT_TRY(out.reserve(sizeof(V) * std::max<uint64_t>(1, g->m)));
+    const uint32_t *eid = pull.peid.as<uint32_t>();
+    const uint64_t m = pull.h_off.empty() ? 0 : pull.h_off.back();
+    if (dtype == GT_F32)
+        return launch_permute<float, V>(g, eid, m, src, w->kind, w->value, out.as<V>());
+    return launch_permute<double, V>(g, eid, m, src, w->kind, w->value, out.as<V>());
+}
+template gt_status stage_pull_weights<double>(gt_graph *, const gt_prop *, Orient &, const void *,
+                                              int, DevBuf &);
+template gt_status stage_pull_weights<float>(gt_graph *, const gt_prop *, Orient &, const void *,
+                                             int, DevBuf &);
 
 gt_status stage_vertex_prop(gt_graph *g, const gt_prop *pm, DevBuf &buf, const double **dev,
                             double *scalar, bool *dense, double default_value) {

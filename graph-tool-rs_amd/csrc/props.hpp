@@ -17,6 +17,11 @@ gt_status stage_degree(gt_graph *g, const gt_prop *w, Orient &push, const void *
 template <typename V>
 gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
                             int dtype, DevBuf &out);
+// Weights in the packed order of `pull`'s wave tiles (ensure_schedule first),
+// converted to V.
+template <typename V>
+gt_status stage_pull_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
+                             int dtype, DevBuf &out);
 
 // A vertex map as a scalar (UNITY / CONSTANT / absent -> default_value) or
 // an f64 device array.

@@ -1,24 +1,27 @@
-// The pull-SpMV tile engine shared by the PageRank, eigenvector and adjoint
-// sweeps.
+// The pull-SpMV engine shared by the PageRank, eigenvector and adjoint sweeps.
 //
-// A workgroup of 256 threads walks tiles blockIdx.x, blockIdx.x + gridDim.x,
-// ...  For each tile:
-//   phase 1  the tile's <= kTileNnz in-entries are streamed (coalesced u32
-//            index loads, eight per thread in flight) and the gathered source
-//            values -- times the pull-ordered weight when weighted -- land in
-//            LDS;
-//   phase 2  row sums out of LDS into an LDS row buffer: one thread per row
-//            when the tile's rows average fewer than 16 entries, one
-//            wavefront per row with a shuffle-tree reduction otherwise; a
-//            split-row chunk is reduced by the whole workgroup into its
-//            partial slot instead;
-//   phase 3  `fin(row, sum)` for every row of the tile, one thread per row,
-//            so the rows' global reads and writes go out together and
-//            coalesced.
-// Three barriers per tile: after phase 1; after the short rows of phase 2
-// (the long-row queue is complete); after the long rows (the LDS tile is free
-// for the next tile's phase 1, and the row buffer is not written again before
-// the next tile's first barrier).
+// Work is cut into wave tiles of <= kWaveTileNnz in-entries and <= kWaveTileRows
+// rows; a row longer than kWaveTileNnz is cut into kWaveTileNnz-entry chunks
+// whose partial sums go to slots of long_part.  Inside a tile the entries are
+// stored sorted by source, each packed as (source << bits) | slot, where slot
+// is the entry's position in row order: the 64 lanes of every gather
+// instruction read ascending addresses, so lanes that touch the same cache
+// line share one request, and the gathered value is dropped into LDS at
+// `slot`, which restores row order for the row sums.  The packing costs no
+// bytes over a plain 4-byte index stream.  With bits == 0 (vertex ids too wide
+// to pack) the entries are the plain sources in row order and an entry's slot
+// is its position.
+//
+// Every wavefront walks its own tiles (wave w of the grid takes tiles w,
+// w + W, ...) with no workgroup barrier:
+//   phase 1  kWaveTileNnz / 64 packed words per lane in flight (coalesced),
+//            as many gathers, each value times the packed-order weight when
+//            weighted, scattered into the wavefront's LDS tile;
+//   phase 2  one lane per row when the row has fewer than kShortRow entries,
+//            the whole wavefront (shuffle tree) for longer rows;
+//   phase 3  `fin(row, sum)`, lane q for rows q, q + 64, ...: the rows'
+//            global reads and writes go out coalesced.
+// A split chunk is summed straight from registers into its slot.
 #pragma once
 
 #include "device_common.hpp"
@@ -26,101 +29,107 @@
 
 namespace gt {
 
-template <typename V, bool WEIGHTED>
-__device__ __forceinline__ void gather_tile(const uint32_t *__restrict__ idx,
-                                            const V *__restrict__ src,
-                                            const V *__restrict__ w, uint32_t nb,
-                                            uint32_t cnt, V *lds) {
-    constexpr int K = kTileNnz / kBlock;
-    uint32_t s[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t j = threadIdx.x + k * kBlock;
-        s[k] = j < cnt ? __builtin_nontemporal_load(idx + nb + j) : 0u;
-    }
-    V v[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t j = threadIdx.x + k * kBlock;
-        v[k] = j < cnt ? src[s[k]] : V(0);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t j = threadIdx.x + k * kBlock;
-        if (j < cnt) {
-            V x = v[k];
-            if (WEIGHTED) x *= __builtin_nontemporal_load(w + nb + j);
-            lds[j] = x;
-        }
-    }
-}
-
-// Rows of at least this many entries are summed by a wavefront.
+static_assert(kWavesPerBlock * kWave == kBlock, "workgroups are whole wavefronts");
+// Rows of at least this many entries are summed by the whole wavefront.
 constexpr uint32_t kShortRow = 32;
 
-// LDS a kernel using spmv_tiles must provide.  nlongq must be zero when the
-// tile loop starts (spmv_tiles resets it after each tile).
 template <typename V>
-struct TileLds {
-    V vals[kTileNnz];
-    double rowsum[kTileRows];
+struct WaveLds {
+    V vals[kWavesPerBlock][kWaveTileNnz];
+    V rowsum[kWavesPerBlock][kWaveTileRows];
     double red[8];
-    uint16_t longq[kTileNnz / kShortRow];
-    uint32_t nlongq;
 };
 
+// Orders this wavefront's LDS accesses: writes before the call are visible to
+// its lanes' reads after it.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename V, bool WEIGHTED, typename Fin>
-__device__ __forceinline__ void spmv_tiles(const Tile *__restrict__ tiles, uint32_t ntiles,
+__device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
-                                           const uint32_t *__restrict__ idx,
+                                           const uint32_t *__restrict__ packed, uint32_t bits,
                                            const V *__restrict__ src, const V *__restrict__ w,
-                                           double *__restrict__ long_part, TileLds<V> &L,
+                                           double *__restrict__ long_part, WaveLds<V> &L,
                                            Fin &fin) {
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wid = tid / kWave;
-    if (tid == 0) L.nlongq = 0;  // published by the first tile's barrier
-    for (uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    constexpr int K = kWaveTileNnz / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t slot_mask = (1u << bits) - 1u;
+    V *vals = L.vals[wid];
+    V *rowsum = L.rowsum[wid];
+    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+    for (uint32_t ti = blockIdx.x * kWavesPerBlock + wid; ti < ntiles; ti += nwaves) {
         const Tile t = tiles[ti];
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
-        gather_tile<V, WEIGHTED>(idx, src, w, nb, cnt, L.vals);
-        __syncthreads();
-        if (t.row_end == kSplit) {
-            double v = 0.0;
-            for (uint32_t k = tid; k < cnt; k += kBlock) v += static_cast<double>(L.vals[k]);
-            v = block_sum(v, L.red);  // ends with a barrier: LDS is free again
-            if (tid == 0) long_part[t.row_begin] = v;
-            continue;
+        uint32_t p[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t j = lane + k * kWave;
+            p[k] = j < cnt ? __builtin_nontemporal_load(packed + nb + j) : 0u;
         }
-        // Rows shorter than kShortRow: one thread each.  Longer rows (at most
-        // kTileNnz / kShortRow of them) are queued and summed by a wavefront
-        // each, so no thread walks a long row alone whatever the tile's mix.
-        const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
-        for (uint32_t q = tid; q < rows; q += kBlock) {
-            const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
-            if (hi - lo < kShortRow) {
-                V acc = V(0);
-                for (uint32_t k = lo; k < hi; ++k) acc += L.vals[k];
-                L.rowsum[q] = static_cast<double>(acc);
-            } else {
-                L.longq[atomicAdd(&L.nlongq, 1u)] = (uint16_t)q;
+        V v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t j = lane + k * kWave;
+            v[k] = j < cnt ? src[p[k] >> bits] : V(0);
+        }
+        if (WEIGHTED) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t j = lane + k * kWave;
+                if (j < cnt) v[k] *= __builtin_nontemporal_load(w + nb + j);
             }
         }
-        __syncthreads();
-        const uint32_t nq = L.nlongq;
-        for (uint32_t i = wid; i < nq; i += kBlock / kWave) {
-            const uint32_t q = L.longq[i];
-            const uint32_t lo = off[rb + q] - nb, hi = off[rb + q + 1] - nb;
+        if (t.row_end == kSplit) {
             V acc = V(0);
-            for (uint32_t k = lo + lane; k < hi; k += kWave) acc += L.vals[k];
-            acc = wave_sum(acc);
-            if (lane == 0) L.rowsum[q] = static_cast<double>(acc);
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc += v[k];
+            const double tot = wave_sum(static_cast<double>(acc));
+            if (lane == 0) long_part[t.row_begin] = tot;
+            continue;
         }
-        __syncthreads();
-        if (tid == 0) L.nlongq = 0;
-        for (uint32_t q = tid; q < rows; q += kBlock) fin(rb + q, L.rowsum[q]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t j = lane + k * kWave;
+            if (j < cnt) vals[bits ? (p[k] & slot_mask) : j] = v[k];
+        }
+        wave_sync();
+        const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
+        for (uint32_t q0 = 0; q0 < rows; q0 += kWave) {
+            const uint32_t q = q0 + lane;
+            uint32_t lo = 0, hi = 0;
+            bool longrow = false;
+            if (q < rows) {
+                lo = off[rb + q] - nb;
+                hi = off[rb + q + 1] - nb;
+                if (hi - lo < kShortRow) {
+                    V acc = V(0);
+                    for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
+                    rowsum[q] = acc;
+                } else {
+                    longrow = true;
+                }
+            }
+            uint64_t lmask = __ballot(longrow);
+            while (lmask) {
+                const int r = __builtin_ctzll(lmask);
+                lmask &= lmask - 1;
+                const uint32_t rlo = __shfl(lo, r, kWave), rhi = __shfl(hi, r, kWave);
+                V acc = V(0);
+                for (uint32_t k = rlo + lane; k < rhi; k += kWave) acc += vals[k];
+                acc = wave_sum(acc);
+                if (lane == 0) rowsum[q0 + r] = acc;
+            }
+        }
+        wave_sync();
+        for (uint32_t q = lane; q < rows; q += kWave)
+            fin(rb + q, static_cast<double>(rowsum[q]));
+        wave_sync();  // the LDS tile is reused by this wavefront's next tile
     }
-    __syncthreads();
 }
 
 // Sum of a split row's partial slots, in chunk order.
