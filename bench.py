@@ -85,12 +85,219 @@ def ev_bytes(n: int, ms: int) -> int:
     return 4 * ms + 4 * (n + 1) + 24 * n
 
 
+def _upload_rmat(gt_hip, scale: int, edgefactor: int, gid: int):
+    n = 1 << scale
+    m = edgefactor * n
+    t0 = time.perf_counter()
+    src, tgt = gt_hip.gen_rmat(scale, m, 0.57, 0.19, 0.19, SEED)
+    adj = gt_hip.host_adjacency(n, src, tgt)
+    dev = gt_hip.DeviceGraph.upload(gid, n, m, *adj)
+    log(f"[bench] R-MAT scale {scale}: n={n} m={m} built and uploaded in "
+        f"{time.perf_counter()-t0:.1f}s")
+    return n, m, src, tgt, dev
+
+
+def _timed_solves(torch, dev, kind: int, steps: int, warmup: int, solve):
+    for _ in range(warmup):
+        solve()
+    dev.set_profiling(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    outs = [solve() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    ms_k, launches, _ = dev.sweep_stats(kind)
+    dev.set_profiling(False)
+    return dt, outs, ms_k, launches
+
+
+def bench_batched(args, torch, gt_hip, _lib) -> int:
+    """k = 64 personalised PageRank vectors as one SpMM per sweep.
+
+    R-MAT scale 20 (1,048,576 vertices, 16,777,216 edges), FP32 weights
+    exp(N(0,1)), 64 distinct uniform seed vertices, damping 0.85, epsilon 1e-6
+    per column.  GTEPS counts m per active column per sweep.  Algorithmic bytes
+    per sweep: B = 8m + 4(n+1) + 16nk + 4n (index + weight stream, offsets,
+    state gather / read / write and contribution write at [n][64] FP32,
+    out-degree)."""
+    scale = 20 if args.scale is None else args.scale
+    k = 64
+    n, m, src, tgt, dev = _upload_rmat(gt_hip, scale, args.edgefactor, 1)
+    rng = np.random.default_rng(SEED & 0xFFFFFFFF)
+    w_host = np.exp(rng.standard_normal(m)).astype(np.float32)
+    seeds = rng.choice(n, k, replace=False).astype(np.uint32)
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    copy = copy_calibration(torch)
+    w = torch.from_numpy(w_host).cuda()
+    X = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    wprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F32, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(w.data_ptr()), m, 1)
+    eps = 1e-6
+    params = _lib.GtPowerIteration(eps, 0, _lib.GtPlan(n, -(-n // 4096)))
+    view = _lib.GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0)
+    iters = np.zeros(k, np.uint64)
+
+    def solve():
+        _lib.call("gt_pagerank_batched", dev.handle, C.byref(view), C.byref(wprop),
+                  seeds.ctypes.data_as(C.POINTER(C.c_uint32)), k, 0.85, C.byref(params),
+                  C.c_void_p(X.data_ptr()), k, _lib.GT_DEVICE,
+                  iters.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return iters.copy()
+
+    dt, outs, ms_k, launches = _timed_solves(torch, dev, _lib.GT_SWEEP_BATCHED, args.steps,
+                                             args.warmup, solve)
+    it = outs[-1].astype(np.int64)
+    col_sweeps = int(it.sum())
+    sweeps = int(it.max())
+    gteps = m * col_sweeps * args.steps / dt / 1e9
+    B = 8 * m + 4 * (n + 1) + 16 * n * k + 4 * n
+    avg = ms_k / max(launches, 1) * 1e-3
+    ach = B * sweeps * args.steps / (ms_k * 1e-3) / 1e9 if launches else 0.0
+    log(f"[bench] batched: {dt/args.steps*1e3:.2f} ms/solve, sweeps {sweeps} (columns "
+        f"{it.min()}..{it.max()}), {launches} launches ({avg*1e6:.1f} us avg)")
+    result = {
+        "metric": "Batched personalised PageRank GTEPS (k=64 SpMM, FP32 state, FP64 residuals)",
+        "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 state/weights/SpMM, f64 teleport/dangling/L1", "data": "synthetic",
+        "config": {"workload": f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}",
+                   "vertices": n, "edges": m, "k": k, "weights": "f32 exp(N(0,1))",
+                   "seeds": "64 distinct uniform", "damping": 0.85, "epsilon": eps},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "ppr_sweep<true> (batched SpMM sweep)", "bytes_per_sweep": B,
+                     "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
+        "column_sweeps": {"min": int(it.min()), "max": int(it.max()), "sum": col_sweeps},
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.Adj(n, src, tgt)
+        w64 = w_host.astype(np.float64)
+        Xh = X.cpu().numpy()
+        t0 = time.perf_counter()
+        errs, sw_ok, done = [], 0, 0
+        for c in range(k):
+            p = np.zeros(n)
+            p[seeds[c]] = 1.0
+            want, ito = o.pagerank(weight=w64, pers=p, epsilon=eps, max_iter=0, grain=4096,
+                                   threads=threads)
+            errs.append(float(np.abs(Xh[:, c].astype(np.float64) - want).sum()))
+            sw_ok += int(abs(ito - int(it[c])) <= 1)
+            done += 1
+            if time.perf_counter() - t0 >= 10.0:
+                break
+        dtc = time.perf_counter() - t0
+        cpu_sweeps = sum(int(it[c]) for c in range(done))
+        result["cpu_baseline"] = {
+            "value": round(m * cpu_sweeps / dtc / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{done} of the 64 columns solved one at a time by the oracle's "
+                      f"pagerank_with (one-hot pers, Plan grain 4096), {dtc:.1f}s"}
+        result["accuracy_vs_fp64_oracle"] = {
+            "columns_checked": done, "max_l1": max(errs),
+            "sweeps_within_1": f"{sw_ok}/{done}"}
+    print(json.dumps(result), flush=True)
+    dev.close()
+    return 0
+
+
+def bench_adjoint(args, torch, gt_hip, _lib) -> int:
+    """Weighted PageRank forward (FP32 form) plus its adjoint.
+
+    R-MAT scale 22 (unless --scale), FP32 weights 1 - 0.9 U[0,1) in (0.1, 1],
+    loss L = c . x with c ~ N(0,1); gradients for every edge weight, the
+    personalisation and the damping.  One step = one forward solve plus one
+    backward (transposed solve and the fused gradient pass); the value is the
+    backward / forward time ratio.  Forward algorithmic bytes per FP32 sweep:
+    8m + 4(n+1) + 20n; adjoint sweep the same on the CSR; gradient pass
+    12m + 16n once."""
+    scale = 22 if args.scale is None else args.scale
+    n, m, src, tgt, dev = _upload_rmat(gt_hip, scale, args.edgefactor, 1)
+    rng = np.random.default_rng((SEED >> 7) & 0xFFFFFFFF)
+    w_host = (1.0 - 0.9 * rng.random(m)).astype(np.float32)
+    c_host = rng.standard_normal(n)
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    w = torch.from_numpy(w_host).cuda()
+    cvec = torch.from_numpy(c_host).cuda()
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    gw = torch.empty(m, dtype=torch.float32, device="cuda")
+    gp = torch.empty(n, dtype=torch.float64, device="cuda")
+    gd = C.c_double()
+    wprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F32, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(w.data_ptr()), m, 1)
+    xprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F64, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(x.data_ptr()), n, 1)
+    cprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F64, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(cvec.data_ptr()), n, 1)
+    eps = 1e-9
+    params = _lib.GtPowerIteration(eps, 0, _lib.GtPlan(n, -(-n // 4096)))
+    view = _lib.GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0)
+    out = _lib.GtVertexOut(C.c_void_p(x.data_ptr()), n, 0, _lib.GT_DEVICE, 0, 1)
+    iters = C.c_uint64()
+    biters = C.c_uint64()
+
+    def forward():
+        out.had = 0
+        _lib.call("gt_pagerank_mixed", dev.handle, C.byref(view), C.byref(wprop), None, 0.85,
+                  C.byref(params), C.byref(out), C.byref(iters))
+        return int(iters.value)
+
+    def backward():
+        _lib.call("gt_pagerank_backward", dev.handle, C.byref(view), C.byref(wprop), None, 0.85,
+                  C.byref(params), C.byref(xprop), C.byref(cprop), _lib.GT_F32,
+                  C.c_void_p(gw.data_ptr()), _lib.GT_F32, _lib.GT_DEVICE,
+                  C.c_void_p(gp.data_ptr()), C.byref(gd),
+                  C.byref(biters))
+        return int(biters.value)
+
+    dtf, fo, ms_f, la_f = _timed_solves(torch, dev, _lib.GT_SWEEP_PAGERANK_F32, args.steps,
+                                        args.warmup, forward)
+    dtb, bo, ms_b, la_b = _timed_solves(torch, dev, _lib.GT_SWEEP_ADJOINT, args.steps,
+                                        args.warmup, backward)
+    ratio = dtb / dtf
+    Bf = 8 * m + 4 * (n + 1) + 20 * n
+    ach_b = Bf * bo[-1] * args.steps / (ms_b * 1e-3) / 1e9 if la_b else 0.0
+    log(f"[bench] adjoint: forward {dtf/args.steps*1e3:.2f} ms ({fo[-1]} sweeps), backward "
+        f"{dtb/args.steps*1e3:.2f} ms ({bo[-1]} sweeps), ratio {ratio:.3f}")
+    result = {
+        "metric": "Adjoint cost / forward solve, weighted PageRank (FP32 form)",
+        "value": round(ratio, 4), "unit": "x forward", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round((dtf + dtb) / args.steps * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 values/weights/SpMV, f64 residuals and sums", "data": "synthetic",
+        "config": {"workload": f"adjoint_rmat{scale}_ef{args.edgefactor}", "vertices": n,
+                   "edges": m, "weights": "f32 1 - 0.9 U[0,1)", "loss": "c . x, c ~ N(0,1)",
+                   "damping": 0.85, "epsilon": eps},
+        "forward_ms": round(dtf / args.steps * 1e3, 4), "forward_sweeps": fo[-1],
+        "backward_ms": round(dtb / args.steps * 1e3, 4), "backward_sweeps": bo[-1],
+        "roofline": {"bound": "hbm", "achieved": round(ach_b, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "adj_sweep<float, true, false> (adjoint sweep)",
+                     "bytes_per_sweep": Bf,
+                     "avg_launch_us": round(ms_b / max(la_b, 1) * 1e3, 2)},
+        "grad_damping": gd.value,
+    }
+    print(json.dumps(result), flush=True)
+    dev.close()
+    return 0
+
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--workload", choices=("pagerank", "batched", "adjoint"),
+                    default="pagerank",
+                    help="pagerank: R-MAT 22 FP32/FP64 solves (the headline); batched: k=64 "
+                         "personalised SpMM on weighted R-MAT 20; adjoint: weighted R-MAT 22 "
+                         "forward + gradient")
+    ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--epsilon", type=float, default=1e-9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle")
@@ -110,6 +317,12 @@ def main() -> int:
     if not torch.cuda.is_available() or gt_hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.workload == "batched":
+        return bench_batched(args, torch, gt_hip, _lib)
+    if args.workload == "adjoint":
+        return bench_adjoint(args, torch, gt_hip, _lib)
+    if args.scale is None:
+        args.scale = 22
 
     n = 1 << args.scale
     m = args.edgefactor * n

@@ -8,12 +8,16 @@
 //   dL/dd   = lambda . (x - p) / d
 //
 // The transposed system is solved by the same power iteration, pulling over
-// the opposite orientation: row s of the out-halves gathers lambda at its
+// the opposite orientation, until its L1 change drops below
+// epsilon * ||dL/dx||_1 (the system is linear in dL/dx, so the tolerance
+// scales with it; FP32 runs stop no later than 2^-22 * ||dL/dx||_1, below
+// which an FP32 iterate's change is rounding noise): row s of the out-halves gathers lambda at its
 // targets (times the out-ordered weight), so the wave engine and the fold
-// kernel of the forward sweep are reused unchanged.  No
-// iterate of the forward solve is stored: only x* is read.  The fused edge
-// kernel then computes mu_s and every out-edge's gradient of row s in one
-// wavefront, writing by edge id.
+// kernel of the forward sweep are reused unchanged.  No iterate of the
+// forward solve is stored: only x* is read.  mu = M^T lambda comes from one
+// more pass of the same engine; the edge gradients are then one coalesced
+// pass over edge ids, reading each edge's endpoints from the graph's edge
+// list.
 #include <cmath>
 #include <type_traits>
 
@@ -38,7 +42,8 @@ struct AdjArgs {
     const double *pers;        // dense personalisation
     double pval;
     double damping;
-    double eps;
+    double eps;                // stop when delta < max(eps, eps_rel) * ||dL/dx||_1
+    double eps_rel;
     uint64_t max_iter;
     const void *l_cur;
     void *l_next;
@@ -125,7 +130,8 @@ __global__ __launch_bounds__(kBlock) void adj_fold(const AdjArgs a, uint32_t sgr
         st->p_sink = p;
         st->iter = it;
         st->launches += 1;
-        st->done = (!(d >= a.eps) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
+        const double thr = fmax(a.eps, a.eps_rel) * (st->aux > 0.0 ? st->aux : 1.0);
+        st->done = (!(d >= thr) || (a.max_iter > 0 && it >= a.max_iter)) ? 1u : 0u;
         st->counter = 0;
     }
 }
@@ -141,48 +147,81 @@ __global__ __launch_bounds__(kBlock) void adj_init(const AdjArgs a, uint32_t n) 
         const V v = static_cast<V>(a.cvec[i]);
         l0[i] = v;
         p += (DPERS ? a.pers[i] : a.pval) * static_cast<double>(v);
+        z += fabs(a.cvec[i]);
     }
     block_sum2(p, z, red);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = p;
+        a.partials[gridDim.x + blockIdx.x] = z;
+    }
     if (!last_to_arrive(&a.st->counter, &flag)) return;
-    double t = 0.0;
-    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += a.partials[k];
-    t = block_sum(t, red);
+    double t = 0.0, c1 = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        t += a.partials[k];
+        c1 += a.partials[gridDim.x + k];
+    }
+    block_sum2(t, c1, red);
     if (threadIdx.x == 0) {
         SweepState *st = a.st;
         *st = SweepState{};
         st->p_sink = t;
-        __threadfence();
+        st->aux = c1;  // ||dL/dx||_1, the scale of the stopping rule
     }
 }
 
-// One wavefront per source row s: mu_s = (sum w_e lambda_t) / deg_s, then
-// dL/dw_e = d * x_s / deg_s * (lambda_t - mu_s) for each out-entry, written
-// at the entry's edge id.  Rows with deg_s == 0 write 0.
+// mu_s = (M^T lambda)_s = (sum over s's out-entries of w_e lambda_t) / deg_s
+// (0 when deg_s == 0): the same wave-engine pass as a transposed sweep,
+// written to mu instead of iterating.
+template <typename V, bool WEIGHTED>
+struct MuFin {
+    const AdjArgs &a;
+    double *mu;
+    __device__ MuFin(const AdjArgs &args, double *m) : a(args), mu(m) {}
+    __device__ __forceinline__ void operator()(uint32_t s, double acc) {
+        const double dg =
+            WEIGHTED ? a.deg[s] : static_cast<double>(a.row_off[s + 1] - a.row_off[s]);
+        mu[s] = dg != 0.0 ? acc / dg : 0.0;
+    }
+};
+
+template <typename V, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void adj_mu(const AdjArgs a, const V *lam, double *mu) {
+    __shared__ WaveLds<V> L;
+    MuFin<V, WEIGHTED> fin(a, mu);
+    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, lam,
+                            static_cast<const V *>(a.w), a.long_part, L, fin);
+}
+
+// The split rows of adj_mu, one thread each.
+template <typename V, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void adj_mu_long(const AdjArgs a, double *mu) {
+    MuFin<V, WEIGHTED> fin(a, mu);
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < a.nlong; k += gridDim.x * kBlock)
+        fin(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
+}
+
+// dL/dw_e = d * x_s / deg_s * (lambda_t - mu_s) for every edge id e < bound,
+// (s, t) = (row, entry) of e in the pull orientation's opposite (es / et:
+// the edge list, swapped for a reversed view).  Ids with no edge, and edges
+// whose deg_s is 0, get 0.  Reads and writes run in edge-id order.
 template <typename V, typename G, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void grad_edges(
-    const uint32_t *__restrict__ off, const uint32_t *__restrict__ idx,
-    const uint32_t *__restrict__ eid, const V *__restrict__ w, const double *__restrict__ deg,
-    const V *__restrict__ lam, const double *__restrict__ x, double damping, uint32_t n,
-    G *__restrict__ grad) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t waves = gridDim.x * (kBlock / kWave);
-    for (uint32_t s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < n; s += waves) {
-        const uint32_t lo = off[s], hi = off[s + 1];
-        if (lo == hi) continue;
-        const double dg = WEIGHTED ? deg[s] : (double)(hi - lo);
-        double acc = 0.0;
-        for (uint32_t j = lo + lane; j < hi; j += kWave)
-            acc += (WEIGHTED ? (double)w[j] : 1.0) * (double)lam[idx[j]];
-        acc = wave_sum(acc);
-        if (dg == 0.0) {
-            for (uint32_t j = lo + lane; j < hi; j += kWave) grad[eid[j]] = G(0);
-            continue;
+    const uint32_t *__restrict__ es, const uint32_t *__restrict__ et,
+    const double *__restrict__ deg, const uint32_t *__restrict__ row_off,
+    const double *__restrict__ mu, const V *__restrict__ lam, const double *__restrict__ x,
+    double damping, uint64_t bound, G *__restrict__ grad) {
+    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < bound;
+         e += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t sv = es[e];
+        G out = G(0);
+        if (sv != kNoEdge) {
+            const double dg =
+                WEIGHTED ? deg[sv] : static_cast<double>(row_off[sv + 1] - row_off[sv]);
+            if (dg != 0.0)
+                out = static_cast<G>(damping * x[sv] / dg *
+                                     (static_cast<double>(lam[et[e]]) - mu[sv]));
         }
-        const double mu = acc / dg;
-        const double cs = damping * x[s] / dg;
-        for (uint32_t j = lo + lane; j < hi; j += kWave)
-            grad[eid[j]] = static_cast<G>(cs * ((double)lam[idx[j]] - mu));
+        grad[e] = out;
     }
 }
 
@@ -249,12 +288,6 @@ static void adj_launch_fold(const AdjArgs &a, int sg, hipStream_t s) {
 static unsigned vgrid(uint64_t n) {
     uint64_t b = (n + kBlock - 1) / kBlock;
     if (b > 2048) b = 2048;
-    return b ? (unsigned)b : 1u;
-}
-// Workgroups for one wavefront per row.
-static unsigned row_wave_grid(uint32_t n) {
-    uint64_t b = ((uint64_t)n + 3) / 4;
-    if (b > 16384) b = 16384;
     return b ? (unsigned)b : 1u;
 }
 
@@ -332,15 +365,9 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         return GT_OK;
     }
     GT_TRY(ensure_schedule(g, *push, n));
-    // The sweeps read weights in packed order, the edge gradients in row order.
-    const V *wrow = nullptr;
     if (weighted) {
         GT_TRY(stage_pull_weights<V>(g, weight, *push, wsrc, wdt, ws.wcsr));
         a.w = ws.wcsr.ptr;
-        if (grad_w) {
-            GT_TRY(stage_csc_weights<V>(g, weight, *push, wsrc, wdt, ws.wrow));
-            wrow = ws.wrow.as<V>();
-        }
     }
     GT_TRY(prof_prepare(g));
     GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
@@ -358,6 +385,7 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     a.cvec = c;
     a.damping = damping;
     a.eps = params->epsilon;
+    a.eps_rel = std::is_same<V, float>::value ? std::ldexp(1.0, -22) : 0.0;
     a.max_iter = params->max_iter;
     a.partials = ws.partials.as<double>();
     a.long_part = ws.long_part.as<double>();
@@ -374,7 +402,7 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
             hipLaunchKernelGGL((adj_init<V, false>), dim3(vgrid(n)), dim3(kBlock), 0, s, b, n);
         GT_HIP(hipGetLastError());
     }
-    g->prof_kind = GT_SWEEP_OTHER;
+    g->prof_kind = GT_SWEEP_ADJOINT;
     AdjArgs b = a;
     int sg = 0;
     auto sweep = [&](uint64_t q) {
@@ -397,7 +425,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     const uint64_t sweeps = g->h_state->iter;
     const V *lf = lam[sweeps & 1];
 
-    // Edge gradients.
+    // Edge gradients: mu = M^T lambda (one more engine pass), then one pass
+    // over edge ids.
     if (grad_w) {
         void *gdst = grad_w;
         const size_t gb = (gw_dtype == GT_F64 ? 8 : 4) * (size_t)g->edge_bound;
@@ -405,26 +434,41 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
             GT_TRY(ws.gw.reserve(std::max<size_t>(gb, 8)));
             gdst = ws.gw.ptr;
         }
-        // Edges outside the view keep 0.
-        if (gb) GT_HIP(hipMemsetAsync(gdst, 0, gb, s));
-        const V *wv = wrow;
-        const dim3 gd(row_wave_grid(n)), bd(kBlock);
-        const uint32_t *po = push->off.as<uint32_t>(), *pi = push->idx.as<uint32_t>(),
-                       *pe = push->eid.as<uint32_t>();
+        GT_TRY(ensure_edge_list(g));
+        GT_TRY(ws.mu.reserve(sizeof(double) * n));
+        double *mu = ws.mu.as<double>();
+        AdjArgs b = a;
+        static const int cap_mu =
+            occupancy_cap((const void *)adj_mu<V, true>, kSweepBlocksPerCU);
+        const dim3 mg(wave_grid(cap_mu, a.ntiles)), lg(fold_grid(a.nlong)), bd(kBlock);
+        if (weighted) {
+            hipLaunchKernelGGL((adj_mu<V, true>), mg, bd, 0, s, b, lf, mu);
+            hipLaunchKernelGGL((adj_mu_long<V, true>), lg, bd, 0, s, b, mu);
+        } else {
+            hipLaunchKernelGGL((adj_mu<V, false>), mg, bd, 0, s, b, lf, mu);
+            hipLaunchKernelGGL((adj_mu_long<V, false>), lg, bd, 0, s, b, mu);
+        }
+        // Rows of the push orientation are sources for a directed view and
+        // targets for a reversed one.
+        const uint32_t *es = (reversed ? g->etgt : g->esrc).as<uint32_t>();
+        const uint32_t *et = (reversed ? g->esrc : g->etgt).as<uint32_t>();
+        const uint64_t eb = g->edge_bound;
+        const dim3 gd((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(16384, (eb + kBlock - 1) / kBlock)));
+        const uint32_t *ro = push->off.as<uint32_t>();
         if (gw_dtype == GT_F64) {
             if (weighted)
-                hipLaunchKernelGGL((grad_edges<V, double, true>), gd, bd, 0, s, po, pi, pe, wv,
-                                   a.deg, lf, x, damping, n, static_cast<double *>(gdst));
+                hipLaunchKernelGGL((grad_edges<V, double, true>), gd, bd, 0, s, es, et, a.deg, ro,
+                                   mu, lf, x, damping, eb, static_cast<double *>(gdst));
             else
-                hipLaunchKernelGGL((grad_edges<V, double, false>), gd, bd, 0, s, po, pi, pe, wv,
-                                   a.deg, lf, x, damping, n, static_cast<double *>(gdst));
+                hipLaunchKernelGGL((grad_edges<V, double, false>), gd, bd, 0, s, es, et, a.deg, ro,
+                                   mu, lf, x, damping, eb, static_cast<double *>(gdst));
         } else {
             if (weighted)
-                hipLaunchKernelGGL((grad_edges<V, float, true>), gd, bd, 0, s, po, pi, pe, wv,
-                                   a.deg, lf, x, damping, n, static_cast<float *>(gdst));
+                hipLaunchKernelGGL((grad_edges<V, float, true>), gd, bd, 0, s, es, et, a.deg, ro,
+                                   mu, lf, x, damping, eb, static_cast<float *>(gdst));
             else
-                hipLaunchKernelGGL((grad_edges<V, float, false>), gd, bd, 0, s, po, pi, pe, wv,
-                                   a.deg, lf, x, damping, n, static_cast<float *>(gdst));
+                hipLaunchKernelGGL((grad_edges<V, float, false>), gd, bd, 0, s, es, et, a.deg, ro,
+                                   mu, lf, x, damping, eb, static_cast<float *>(gdst));
         }
         GT_HIP(hipGetLastError());
         if (grad_loc == GT_HOST && gb)

@@ -10,9 +10,10 @@
 // is one coalesced 256-byte load.  Values, weights and the SpMM accumulation
 // are FP32; the teleport update, the per-column dangling mass and the
 // per-column L1 change are FP64.  Tiles of <= 1024 in-entries; longer rows
-// are split into chunks whose [64] partial sums the last workgroup combines
-// in chunk order, as it folds the per-workgroup, per-column partials in index
-// order.  k > 64 runs as consecutive blocks of 64 columns.
+// are split into chunks whose [64] partial sums a second kernel per sweep
+// (ppr_fold) combines in chunk order before folding the per-workgroup,
+// per-column partials in index order.  k > 64 runs as consecutive blocks of
+// 64 columns.
 #include <cmath>
 
 #include "device_common.hpp"
@@ -51,7 +52,7 @@ struct BArgs {
     float *x_next;
     const float *c_cur;
     float *c_next;
-    double *partials;         // [2][gridDim.x][64]
+    double *partials;         // [kPartials][64]: sweep partials, then fold partials
     float *long_part;         // [nslots][64]
     const uint32_t *long_rows;
     const uint32_t *long_slot;
@@ -113,7 +114,6 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
     if (a.st->h.done) return;
     __shared__ float s_part[kBlock / kWave][kCols];
     __shared__ double s_red[2][kBlock / kWave][kCols];
-    __shared__ uint32_t flag;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid / kWave;
@@ -153,21 +153,58 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
         a.partials[((size_t)blockIdx.x) * kCols + lane] = d;
         a.partials[((size_t)gridDim.x + blockIdx.x) * kCols + lane] = s;
     }
-    if (!last_to_arrive(&a.st->h.counter, &flag)) return;
+}
 
-    // Last workgroup: split rows, then the per-column folds in index order.
+// Completes the preceding ppr_sweep launch (`sgrid` workgroups): one
+// wavefront per split row combines its chunk partials in chunk order and
+// finishes the row, and workgroup b also sums its contiguous slice of the
+// sweep's per-column partials; the last workgroup to arrive folds the
+// workgroup sums in index order and updates the per-column state.  (Two
+// levels, so no workgroup walks all of the sweep's partials alone.)
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid) {
+    if (a.st->h.done) return;
+    __shared__ double s_red[2][kBlock / kWave][kCols];
+    __shared__ uint32_t flag;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid / kWave;
+    const uint64_t active = a.st->active;
+    const bool col_on = (active >> lane) & 1ull;
+    const uint32_t seed = a.seeds[lane];
+    const double psink = a.st->p_sink[lane];
     double d = 0.0, s = 0.0;
-    for (uint32_t b = wid; b < gridDim.x; b += kBlock / kWave) {
+    const uint32_t b0 = (uint32_t)((uint64_t)sgrid * blockIdx.x / gridDim.x);
+    const uint32_t b1 = (uint32_t)((uint64_t)sgrid * (blockIdx.x + 1) / gridDim.x);
+    for (uint32_t b = b0 + wid; b < b1; b += kBlock / kWave) {
         d += a.partials[(size_t)b * kCols + lane];
-        s += a.partials[((size_t)gridDim.x + b) * kCols + lane];
+        s += a.partials[((size_t)sgrid + b) * kCols + lane];
     }
-    for (uint32_t k = wid; k < a.nlong; k += kBlock / kWave) {
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t k = blockIdx.x * (kBlock / kWave) + wid; k < a.nlong; k += waves) {
         float acc = 0.f;
         for (uint32_t q = a.long_slot[k]; q < a.long_slot[k + 1]; ++q)
             acc += a.long_part[(size_t)q * kCols + lane];
         finish_row<WEIGHTED>(a, a.long_rows[k], acc, lane, col_on, seed, psink, d, s);
     }
+    s_red[0][wid][lane] = d;
+    s_red[1][wid][lane] = s;
     __syncthreads();
+    double *fp = a.partials + 2 * (size_t)kMaxSweepGrid * kCols;
+    if (wid == 0) {
+        d = ((s_red[0][0][lane] + s_red[0][1][lane]) + s_red[0][2][lane]) + s_red[0][3][lane];
+        s = ((s_red[1][0][lane] + s_red[1][1][lane]) + s_red[1][2][lane]) + s_red[1][3][lane];
+        fp[((size_t)blockIdx.x) * kCols + lane] = d;
+        fp[((size_t)gridDim.x + blockIdx.x) * kCols + lane] = s;
+    }
+    if (!last_to_arrive(&a.st->h.counter, &flag)) return;
+
+    d = 0.0;
+    s = 0.0;
+    for (uint32_t b = wid; b < gridDim.x; b += kBlock / kWave) {
+        d += fp[(size_t)b * kCols + lane];
+        s += fp[((size_t)gridDim.x + b) * kCols + lane];
+    }
     s_red[0][wid][lane] = d;
     s_red[1][wid][lane] = s;
     __syncthreads();
@@ -197,7 +234,6 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
             st->h.done = mask == 0 ? 1u : 0u;
             st->h.counter = 0;
         }
-        __threadfence();
     }
 }
 
@@ -292,13 +328,17 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
     static const int cap_w = occupancy_cap((const void *)ppr_sweep<true>);
     static const int cap_u = occupancy_cap((const void *)ppr_sweep<false>);
     const int grid_w = fit_grid(cap_w, S.ntiles), grid_u = fit_grid(cap_u, S.ntiles);
+    // At least 64 fold workgroups: each sums at most 32 of the sweep's rows of
+    // partials before the last one folds the workgroup sums.
+    const unsigned fgrid =
+        (unsigned)fit_grid(kMaxSweepGrid, std::max<uint32_t>(64, (S.nlong + 3) / 4));
     GT_TRY(prof_prepare(g));
     const size_t vb = sizeof(float) * kCols * (size_t)n;
     GT_TRY(ws.bx[0].reserve(vb));
     GT_TRY(ws.bx[1].reserve(vb));
     GT_TRY(ws.bc[0].reserve(vb));
     GT_TRY(ws.bc[1].reserve(vb));
-    GT_TRY(ws.bpart.reserve(sizeof(double) * 2 * kCols * 2048));
+    GT_TRY(ws.bpart.reserve(sizeof(double) * kCols * kPartials));
     GT_TRY(ws.blong.reserve(sizeof(float) * kCols * std::max<uint32_t>(1, S.nslots)));
     GT_TRY(ws.bstate.reserve(sizeof(BatchState) + sizeof(uint32_t) * kCols));
     BatchState *st = ws.bstate.as<BatchState>();
@@ -326,7 +366,7 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
     a.long_rows = S.long_rows.as<uint32_t>();
     a.long_slot = S.long_slot.as<uint32_t>();
     a.st = st;
-    g->prof_kind = GT_SWEEP_OTHER;
+    g->prof_kind = GT_SWEEP_BATCHED;
     uint64_t total = 0;
     for (uint32_t c0 = 0; c0 < k; c0 += kCols) {
         const uint32_t kb = std::min<uint32_t>(kCols, k - c0);
@@ -344,9 +384,9 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
             hipLaunchKernelGGL(ppr_init<false>, dim3(row_waves_grid(n)), dim3(kBlock), 0, s, ai,
                                n, mask);
         GT_HIP(hipGetLastError());
-        auto launch = [&](uint64_t q) {
+        BArgs b = a;
+        auto sweep = [&](uint64_t q) {
             const int par = (int)(q & 1);
-            BArgs b = a;
             b.x_cur = xb[par];
             b.x_next = xb[par ^ 1];
             b.c_cur = cb[par];
@@ -356,9 +396,17 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
             else
                 hipLaunchKernelGGL(ppr_sweep<false>, dim3(grid_u), dim3(kBlock), 0, s, b);
         };
+        auto fold = [&](uint64_t) {
+            if (weighted)
+                hipLaunchKernelGGL(ppr_fold<true>, dim3(fgrid), dim3(kBlock), 0, s, b,
+                                   (uint32_t)grid_w);
+            else
+                hipLaunchKernelGGL(ppr_fold<false>, dim3(fgrid), dim3(kBlock), 0, s, b,
+                                   (uint32_t)grid_u);
+        };
         uint64_t launched = 0;
         GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), params->epsilon, params->max_iter,
-                     launch, [](uint64_t) {}, &launched));
+                     sweep, fold, &launched));
         const uint64_t sweeps = g->h_state->iter;
         total += sweeps;
         uint64_t hit[kCols];

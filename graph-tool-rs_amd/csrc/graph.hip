@@ -252,6 +252,53 @@ gt_status ensure_schedule_mm(Orient &o, uint32_t n) {
     return GT_OK;
 }
 
+// esrc[e] = kNoEdge for every id, then the out-halves' entries (row s,
+// entry (t, e)) set esrc[e] = s, etgt[e] = t: one wavefront per row.
+__global__ __launch_bounds__(kBlock) void fill_u32(uint32_t *p, uint64_t len, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len;
+         i += (uint64_t)gridDim.x * kBlock)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_edges(const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ idx,
+                                                        const uint32_t *__restrict__ eid,
+                                                        uint32_t n, uint32_t *__restrict__ es,
+                                                        uint32_t *__restrict__ et) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * kWavesPerBlock;
+    for (uint32_t r = blockIdx.x * kWavesPerBlock + threadIdx.x / 64; r < n; r += waves)
+        for (uint32_t j = off[r] + lane; j < off[r + 1]; j += 64) {
+            const uint32_t e = eid[j];
+            es[e] = r;
+            et[e] = idx[j];
+        }
+}
+
+gt_status ensure_edge_list(gt_graph *g) {
+    if (g->edges_ready) return GT_OK;
+    const uint64_t eb = g->edge_bound;
+    GT_TRY(g->esrc.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, eb)));
+    GT_TRY(g->etgt.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, eb)));
+    if (eb) {
+        const unsigned fb = (unsigned)std::min<uint64_t>(8192, (eb + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(fill_u32, dim3(fb), dim3(kBlock), 0, g->stream,
+                           g->esrc.as<uint32_t>(), eb, kNoEdge);
+        hipLaunchKernelGGL(fill_u32, dim3(fb), dim3(kBlock), 0, g->stream,
+                           g->etgt.as<uint32_t>(), eb, kNoEdge);
+        if (g->n && g->m) {
+            const unsigned rb = (unsigned)std::min<uint64_t>(16384, ((uint64_t)g->n + 3) / 4);
+            hipLaunchKernelGGL(scatter_edges, dim3(rb), dim3(kBlock), 0, g->stream,
+                               g->out.off.as<uint32_t>(), g->out.idx.as<uint32_t>(),
+                               g->out.eid.as<uint32_t>(), g->n, g->esrc.as<uint32_t>(),
+                               g->etgt.as<uint32_t>());
+        }
+        GT_HIP(hipGetLastError());
+    }
+    g->edges_ready = true;
+    return GT_OK;
+}
+
 static gt_status upload_orient(Orient &o, uint32_t n, uint64_t m, const uint32_t *off,
                                const uint32_t *idx, const uint32_t *eid) {
     GT_TRY(o.off.reserve(sizeof(uint32_t) * ((size_t)n + 1)));

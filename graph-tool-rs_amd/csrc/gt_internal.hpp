@@ -74,6 +74,8 @@ struct Tile {
     uint32_t nnz_end;
 };
 constexpr uint32_t kSplit = 0xffffffffu;
+// Edge-list slot of an edge id with no edge.
+constexpr uint32_t kNoEdge = 0xffffffffu;
 
 // Pull-sweep wave tiles: at most kWaveTileNnz entries (16 per lane) and
 // kWaveTileRows rows (4 per lane); longer rows are split into chunks of
@@ -116,9 +118,9 @@ struct Orient {
     uint32_t bits = 0;       // slot bits of `packed` (0: plain sources, row order)
     bool scheduled = false;
     Schedule mm;             // SpMM schedule (batched personalisation)
-    // Rows too long for one thread's sequential degree sum.
-    DevBuf heavy;
-    uint32_t nheavy = 0;
+    // Rows for the wavefront (mid) and workgroup (heavy) degree sums.
+    DevBuf heavy, mid;
+    uint32_t nheavy = 0, nmid = 0;
     bool heavy_ready = false;
 };
 
@@ -156,7 +158,7 @@ struct Workspace {
     DevBuf lam[2];      // adjoint iterates
     DevBuf ax, ac;      // adjoint: staged x* and dL/dx (f64)
     DevBuf wcsr;        // adjoint: weights in the push orientation's packed order
-    DevBuf wrow;        // adjoint: weights in push row order (edge gradients)
+    DevBuf mu;          // adjoint: M^T lambda, f64[n]
     DevBuf gw, gp;      // adjoint: staged gradients for host outputs
     DevBuf bx[2];       // batched: [n][64] f32 iterates
     DevBuf bc[2];       // batched: [n][64] f32 contributions
@@ -194,6 +196,10 @@ struct gt_graph {
     gt::Orient in;   // CSC: in-halves
     gt::Orient out;  // CSR: out-halves
     gt::Relabeled rel[2];  // [0]: pull = in-halves, [1]: pull = out-halves
+    // Edge list by edge id (source, target), kNoEdge where an id has no edge;
+    // built on first use from the out-halves.
+    gt::DevBuf esrc, etgt;
+    bool edges_ready = false;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     gt::Workspace ws;
@@ -201,8 +207,8 @@ struct gt_graph {
     bool profiling = false;
     // Summed event time and launch count of sweep launches, per kernel kind
     // (gt_sweep_kind); prof_kind is the kind being launched.
-    double prof_ms[4] = {0, 0, 0, 0};
-    uint64_t prof_launches[4] = {0, 0, 0, 0};
+    double prof_ms[GT_SWEEP_KINDS] = {};
+    uint64_t prof_launches[GT_SWEEP_KINDS] = {};
     int prof_kind = 0;
     uint64_t last_sweeps = 0;
     std::vector<hipEvent_t> ev;  // profiling events
@@ -242,5 +248,7 @@ int fold_grid(uint32_t nlong);
 int wave_grid(int cap, uint32_t ntiles);
 // The relabelled pull orientation (0: in-halves, 1: out-halves), built once.
 gt_status ensure_relabeled(gt_graph *g, int which, Relabeled **out);
+// The edge list (g->esrc, g->etgt over [0, edge_bound)), built once.
+gt_status ensure_edge_list(gt_graph *g);
 
 }  // namespace gt

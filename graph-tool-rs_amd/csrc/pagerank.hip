@@ -668,7 +668,7 @@ gt_status gt_graph_set_profiling(gt_graph *g, int32_t enable) {
     clear_error();
     if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
     g->profiling = enable != 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < GT_SWEEP_KINDS; ++k) {
         g->prof_ms[k] = 0.0;
         g->prof_launches[k] = 0;
     }
@@ -679,7 +679,7 @@ gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms, uint
                          uint64_t *last_sweeps) {
     clear_error();
     if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
-    if (kind < 0 || kind > 3) return fail(GT_ERR_INVALID_ARG, "unknown sweep kind");
+    if (kind < 0 || kind >= GT_SWEEP_KINDS) return fail(GT_ERR_INVALID_ARG, "unknown sweep kind");
     if (total_ms) *total_ms = g->prof_ms[kind];
     if (launches) *launches = g->prof_launches[kind];
     if (last_sweeps) *last_sweeps = g->last_sweeps;

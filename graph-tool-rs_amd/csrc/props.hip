@@ -4,15 +4,18 @@
 //
 // The weighted out-degree restates crates/gt-algo/src/centrality.rs:281-297:
 // deg(v) = sum of w(e) over v's out-entries in storage order, starting from
-// 0.0.  Rows up to kSeqDeg entries are summed sequentially by one thread, so
-// the value is the reference's bit for bit; longer rows use a fixed
-// shuffle-tree order (deterministic, within rounding of the sequential sum).
+// 0.0.  Rows up to kSeqDeg entries are summed in that order -- rows shorter
+// than kWaveDeg by one thread, longer ones by a wavefront that loads 64
+// weights at a time and folds them lane by lane -- so the value is the
+// reference's bit for bit; longer rows use a fixed shuffle-tree order
+// (deterministic, within rounding of the sequential sum).
 #include "device_common.hpp"
 #include "gt_internal.hpp"
 #include "props.hpp"
 
 namespace gt {
 
+constexpr uint32_t kWaveDeg = 32;
 constexpr uint32_t kSeqDeg = 4096;
 
 template <typename S>
@@ -21,7 +24,8 @@ __device__ __forceinline__ double read_w(const void *w, uint32_t e, int kind, do
     return static_cast<double>(static_cast<const S *>(w)[e]);
 }
 
-// One thread per row; rows longer than kSeqDeg are left for deg_long.
+// One thread per row shorter than kWaveDeg, eight weight loads in flight,
+// added in storage order.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void deg_short(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ eid,
@@ -29,14 +33,48 @@ __global__ __launch_bounds__(kBlock) void deg_short(const uint32_t *__restrict__
                                                     uint32_t n, double *__restrict__ deg) {
     for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
         const uint32_t lo = off[v], hi = off[v + 1];
-        if (hi - lo > kSeqDeg) continue;
+        if (hi - lo >= kWaveDeg) continue;
         double acc = 0.0;
-        for (uint32_t j = lo; j < hi; ++j) acc += read_w<S>(w, eid[j], kind, value);
+        uint32_t j = lo;
+        for (; j + 8 <= hi; j += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = read_w<S>(w, eid[j + u], kind, value);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += x[u];
+        }
+        for (; j < hi; ++j) acc += read_w<S>(w, eid[j], kind, value);
         deg[v] = acc;
     }
 }
 
-// One workgroup per long row (list computed on the host).
+// One wavefront per listed row (kWaveDeg <= length <= kSeqDeg): 64 weights
+// loaded at once, then added in storage order by a lane-by-lane fold that
+// every lane repeats.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void deg_mid(const uint32_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ eid,
+                                                  const void *w, int kind, double value,
+                                                  const uint32_t *__restrict__ rows,
+                                                  uint32_t nrows, double *__restrict__ deg) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t k = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; k < nrows;
+         k += waves) {
+        const uint32_t v = rows[k];
+        const uint32_t lo = off[v], hi = off[v + 1];
+        double acc = 0.0;
+        for (uint32_t j0 = lo; j0 < hi; j0 += kWave) {
+            const uint32_t j = j0 + lane;
+            const double x = j < hi ? read_w<S>(w, eid[j], kind, value) : 0.0;
+            const uint32_t cnt = hi - j0 < (uint32_t)kWave ? hi - j0 : (uint32_t)kWave;
+            for (uint32_t u = 0; u < cnt; ++u) acc += __shfl(x, (int)u, kWave);
+        }
+        if (lane == 0) deg[v] = acc;
+    }
+}
+
+// One workgroup per row longer than kSeqDeg (list computed on the host).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void deg_long(const uint32_t *__restrict__ off,
                                                    const uint32_t *__restrict__ eid,
@@ -76,15 +114,24 @@ static unsigned grid_for(uint64_t n) {
 }
 
 // Long rows of an orientation (> kSeqDeg entries), cached.
+// Host lists of the rows deg_mid and deg_long take, built once per orientation.
 static gt_status heavy_rows(Orient &o, uint32_t n) {
     if (o.heavy_ready) return GT_OK;
-    std::vector<uint32_t> rows;
-    for (uint32_t v = 0; v < n; ++v)
-        if (o.h_off[v + 1] - o.h_off[v] > kSeqDeg) rows.push_back(v);
-    o.nheavy = (uint32_t)rows.size();
-    GT_TRY(o.heavy.reserve(sizeof(uint32_t) * std::max<size_t>(1, rows.size())));
-    if (!rows.empty())
-        GT_HIP(hipMemcpy(o.heavy.ptr, rows.data(), sizeof(uint32_t) * rows.size(),
+    std::vector<uint32_t> mid, heavy;
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t len = o.h_off[v + 1] - o.h_off[v];
+        if (len > kSeqDeg) heavy.push_back(v);
+        else if (len >= kWaveDeg) mid.push_back(v);
+    }
+    o.nheavy = (uint32_t)heavy.size();
+    o.nmid = (uint32_t)mid.size();
+    GT_TRY(o.heavy.reserve(sizeof(uint32_t) * std::max<size_t>(1, heavy.size())));
+    GT_TRY(o.mid.reserve(sizeof(uint32_t) * std::max<size_t>(1, mid.size())));
+    if (!heavy.empty())
+        GT_HIP(hipMemcpy(o.heavy.ptr, heavy.data(), sizeof(uint32_t) * heavy.size(),
+                         hipMemcpyHostToDevice));
+    if (!mid.empty())
+        GT_HIP(hipMemcpy(o.mid.ptr, mid.data(), sizeof(uint32_t) * mid.size(),
                          hipMemcpyHostToDevice));
     o.heavy_ready = true;
     return GT_OK;
@@ -124,6 +171,12 @@ static gt_status launch_deg(gt_graph *g, Orient &push, const void *src, int kind
     hipLaunchKernelGGL((deg_short<S>), dim3(grid_for(n)), dim3(kBlock), 0, s,
                        push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value, n, deg);
     GT_TRY(heavy_rows(push, n));
+    if (push.nmid) {
+        const unsigned mg = (unsigned)std::min<uint32_t>(8192, (push.nmid + 3) / 4);
+        hipLaunchKernelGGL((deg_mid<S>), dim3(mg), dim3(kBlock), 0, s, push.off.as<uint32_t>(),
+                           push.eid.as<uint32_t>(), src, kind, value, push.mid.as<uint32_t>(),
+                           push.nmid, deg);
+    }
     if (push.nheavy)
         hipLaunchKernelGGL((deg_long<S>), dim3(push.nheavy), dim3(kBlock), 0, s,
                            push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value,

@@ -55,7 +55,8 @@ class GtView(C.Structure):
                 ("vmask_len", C.c_uint64), ("emask", C.c_void_p), ("emask_len", C.c_uint64)]
 
 
-GT_SWEEP_PAGERANK_F64, GT_SWEEP_PAGERANK_F32, GT_SWEEP_EIGENVECTOR, GT_SWEEP_OTHER = range(4)
+(GT_SWEEP_PAGERANK_F64, GT_SWEEP_PAGERANK_F32, GT_SWEEP_EIGENVECTOR, GT_SWEEP_OTHER,
+ GT_SWEEP_BATCHED, GT_SWEEP_ADJOINT) = range(6)
 
 GraphP = C.c_void_p
 

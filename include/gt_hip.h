@@ -225,8 +225,11 @@ typedef enum gt_sweep_kind {
     GT_SWEEP_PAGERANK_F64 = 0,
     GT_SWEEP_PAGERANK_F32 = 1,
     GT_SWEEP_EIGENVECTOR = 2,
-    GT_SWEEP_OTHER = 3
+    GT_SWEEP_OTHER = 3,
+    GT_SWEEP_BATCHED = 4,   /* personalised SpMM sweep */
+    GT_SWEEP_ADJOINT = 5    /* transposed (adjoint) sweep */
 } gt_sweep_kind;
+#define GT_SWEEP_KINDS 6
 
 /* Extension.  While enabled, every sweep launch on g is bracketed by HIP
  * events recorded on the stream it runs on; gt_sweep_stats reports, for one

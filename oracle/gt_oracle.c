@@ -588,10 +588,17 @@ GT_EXPORT int64_t oracle_pagerank_grad(const gt_adj *g, int reversed, int wkind,
         for (uint32_t j = 0; j < cnt; ++j) k += map_get(&wm, out[j].idx);
         deg[v] = wkind == MAP_UNITY ? (double)cnt : k;
     }
-    for (uint32_t v = 0; v < n; ++v) lambda[v] = cvec[v];
-    double delta = epsilon + 1.0;
+    /* The system is linear in cvec: stop when the L1 change drops below
+     * epsilon * ||cvec||_1 (epsilon itself when cvec is zero). */
+    double c1 = 0.0;
+    for (uint32_t v = 0; v < n; ++v) {
+        lambda[v] = cvec[v];
+        c1 += fabs(cvec[v]);
+    }
+    const double thr = epsilon * (c1 > 0.0 ? c1 : 1.0);
+    double delta = thr + 1.0;
     uint64_t iter = 0;
-    while (delta >= epsilon) {
+    while (delta >= thr) {
         double pl = 0.0;
         for (uint32_t v = 0; v < n; ++v) pl += map_get(&pm, v) * lambda[v];
         delta = 0.0;

@@ -48,6 +48,22 @@ def test_backward_matches_oracle_adjoint(gpu, precision, tol):
         assert abs(it - it_o) <= 1
 
 
+def test_backward_on_reversed_view_matches_oracle(gpu):
+    # Reversed view: the transposed sweep pulls over the in-halves and every
+    # edge's gradient row is its target.
+    n, s, t, w, c = _problem(11, 9)
+    g = gt_hip.AdjList.with_vertices(n)
+    g.add_edges(s, t)
+    o = oracle.Adj(n, s, t)
+    x, _ = o.pagerank(weight=w, epsilon=1e-15, max_iter=10_000, reversed=True)
+    gw_o, gp_o, gd_o, _, _ = o.pagerank_grad(x, c, weight=w, epsilon=1e-13, reversed=True)
+    wm = gt_hip.DenseProp.from_vec(g.graph_id, w)
+    gw, gp, gd, _ = gt_hip.pagerank_grad(g.reversed(), wm, None, 0.85, _params(n, 1e-13), x, c)
+    assert np.abs(gw - gw_o).max() <= 1e-9 * np.abs(gw_o).max()
+    assert np.abs(gp - gp_o).max() <= 1e-9 * np.abs(gp_o).max()
+    assert abs(gd - gd_o) <= 1e-9 * max(1.0, abs(gd_o))
+
+
 def test_backward_matches_finite_differences_on_gpu(gpu):
     n, s, t, w, c = _problem(10, 11)
     g = gt_hip.AdjList.with_vertices(n)
