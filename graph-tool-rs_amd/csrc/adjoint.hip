@@ -55,7 +55,7 @@ struct AdjArgs {
 };
 
 template <typename V, bool WEIGHTED, bool DPERS>
-struct AdjFin {
+struct AdjFin : TileAll {
     const AdjArgs &a;
     double pl;
     double dsum = 0.0;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void adj_init(const AdjArgs a, uint32_t n) 
 // (0 when deg_s == 0): the same wave-engine pass as a transposed sweep,
 // written to mu instead of iterating.
 template <typename V, bool WEIGHTED>
-struct MuFin {
+struct MuFin : TileAll {
     const AdjArgs &a;
     double *mu;
     __device__ MuFin(const AdjArgs &args, double *m) : a(args), mu(m) {}

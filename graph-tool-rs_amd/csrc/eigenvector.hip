@@ -64,7 +64,7 @@ struct EvArgs {
 };
 
 template <bool MASK>
-struct EvFin {
+struct EvFin : TileAll {
     const EvArgs &a;
     double sc_cur, sc_prev;
     bool first;

@@ -139,16 +139,24 @@ __global__ __launch_bounds__(kBlock) void pack_tiles(const Tile *__restrict__ ti
 
 // Wave tiles of at most kWaveTileNnz entries and kWaveTileRows rows; a row
 // longer than kWaveTileNnz becomes ceil(len / kWaveTileNnz) split chunks with
-// one partial slot each.
-gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
-    if (o.scheduled) return GT_OK;
+// one partial slot each.  With `bounds` (ascending row indices) no tile spans
+// a bound, and the host keeps the tile list.
+gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
+                          const std::vector<uint32_t> *bounds) {
+    if (o.scheduled && (!bounds || o.bounds == *bounds)) return GT_OK;
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;
     uint32_t nslots = 0;
     tiles.reserve(n / 64 + 16);
     uint32_t r = 0;
+    size_t bi = 0;
     while (r < n) {
+        uint32_t stop = n;
+        if (bounds) {
+            while (bi < bounds->size() && (*bounds)[bi] <= r) ++bi;
+            if (bi < bounds->size()) stop = std::min(n, (*bounds)[bi]);
+        }
         const uint32_t len = off[r + 1] - off[r];
         if (len > kWaveTileNnz) {
             long_rows.push_back(r);
@@ -162,7 +170,7 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
         }
         const uint32_t r0 = r;
         uint32_t nnz = 0;
-        while (r < n && r - r0 < kWaveTileRows) {
+        while (r < stop && r - r0 < kWaveTileRows) {
             const uint32_t l = off[r + 1] - off[r];
             if (l > kWaveTileNnz || nnz + l > kWaveTileNnz) break;
             nnz += l;
@@ -171,6 +179,15 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n) {
         tiles.push_back(Tile{r0, r, off[r0], off[r]});
     }
     long_slot.push_back(nslots);
+    if (bounds) {
+        o.bounds = *bounds;
+        o.h_tiles = tiles;
+        o.h_long_rows = long_rows;
+    } else {
+        o.bounds.clear();
+        o.h_tiles.clear();
+        o.h_long_rows.clear();
+    }
     o.ntiles = (uint32_t)tiles.size();
     o.nlong = (uint32_t)long_rows.size();
     o.nslots = nslots;

@@ -117,6 +117,11 @@ struct Orient {
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
     uint32_t bits = 0;       // slot bits of `packed` (0: plain sources, row order)
     bool scheduled = false;
+    // Set when tiles were cut at these row bounds (a batch of graphs): the
+    // host copies of the tile list and the split rows.
+    std::vector<uint32_t> bounds;
+    std::vector<Tile> h_tiles;
+    std::vector<uint32_t> h_long_rows;
     Schedule mm;             // SpMM schedule (batched personalisation)
     // Rows for the wavefront (mid) and workgroup (heavy) degree sums.
     DevBuf heavy, mid;
@@ -166,6 +171,7 @@ struct Workspace {
     DevBuf blong;       // batched: split-row [64] partials
     DevBuf bstate;      // batched: loop state + seeds
     DevBuf masks;       // staged vertex + edge masks
+    DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
     Orient fpull;       // filtered view: compacted pull orientation
     Orient fpush;       // filtered view: compacted push orientation
 };
@@ -219,7 +225,8 @@ namespace gt {
 
 // Build (once) the pull-sweep schedule of an orientation: wave tiles from the
 // host offsets, then the packed entry stream and its edge ids on the device.
-gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n);
+gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
+                          const std::vector<uint32_t> *bounds = nullptr);
 // Build (once) the SpMM schedule of an orientation.
 gt_status ensure_schedule_mm(Orient &o, uint32_t n);
 // Allocate a graph shell with its stream.

@@ -53,6 +53,86 @@ void parallel_for(uint64_t n, F f) {
 
 extern "C" {
 
+// A batch of directed Chung-Lu graphs (the uneven-batch workload; no
+// counterpart in the reference).  Graph g draws from the stream seeded with
+// mix(seed + g):
+//   draw 0      n_g = round(exp(U[ln n_min, ln n_max]))
+//   draw 1      damping_g = d_lo + (d_hi - d_lo) U
+//   per edge i  draws 2 + 3i, 3 + 3i, 4 + 3i: source, target, weight
+// with m_g = round(mean_degree * n_g).  Vertex v of graph g has weight
+// theta_v = (v + 1)^(-1 / (exponent - 1)) (expected degrees of a power law with
+// that exponent); the source and the target of an edge are drawn
+// independently with probability proportional to theta (inverse CDF over the
+// prefix sums), so the mean out-degree is mean_degree.  Weights are
+// 1 - U[0,1), in (0, 1].  Vertices and edges of graph g follow those of
+// graphs 0..g-1 (ids are global).
+//
+// Two calls: with src == NULL only n_g, m_g and damping_g are written; then,
+// with arrays of sum(m_g) entries, the edges.
+gt_status gt_gen_chung_lu_batch(uint32_t num_graphs, uint64_t seed, uint32_t n_min,
+                                uint32_t n_max, double mean_degree, double exponent,
+                                double d_lo, double d_hi, uint32_t *n_g, uint64_t *m_g,
+                                double *damping, uint32_t *src, uint32_t *tgt, float *weight) {
+    gt::clear_error();
+    if (!n_g || !m_g || !damping) return gt::fail(GT_ERR_INVALID_ARG, "null size output");
+    if (n_min == 0 || n_max < n_min) return gt::fail(GT_ERR_INVALID_ARG, "bad vertex range");
+    if (!(exponent > 1.0) || !(mean_degree >= 0.0))
+        return gt::fail(GT_ERR_INVALID_ARG, "exponent must exceed 1, mean degree be >= 0");
+    uint64_t tn = 0;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+        const uint64_t gs = mix(seed + g);
+        const double l0 = std::log((double)n_min), l1 = std::log((double)n_max);
+        double nv = std::round(std::exp(l0 + (l1 - l0) * unit(gs, 0)));
+        nv = std::min<double>(std::max<double>(nv, n_min), n_max);
+        n_g[g] = (uint32_t)nv;
+        m_g[g] = (uint64_t)std::llround(mean_degree * nv);
+        damping[g] = d_lo + (d_hi - d_lo) * unit(gs, 1);
+        tn += n_g[g];
+        if (tn > 0xffffffffull) return gt::fail(GT_ERR_INVALID_ARG, "batch exceeds 2^32 vertices");
+    }
+    if (!src) return GT_OK;
+    if (!tgt || !weight) return gt::fail(GT_ERR_INVALID_ARG, "null edge output");
+    std::vector<uint64_t> eoff(num_graphs + 1, 0);
+    std::vector<uint32_t> voff(num_graphs + 1, 0);
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+        eoff[g + 1] = eoff[g] + m_g[g];
+        voff[g + 1] = voff[g] + n_g[g];
+    }
+    const double beta = 1.0 / (exponent - 1.0);
+    unsigned t = std::thread::hardware_concurrency();
+    if (t == 0) t = 1;
+    if (t > 16) t = 16;
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < t; ++k) {
+        th.emplace_back([&, k] {
+            std::vector<double> cdf;
+            for (uint32_t g = k; g < num_graphs; g += t) {
+                const uint64_t gs = mix(seed + g);
+                const uint32_t n = n_g[g];
+                cdf.resize(n);
+                double acc = 0.0;
+                for (uint32_t v = 0; v < n; ++v) {
+                    acc += std::pow((double)(v + 1), -beta);
+                    cdf[v] = acc;
+                }
+                auto pick = [&](double u) {
+                    const double x = u * acc;
+                    uint32_t v = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), x) - cdf.begin());
+                    return v < n ? v : n - 1;
+                };
+                for (uint64_t i = 0; i < m_g[g]; ++i) {
+                    const uint64_t e = eoff[g] + i;
+                    src[e] = voff[g] + pick(unit(gs, 2 + 3 * i));
+                    tgt[e] = voff[g] + pick(unit(gs, 3 + 3 * i));
+                    weight[e] = (float)(1.0 - unit(gs, 4 + 3 * i));
+                }
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    return GT_OK;
+}
+
 // Directed G(n, m) as crates/gt-algo/benches/kernels.rs:49-73 draws it: per
 // edge, source = next % n then target = next % n.
 gt_status gt_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *src, uint32_t *tgt) {

@@ -59,7 +59,7 @@ struct PrArgs {
 };
 
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
-struct PrFin {
+struct PrFin : TileAll {
     const PrArgs &a;
     double p_sink;
     double dsum = 0.0;

@@ -22,6 +22,11 @@
 //   phase 3  `fin(row, sum)`, lane q for rows q, q + 64, ...: the rows'
 //            global reads and writes go out coalesced.
 // A split chunk is summed straight from registers into its slot.
+//
+// The finisher `fin` is called as fin(row, sum) for every whole row, and
+// brackets every tile: fin.tile(t) (wave-uniform) before it -- returning false
+// skips the tile, loads included -- and fin.tile_end(t) after it.  TileAll is
+// the pair for finishers that need neither.
 #pragma once
 
 #include "device_common.hpp"
@@ -38,6 +43,12 @@ struct WaveLds {
     V vals[kWavesPerBlock][kWaveTileNnz];
     V rowsum[kWavesPerBlock][kWaveTileRows];
     double red[8];
+};
+
+// Default tile hooks: every tile is processed, nothing is done at its end.
+struct TileAll {
+    __device__ __forceinline__ bool tile(uint32_t) const { return true; }
+    __device__ __forceinline__ void tile_end(uint32_t) const {}
 };
 
 // Orders this wavefront's LDS accesses: writes before the call are visible to
@@ -63,6 +74,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     V *rowsum = L.rowsum[wid];
     const uint32_t nwaves = gridDim.x * kWavesPerBlock;
     for (uint32_t ti = blockIdx.x * kWavesPerBlock + wid; ti < ntiles; ti += nwaves) {
+        if (!fin.tile(ti)) continue;
         const Tile t = tiles[ti];
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
         uint32_t p[K];
@@ -90,6 +102,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             for (int k = 0; k < K; ++k) acc += v[k];
             const double tot = wave_sum(static_cast<double>(acc));
             if (lane == 0) long_part[t.row_begin] = tot;
+            fin.tile_end(ti);
             continue;
         }
 #pragma unroll
@@ -128,6 +141,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
         wave_sync();
         for (uint32_t q = lane; q < rows; q += kWave)
             fin(rb + q, static_cast<double>(rowsum[q]));
+        fin.tile_end(ti);
         wave_sync();  // the LDS tile is reused by this wavefront's next tile
     }
 }

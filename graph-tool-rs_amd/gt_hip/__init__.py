@@ -37,6 +37,7 @@ __all__ = [
     "Plan", "PowerIteration", "Unity", "Constant", "DenseProp", "AdjList", "View",
     "DeviceGraph", "pagerank", "pagerank_with", "pagerank_mixed", "eigenvector",
     "gen_er", "gen_rmat", "host_adjacency", "GtError", "GtValueError", "device_count",
+    "pagerank_uneven", "pagerank_uneven_grad", "gen_chung_lu_batch",
 ]
 
 _u32p = C.POINTER(C.c_uint32)
@@ -641,3 +642,89 @@ def gen_rmat(scale: int, m: int, a: float, b: float, c: float,
     tgt = np.zeros(max(m, 1), np.uint32)
     call("gt_gen_rmat", scale, m, a, b, c, seed & (2**64 - 1), _ptr(src), _ptr(tgt))
     return src[:m], tgt[:m]
+
+
+def gen_chung_lu_batch(num_graphs: int, seed: int, n_min: int = 64, n_max: int = 65536,
+                       mean_degree: float = 8.0, exponent: float = 2.1, d_lo: float = 0.5,
+                       d_hi: float = 0.95):
+    """A batch of directed Chung-Lu graphs with global vertex and edge ids
+    (see gt_gen_chung_lu_batch).  Returns (vertex_offsets [G+1] uint32,
+    edge_offsets [G+1] int64, damping [G], src, tgt, weight float32)."""
+    G = int(num_graphs)
+    ng = np.zeros(max(G, 1), np.uint32)
+    mg = np.zeros(max(G, 1), np.uint64)
+    dmp = np.zeros(max(G, 1), np.float64)
+    args = (G, seed & (2**64 - 1), n_min, n_max, float(mean_degree), float(exponent),
+            float(d_lo), float(d_hi), _ptr(ng), _ptr(mg, C.POINTER(C.c_uint64)),
+            _ptr(dmp, C.POINTER(C.c_double)))
+    call("gt_gen_chung_lu_batch", *args, None, None, None)
+    m = int(mg[:G].sum())
+    src = np.zeros(max(m, 1), np.uint32)
+    tgt = np.zeros(max(m, 1), np.uint32)
+    w = np.zeros(max(m, 1), np.float32)
+    call("gt_gen_chung_lu_batch", *args, _ptr(src), _ptr(tgt), _ptr(w, C.POINTER(C.c_float)))
+    voff = np.zeros(G + 1, np.uint32)
+    voff[1:] = np.cumsum(ng[:G], dtype=np.uint64).astype(np.uint32)
+    eoff = np.zeros(G + 1, np.int64)
+    eoff[1:] = np.cumsum(mg[:G].astype(np.int64))
+    return voff, eoff, dmp[:G].copy(), src[:m], tgt[:m], w[:m]
+
+
+def _uneven_args(dev: "DeviceGraph", vertex_offsets, damping):
+    voff = np.ascontiguousarray(vertex_offsets, dtype=np.uint32)
+    dmp = np.ascontiguousarray(damping, dtype=np.float64)
+    G = int(voff.shape[0]) - 1
+    if G < 1 or dmp.shape[0] != G:
+        raise GtValueError(_lib.GT_ERR_INVALID_ARG,
+                           "vertex_offsets needs G+1 entries and damping G")
+    n = dev.info()[0]
+    return voff, dmp, G, n
+
+
+def pagerank_uneven(dev: "DeviceGraph", vertex_offsets, damping, weight=None,
+                    epsilon: float = 1e-6, max_iter: int = 0, out=None):
+    """PageRank of every graph of a block-diagonal batch (extension; see
+    gt_pagerank_uneven).  Returns (rank [n] float64, iters [G]); rank is the
+    CUDA tensor ``out`` when one is given, else numpy."""
+    voff, dmp, G, n = _uneven_args(dev, vertex_offsets, damping)
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, dev.graph_id, keep)
+    params = PowerIteration(epsilon, max_iter, Plan.from_parts(n, max(1, -(-n // 4096))))
+    iters = np.zeros(G, np.uint64)
+    if out is not None and _is_torch_cuda(out):
+        _bind_stream(dev, out)
+        ptr, loc, res = C.c_void_p(out.data_ptr()), GT_DEVICE, out
+    else:
+        res = np.zeros(max(n, 1), np.float64)
+        dev.set_stream(None)
+        ptr, loc = res.ctypes.data_as(C.c_void_p), GT_HOST
+    call("gt_pagerank_uneven", dev.handle, G, _ptr(voff), _ptr(dmp, C.POINTER(C.c_double)),
+         C.byref(wp), C.byref(params._c()), ptr, loc, _ptr(iters, C.POINTER(C.c_uint64)))
+    return (res if loc == GT_DEVICE else res[:n]), iters.astype(np.int64)
+
+
+def pagerank_uneven_grad(dev: "DeviceGraph", vertex_offsets, damping, x, dl_dx, weight=None,
+                         epsilon: float = 1e-6, max_iter: int = 0):
+    """Gradients of L = dl_dx . x through every graph's fixed point
+    (extension; see gt_pagerank_uneven_backward).  Host numpy in and out:
+    returns (grad_w [edge_bound] float32, grad_pers [n], grad_damping [G],
+    iters [G])."""
+    voff, dmp, G, n = _uneven_args(dev, vertex_offsets, damping)
+    _, _, eb, _ = dev.info()
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, dev.graph_id, keep)
+    params = PowerIteration(epsilon, max_iter, Plan.from_parts(n, max(1, -(-n // 4096))))
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    ca = np.ascontiguousarray(dl_dx, dtype=np.float64)
+    gw = np.zeros(max(eb, 1), np.float32)
+    gp = np.zeros(max(n, 1), np.float64)
+    gd = np.zeros(G, np.float64)
+    iters = np.zeros(G, np.uint64)
+    dev.set_stream(None)
+    call("gt_pagerank_uneven_backward", dev.handle, G, _ptr(voff),
+         _ptr(dmp, C.POINTER(C.c_double)), C.byref(wp), C.byref(params._c()),
+         xa.ctypes.data_as(C.c_void_p), ca.ctypes.data_as(C.c_void_p), GT_HOST,
+         gw.ctypes.data_as(C.c_void_p), gp.ctypes.data_as(C.c_void_p),
+         _ptr(gd, C.POINTER(C.c_double)), _ptr(iters, C.POINTER(C.c_uint64)))
+    return gw[:eb], gp[:n], gd, iters.astype(np.int64)
+

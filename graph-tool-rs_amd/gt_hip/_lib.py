@@ -92,6 +92,17 @@ _PROTOS = {
     "gt_sweep_stats": [GraphP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                        C.POINTER(C.c_uint64)],
     "gt_gen_er": [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p],
+    "gt_pagerank_uneven": [GraphP, C.c_uint32, u32p, C.POINTER(C.c_double), C.POINTER(GtProp),
+                           C.POINTER(GtPowerIteration), C.c_void_p, C.c_int32,
+                           C.POINTER(C.c_uint64)],
+    "gt_pagerank_uneven_backward": [GraphP, C.c_uint32, u32p, C.POINTER(C.c_double),
+                                    C.POINTER(GtProp), C.POINTER(GtPowerIteration), C.c_void_p,
+                                    C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "gt_gen_chung_lu_batch": [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                              C.c_double, C.c_double, C.c_double, u32p,
+                              C.POINTER(C.c_uint64), C.POINTER(C.c_double), u32p, u32p,
+                              C.POINTER(C.c_float)],
     "gt_gen_rmat": [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64,
                     u32p, u32p],
     "gt_host_adjacency": [C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p, u32p,

@@ -211,6 +211,37 @@ gt_status gt_pagerank_backward(gt_graph *g, const gt_view *view, const gt_prop *
                                int32_t grad_w_dtype, int32_t grad_loc, double *grad_pers,
                                double *grad_damping, uint64_t *iters);
 
+/* Extension (no counterpart in the reference): an uneven batch of independent
+ * graphs solved together.  g holds them as the vertex ranges
+ * [vertex_offsets[k], vertex_offsets[k+1]) of one block-diagonal graph (an edge
+ * joining two ranges is an error); vertex_offsets has num_graphs + 1 entries
+ * from 0 to the vertex bound.  Graph k runs pagerank (pers = 1 / its vertex
+ * count) with damping[k], in FP32 values, weights and SpMV accumulation with
+ * FP64 dangling mass, teleport update and L1 test; it stops when its own L1
+ * change drops below max(epsilon, 2^-22) (the FP32 floor of a unit-mass
+ * iterate) or at max_iter sweeps, and is frozen after.  params->plan.len must
+ * equal the vertex bound.  rank: [n] f64 in rank_loc memory (overwritten,
+ * cold start); iters: [num_graphs] host.  The sweeps run as replays of a
+ * captured HIP graph of 8 sweeps each. */
+gt_status gt_pagerank_uneven(gt_graph *g, uint32_t num_graphs, const uint32_t *vertex_offsets,
+                             const double *damping, const gt_prop *weight,
+                             const gt_power_iteration *params, double *rank, int32_t rank_loc,
+                             uint64_t *iters);
+
+/* Extension.  The adjoint of gt_pagerank_uneven for losses L = sum_k c_k . x_k:
+ * per graph, the transposed system of gt_pagerank_backward with damping[k],
+ * FP32 values, stopped when its L1 change drops below
+ * max(epsilon, 2^-22) * ||dl_dx restricted to the graph||_1.  x (the forward
+ * result) and dl_dx are [n] f64, grad_w [edge_bound] f32 and grad_pers [n] f64
+ * (both optional), all in `loc` memory; grad_damping [num_graphs] and iters
+ * [num_graphs] are host arrays. */
+gt_status gt_pagerank_uneven_backward(gt_graph *g, uint32_t num_graphs,
+                                      const uint32_t *vertex_offsets, const double *damping,
+                                      const gt_prop *weight, const gt_power_iteration *params,
+                                      const double *x, const double *dl_dx, int32_t loc,
+                                      float *grad_w, double *grad_pers, double *grad_damping,
+                                      uint64_t *iters);
+
 /* Eigenvector centrality, FP64: the drop-in for eigenvector.  *eig is the
  * last norm (the eigenvalue estimate), *iters the sweep count. */
 gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,
@@ -251,6 +282,19 @@ gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms,
 gt_status gt_gen_er(uint32_t n, uint64_t m, uint64_t seed, uint32_t *src, uint32_t *tgt);
 gt_status gt_gen_rmat(uint32_t scale, uint64_t m, double a, double b, double c,
                       uint64_t seed, uint32_t *src, uint32_t *tgt);
+
+/* A batch of directed Chung-Lu graphs (uneven-batch workload).  Graph k draws
+ * from the stream seeded with mix(seed + k): n_k = round(exp(U[ln n_min,
+ * ln n_max])), damping_k = d_lo + (d_hi - d_lo) U, m_k = round(mean_degree *
+ * n_k) edges whose source and target are drawn independently with
+ * probability proportional to (v + 1)^(-1 / (exponent - 1)) over the graph's
+ * vertices, weight 1 - U in (0, 1].  Vertex and edge ids are global (graph k
+ * after graphs 0..k-1).  With src == NULL only n_g, m_g and damping are
+ * written; then call again with sum(m_g)-entry src, tgt and weight. */
+gt_status gt_gen_chung_lu_batch(uint32_t num_graphs, uint64_t seed, uint32_t n_min,
+                                uint32_t n_max, double mean_degree, double exponent,
+                                double d_lo, double d_hi, uint32_t *n_g, uint64_t *m_g,
+                                double *damping, uint32_t *src, uint32_t *tgt, float *weight);
 
 /* The host adjacency AdjList::add_edge(src[i], tgt[i]) for i = 0..m builds
  * (crates/gt-core/src/adj/list.rs:258-280, 471-500), flattened into the

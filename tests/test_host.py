@@ -111,3 +111,34 @@ def test_dense_prop_grows_and_never_shrinks():
 
 def test_device_count_without_gpu_is_not_an_error():
     assert gt_hip.device_count() >= 0
+
+
+def test_chung_lu_batch_generator():
+    voff, eoff, d, s, t, w = gt_hip.gen_chung_lu_batch(300, 0x5EED, n_min=64, n_max=4096)
+    G = 300
+    ng = np.diff(voff.astype(np.int64))
+    mg = np.diff(eoff)
+    assert voff[0] == 0 and len(voff) == G + 1
+    assert ng.min() >= 64 and ng.max() <= 4096
+    # log-uniform sizes: about half the graphs below the geometric midpoint 512
+    assert 0.35 < np.mean(ng < 512) < 0.65
+    assert np.array_equal(mg, np.rint(8.0 * ng).astype(np.int64))
+    assert np.all((d >= 0.5) & (d <= 0.95))
+    assert np.all((w > 0) & (w <= 1))
+    # every edge stays inside its graph's vertex range
+    gid = np.repeat(np.arange(G), mg)
+    assert np.all(s >= voff[gid]) and np.all(s < voff[gid + 1])
+    assert np.all(t >= voff[gid]) and np.all(t < voff[gid + 1])
+    # power-law sources: the lowest vertex of a graph is drawn far more often
+    # than the median one (theta_0 / theta_{n/2} = (n/2)^(1/1.1))
+    big = int(np.argmax(ng))
+    ls = s[eoff[big]:eoff[big + 1]] - voff[big]
+    cnt = np.bincount(ls, minlength=ng[big])
+    assert cnt[0] > 20 * max(1, cnt[ng[big] // 2])
+    # deterministic, and graph k depends only on (seed, k)
+    again = gt_hip.gen_chung_lu_batch(300, 0x5EED, n_min=64, n_max=4096)
+    assert np.array_equal(again[3], s) and np.array_equal(again[4], t)
+    assert np.array_equal(again[5], w)
+    few = gt_hip.gen_chung_lu_batch(10, 0x5EED, n_min=64, n_max=4096)
+    assert np.array_equal(few[0], voff[:11])
+    assert np.array_equal(few[3], s[:eoff[10]])
