@@ -287,16 +287,145 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
 
 
 
+def bench_uneven(args, torch, gt_hip, _lib) -> int:
+    """An uneven batch: 4096 directed Chung-Lu graphs, vertex counts
+    log-uniform in [64, 65536], expected out-degrees a power law (exponent
+    2.1) with mean 8, FP32 weights in (0, 1], damping uniform in [0.5, 0.95],
+    epsilon 1e-6; PageRank forward plus the adjoint of L = c . x (c ~ N(0,1))
+    for every graph, each call one hipGraph-captured sweep loop.  One step =
+    forward + backward.  GTEPS counts each graph's edges once per sweep it ran
+    (forward and adjoint sweeps); algorithmic bytes per graph sweep:
+    8 m_g + 4 (n_g + 1) + 20 n_g."""
+    G = 4096
+    t0 = time.perf_counter()
+    voff, eoff, damp, src, tgt, w_host = gt_hip.gen_chung_lu_batch(
+        G, SEED, 64, 65536, 8.0, 2.1, 0.5, 0.95)
+    n, m = int(voff[-1]), int(src.shape[0])
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dev = gt_hip.DeviceGraph.build(n, src, tgt, graph_id=1)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    log(f"[bench] uneven batch: {G} graphs, n={n} m={m}, generated in {t_gen:.1f}s, "
+        f"device build {t_build:.2f}s")
+    rng = np.random.default_rng(SEED & 0xFFFFFFFF)
+    c_host = rng.standard_normal(n)
+    w = torch.from_numpy(w_host).cuda()
+    c = torch.from_numpy(c_host).cuda()
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    gw = torch.empty(m, dtype=torch.float32, device="cuda")
+    gp = torch.empty(n, dtype=torch.float64, device="cuda")
+    gd = np.zeros(G)
+    wprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F32, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(w.data_ptr()), m, 1)
+    eps = 1e-6
+    params = _lib.GtPowerIteration(eps, 0, _lib.GtPlan(n, -(-n // 4096)))
+    it_f = np.zeros(G, np.uint64)
+    it_b = np.zeros(G, np.uint64)
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    f64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    u64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def forward():
+        _lib.call("gt_pagerank_uneven", dev.handle, G, u32(voff), f64(damp), C.byref(wprop),
+                  C.byref(params), C.c_void_p(x.data_ptr()), _lib.GT_DEVICE, u64(it_f))
+
+    def backward():
+        _lib.call("gt_pagerank_uneven_backward", dev.handle, G, u32(voff), f64(damp),
+                  C.byref(wprop), C.byref(params), C.c_void_p(x.data_ptr()),
+                  C.c_void_p(c.data_ptr()), _lib.GT_DEVICE, C.c_void_p(gw.data_ptr()),
+                  C.c_void_p(gp.data_ptr()), f64(gd), u64(it_b))
+
+    for _ in range(args.warmup):
+        forward()
+        backward()
+    tf = tb = 0.0
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        forward()
+        torch.cuda.synchronize()
+        tf += time.perf_counter() - t
+        t = time.perf_counter()
+        backward()
+        torch.cuda.synchronize()
+        tb += time.perf_counter() - t
+    mg = np.diff(eoff).astype(np.float64)
+    ng = np.diff(voff.astype(np.int64)).astype(np.float64)
+    itf, itb = it_f.astype(np.float64), it_b.astype(np.float64)
+    edges_f = float((mg * itf).sum())
+    edges_b = float((mg * itb).sum())
+    bytes_f = float(((8 * mg + 4 * (ng + 1) + 20 * ng) * itf).sum())
+    dt = tf + tb
+    gteps = (edges_f + edges_b) * args.steps / dt / 1e9
+    ach = bytes_f * args.steps / tf / 1e9
+    log(f"[bench] uneven: forward {tf/args.steps*1e3:.2f} ms (sweeps {int(itf.min())}.."
+        f"{int(itf.max())}), backward {tb/args.steps*1e3:.2f} ms (sweeps {int(itb.min())}.."
+        f"{int(itb.max())}), {gteps:.1f} GTEPS")
+    result = {
+        "metric": "Uneven batch PageRank forward + adjoint GTEPS (4096 Chung-Lu graphs)",
+        "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 values/weights/SpMV, f64 residuals", "data": "synthetic",
+        "config": {"workload": "uneven_chung_lu_4096", "graphs": G, "vertices": n,
+                   "edges": m, "n_range": [64, 65536], "exponent": 2.1, "mean_degree": 8,
+                   "damping": [0.5, 0.95], "weights": "f32 1 - U[0,1)", "epsilon": eps},
+        "forward_ms": round(tf / args.steps * 1e3, 4),
+        "backward_ms": round(tb / args.steps * 1e3, 4),
+        "backward_over_forward": round(tb / tf, 4),
+        "forward_sweeps": {"min": int(itf.min()), "max": int(itf.max()),
+                           "mean": round(float(itf.mean()), 2)},
+        "adjoint_sweeps": {"min": int(itb.min()), "max": int(itb.max()),
+                           "mean": round(float(itb.mean()), 2)},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "u_sweep<UFwdFin<true>> (forward, wall time of the call)",
+                     "bytes_per_solve": int(bytes_f)},
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        xh = x.cpu().numpy()
+        t0 = time.perf_counter()
+        done, edges, worst, sw_ok = 0, 0.0, 0.0, 0
+        order = np.random.default_rng(1).permutation(G)
+        for k in order:
+            lo, hi = int(eoff[k]), int(eoff[k + 1])
+            o = oracle.Adj(int(ng[k]), src[lo:hi] - voff[k], tgt[lo:hi] - voff[k])
+            want, ito = o.pagerank(weight=w_host[lo:hi].astype(np.float64),
+                                   damping=float(damp[k]), epsilon=eps, grain=4096,
+                                   threads=threads)
+            worst = max(worst, float(np.abs(xh[voff[k]:voff[k + 1]] - want).sum()))
+            sw_ok += int(abs(ito - int(it_f[k])) <= 1)
+            edges += mg[k] * ito
+            done += 1
+            if time.perf_counter() - t0 >= 10.0:
+                break
+        dtc = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": round(edges / dtc / 1e9, 4), "unit": "GTEPS (forward only)",
+            "cores": threads, "kind": "port",
+            "sample": f"{done} random graphs of the batch, each solved alone by the oracle's "
+                      f"pagerank_with (forward only, graph build included), {dtc:.1f}s"}
+        result["accuracy_vs_fp64_oracle"] = {"graphs_checked": done, "max_l1": worst,
+                                             "sweeps_within_1": f"{sw_ok}/{done}"}
+    print(json.dumps(result), flush=True)
+    dev.close()
+    return 0
+
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("pagerank", "batched", "adjoint"),
+    ap.add_argument("--workload", choices=("pagerank", "batched", "adjoint", "uneven"),
                     default="pagerank",
                     help="pagerank: R-MAT 22 FP32/FP64 solves (the headline); batched: k=64 "
                          "personalised SpMM on weighted R-MAT 20; adjoint: weighted R-MAT 22 "
-                         "forward + gradient")
+                         "forward + gradient; uneven: 4096 Chung-Lu graphs, forward + adjoint")
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--epsilon", type=float, default=1e-9)
@@ -321,6 +450,8 @@ def main() -> int:
         return bench_batched(args, torch, gt_hip, _lib)
     if args.workload == "adjoint":
         return bench_adjoint(args, torch, gt_hip, _lib)
+    if args.workload == "uneven":
+        return bench_uneven(args, torch, gt_hip, _lib)
     if args.scale is None:
         args.scale = 22
 
