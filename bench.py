@@ -97,6 +97,21 @@ def _upload_rmat(gt_hip, scale: int, edgefactor: int, gid: int):
     return n, m, src, tgt, dev
 
 
+def _tune(args, dev, workload: str, weight) -> dict:
+    """Autotune the graph's launch parameters before the timed region (the
+    sweeps timed afterwards run with the winner); report what was chosen."""
+    if args.no_tune:
+        return {"autotuned": False, "params": dev.tuning()}
+    from gt_hip import autotune
+    t0 = time.perf_counter()
+    rec = autotune.autotune(dev, workload, weight=weight, cache=None)
+    log(f"[bench] autotune ({workload}): {rec['tuning']} -> {rec['sweep_us']:.1f} us/sweep "
+        f"(defaults {rec['default_sweep_us']:.1f}), {len(rec['trials'])} trials in "
+        f"{time.perf_counter()-t0:.1f}s")
+    return {"autotuned": True, "params": dev.tuning(), "sweep_us": rec["sweep_us"],
+            "default_sweep_us": rec["default_sweep_us"], "trials": len(rec["trials"])}
+
+
 def _timed_solves(torch, dev, kind: int, steps: int, warmup: int, solve):
     for _ in range(warmup):
         solve()
@@ -133,6 +148,7 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
     X = torch.empty((n, k), dtype=torch.float32, device="cuda")
     wprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F32, _lib.GT_DEVICE, 0, 0.0,
                         C.c_void_p(w.data_ptr()), m, 1)
+    tuning = _tune(args, dev, "batched", w)
     eps = 1e-6
     params = _lib.GtPowerIteration(eps, 0, _lib.GtPlan(n, -(-n // 4096)))
     view = _lib.GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0)
@@ -170,6 +186,7 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                      "kernel": "ppr_sweep<true> (batched SpMM sweep)", "bytes_per_sweep": B,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
         "column_sweeps": {"min": int(it.min()), "max": int(it.max()), "sum": col_sweeps},
+        "tuning": tuning,
     }
     if not args.no_cpu:
         import oracle
@@ -431,6 +448,8 @@ def main() -> int:
     ap.add_argument("--epsilon", type=float, default=1e-9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle")
     ap.add_argument("--no-eigen", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in launch parameters instead of autotuning them")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -475,6 +494,7 @@ def main() -> int:
     copy = copy_calibration(torch)
     log(f"[bench] copy calibration: {copy['bytes']/2**30:.0f} GiB in {copy['us']:.0f} us "
         f"= {copy['gbs']:.0f} GB/s")
+    tuning = _tune(args, dev, "pagerank", None)
     rank = torch.empty(n, dtype=torch.float64, device="cuda")
     params = _lib.GtPowerIteration(args.epsilon, 0, _lib.GtPlan(n, -(-n // 4096)))
     view = _lib.GtView(_lib.GT_VIEW_DIRECTED, 0, None, 0, None, 0)
@@ -556,6 +576,7 @@ def main() -> int:
             "bytes_per_sweep": b32, "avg_launch_us": round(avg32 * 1e6, 2),
             "traffic_source": traffic_src, "copy_calibration": copy,
         },
+        "tuning": tuning,
         "fp64_dropin": {
             "gteps": round(gteps64, 3), "ms_per_solve": round(dt64 / args.steps * 1e3, 4),
             "sweeps": sw64[0], "achieved_gbs": round(ach64, 1),

@@ -36,6 +36,7 @@ struct AdjArgs {
     const uint32_t *row_off;   // push orientation (rows = sources)
     const uint32_t *packed;    // packed targets of the push orientation's wave tiles
     uint32_t bits;             // slot bits of `packed`
+    uint32_t short_row;        // rows with fewer entries are summed by one lane
     const void *w;             // weights in packed order (value type)
     const double *deg;         // weighted out-degree (weighted)
     const double *cvec;        // dL/dx, f64
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
     AdjFin<V, WEIGHTED, DPERS> fin(a, a.st->p_sink);
     spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits,
                             static_cast<const V *>(a.l_cur), static_cast<const V *>(a.w),
-                            a.long_part, L, fin);
+                            a.long_part, a.short_row, L, fin);
     double d = fin.dsum, p = fin.psum;
     block_sum2(d, p, L.red);
     if (threadIdx.x == 0) {
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void adj_mu(const AdjArgs a, const V *lam, 
     __shared__ WaveLds<V> L;
     MuFin<V, WEIGHTED> fin(a, mu);
     spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, lam,
-                            static_cast<const V *>(a.w), a.long_part, L, fin);
+                            static_cast<const V *>(a.w), a.long_part, a.short_row, L, fin);
 }
 
 // The split rows of adj_mu, one thread each.
@@ -273,9 +274,8 @@ __global__ __launch_bounds__(kBlock) void f32_to_f64_v(const float *a, uint32_t 
 }
 
 template <typename V, bool W, bool P>
-static int adj_launch(const AdjArgs &a, hipStream_t s) {
-    static const int cap = occupancy_cap((const void *)adj_sweep<V, W, P>, kSweepBlocksPerCU);
-    const int sg = wave_grid(cap, a.ntiles);
+static int adj_launch(const AdjArgs &a, int per_cu, hipStream_t s) {
+    const int sg = wave_grid(occupancy_cap((const void *)adj_sweep<V, W, P>, per_cu), a.ntiles);
     hipLaunchKernelGGL((adj_sweep<V, W, P>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
 }
@@ -382,6 +382,7 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     a.row_off = push->off.as<uint32_t>();
     a.packed = push->packed.as<uint32_t>();
     a.bits = push->bits;
+    a.short_row = (uint32_t)g->tune.short_row;
     a.cvec = c;
     a.damping = damping;
     a.eps = params->epsilon;
@@ -405,14 +406,15 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     g->prof_kind = GT_SWEEP_ADJOINT;
     AdjArgs b = a;
     int sg = 0;
+    const int pc = g->tune.sweep_blocks_per_cu;
     auto sweep = [&](uint64_t q) {
         const int par = (int)(q & 1);
         b.l_cur = lam[par];
         b.l_next = lam[par ^ 1];
-        if (weighted && dpers) sg = adj_launch<V, true, true>(b, s);
-        else if (weighted) sg = adj_launch<V, true, false>(b, s);
-        else if (dpers) sg = adj_launch<V, false, true>(b, s);
-        else sg = adj_launch<V, false, false>(b, s);
+        if (weighted && dpers) sg = adj_launch<V, true, true>(b, pc, s);
+        else if (weighted) sg = adj_launch<V, true, false>(b, pc, s);
+        else if (dpers) sg = adj_launch<V, false, true>(b, pc, s);
+        else sg = adj_launch<V, false, false>(b, pc, s);
     };
     auto fold = [&](uint64_t) {
         if (weighted && dpers) adj_launch_fold<V, true, true>(b, sg, s);
@@ -438,9 +440,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         GT_TRY(ws.mu.reserve(sizeof(double) * n));
         double *mu = ws.mu.as<double>();
         AdjArgs b = a;
-        static const int cap_mu =
-            occupancy_cap((const void *)adj_mu<V, true>, kSweepBlocksPerCU);
-        const dim3 mg(wave_grid(cap_mu, a.ntiles)), lg(fold_grid(a.nlong)), bd(kBlock);
+        const dim3 mg(wave_grid(occupancy_cap((const void *)adj_mu<V, true>, pc), a.ntiles)),
+            lg(fold_grid(a.nlong)), bd(kBlock);
         if (weighted) {
             hipLaunchKernelGGL((adj_mu<V, true>), mg, bd, 0, s, b, lf, mu);
             hipLaunchKernelGGL((adj_mu_long<V, true>), lg, bd, 0, s, b, mu);

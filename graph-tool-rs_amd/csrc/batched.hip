@@ -323,10 +323,10 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
         a.deg = ws.deg.as<double>();
         a.w = ws.wcsc.as<float>();
     }
-    GT_TRY(ensure_schedule_mm(*pull, n));
+    GT_TRY(ensure_schedule_mm(*pull, n, (uint32_t)g->tune.mm_tile_nnz));
     const Schedule &S = pull->mm;
-    static const int cap_w = occupancy_cap((const void *)ppr_sweep<true>);
-    static const int cap_u = occupancy_cap((const void *)ppr_sweep<false>);
+    const int cap_w = occupancy_cap((const void *)ppr_sweep<true>, g->tune.mm_blocks_per_cu);
+    const int cap_u = occupancy_cap((const void *)ppr_sweep<false>, g->tune.mm_blocks_per_cu);
     const int grid_w = fit_grid(cap_w, S.ntiles), grid_u = fit_grid(cap_u, S.ntiles);
     // At least 64 fold workgroups: each sums at most 32 of the sweep's rows of
     // partials before the last one folds the workgroup sums.

@@ -49,6 +49,7 @@ struct EvArgs {
     const uint32_t *in_off;
     const uint32_t *packed;   // packed entry stream of the wave tiles
     uint32_t bits;            // slot bits of `packed`
+    uint32_t short_row;        // rows with fewer entries are summed by one lane
     const double *w;          // weights in packed order
     const uint8_t *active;
     const double *y_cur;      // y_{L-1}
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     const bool first = a.st->launches == 0;
     EvFin<MASK> fin(a, a.st->sc_cur, a.st->sc_prev, first);
     spmv_waves<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur, a.w,
-                                 a.long_part, L, fin);
+                                 a.long_part, a.short_row, L, fin);
     double sq = fin.sq, dl = fin.dl;
     block_sum2(sq, dl, L.red);
     if (threadIdx.x == 0) {
@@ -204,9 +205,8 @@ __global__ __launch_bounds__(kBlock) void ev_emit(const double *y, const uint8_t
 }
 
 template <bool W, bool M>
-static int ev_launch(const EvArgs &a, int grid, hipStream_t s) {
-    static const int cap = occupancy_cap((const void *)ev_sweep<W, M>, kSweepBlocksPerCU);
-    const int sg = wave_grid(grid > 0 ? grid : cap, a.ntiles);
+static int ev_launch(const EvArgs &a, int per_cu, hipStream_t s) {
+    const int sg = wave_grid(occupancy_cap((const void *)ev_sweep<W, M>, per_cu), a.ntiles);
     hipLaunchKernelGGL((ev_sweep<W, M>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
 }
@@ -308,6 +308,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.in_off = pull->off.as<uint32_t>();
         a.packed = pull->packed.as<uint32_t>();
         a.bits = pull->bits;
+        a.short_row = (uint32_t)g->tune.short_row;
         a.w = wp;
         a.active = active;
         a.epsilon = eps;
@@ -319,16 +320,17 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.st = st;
         EvArgs b = a;
         int sg = 0;
+        const int pc = g->tune.sweep_blocks_per_cu;
         auto sweep = [&](uint64_t k) {
             // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
             const uint64_t L = k + 1;
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            if (weighted && masked) sg = ev_launch<true, true>(b, g->grid, s);
-            else if (weighted) sg = ev_launch<true, false>(b, g->grid, s);
-            else if (masked) sg = ev_launch<false, true>(b, g->grid, s);
-            else sg = ev_launch<false, false>(b, g->grid, s);
+            if (weighted && masked) sg = ev_launch<true, true>(b, pc, s);
+            else if (weighted) sg = ev_launch<true, false>(b, pc, s);
+            else if (masked) sg = ev_launch<false, true>(b, pc, s);
+            else sg = ev_launch<false, false>(b, pc, s);
         };
         auto fold = [&](uint64_t) {
             if (weighted && masked) ev_launch_fold<true, true>(b, sg, s);

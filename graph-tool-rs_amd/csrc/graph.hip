@@ -2,6 +2,8 @@
 // schedule, streams and the error channel of the C ABI.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "gt_internal.hpp"
 
@@ -40,15 +42,46 @@ gt_status DevBuf::reserve(size_t b) {
 }
 
 int occupancy_cap(const void *kernel, int per_cu) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 2048;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess ||
-        cus <= 0 || per <= 0)
-        return 2048;
+    // Compute-unit count and per-kernel occupancy are looked up once.
+    static std::mutex mu;
+    static std::unordered_map<const void *, int> occ;
+    static int cus = -1;
+    int per = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (cus < 0) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                    hipSuccess)
+                cus = 0;
+        }
+        auto it = occ.find(kernel);
+        if (it == occ.end()) {
+            int p = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, kernel, kBlock, 0) != hipSuccess)
+                p = 0;
+            it = occ.emplace(kernel, p).first;
+        }
+        per = it->second;
+    }
+    if (cus <= 0 || per <= 0) return 2048;
     if (per > per_cu) per = per_cu;
     const int g = per * cus;
     return g < 2048 ? g : 2048;
+}
+
+gt_tuning resolve_tuning(const gt_tuning &t) {
+    gt_tuning r{};
+    r.sweep_blocks_per_cu = t.sweep_blocks_per_cu > 0 ? std::min(t.sweep_blocks_per_cu, 8)
+                                                      : kSweepBlocksPerCU;
+    r.tile_rows = t.tile_rows > 0 ? std::max(16, std::min<int>(t.tile_rows, kWaveTileRows))
+                                  : (int)kWaveTileRows;
+    r.short_row = t.short_row > 0 ? std::min(t.short_row, 1024) : (int)kShortRowDefault;
+    r.mm_tile_nnz = t.mm_tile_nnz > 0 ? std::max(256, std::min(t.mm_tile_nnz, 8192))
+                                      : (int)kMmTileNnz;
+    r.mm_blocks_per_cu = t.mm_blocks_per_cu > 0 ? std::min(t.mm_blocks_per_cu, 8) : 8;
+    return r;
 }
 
 int fit_grid(int cap, uint32_t units) {
@@ -81,6 +114,7 @@ gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bou
         return hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
     }
     g->stream = g->own_stream;
+    g->tune = resolve_tuning(gt_tuning{});
     e = hipHostMalloc((void **)&g->h_state, sizeof(SweepState), hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(g->own_stream);
@@ -143,7 +177,8 @@ __global__ __launch_bounds__(kBlock) void pack_tiles(const Tile *__restrict__ ti
 // a bound, and the host keeps the tile list.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                           const std::vector<uint32_t> *bounds) {
-    if (o.scheduled && (!bounds || o.bounds == *bounds)) return GT_OK;
+    const uint32_t max_rows = (uint32_t)g->tune.tile_rows;
+    if (o.scheduled && o.sched_rows == max_rows && (!bounds || o.bounds == *bounds)) return GT_OK;
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;
@@ -170,7 +205,7 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
         }
         const uint32_t r0 = r;
         uint32_t nnz = 0;
-        while (r < stop && r - r0 < kWaveTileRows) {
+        while (r < stop && r - r0 < max_rows) {
             const uint32_t l = off[r + 1] - off[r];
             if (l > kWaveTileNnz || nnz + l > kWaveTileNnz) break;
             nnz += l;
@@ -191,6 +226,7 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
     o.ntiles = (uint32_t)tiles.size();
     o.nlong = (uint32_t)long_rows.size();
     o.nslots = nslots;
+    o.sched_rows = max_rows;
     // Sources and a kWaveSlotBits-bit slot share one 32-bit word when every
     // vertex id fits in the remaining bits.
     o.bits = (uint64_t)n <= (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
@@ -221,20 +257,22 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
     return GT_OK;
 }
 
-gt_status ensure_schedule_mm(Orient &o, uint32_t n) {
+gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz) {
     Schedule &S = o.mm;
-    if (S.ready) return GT_OK;
+    if (S.ready && S.tile_nnz == tile_nnz) return GT_OK;
+    const uint32_t tnnz = tile_nnz;
+    const uint32_t trows = std::max<uint32_t>(1, tile_nnz / 4);
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;
     uint32_t nslots = 0, r = 0;
     while (r < n) {
         const uint32_t len = off[r + 1] - off[r];
-        if (len > kMmTileNnz) {
+        if (len > tnnz) {
             long_rows.push_back(r);
             long_slot.push_back(nslots);
-            for (uint32_t b = off[r]; b < off[r + 1]; b += kMmTileNnz) {
-                const uint32_t e = std::min<uint64_t>((uint64_t)b + kMmTileNnz, off[r + 1]);
+            for (uint32_t b = off[r]; b < off[r + 1]; b += tnnz) {
+                const uint32_t e = std::min<uint64_t>((uint64_t)b + tnnz, off[r + 1]);
                 tiles.push_back(Tile{nslots++, kSplit, b, e});
             }
             ++r;
@@ -242,9 +280,9 @@ gt_status ensure_schedule_mm(Orient &o, uint32_t n) {
         }
         const uint32_t r0 = r;
         uint32_t nnz = 0;
-        while (r < n && r - r0 < kMmTileRows) {
+        while (r < n && r - r0 < trows) {
             const uint32_t l = off[r + 1] - off[r];
-            if (l > kMmTileNnz || nnz + l > kMmTileNnz) break;
+            if (l > tnnz || nnz + l > tnnz) break;
             nnz += l;
             ++r;
         }
@@ -265,6 +303,7 @@ gt_status ensure_schedule_mm(Orient &o, uint32_t n) {
                          hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(S.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
                      hipMemcpyHostToDevice));
+    S.tile_nnz = tile_nnz;
     S.ready = true;
     return GT_OK;
 }
@@ -477,6 +516,20 @@ gt_status gt_graph_set_stream(gt_graph *g, void *hip_stream) {
     clear_error();
     if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
     g->stream = hip_stream ? (hipStream_t)hip_stream : g->own_stream;
+    return GT_OK;
+}
+
+gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t) {
+    clear_error();
+    if (!g || !t) return fail(GT_ERR_INVALID_ARG, "null argument");
+    g->tune = resolve_tuning(*t);
+    return GT_OK;
+}
+
+gt_status gt_graph_get_tuning(const gt_graph *g, gt_tuning *t) {
+    clear_error();
+    if (!g || !t) return fail(GT_ERR_INVALID_ARG, "null argument");
+    *t = g->tune;
     return GT_OK;
 }
 

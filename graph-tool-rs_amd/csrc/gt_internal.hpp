@@ -93,13 +93,14 @@ struct Schedule {
     DevBuf long_rows;        // uint32[nlong]: row of each split row
     DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
+    uint32_t tile_nnz = 0;   // entry bound the schedule was built with
     bool ready = false;
 };
 
 // SpMM tiles: each in-entry gathers a 64-column row (256 bytes), so tiles are
-// smaller and long rows split sooner.
+// smaller and long rows split sooner.  Default entries per tile (at most a
+// quarter as many rows).
 constexpr uint32_t kMmTileNnz = 1024;
-constexpr uint32_t kMmTileRows = 256;
 
 // One orientation of the adjacency: row r lists the entries a kernel pulls
 // from (in-halves for the CSC, out-halves for the CSR), in storage order.
@@ -116,6 +117,7 @@ struct Orient {
     DevBuf peid;             // uint32[m]: edge id of each packed entry
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
     uint32_t bits = 0;       // slot bits of `packed` (0: plain sources, row order)
+    uint32_t sched_rows = 0; // rows-per-tile bound the schedule was built with
     bool scheduled = false;
     // Set when tiles were cut at these row bounds (a batch of graphs): the
     // host copies of the tile list and the split rows.
@@ -209,7 +211,7 @@ struct gt_graph {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     gt::Workspace ws;
-    int grid = 0;           // sweep workgroups (0: resident count), at most 2048
+    gt_tuning tune{};       // effective launch / schedule parameters
     bool profiling = false;
     // Summed event time and launch count of sweep launches, per kernel kind
     // (gt_sweep_kind); prof_kind is the kind being launched.
@@ -227,8 +229,8 @@ namespace gt {
 // host offsets, then the packed entry stream and its edge ids on the device.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                           const std::vector<uint32_t> *bounds = nullptr);
-// Build (once) the SpMM schedule of an orientation.
-gt_status ensure_schedule_mm(Orient &o, uint32_t n);
+// Build (once per tile size) the SpMM schedule of an orientation.
+gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz);
 // Allocate a graph shell with its stream.
 gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
                     gt_graph **out);
@@ -237,11 +239,16 @@ gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bou
 // tile loop launched with this many workgroups has no second wave of
 // workgroups.
 int occupancy_cap(const void *kernel, int per_cu = 8);
-// Resident workgroups per compute unit for the wave-engine sweeps.  More
-// wavefronts than 12 per CU only add contention for the L2 the gathers live
-// in (R-MAT scale 22: 3 per CU is the fastest f32 and f64 sweep, 8 per CU is
-// 40% slower).
+// Default resident workgroups per compute unit for the wave-engine sweeps.
+// More wavefronts than 12 per CU only add contention for the L2 the gathers
+// live in (R-MAT scale 22: 3 per CU is the fastest f32 and f64 sweep, 8 per
+// CU is 40% slower).
 constexpr int kSweepBlocksPerCU = 3;
+// Default short-row threshold: rows with fewer entries are summed by one lane.
+constexpr uint32_t kShortRowDefault = 32;
+// The tuning in effect for `t` (zero fields replaced by defaults, others
+// clamped to their ranges).
+gt_tuning resolve_tuning(const gt_tuning &t);
 // min(cap, units), at least 1.
 int fit_grid(int cap, uint32_t units);
 // Most workgroups of a sweep launch; the partials buffer holds two f64

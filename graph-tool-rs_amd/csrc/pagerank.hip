@@ -36,6 +36,7 @@ struct PrArgs {
     const uint32_t *in_off;
     const uint32_t *packed;   // packed entry stream of the wave tiles
     uint32_t bits;            // slot bits of `packed`
+    uint32_t short_row;        // rows with fewer entries are summed by one lane
     const void *w;            // weights in packed order (value type), weighted only
     const uint32_t *out_off;  // unity out-degree source
     const double *deg;        // weighted out-degree
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     PrFin<V, WEIGHTED, DPERS, MASK> fin(a, a.st->p_sink);
     spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits,
                             static_cast<const V *>(a.c_cur), static_cast<const V *>(a.w),
-                            a.long_part, L, fin);
+                            a.long_part, a.short_row, L, fin);
     double dsum = fin.dsum, ssum = fin.ssum;
     block_sum2(dsum, ssum, L.red);
     if (threadIdx.x == 0) {
@@ -244,12 +245,11 @@ struct PrPlan {
     bool masked = false;
 };
 
-// The sweep kernel; returns its workgroup count.  `grid` > 0 overrides the
-// resident-workgroup count (at most kMaxSweepGrid).
+// The sweep kernel with at most `per_cu` resident workgroups per compute
+// unit; returns its workgroup count.
 template <typename V, bool W, bool P, bool M>
-int launch_sweep(const PrArgs &a, int grid, hipStream_t s) {
-    static const int cap = occupancy_cap((const void *)pr_sweep<V, W, P, M>, kSweepBlocksPerCU);
-    const int sg = wave_grid(grid > 0 ? grid : cap, a.ntiles);
+int launch_sweep(const PrArgs &a, int per_cu, hipStream_t s) {
+    const int sg = wave_grid(occupancy_cap((const void *)pr_sweep<V, W, P, M>, per_cu), a.ntiles);
     hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
 }
@@ -460,6 +460,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     a.in_off = S.pull->off.as<uint32_t>();
     a.packed = S.pull->packed.as<uint32_t>();
     a.bits = S.pull->bits;
+    a.short_row = (uint32_t)g->tune.short_row;
     a.damping = damping;
     a.eps_abs = params->epsilon;
     a.eps_rel = 0.0;
@@ -493,7 +494,7 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        sg = dispatch_sweep<V>(S.plan, b, g->grid, s);
+        sg = dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
     };
     auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };
     GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched));

@@ -17,8 +17,9 @@
 //   phase 1  kWaveTileNnz / 64 packed words per lane in flight (coalesced),
 //            as many gathers, each value times the packed-order weight when
 //            weighted, scattered into the wavefront's LDS tile;
-//   phase 2  one lane per row when the row has fewer than kShortRow entries,
-//            the whole wavefront (shuffle tree) for longer rows;
+//   phase 2  one lane per row when the row has fewer than `short_row` entries
+//            (a tuning parameter), the whole wavefront (shuffle tree) for
+//            longer rows;
 //   phase 3  `fin(row, sum)`, lane q for rows q, q + 64, ...: the rows'
 //            global reads and writes go out coalesced.
 // A split chunk is summed straight from registers into its slot.
@@ -35,8 +36,6 @@
 namespace gt {
 
 static_assert(kWavesPerBlock * kWave == kBlock, "workgroups are whole wavefronts");
-// Rows of at least this many entries are summed by the whole wavefront.
-constexpr uint32_t kShortRow = 32;
 
 template <typename V>
 struct WaveLds {
@@ -64,8 +63,8 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
                                            const V *__restrict__ src, const V *__restrict__ w,
-                                           double *__restrict__ long_part, WaveLds<V> &L,
-                                           Fin &fin) {
+                                           double *__restrict__ long_part, uint32_t short_row,
+                                           WaveLds<V> &L, Fin &fin) {
     constexpr int K = kWaveTileNnz / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -119,7 +118,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             if (q < rows) {
                 lo = off[rb + q] - nb;
                 hi = off[rb + q + 1] - nb;
-                if (hi - lo < kShortRow) {
+                if (hi - lo < short_row) {
                     V acc = V(0);
                     for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
                     rowsum[q] = acc;

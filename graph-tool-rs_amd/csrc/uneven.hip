@@ -53,6 +53,7 @@ struct UArgs {
     const uint32_t *off;        // pull (forward) / push (adjoint) row offsets
     const uint32_t *packed;
     uint32_t bits;
+    uint32_t short_row;        // rows with fewer entries are summed by one lane
     const float *w;             // packed-order weights (weighted)
     const uint32_t *deg_off;    // unity out-degree: push orientation offsets
     const double *deg;          // weighted out-degree
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void u_sweep(const UArgs a) {
     __shared__ WaveLds<float> L;
     Fin fin(a);
     spmv_waves<float, WEIGHTED>(a.tiles, a.ntiles, a.off, a.packed, a.bits, a.src, a.w,
-                                a.long_part, L, fin);
+                                a.long_part, a.short_row, L, fin);
 }
 
 // One wavefront per graph: the graph's tile partials in tile order, its split
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void u_mu(const UArgs a, double *mu) {
     __shared__ WaveLds<float> L;
     UMuFin<WEIGHTED> fin(a, mu);
     spmv_waves<float, WEIGHTED>(a.tiles, a.ntiles, a.off, a.packed, a.bits, a.src, a.w,
-                                a.long_part, L, fin);
+                                a.long_part, a.short_row, L, fin);
 }
 
 template <bool WEIGHTED>
@@ -498,6 +499,7 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     a.off = o.off.as<uint32_t>();
     a.packed = o.packed.as<uint32_t>();
     a.bits = o.bits;
+    a.short_row = (uint32_t)g->tune.short_row;
     a.long_rows = o.long_rows.as<uint32_t>();
     a.long_slot = o.long_slot.as<uint32_t>();
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, o.nslots)));
@@ -650,10 +652,10 @@ gt_status pagerank_uneven_impl(gt_graph *g, uint32_t G, const uint32_t *voff,
         hipLaunchKernelGGL(u_fwd_psink, dim3(ugrid(G)), dim3(kBlock), 0, s, b);
         GT_HIP(hipGetLastError());
     }
-    static const int cap_w =
-        occupancy_cap((const void *)u_sweep<UFwdFin<true>, true>, kSweepBlocksPerCU);
-    static const int cap_u =
-        occupancy_cap((const void *)u_sweep<UFwdFin<false>, false>, kSweepBlocksPerCU);
+    const int cap_w = occupancy_cap(
+        (const void *)u_sweep<UFwdFin<true>, true>, g->tune.sweep_blocks_per_cu);
+    const int cap_u = occupancy_cap(
+        (const void *)u_sweep<UFwdFin<false>, false>, g->tune.sweep_blocks_per_cu);
     const dim3 sg(wave_grid(weighted ? cap_w : cap_u, a.ntiles)),
         fg((unsigned)std::max<uint32_t>(1, (G + kWavesPerBlock - 1) / kWavesPerBlock)),
         bd(kBlock);
@@ -732,10 +734,10 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     hipLaunchKernelGGL(u_graph_sums<AdjStartSums>, dim3(gg), dim3(kBlock), 0, s, a.voff, G,
                        AdjStartSums{cd, a.voff}, a.st.p_sink, a.st.scale);
     GT_HIP(hipGetLastError());
-    static const int cap_w =
-        occupancy_cap((const void *)u_sweep<UAdjFin<true>, true>, kSweepBlocksPerCU);
-    static const int cap_u =
-        occupancy_cap((const void *)u_sweep<UAdjFin<false>, false>, kSweepBlocksPerCU);
+    const int cap_w = occupancy_cap(
+        (const void *)u_sweep<UAdjFin<true>, true>, g->tune.sweep_blocks_per_cu);
+    const int cap_u = occupancy_cap(
+        (const void *)u_sweep<UAdjFin<false>, false>, g->tune.sweep_blocks_per_cu);
     const dim3 sg(wave_grid(weighted ? cap_w : cap_u, a.ntiles)),
         fg((unsigned)std::max<uint32_t>(1, (G + kWavesPerBlock - 1) / kWavesPerBlock)),
         bd(kBlock);

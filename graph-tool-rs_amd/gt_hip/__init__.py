@@ -286,6 +286,27 @@ class DeviceGraph:
     def set_stream(self, stream_ptr: Optional[int]) -> None:
         call("gt_graph_set_stream", self._h, C.c_void_p(stream_ptr or 0))
 
+    TUNING_FIELDS = ("sweep_blocks_per_cu", "tile_rows", "short_row", "mm_tile_nnz",
+                     "mm_blocks_per_cu")
+
+    def tuning(self) -> dict:
+        """Launch / schedule parameters in effect (gt_graph_get_tuning)."""
+        t = _lib.GtTuning()
+        call("gt_graph_get_tuning", self.handle, C.byref(t))
+        return {f: int(getattr(t, f)) for f in self.TUNING_FIELDS}
+
+    def set_tuning(self, **kw) -> dict:
+        """Set some parameters (others keep their values); 0 restores a default.
+        Returns the values in effect."""
+        cur = self.tuning()
+        for k, v in kw.items():
+            if k not in self.TUNING_FIELDS:
+                raise GtValueError(_lib.GT_ERR_INVALID_ARG, f"unknown tuning field {k}")
+            cur[k] = int(v)
+        t = _lib.GtTuning(*[cur[f] for f in self.TUNING_FIELDS])
+        call("gt_graph_set_tuning", self.handle, C.byref(t))
+        return self.tuning()
+
     def set_profiling(self, enable: bool) -> None:
         call("gt_graph_set_profiling", self._h, 1 if enable else 0)
 

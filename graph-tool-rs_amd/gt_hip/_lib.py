@@ -50,6 +50,12 @@ class GtVertexOut(C.Structure):
                 ("loc", C.c_int32), ("reserved", C.c_int32), ("graph_id", C.c_uint64)]
 
 
+class GtTuning(C.Structure):
+    _fields_ = [("sweep_blocks_per_cu", C.c_int32), ("tile_rows", C.c_int32),
+                ("short_row", C.c_int32), ("mm_tile_nnz", C.c_int32),
+                ("mm_blocks_per_cu", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
 class GtView(C.Structure):
     _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("vmask", C.c_void_p),
                 ("vmask_len", C.c_uint64), ("emask", C.c_void_p), ("emask_len", C.c_uint64)]
@@ -99,6 +105,8 @@ _PROTOS = {
                                     C.POINTER(GtProp), C.POINTER(GtPowerIteration), C.c_void_p,
                                     C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "gt_graph_set_tuning": [GraphP, C.POINTER(GtTuning)],
+    "gt_graph_get_tuning": [GraphP, C.POINTER(GtTuning)],
     "gt_gen_chung_lu_batch": [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
                               C.c_double, C.c_double, C.c_double, u32p,
                               C.POINTER(C.c_uint64), C.POINTER(C.c_double), u32p, u32p,

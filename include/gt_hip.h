@@ -249,6 +249,24 @@ gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,
                              const gt_power_iteration *params,
                              gt_vertex_out *x, double *eig, uint64_t *iters);
 
+/* ------------------------------------------------------------- tuning */
+
+/* Extension: launch and schedule parameters of one graph, for an autotuner.
+ * A zero field means the built-in default.  Changing tile_rows or mm_tile_nnz
+ * rebuilds the affected schedules on the next call. */
+typedef struct gt_tuning {
+    int32_t sweep_blocks_per_cu; /* resident workgroups per CU, pull sweeps: 1..8 (3) */
+    int32_t tile_rows;           /* rows per wave tile: 16..256 (256) */
+    int32_t short_row;           /* rows with fewer entries are summed by one lane: 1..1024 (32) */
+    int32_t mm_tile_nnz;         /* batched SpMM: entries per tile, 256..8192 (1024) */
+    int32_t mm_blocks_per_cu;    /* batched SpMM: resident workgroups per CU: 1..8 (8) */
+    int32_t reserved[3];
+} gt_tuning;
+
+gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);
+/* The values in effect (defaults filled in). */
+gt_status gt_graph_get_tuning(const gt_graph *g, gt_tuning *t);
+
 /* ------------------------------------------------------------- profiling */
 
 /* Sweep kernels, for gt_sweep_stats. */
