@@ -536,6 +536,24 @@ def main() -> int:
     log(f"[bench] fp64: {dt64/args.steps*1e3:.3f} ms/solve, sweeps {sw64[0]}, "
         f"launches {la64} ({ms64/max(la64,1)*1e3:.1f} us avg)")
 
+    # The same solve handing over a host rank map (copy in / out over PCIe
+    # inside the call): reported beside `value`, never as it.
+    rank_h = np.zeros(n, np.float64)
+    out_h = _lib.GtVertexOut(rank_h.ctypes.data_as(C.c_void_p), n, 0, _lib.GT_HOST, 0, gid)
+
+    def solve_host() -> None:
+        out_h.had = 0
+        _lib.call("gt_pagerank_mixed", dev.handle, C.byref(view), C.byref(unity), None, 0.85,
+                  C.byref(params), C.byref(out_h), C.byref(iters))
+
+    solve_host()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        solve_host()
+    host_ms = (time.perf_counter() - t) / 3 * 1e3
+    log(f"[bench] mixed with a host rank map: {host_ms:.3f} ms/solve")
+
     # Work per solve: every sweep of the mixed form is a full pass over m edges.
     gteps32 = m * sw32[0] * args.steps / dt32 / 1e9
     gteps64 = m * sw64[0] * args.steps / dt64 / 1e9
@@ -584,6 +602,7 @@ def main() -> int:
             "avg_launch_us": round(avg64 * 1e6, 2),
         },
         "mixed_vs_fp64_speedup": round(dt64 / dt32, 3),
+        "host_rank_map_ms_per_solve": round(host_ms, 4),
         "mixed_sweeps": {"total": sw32[0], "fp64_refinement": f64_in_mixed[1] // max(args.steps, 1)},
         "host_adjacency_s": round(t_adj, 2), "upload_s": round(t_up, 3),
     }
