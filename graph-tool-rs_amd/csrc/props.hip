@@ -30,11 +30,12 @@ template <typename S>
 __global__ __launch_bounds__(kBlock) void deg_short(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ eid,
                                                     const void *w, int kind, double value,
-                                                    uint32_t n, double *__restrict__ deg) {
+                                                    uint32_t n, bool accumulate,
+                                                    double *__restrict__ deg) {
     for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
         const uint32_t lo = off[v], hi = off[v + 1];
         if (hi - lo >= kWaveDeg) continue;
-        double acc = 0.0;
+        double acc = accumulate ? deg[v] : 0.0;
         uint32_t j = lo;
         for (; j + 8 <= hi; j += 8) {
             double x[8];
@@ -56,14 +57,15 @@ __global__ __launch_bounds__(kBlock) void deg_mid(const uint32_t *__restrict__ o
                                                   const uint32_t *__restrict__ eid,
                                                   const void *w, int kind, double value,
                                                   const uint32_t *__restrict__ rows,
-                                                  uint32_t nrows, double *__restrict__ deg) {
+                                                  uint32_t nrows, bool accumulate,
+                                                  double *__restrict__ deg) {
     const int lane = threadIdx.x & (kWave - 1);
     const uint32_t waves = gridDim.x * (kBlock / kWave);
     for (uint32_t k = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; k < nrows;
          k += waves) {
         const uint32_t v = rows[k];
         const uint32_t lo = off[v], hi = off[v + 1];
-        double acc = 0.0;
+        double acc = accumulate ? deg[v] : 0.0;
         for (uint32_t j0 = lo; j0 < hi; j0 += kWave) {
             const uint32_t j = j0 + lane;
             const double x = j < hi ? read_w<S>(w, eid[j], kind, value) : 0.0;
@@ -80,14 +82,14 @@ __global__ __launch_bounds__(kBlock) void deg_long(const uint32_t *__restrict__ 
                                                    const uint32_t *__restrict__ eid,
                                                    const void *w, int kind, double value,
                                                    const uint32_t *__restrict__ rows,
-                                                   double *__restrict__ deg) {
+                                                   bool accumulate, double *__restrict__ deg) {
     __shared__ double red[8];
     const uint32_t v = rows[blockIdx.x];
     const uint32_t lo = off[v], hi = off[v + 1];
     double acc = 0.0;
     for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) acc += read_w<S>(w, eid[j], kind, value);
     acc = block_sum(acc, red);
-    if (threadIdx.x == 0) deg[v] = acc;
+    if (threadIdx.x == 0) deg[v] = accumulate ? deg[v] + acc : acc;
 }
 
 // wcsc[j] = w(eid[j]) in the value type of the sweep.
@@ -165,22 +167,23 @@ gt_status stage_edge_source(gt_graph *g, const gt_prop *w, const void **src, int
 
 template <typename S>
 static gt_status launch_deg(gt_graph *g, Orient &push, const void *src, int kind, double value,
-                            double *deg) {
+                            double *deg, bool accumulate) {
     hipStream_t s = g->stream;
     const uint32_t n = g->n;
     hipLaunchKernelGGL((deg_short<S>), dim3(grid_for(n)), dim3(kBlock), 0, s,
-                       push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value, n, deg);
+                       push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value, n,
+                       accumulate, deg);
     GT_TRY(heavy_rows(push, n));
     if (push.nmid) {
         const unsigned mg = (unsigned)std::min<uint32_t>(8192, (push.nmid + 3) / 4);
         hipLaunchKernelGGL((deg_mid<S>), dim3(mg), dim3(kBlock), 0, s, push.off.as<uint32_t>(),
                            push.eid.as<uint32_t>(), src, kind, value, push.mid.as<uint32_t>(),
-                           push.nmid, deg);
+                           push.nmid, accumulate, deg);
     }
     if (push.nheavy)
         hipLaunchKernelGGL((deg_long<S>), dim3(push.nheavy), dim3(kBlock), 0, s,
                            push.off.as<uint32_t>(), push.eid.as<uint32_t>(), src, kind, value,
-                           push.heavy.as<uint32_t>(), deg);
+                           push.heavy.as<uint32_t>(), accumulate, deg);
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -200,8 +203,16 @@ gt_status stage_degree(gt_graph *g, const gt_prop *w, Orient &push, const void *
                        DevBuf &out) {
     GT_TRY(out.reserve(sizeof(double) * std::max<uint32_t>(1, g->n)));
     if (dtype == GT_F32)
-        return launch_deg<float>(g, push, src, w->kind, w->value, out.as<double>());
-    return launch_deg<double>(g, push, src, w->kind, w->value, out.as<double>());
+        return launch_deg<float>(g, push, src, w->kind, w->value, out.as<double>(), false);
+    return launch_deg<double>(g, push, src, w->kind, w->value, out.as<double>(), false);
+}
+
+gt_status stage_degree_into(gt_graph *g, const gt_prop *w, Orient &o, const void *src, int dtype,
+                            double *out, bool accumulate) {
+    if (g->n == 0) return GT_OK;
+    if (dtype == GT_F32)
+        return launch_deg<float>(g, o, src, w->kind, w->value, out, accumulate);
+    return launch_deg<double>(g, o, src, w->kind, w->value, out, accumulate);
 }
 
 template <typename V>

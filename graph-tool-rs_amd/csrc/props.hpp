@@ -13,6 +13,11 @@ gt_status stage_edge_source(gt_graph *g, const gt_prop *w, const void **src, int
 gt_status stage_degree(gt_graph *g, const gt_prop *w, Orient &push, const void *src, int dtype,
                        DevBuf &out);
 
+// out[v] (+)= sum of w over row v of `o` in storage order; with `accumulate`
+// the sum continues from out[v] (a second half of the same block).
+gt_status stage_degree_into(gt_graph *g, const gt_prop *w, Orient &o, const void *src, int dtype,
+                            double *out, bool accumulate);
+
 // Weights permuted into the storage order of `pull`, converted to V.
 template <typename V>
 gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,

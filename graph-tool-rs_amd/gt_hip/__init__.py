@@ -38,6 +38,9 @@ __all__ = [
     "DeviceGraph", "pagerank", "pagerank_with", "pagerank_mixed", "eigenvector",
     "gen_er", "gen_rmat", "host_adjacency", "GtError", "GtValueError", "device_count",
     "pagerank_uneven", "pagerank_uneven_grad", "gen_chung_lu_batch",
+    "Document", "read_gt", "read_edge_list",
+    "weighted_out_degree", "weighted_in_degree", "weighted_degree", "weighted_out_degrees",
+    "weighted_in_degrees", "weighted_degrees", "out_degrees", "degree_histogram",
 ]
 
 _u32p = C.POINTER(C.c_uint32)
@@ -748,4 +751,209 @@ def pagerank_uneven_grad(dev: "DeviceGraph", vertex_offsets, damping, x, dl_dx, 
          gw.ctypes.data_as(C.c_void_p), gp.ctypes.data_as(C.c_void_p),
          _ptr(gd, C.POINTER(C.c_double)), _ptr(iters, C.POINTER(C.c_uint64)))
     return gw[:eb], gp[:n], gd, iters.astype(np.int64)
+
+
+# ------------------------------------------------------------------ formats
+
+# Value-type indices of the .gt format (graph-tool's value_types order).
+VALUE_KINDS = ("bool", "int16_t", "int32_t", "int64_t", "double", "long double", "string",
+               "vector<bool>", "vector<int16_t>", "vector<int32_t>", "vector<int64_t>",
+               "vector<double>", "vector<long double>", "vector<string>", "python::object")
+DOMAINS = ("graph", "vertex", "edge")
+
+
+class Document:
+    """A parsed graph file (gt_read_gt / gt_read_edge_list): the edge list in
+    edge-id order, the property columns and the vertex names.  ``adjlist()``
+    replays it into an AdjList, ``device()`` builds the device graph."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        n, m, nn = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        d, npr = C.c_int32(), C.c_uint32()
+        call("gt_doc_info", self._h, C.byref(n), C.byref(m), C.byref(d), C.byref(npr),
+             C.byref(nn))
+        self.n, self.m, self.directed = int(n.value), int(m.value), bool(d.value)
+        self._nprops, self._nnames = int(npr.value), int(nn.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.lib().gt_doc_free(h)
+            self._h = None
+
+    @property
+    def comment(self) -> str:
+        p, ln = C.c_void_p(), C.c_uint64()
+        call("gt_doc_comment", self._h, C.byref(p), C.byref(ln))
+        return C.string_at(p.value, ln.value).decode("utf-8", errors="replace") if ln.value else ""
+
+    def edges(self) -> Tuple[np.ndarray, np.ndarray]:
+        src = np.zeros(max(self.m, 1), np.uint32)
+        tgt = np.zeros(max(self.m, 1), np.uint32)
+        call("gt_doc_edges", self._h, _ptr(src), _ptr(tgt))
+        return src[:self.m], tgt[:self.m]
+
+    def names(self) -> list:
+        out = []
+        for v in range(self._nnames):
+            p, ln = C.c_void_p(), C.c_uint64()
+            call("gt_doc_vertex_name", self._h, v, C.byref(p), C.byref(ln))
+            out.append(C.string_at(p.value, ln.value).decode("utf-8") if ln.value else "")
+        return out
+
+    def _cell(self, k: int, i: int) -> bytes:
+        p, ln = C.c_void_p(), C.c_uint64()
+        call("gt_doc_prop_cell", self._h, k, i, C.byref(p), C.byref(ln))
+        return C.string_at(p.value, ln.value) if ln.value else b""
+
+    @staticmethod
+    def _decode(kind: int, b: bytes):
+        name = VALUE_KINDS[kind]
+        if name == "string":
+            return b.decode("utf-8")
+        if name in ("python::object", "long double"):
+            return b  # opaque: never unpickled or converted
+        if name == "vector<long double>":
+            return [b[i:i + 16] for i in range(0, len(b), 16)]
+        if name == "vector<string>":
+            k = int.from_bytes(b[:8], "little")
+            out, pos = [], 8
+            for _ in range(k):
+                ln = int.from_bytes(b[pos:pos + 8], "little")
+                out.append(b[pos + 8:pos + 8 + ln].decode("utf-8"))
+                pos += 8 + ln
+            return out
+        return np.frombuffer(b, np.float64).copy()  # numeric vectors, widened to f64
+
+    def properties(self) -> list:
+        """[{name, domain, kind, values}] in file order; numeric scalar columns
+        as float64 arrays, the others as lists (see _decode)."""
+        out = []
+        for k in range(self._nprops):
+            p, ln = C.c_void_p(), C.c_uint64()
+            dom, kind, cnt = C.c_int32(), C.c_int32(), C.c_uint64()
+            call("gt_doc_prop_info", self._h, k, C.byref(p), C.byref(ln), C.byref(dom),
+                 C.byref(kind), C.byref(cnt))
+            name = C.string_at(p.value, ln.value).decode("utf-8") if ln.value else ""
+            kd = int(kind.value)
+            if kd in (0, 1, 2, 3, 4):
+                vals = np.zeros(max(int(cnt.value), 1), np.float64)
+                call("gt_doc_prop_f64", self._h, k, _ptr(vals, C.POINTER(C.c_double)))
+                vals = vals[:int(cnt.value)]
+            else:
+                vals = [self._decode(kd, self._cell(k, i)) for i in range(int(cnt.value))]
+            out.append({"name": name, "domain": DOMAINS[int(dom.value)],
+                        "kind": VALUE_KINDS[kd], "values": vals})
+        return out
+
+    def adjlist(self) -> "AdjList":
+        g = AdjList.with_vertices(self.n)
+        s, t = self.edges()
+        g.add_edges(s, t)
+        return g
+
+    def device(self, graph_id: Optional[int] = None) -> DeviceGraph:
+        gid = next(_graph_ids) if graph_id is None else graph_id
+        h = _lib.GraphP()
+        call("gt_graph_from_doc", self._h, gid, C.byref(h))
+        return DeviceGraph(h.value, gid)
+
+
+def _bytes_of(src) -> bytes:
+    if isinstance(src, (bytes, bytearray, memoryview)):
+        return bytes(src)
+    with open(src, "rb") as f:
+        return f.read()
+
+
+def read_gt(src) -> Document:
+    """Parse a .gt file (a path or its bytes)."""
+    b = _bytes_of(src)
+    h = C.c_void_p()
+    call("gt_read_gt", C.c_char_p(b), len(b), C.byref(h))
+    return Document(h.value)
+
+
+def read_edge_list(src, delimiter: str = ",", header: bool = False,
+                   symbolic: bool = True) -> Document:
+    """Parse a delimited edge list (a path or its bytes)."""
+    b = _bytes_of(src)
+    d = delimiter.encode("latin-1")
+    if len(d) != 1:
+        raise GtValueError(_lib.GT_ERR_INVALID_ARG, "the delimiter must be one byte")
+    h = C.c_void_p()
+    call("gt_read_edge_list", C.c_char_p(b), len(b), d[0], int(header), int(symbolic),
+         C.byref(h))
+    return Document(h.value)
+
+
+# ------------------------------------------------------------------ degrees
+
+def _degrees(g, which: int, weight) -> np.ndarray:
+    dev, view, gid, bound, nv = _resolve(g)
+    keep = []
+    wp = _prop(weight if weight is not None else Unity.NEW, gid, keep)
+    out = np.zeros(max(bound, 1), np.float64)
+    dev.set_stream(None)
+    call("gt_degrees", dev.handle, C.byref(view), which, C.byref(wp),
+         out.ctypes.data_as(C.c_void_p), GT_HOST)
+    return out[:bound]
+
+
+def weighted_out_degrees(g, weight=None) -> np.ndarray:
+    """``weighted_out_degree`` of every vertex (crates/gt-algo/src/degree.rs:122-140)."""
+    return _degrees(g, 0, weight)
+
+
+def weighted_in_degrees(g, weight=None) -> np.ndarray:
+    """``weighted_in_degree`` of every vertex (degree.rs:146-162)."""
+    return _degrees(g, 1, weight)
+
+
+def weighted_degrees(g, weight=None) -> np.ndarray:
+    """``weighted_degree`` of every vertex (degree.rs:172-185)."""
+    return _degrees(g, 2, weight)
+
+
+def weighted_out_degree(g, v: int, weight=None) -> float:
+    return float(weighted_out_degrees(g, weight)[v])
+
+
+def weighted_in_degree(g, v: int, weight=None) -> float:
+    return float(weighted_in_degrees(g, weight)[v])
+
+
+def weighted_degree(g, v: int, weight=None) -> float:
+    return float(weighted_degrees(g, weight)[v])
+
+
+def out_degrees(g, out: Optional[DenseProp] = None) -> DenseProp:
+    """``out_degrees`` (degree.rs:199-216): an int64 map grown to the vertex
+    bound, every slot rewritten."""
+    dev, view, gid, bound, nv = _resolve(g)
+    if out is None:
+        out = DenseProp(gid, np.zeros(0, np.int64), dtype=np.int64)
+    out.sized_for(gid, bound)
+    buf = np.zeros(max(bound, 1), np.int64)
+    dev.set_stream(None)
+    call("gt_out_degrees", dev.handle, C.byref(view), buf.ctypes.data_as(C.c_void_p), GT_HOST)
+    out.data = np.ascontiguousarray(out.data).astype(np.int64)
+    out.data[:bound] = buf[:bound]
+    return out
+
+
+def degree_histogram(g) -> list:
+    """``degree_histogram`` (degree.rs:230-247): hist[d] = number of the view's
+    vertices of total degree d."""
+    dev, view, gid, bound, nv = _resolve(g)
+    ln = C.c_uint64()
+    dev.set_stream(None)
+    call("gt_degree_histogram", dev.handle, C.byref(view), None, 0, C.byref(ln))
+    if ln.value == 0:
+        return []
+    h = np.zeros(ln.value, np.uint64)
+    call("gt_degree_histogram", dev.handle, C.byref(view), _ptr(h, C.POINTER(C.c_uint64)),
+         ln.value, C.byref(ln))
+    return [int(x) for x in h[:ln.value]]
 

@@ -105,6 +105,27 @@ _PROTOS = {
                                     C.POINTER(GtProp), C.POINTER(GtPowerIteration), C.c_void_p,
                                     C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "gt_degrees": [GraphP, C.POINTER(GtView), C.c_int32, C.POINTER(GtProp), C.c_void_p,
+                   C.c_int32],
+    "gt_out_degrees": [GraphP, C.POINTER(GtView), C.c_void_p, C.c_int32],
+    "gt_degree_histogram": [GraphP, C.POINTER(GtView), C.POINTER(C.c_uint64), C.c_uint64,
+                            C.POINTER(C.c_uint64)],
+    "gt_read_gt": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)],
+    "gt_read_edge_list": [C.c_void_p, C.c_uint64, C.c_uint8, C.c_int32, C.c_int32,
+                          C.POINTER(C.c_void_p)],
+    "gt_doc_free": [C.c_void_p],
+    "gt_doc_info": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                    C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)],
+    "gt_doc_edges": [C.c_void_p, u32p, u32p],
+    "gt_doc_comment": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)],
+    "gt_doc_prop_info": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
+                         C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)],
+    "gt_doc_prop_f64": [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)],
+    "gt_doc_prop_cell": [C.c_void_p, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p),
+                         C.POINTER(C.c_uint64)],
+    "gt_doc_vertex_name": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p),
+                           C.POINTER(C.c_uint64)],
+    "gt_graph_from_doc": [C.c_void_p, C.c_uint64, C.POINTER(GraphP)],
     "gt_graph_set_tuning": [GraphP, C.POINTER(GtTuning)],
     "gt_graph_get_tuning": [GraphP, C.POINTER(GtTuning)],
     "gt_gen_chung_lu_batch": [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,

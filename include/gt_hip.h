@@ -249,6 +249,70 @@ gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,
                              const gt_power_iteration *params,
                              gt_vertex_out *x, double *eig, uint64_t *iters);
 
+/* ------------------------------------------------------------- degrees */
+
+/* The degree kernels of crates/gt-algo/src/degree.rs:122-237, on the device.
+ * which: 0 = weighted_out_degree, 1 = weighted_in_degree, 2 = weighted_degree
+ * (in + out).  weight UNITY: the degree; CONSTANT c: degree * c (0 without
+ * incident edges); DENSE: the sum in storage order (the total walks the
+ * out-half, then the in-half).  out: [n] f64 in out_loc memory; vertices a
+ * filtered view drops get 0.  Views: the four directed kinds and Undirected
+ * (whose out-degree is the whole block; its in-degree is refused). */
+gt_status gt_degrees(gt_graph *g, const gt_view *view, int32_t which, const gt_prop *weight,
+                     double *out, int32_t out_loc);
+/* out_degrees (degree.rs:199-216): [n] int64, every slot written. */
+gt_status gt_out_degrees(gt_graph *g, const gt_view *view, int64_t *out, int32_t out_loc);
+/* degree_histogram (degree.rs:230-247): hist[d] = number of the view's
+ * vertices of total degree d.  *len = max degree + 1 (0 without vertices);
+ * with hist == NULL or cap < *len only *len is written. */
+gt_status gt_degree_histogram(gt_graph *g, const gt_view *view, uint64_t *hist, uint64_t cap,
+                              uint64_t *len);
+
+/* ------------------------------------------------------------- input formats */
+
+/* A parsed graph file: the edge list in edge-id order, property columns and
+ * (for symbolic edge lists) vertex names.  Build the device graph from it
+ * with gt_graph_from_doc. */
+typedef struct gt_doc gt_doc;
+
+/* graph-tool's binary .gt format (crates/gt-io/src/gt.rs:555 `read`, layout
+ * :1-58): the adjacency in file order (edge i = the i-th entry), every
+ * property column (domain 0 graph / 1 vertex / 2 edge, value-type index of
+ * graph-tool's value_types: 0 bool, 1 int16, 2 int32, 3 int64, 4 double,
+ * 5 long double, 6 string, 7..13 vectors of those, 14 python object).
+ * Truncated or malformed input is an error with the byte offset. */
+gt_status gt_read_gt(const void *bytes, uint64_t len, gt_doc **out);
+
+/* A delimited edge list (crates/gt-io/src/csv.rs:129 `read_edge_list`):
+ * Python's default csv dialect, fields stripped, blank records skipped,
+ * `header` drops the first record; `symbolic` interns endpoint names in
+ * first-seen order (source before target), otherwise endpoints are indices
+ * and the graph has max + 1 vertices.  Record i is edge i.  Errors name the
+ * line a record started on. */
+gt_status gt_read_edge_list(const void *bytes, uint64_t len, uint8_t delimiter, int32_t header,
+                            int32_t symbolic, gt_doc **out);
+
+gt_status gt_doc_free(gt_doc *d);
+gt_status gt_doc_info(const gt_doc *d, uint64_t *n, uint64_t *m, int32_t *directed,
+                      uint32_t *nprops, uint64_t *nnames);
+/* src / tgt: m entries each, in edge-id order. */
+gt_status gt_doc_edges(const gt_doc *d, uint32_t *src, uint32_t *tgt);
+gt_status gt_doc_comment(const gt_doc *d, const char **s, uint64_t *len);
+/* Property k: name (not NUL-terminated), domain, value-type index, element count. */
+gt_status gt_doc_prop_info(const gt_doc *d, uint32_t k, const char **name, uint64_t *name_len,
+                           int32_t *domain, int32_t *kind, uint64_t *count);
+/* A numeric scalar column (bool, int16, int32, int64, double) as f64. */
+gt_status gt_doc_prop_f64(const gt_doc *d, uint32_t k, double *out);
+/* Element i of any other column: strings and python objects as their bytes,
+ * long doubles as 16 bytes, numeric vectors as packed f64, vectors of long
+ * double as packed 16-byte values, vectors of strings as a u64 count then
+ * (u64 length, bytes) per string. */
+gt_status gt_doc_prop_cell(const gt_doc *d, uint32_t k, uint64_t i, const void **data,
+                           uint64_t *len);
+gt_status gt_doc_vertex_name(const gt_doc *d, uint64_t v, const char **s, uint64_t *len);
+/* The device graph of the document's edge list (gt_graph_build_edges). */
+gt_status gt_graph_from_doc(const gt_doc *d, uint64_t graph_id, gt_graph **out);
+
 /* ------------------------------------------------------------- tuning */
 
 /* Extension: launch and schedule parameters of one graph, for an autotuner.
