@@ -269,13 +269,31 @@ __global__ __launch_bounds__(kBlock) void ppr_init(const BArgs a, uint32_t n, ui
     }
 }
 
-// Copy column block [n][64] into the caller's [n][ldx] layout at column c0.
-__global__ __launch_bounds__(kBlock) void ppr_emit(const float *x, uint32_t n, uint32_t kb,
-                                                   uint64_t ldx, uint32_t c0, float *out) {
+// Copy column block [n][64] (sweep labels) into the caller's [n][ldx] layout
+// at column c0: caller row o is sweep row perm[o] (perm null: identity).
+__global__ __launch_bounds__(kBlock) void ppr_emit(const float *x, const uint32_t *perm,
+                                                   uint32_t n, uint32_t kb, uint64_t ldx,
+                                                   uint32_t c0, float *out) {
     const int lane = threadIdx.x & (kWave - 1);
     const uint32_t waves = gridDim.x * (kBlock / kWave);
-    for (uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves)
-        if ((uint32_t)lane < kb) out[(size_t)r * ldx + c0 + lane] = x[(size_t)r * kCols + lane];
+    for (uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves) {
+        const size_t sr = perm ? perm[r] : r;
+        if ((uint32_t)lane < kb) out[(size_t)r * ldx + c0 + lane] = x[sr * kCols + lane];
+    }
+}
+
+// seeds[c] <- perm[seeds[c]] for the used columns: seeds into sweep labels.
+__global__ void ppr_map_seeds(uint32_t *seeds, const uint32_t *perm) {
+    const int c = threadIdx.x;
+    if (c < kCols && seeds[c] != kNoSeed) seeds[c] = perm[seeds[c]];
+}
+
+// out[k] = in[iperm[k]]: a vertex map into sweep labels.
+__global__ __launch_bounds__(kBlock) void ppr_permute_f64(const double *__restrict__ in,
+                                                          const uint32_t *__restrict__ iperm,
+                                                          uint32_t n, double *__restrict__ out) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        out[k] = in[iperm[k]];
 }
 
 static unsigned row_waves_grid(uint32_t n) {
@@ -312,15 +330,26 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
     hipStream_t s = g->stream;
     Workspace &ws = g->ws;
     BArgs a{};
+    // The sweeps run in the locality-friendly labelling: hub rows (256 bytes
+    // each) sit together and stay cached across the entries that read them.
+    Relabeled *R = nullptr;
+    GT_TRY(ensure_relabeled(g, reversed ? 1 : 0, &R));
+    Orient *push_caller = push;
+    pull = &R->pull;
+    const uint32_t *perm = R->perm.as<uint32_t>(), *iperm = R->iperm.as<uint32_t>();
     bool weighted = false;
     if (weight && weight->kind != GT_MAP_UNITY) {
         weighted = true;
         const void *src = nullptr;
         int dt = GT_F64;
         GT_TRY(stage_edge_source(g, weight, &src, &dt));
-        GT_TRY(stage_degree(g, weight, *push, src, dt, ws.deg));
+        GT_TRY(stage_degree(g, weight, *push_caller, src, dt, ws.deg));
         GT_TRY(stage_csc_weights<float>(g, weight, *pull, src, dt, ws.wcsc));
-        a.deg = ws.deg.as<double>();
+        GT_TRY(ws.deg2.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
+        hipLaunchKernelGGL(ppr_permute_f64, dim3(row_waves_grid(n)), dim3(kBlock), 0, s,
+                           ws.deg.as<double>(), iperm, n, ws.deg2.as<double>());
+        GT_HIP(hipGetLastError());
+        a.deg = ws.deg2.as<double>();
         a.w = ws.wcsc.as<float>();
     }
     GT_TRY(ensure_schedule_mm(*pull, n, (uint32_t)g->tune.mm_tile_nnz));
@@ -356,7 +385,7 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
     a.nlong = S.nlong;
     a.in_off = pull->off.as<uint32_t>();
     a.in_idx = pull->idx.as<uint32_t>();
-    a.out_off = push->off.as<uint32_t>();
+    a.out_off = R->push_off.as<uint32_t>();
     a.seeds = dseeds;
     a.damping = damping;
     a.eps = params->epsilon;
@@ -373,6 +402,7 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
         uint32_t hs[kCols];
         for (uint32_t c = 0; c < (uint32_t)kCols; ++c) hs[c] = c < kb ? seeds[c0 + c] : kNoSeed;
         GT_HIP(hipMemcpyAsync(dseeds, hs, sizeof(hs), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(ppr_map_seeds, dim3(1), dim3(kCols), 0, s, dseeds, perm);
         const uint64_t mask = kb == 64 ? ~0ull : ((1ull << kb) - 1);
         BArgs ai = a;
         ai.x_next = xb[0];
@@ -412,7 +442,7 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
         uint64_t hit[kCols];
         GT_HIP(hipMemcpyAsync(hit, st->iters, sizeof(hit), hipMemcpyDeviceToHost, s));
         hipLaunchKernelGGL(ppr_emit, dim3(row_waves_grid(n)), dim3(kBlock), 0, s, xb[sweeps & 1],
-                           n, kb, ldx, c0, xout);
+                           perm, n, kb, ldx, c0, xout);
         GT_HIP(hipGetLastError());
         GT_HIP(hipStreamSynchronize(s));
         for (uint32_t c = 0; c < kb; ++c) iters[c0 + c] = hit[c];
