@@ -174,6 +174,7 @@ struct Workspace {
     DevBuf bstate;      // batched: loop state + seeds
     DevBuf masks;       // staged vertex + edge masks
     DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
+    DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces
     Orient fpull;       // filtered view: compacted pull orientation
     Orient fpush;       // filtered view: compacted push orientation
 };

@@ -651,6 +651,46 @@ def eigenvector(g, weight, params: PowerIteration, x: DenseProp) -> Tuple[float,
     return float(eig.value), int(iters.value)
 
 
+def _bc_map(m: DenseProp, gid: int, bound: int, what: str):
+    if m.graph_id != gid:
+        raise GtValueError(_lib.GT_ERR_WRONG_GRAPH,
+                           f"betweenness: {what} map does not belong to this graph")
+    m.sized_for(gid, bound)
+    d = m.data
+    if _is_torch_cuda(d):
+        import torch
+        if d.dtype != torch.float64 or not d.is_contiguous():
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, f"{what} map must be contiguous float64")
+        return C.c_void_p(d.data_ptr()), int(d.numel()), GT_DEVICE
+    if d.dtype != np.float64 or not d.flags.c_contiguous:
+        m.data = d = np.ascontiguousarray(d, dtype=np.float64)
+    return d.ctypes.data_as(C.c_void_p), int(d.shape[0]), GT_HOST
+
+
+def betweenness(g, vertex_bc: DenseProp, edge_bc: Optional[DenseProp], pivots=None) -> dict:
+    """Brandes betweenness (crates/gt-algo/src/centrality.rs:614-689) over any
+    view: both maps are grown to their bounds and overwritten with the
+    unnormalised vertex and edge betweenness (halved on an undirected view).
+    ``edge_bc=None`` skips the edge map and ``pivots`` restricts the sources
+    (both extensions).  The maps are numpy arrays or CUDA tensors, both in the
+    same memory.  Returns the call's work counters."""
+    dev, view, gid, bound, nv = _resolve(g)
+    _, _, eb, _ = dev.info()
+    vptr, vlen, vloc = _bc_map(vertex_bc, gid, bound, "vertex")
+    out = _lib.GtBcOut(vptr, vlen, None, 0, vloc, 0, gid, 0)
+    if edge_bc is not None:
+        eptr, elen, eloc = _bc_map(edge_bc, gid, eb, "edge")
+        if eloc != vloc:
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG, "both maps must be in the same memory")
+        out.edge, out.edge_len, out.edge_graph_id = eptr, elen, gid
+    _bind_stream(dev, vertex_bc)
+    pv = None if pivots is None else np.ascontiguousarray(pivots, dtype=np.uint32)
+    st = _lib.GtBcStats()
+    call("gt_betweenness", dev.handle, C.byref(view), None if pv is None else _ptr(pv),
+         0 if pv is None else int(pv.shape[0]), C.byref(out), C.byref(st))
+    return {k: getattr(st, k) for k, _ in _lib.GtBcStats._fields_}
+
+
 # ------------------------------------------------------------- generators
 
 def gen_er(n: int, m: int, seed: int) -> Tuple[np.ndarray, np.ndarray]:

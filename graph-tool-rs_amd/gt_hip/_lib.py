@@ -62,7 +62,19 @@ class GtView(C.Structure):
 
 
 (GT_SWEEP_PAGERANK_F64, GT_SWEEP_PAGERANK_F32, GT_SWEEP_EIGENVECTOR, GT_SWEEP_OTHER,
- GT_SWEEP_BATCHED, GT_SWEEP_ADJOINT) = range(6)
+ GT_SWEEP_BATCHED, GT_SWEEP_ADJOINT, GT_SWEEP_BC_FORWARD, GT_SWEEP_BC_BACKWARD) = range(8)
+
+
+class GtBcOut(C.Structure):
+    _fields_ = [("vertex", C.c_void_p), ("vertex_len", C.c_uint64), ("edge", C.c_void_p),
+                ("edge_len", C.c_uint64), ("loc", C.c_int32), ("reserved", C.c_int32),
+                ("vertex_graph_id", C.c_uint64), ("edge_graph_id", C.c_uint64)]
+
+
+class GtBcStats(C.Structure):
+    _fields_ = [("sources", C.c_uint64), ("batches", C.c_uint64), ("levels", C.c_uint64),
+                ("entries_scanned", C.c_uint64), ("lane_hits", C.c_uint64),
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
 
 GraphP = C.c_void_p
 
@@ -105,6 +117,8 @@ _PROTOS = {
                                     C.POINTER(GtProp), C.POINTER(GtPowerIteration), C.c_void_p,
                                     C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
+    "gt_betweenness": [GraphP, C.POINTER(GtView), u32p, C.c_uint64, C.POINTER(GtBcOut),
+                       C.POINTER(GtBcStats)],
     "gt_degrees": [GraphP, C.POINTER(GtView), C.c_int32, C.POINTER(GtProp), C.c_void_p,
                    C.c_int32],
     "gt_out_degrees": [GraphP, C.POINTER(GtView), C.c_void_p, C.c_int32],

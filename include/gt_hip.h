@@ -7,6 +7,11 @@
  *   gt_pagerank_f64 ....... crates/gt-algo/src/centrality.rs:234-248 (pagerank)
  *                           and :261-384 (pagerank_with)
  *   gt_eigenvector_f64 .... crates/gt-algo/src/centrality.rs:405-497 (eigenvector)
+ *   gt_betweenness ........ crates/gt-algo/src/centrality.rs:500-689 (Brandes, betweenness)
+ *   gt_degrees, gt_out_degrees, gt_degree_histogram
+ *                           crates/gt-algo/src/degree.rs:122-247
+ *   gt_read_gt ............ crates/gt-io/src/gt.rs:555 (read)
+ *   gt_read_edge_list ..... crates/gt-io/src/csv.rs:129 (read_edge_list)
  *   gt_power_iteration .... crates/gt-algo/src/centrality.rs:79-99 (PowerIteration)
  *   gt_plan ............... crates/gt-core/src/par/plan.rs:38-101 (Plan{n, chunks})
  *   gt_prop ............... crates/gt-core/src/prop/map.rs:28-56 (ReadProp with
@@ -249,6 +254,44 @@ gt_status gt_eigenvector_f64(gt_graph *g, const gt_view *view,
                              const gt_power_iteration *params,
                              gt_vertex_out *x, double *eig, uint64_t *iters);
 
+/* ------------------------------------------------------------- betweenness */
+
+/* Output maps of gt_betweenness: vertex_bc [vertex_bound] and edge_bc
+ * [edge_bound] f64 in `loc` memory (DenseProp<f64, VertexTag / EdgeTag> after
+ * sized_for).  edge == NULL skips the edge map (extension). */
+typedef struct gt_bc_out {
+    double *vertex;
+    uint64_t vertex_len;
+    double *edge;
+    uint64_t edge_len;
+    int32_t loc;             /* gt_loc */
+    int32_t reserved;
+    uint64_t vertex_graph_id;  /* owner graph of each map; 0 = not checked */
+    uint64_t edge_graph_id;
+} gt_bc_out;
+
+/* Extension: what one gt_betweenness call did. */
+typedef struct gt_bc_stats {
+    uint64_t sources;          /* pivots with at least one out-edge (the others add nothing) */
+    uint64_t batches;          /* groups of 64 sources run together */
+    uint64_t levels;           /* BFS levels over all batches */
+    uint64_t entries_scanned;  /* adjacency entries read by the level kernels */
+    uint64_t lane_hits;        /* (entry, source) pairs on a shortest-path DAG, both phases */
+    double forward_ms;         /* device time of the forward phases (profiling only) */
+    double backward_ms;        /* device time of the backward phases (profiling only) */
+} gt_bc_stats;
+
+/* betweenness (crates/gt-algo/src/centrality.rs:614-689): Brandes over the
+ * view's out_edges (any of the six view kinds; undirected = the whole block),
+ * unweighted, unnormalised, both maps zero-filled first and halved on an
+ * undirected view; errors "betweenness: vertex/edge map does not belong to
+ * this graph" as the reference panics (:644-651).  pivots == NULL (npivots 0)
+ * means every vertex of the view, the reference's only mode; a pivot list is
+ * an extension that restricts the sources (each must be a vertex of the
+ * view).  stats may be NULL. */
+gt_status gt_betweenness(gt_graph *g, const gt_view *view, const uint32_t *pivots,
+                         uint64_t npivots, const gt_bc_out *out, gt_bc_stats *stats);
+
 /* ------------------------------------------------------------- degrees */
 
 /* The degree kernels of crates/gt-algo/src/degree.rs:122-237, on the device.
@@ -340,9 +383,11 @@ typedef enum gt_sweep_kind {
     GT_SWEEP_EIGENVECTOR = 2,
     GT_SWEEP_OTHER = 3,
     GT_SWEEP_BATCHED = 4,   /* personalised SpMM sweep */
-    GT_SWEEP_ADJOINT = 5    /* transposed (adjoint) sweep */
+    GT_SWEEP_ADJOINT = 5,   /* transposed (adjoint) sweep */
+    GT_SWEEP_BC_FORWARD = 6,  /* betweenness: forward (path-count) level launches */
+    GT_SWEEP_BC_BACKWARD = 7  /* betweenness: backward (dependency) level launches */
 } gt_sweep_kind;
-#define GT_SWEEP_KINDS 6
+#define GT_SWEEP_KINDS 8
 
 /* Extension.  While enabled, every sweep launch on g is bracketed by HIP
  * events recorded on the stream it runs on; gt_sweep_stats reports, for one

@@ -61,6 +61,9 @@ def lib():
                                            C.c_int, C.c_double, _f64p, C.c_double, _f64p, _f64p,
                                            C.c_double, C.c_uint64, _f64p, _f64p, _f64p, _f64p]
         L.oracle_pagerank_grad.restype = C.c_int64
+        L.oracle_betweenness.argtypes = [C.c_void_p, C.c_int, C.c_int, _u8p, _u8p, _u32p,
+                                         C.c_uint64, C.c_int, _f64p, _f64p]
+        L.oracle_betweenness.restype = C.c_int
         _lib = L
     return _lib
 
@@ -241,3 +244,22 @@ def dense_pagerank(n: int, edges, pers, d: float, sweeps: int) -> np.ndarray:
         p_sink = r[sink].sum()
         r = (1.0 - d) * pers + d * (p_sink * pers + M @ r)
     return r
+
+
+def betweenness(adj: "Adj", reversed: bool = False, undirected: bool = False, vmask=None,
+                emask=None, pivots=None, threads: int = 1):
+    """betweenness (centrality.rs:630-689) on a view of ``adj``: returns
+    (vertex_bc [n], edge_bc [edge_bound]).  ``pivots`` None means every vertex
+    of the view (the reference's behaviour); a list restricts the sources."""
+    L = lib()
+    vm = None if vmask is None else np.ascontiguousarray(vmask, dtype=np.uint8)
+    em = None if emask is None else np.ascontiguousarray(emask, dtype=np.uint8)
+    pv = None if pivots is None else np.ascontiguousarray(pivots, dtype=np.uint32)
+    vb = np.zeros(max(adj.n, 1))
+    eb = np.zeros(max(adj.m, 1))
+    rc = L.oracle_betweenness(adj._h, int(reversed), int(undirected), _p(vm, _u8p), _p(em, _u8p),
+                              _p(pv), 0 if pv is None else len(pv), threads, _p(vb, _f64p),
+                              _p(eb, _f64p))
+    if rc != 0:
+        raise ValueError("pivot out of range")
+    return vb[: adj.n], eb[: adj.m]

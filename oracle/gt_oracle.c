@@ -24,6 +24,9 @@
  *                        per-chunk partial, folded left to right from 0.0
  *   oracle_pagerank .... crates/gt-algo/src/centrality.rs:234-384
  *   oracle_eigenvector . crates/gt-algo/src/centrality.rs:405-497
+ *   oracle_betweenness . crates/gt-algo/src/centrality.rs:500-689 (Brandes::run,
+ *                        betweenness), crates/gt-core/src/view/undirected.rs:110-112
+ *                        (undirected out_edges = the whole block)
  *
  * Pinning: every function above is checked by tests/test_oracle.py against the
  * reference's own known answers (u16_centrality.rs, the LAYOUT fixture of
@@ -32,7 +35,13 @@
  * force (a dense n x n power iteration).  oracle_pagerank_grad (a build
  * extension: implicit differentiation of the fixed point) is pinned by central
  * finite differences of oracle_pagerank and by the row-scale identity
- * sum_{e in out(s)} w_e dL/dw_e = 0.
+ * sum_{e in out(s)} w_e dL/dw_e = 0.  oracle_betweenness is pinned by
+ * tests/test_oracle_betweenness.py: the reference's own 100 seeded graphs of
+ * u16_centrality.rs:707-770 against an independent restatement of its
+ * reference_brandes helper (:628-690), the path closed form (:772-799), the
+ * relabelling case (:801-831), the unreachable case (:870-887), thread-count
+ * independence (:846-868) and brute force from the definition (all-pairs BFS
+ * distances and path counts, sigma_st(v) = sigma_sv * sigma_vt).
  */
 #include <math.h>
 #include <pthread.h>
@@ -642,4 +651,187 @@ GT_EXPORT int64_t oracle_pagerank_grad(const gt_adj *g, int reversed, int wkind,
     free(nl);
     free(mu);
     return (int64_t)iter;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Betweenness, Brandes (centrality.rs:500-689)                              */
+/* ------------------------------------------------------------------------ */
+
+/* out_edges(u) of the view: the out-half (the in-half when reversed); on an
+ * undirected view the whole block, out-half first (undirected.rs:110-112).
+ * Filtered incidences are dropped (filtered.rs:122-124). */
+typedef struct {
+    const gt_view *w;
+    int undirected;
+    const uint32_t *verts; /* pivots, in order */
+    uint64_t nverts;
+    uint64_t chunks;
+    double *vb;            /* [chunks][n] partials */
+    double *eb;            /* [chunks][edge_bound] partials */
+    atomic_ulong next;
+} bc_job;
+
+/* One source (Brandes::run, centrality.rs:545-611): BFS examining vertices on
+ * dequeue, self-loops skipped, every parallel edge a distinct shortest path;
+ * then dependencies accumulated in reverse dequeue order. */
+static void bc_source(const bc_job *j, uint32_t s, int64_t *dist, double *sigma, double *dep,
+                      uint32_t *order, uint64_t *in_head, uint64_t *in_tail, uint64_t *in_next,
+                      uint32_t *in_pred, uint32_t *in_edge, double *vb, double *eb) {
+    const gt_view *w = j->w;
+    const uint32_t n = w->g->n;
+    for (uint32_t v = 0; v < n; ++v) {
+        dist[v] = -1;
+        sigma[v] = 0.0;
+        dep[v] = 0.0;
+        in_head[v] = UINT64_MAX;
+    }
+    uint64_t npred = 0;
+    uint64_t head = 0, tail = 0;
+    sigma[s] = 1.0;
+    dist[s] = 0;
+    order[tail++] = s;
+    while (head < tail) {
+        const uint32_t u = order[head++];
+        const int64_t du = dist[u];
+        const double su = sigma[u];
+        for (int half = 0; half < (j->undirected ? 2 : 1); ++half) {
+            uint32_t cnt;
+            const gt_entry *es = half == 0 ? view_out(w, u, &cnt) : view_in(w, u, &cnt);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                if (!view_keeps(w, es[k])) continue;
+                const uint32_t x = es[k].other;
+                if (x == u) continue;
+                if (dist[x] < 0) {
+                    dist[x] = du + 1;
+                    sigma[x] = su;
+                    order[tail++] = x;
+                } else if (dist[x] == du + 1) {
+                    sigma[x] += su;
+                } else {
+                    continue;
+                }
+                /* incoming[x].push((u, e)), kept in push order */
+                in_pred[npred] = u;
+                in_edge[npred] = es[k].idx;
+                in_next[npred] = UINT64_MAX;
+                if (in_head[x] == UINT64_MAX) in_head[x] = npred;
+                else in_next[in_tail[x]] = npred;
+                in_tail[x] = npred;
+                ++npred;
+            }
+        }
+    }
+    for (uint64_t i = tail; i-- > 0;) {
+        const uint32_t u = order[i];
+        const double su = sigma[u];
+        for (uint64_t p = in_head[u]; p != UINT64_MAX; p = in_next[p]) {
+            const uint32_t v = in_pred[p];
+            const double factor = (sigma[v] / su) * (1.0 + dep[u]);
+            dep[v] += factor;
+            eb[in_edge[p]] += factor;
+        }
+        if (u != s) vb[u] += dep[u];
+    }
+}
+
+static void *bc_worker(void *arg) {
+    bc_job *j = (bc_job *)arg;
+    const gt_adj *g = j->w->g;
+    const uint32_t n = g->n;
+    const uint64_t ents = 2 * (g->start[n] + 1);
+    int64_t *dist = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+    double *sigma = (double *)malloc(sizeof(double) * (n + 1));
+    double *dep = (double *)malloc(sizeof(double) * (n + 1));
+    uint32_t *order = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint64_t *in_head = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *in_tail = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *in_next = (uint64_t *)malloc(sizeof(uint64_t) * ents);
+    uint32_t *in_pred = (uint32_t *)malloc(sizeof(uint32_t) * ents);
+    uint32_t *in_edge = (uint32_t *)malloc(sizeof(uint32_t) * ents);
+    for (;;) {
+        const uint64_t k = atomic_fetch_add(&j->next, 1);
+        if (k >= j->chunks) break;
+        uint64_t lo, hi;
+        plan_range(j->nverts, j->chunks, k, &lo, &hi);
+        double *vb = j->vb + k * (uint64_t)n;
+        double *eb = j->eb + k * g->edge_bound;
+        for (uint64_t i = lo; i < hi; ++i)
+            bc_source(j, j->verts[i], dist, sigma, dep, order, in_head, in_tail, in_next, in_pred,
+                      in_edge, vb, eb);
+    }
+    free(dist);
+    free(sigma);
+    free(dep);
+    free(order);
+    free(in_head);
+    free(in_tail);
+    free(in_next);
+    free(in_pred);
+    free(in_edge);
+    return NULL;
+}
+
+/* betweenness (centrality.rs:630-689): maps zero-filled, the pivot list cut by
+ * Plan::new(len, ceil(len / 16)) (BETWEENNESS_CHUNKS, :107-115), one private
+ * accumulator pair per chunk, partials summed in chunk order, both maps
+ * halved on an undirected view.  pivots == NULL means every vertex of the
+ * view, in index order (the reference's only mode); a pivot list is a build
+ * extension used to time a bounded sample.  Returns 0, or -1 for a pivot out
+ * of range. */
+GT_EXPORT int oracle_betweenness(const gt_adj *g, int reversed, int undirected,
+                                 const uint8_t *vmask, const uint8_t *emask,
+                                 const uint32_t *pivots, uint64_t npivots, int threads,
+                                 double *vb, double *eb) {
+    const uint32_t n = g->n;
+    gt_view view = {g, reversed, vmask, emask};
+    uint32_t *verts = NULL;
+    uint64_t nv = 0;
+    if (pivots) {
+        for (uint64_t i = 0; i < npivots; ++i)
+            if (pivots[i] >= n) return -1;
+        verts = (uint32_t *)malloc(sizeof(uint32_t) * (npivots + 1));
+        memcpy(verts, pivots, sizeof(uint32_t) * npivots);
+        nv = npivots;
+    } else {
+        verts = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+        for (uint32_t v = 0; v < n; ++v)
+            if (!vmask || vmask[v]) verts[nv++] = v;
+    }
+    for (uint32_t v = 0; v < n; ++v) vb[v] = 0.0;
+    for (uint64_t e = 0; e < g->edge_bound; ++e) eb[e] = 0.0;
+    const uint64_t grain = nv / 16 + (nv % 16 != 0);
+    bc_job j;
+    j.w = &view;
+    j.undirected = undirected;
+    j.verts = verts;
+    j.nverts = nv;
+    j.chunks = plan_chunks_for(nv, grain ? grain : 1);
+    j.vb = (double *)calloc(j.chunks * (uint64_t)n + 1, sizeof(double));
+    j.eb = (double *)calloc(j.chunks * g->edge_bound + 1, sizeof(double));
+    atomic_init(&j.next, 0);
+    int t = threads < 1 ? 1 : threads;
+    if ((uint64_t)t > j.chunks) t = (int)j.chunks;
+    if (t <= 1) {
+        bc_worker(&j);
+    } else {
+        pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)t);
+        for (int i = 1; i < t; ++i) pthread_create(&th[i], NULL, bc_worker, &j);
+        bc_worker(&j);
+        for (int i = 1; i < t; ++i) pthread_join(th[i], NULL);
+        free(th);
+    }
+    for (uint64_t k = 0; k < j.chunks; ++k) {
+        const double *pv = j.vb + k * (uint64_t)n;
+        const double *pe = j.eb + k * g->edge_bound;
+        for (uint32_t v = 0; v < n; ++v) vb[v] += pv[v];
+        for (uint64_t e = 0; e < g->edge_bound; ++e) eb[e] += pe[e];
+    }
+    if (undirected) {
+        for (uint32_t v = 0; v < n; ++v) vb[v] /= 2.0;
+        for (uint64_t e = 0; e < g->edge_bound; ++e) eb[e] /= 2.0;
+    }
+    free(j.vb);
+    free(j.eb);
+    free(verts);
+    return 0;
 }

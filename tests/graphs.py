@@ -4,6 +4,8 @@ thousand()  -- crates/gt-algo/tests/u16_centrality.rs:93-107
 sixty()     -- crates/gt-algo/tests/u16_centrality.rs:495-503 (eigenvector residual graph)
 bench()     -- crates/gt-algo/benches/kernels.rs:49-73 (ring + SplitMix64 chords)
 LAYOUT      -- crates/gt-core/tests/u34_diff_adjacency.rs:74-83
+brandes_cases() -- crates/gt-algo/tests/u16_centrality.rs:707-770 (100 small
+                   multigraphs with self-loops and parallel edges)
 """
 import numpy as np
 
@@ -82,3 +84,21 @@ def c1():
     seed 0x243f6a8885a308d3, source then target per edge."""
     s, t = oracle.gen_er(10_000, 100_000, 0x243F6A8885A308D3)
     return 10_000, s, t
+
+
+def brandes_cases():
+    """The 100 (n, edges) of u16_centrality.rs:707-770, drawn in its order:
+    seed 0x0123456789abcdef, n = 3 + below(10), m = below(3n + 1), each edge
+    (below(n), below(n))."""
+    r = Rng(0x0123456789ABCDEF)
+    cases = []
+    for _ in range(100):
+        n = 3 + r.below(10)
+        m = r.below(3 * n + 1)
+        edges = []
+        for _ in range(m):
+            s = r.below(n)
+            t = r.below(n)
+            edges.append((s, t))
+        cases.append((n, edges))
+    return cases
