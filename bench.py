@@ -433,16 +433,122 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
 
 
 
+def bench_betweenness(args, torch, gt_hip, _lib) -> int:
+    """All-pivot Brandes betweenness (vertex and edge maps).
+
+    R-MAT scale 16 (unless --scale): 65,536 vertices, 1,048,576 edges, the
+    same generator and seed as the PageRank workload, on the undirected view;
+    every vertex with an incident edge is a source.  One step = one call.
+    GTEPS = sources x edges / time (each BFS traverses the graph once).
+    Algorithmic bytes, from the call's own work counters: 12 B per adjacency
+    entry a level kernel reads (4 B index + 8 B lane mask), 8 B per forward
+    (entry, source) hit (a path count), 16 B per backward hit (path count and
+    dependency), and 32 B per vertex per level (lane masks in and out, level
+    range, row offsets)."""
+    scale = 16 if args.scale is None else args.scale
+    n, m, src, tgt, dev = _upload_rmat(gt_hip, scale, args.edgefactor, 1)
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    copy = copy_calibration(torch)
+    vb = torch.zeros(n, dtype=torch.float64, device="cuda")
+    eb = torch.zeros(m, dtype=torch.float64, device="cuda")
+    out = _lib.GtBcOut(C.c_void_p(vb.data_ptr()), n, C.c_void_p(eb.data_ptr()), m,
+                       _lib.GT_DEVICE, 0, 1, 1)
+    view = _lib.GtView(_lib.GT_VIEW_UNDIRECTED, 0, None, 0, None, 0)
+    st = _lib.GtBcStats()
+
+    def solve(pivots=None):
+        pp = None if pivots is None else pivots.ctypes.data_as(C.POINTER(C.c_uint32))
+        _lib.call("gt_betweenness", dev.handle, C.byref(view), pp,
+                  0 if pivots is None else int(pivots.shape[0]), C.byref(out), C.byref(st))
+        return {k: getattr(st, k) for k, _ in _lib.GtBcStats._fields_}
+
+    steps = args.steps
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        solve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    dev.set_profiling(True)
+    prof = solve()
+    dev.set_profiling(False)
+    sources = prof["sources"]
+    gteps = sources * m * steps / dt / 1e9
+    kern_ms = prof["forward_ms"] + prof["backward_ms"]
+    B = (12 * (prof["forward_entries"] + prof["backward_entries"]) + 8 * prof["forward_hits"]
+         + 16 * prof["backward_hits"] + 32 * n * prof["levels"])
+    ach = B / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    log(f"[bench] betweenness: {dt/steps*1e3:.1f} ms/call, {sources} sources in "
+        f"{prof['batches']} batches, {prof['levels']} levels, kernels {kern_ms:.1f} ms "
+        f"(forward {prof['forward_ms']:.1f}, backward {prof['backward_ms']:.1f}), "
+        f"{gteps:.2f} GTEPS")
+    result = {
+        "metric": "Betweenness GTEPS (all-pivot Brandes, vertex + edge maps, f64)",
+        "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 path counts and dependencies, u32 levels", "data": "synthetic",
+        "config": {"workload": f"betweenness_rmat{scale}_ef{args.edgefactor}_undirected",
+                   "vertices": n, "edges": m, "view": "undirected", "pivots": "all",
+                   "sources": sources, "rmat": [0.57, 0.19, 0.19, 0.05], "seed": SEED},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "bc_forward + bc_backward level launches (one call)",
+                     "bytes_per_call": int(B), "kernel_ms_per_call": round(kern_ms, 3),
+                     "copy_calibration": copy},
+        "work": prof,
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.Adj(n, src, tgt)
+        deg = np.bincount(src, minlength=n) + np.bincount(tgt, minlength=n)
+        cand = np.flatnonzero(deg > 0).astype(np.uint32)
+        rng = np.random.default_rng(5)
+        k = 64
+        while True:  # grow the pivot sample until the oracle takes ~10 s
+            piv = np.sort(rng.choice(cand, min(k, cand.shape[0]), replace=False)).astype(np.uint32)
+            t0 = time.perf_counter()
+            wv, we = oracle.betweenness(o, undirected=True, pivots=piv, threads=threads)
+            dtc = time.perf_counter() - t0
+            if dtc >= 10.0 or k >= cand.shape[0] or dtc * (2 * k) / k > 40.0:
+                break
+            k *= 2
+        # The device on the same pivot sample, for the accuracy line.
+        solve(piv)
+        torch.cuda.synchronize()
+        gv, ge = vb.cpu().numpy(), eb.cpu().numpy()
+        nz_v, nz_e = wv != 0, we != 0
+        result["cpu_baseline"] = {
+            "value": round(piv.shape[0] * m / dtc / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{piv.shape[0]} random pivots of the same graph and view through the "
+                      f"oracle's Brandes (16 chunks, C restatement of the reference), {dtc:.1f}s"}
+        result["accuracy_vs_oracle"] = {
+            "pivots": int(piv.shape[0]),
+            "vertex_max_rel": float((np.abs(gv - wv)[nz_v] / np.abs(wv[nz_v])).max()),
+            "edge_max_rel": float((np.abs(ge - we)[nz_e] / np.abs(we[nz_e])).max()),
+            "zeros_match": bool(np.all(gv[~nz_v] == 0) and np.all(ge[~nz_e] == 0))}
+    print(json.dumps(result), flush=True)
+    dev.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("pagerank", "batched", "adjoint", "uneven"),
+    ap.add_argument("--workload",
+                    choices=("pagerank", "batched", "adjoint", "uneven", "betweenness"),
                     default="pagerank",
                     help="pagerank: R-MAT 22 FP32/FP64 solves (the headline); batched: k=64 "
                          "personalised SpMM on weighted R-MAT 20; adjoint: weighted R-MAT 22 "
-                         "forward + gradient; uneven: 4096 Chung-Lu graphs, forward + adjoint")
+                         "forward + gradient; uneven: 4096 Chung-Lu graphs, forward + adjoint; "
+                         "betweenness: all-pivot Brandes on undirected R-MAT 16")
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--epsilon", type=float, default=1e-9)
@@ -471,6 +577,8 @@ def main() -> int:
         return bench_adjoint(args, torch, gt_hip, _lib)
     if args.workload == "uneven":
         return bench_uneven(args, torch, gt_hip, _lib)
+    if args.workload == "betweenness":
+        return bench_betweenness(args, torch, gt_hip, _lib)
     if args.scale is None:
         args.scale = 22
 

@@ -102,6 +102,19 @@ struct Schedule {
 // quarter as many rows).
 constexpr uint32_t kMmTileNnz = 1024;
 
+// Row tiles of the betweenness level kernels: pieces of the rows longer than
+// tile_nnz entries first (row, kSplit, first, end; pieces of one row adjacent),
+// then runs of at most 64 whole rows and tile_nnz entries (row_begin,
+// row_end, first, end).
+struct BcSchedule {
+    DevBuf tiles;            // uint4[ntiles]
+    DevBuf long_rows;        // uint32[nlong]
+    DevBuf long_first;       // uint32[nlong + 1]: first piece of each long row
+    uint32_t ntiles = 0, npieces = 0, nlong = 0;
+    uint32_t tile_nnz = 0;
+    bool ready = false;
+};
+
 // One orientation of the adjacency: row r lists the entries a kernel pulls
 // from (in-halves for the CSC, out-halves for the CSR), in storage order.
 struct Orient {
@@ -129,6 +142,10 @@ struct Orient {
     DevBuf heavy, mid;
     uint32_t nheavy = 0, nmid = 0;
     bool heavy_ready = false;
+    BcSchedule bcs;          // betweenness row tiles
+    // Set on an orientation that merges an out-half and an in-half per row
+    // (out-half first): the out-half length of each row.
+    DevBuf first_len;
 };
 
 // A pull orientation in a locality-friendly labelling: vertices renumbered by
@@ -177,6 +194,7 @@ struct Workspace {
     DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces
     Orient fpull;       // filtered view: compacted pull orientation
     Orient fpush;       // filtered view: compacted push orientation
+    Orient fund;        // undirected filtered view: merged halves
 };
 
 struct gt_graph_impl;
@@ -205,6 +223,8 @@ struct gt_graph {
     gt::Orient in;   // CSC: in-halves
     gt::Orient out;  // CSR: out-halves
     gt::Relabeled rel[2];  // [0]: pull = in-halves, [1]: pull = out-halves
+    gt::Orient und;        // whole blocks (out-half then in-half), built on first use
+    bool und_ready = false;
     // Edge list by edge id (source, target), kNoEdge where an id has no edge;
     // built on first use from the out-halves.
     gt::DevBuf esrc, etgt;

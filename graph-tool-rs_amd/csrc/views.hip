@@ -89,6 +89,7 @@ static gt_status compact(gt_graph *g, const Orient &base, Orient &out, const uin
     out.scheduled = false;
     out.mm.ready = false;
     out.heavy_ready = false;
+    out.bcs.ready = false;
     return GT_OK;
 }
 

@@ -73,7 +73,10 @@ class GtBcOut(C.Structure):
 
 class GtBcStats(C.Structure):
     _fields_ = [("sources", C.c_uint64), ("batches", C.c_uint64), ("levels", C.c_uint64),
-                ("entries_scanned", C.c_uint64), ("lane_hits", C.c_uint64),
+                ("forward_entries", C.c_uint64), ("forward_hits", C.c_uint64),
+                ("forward_hit_entries", C.c_uint64),
+                ("backward_entries", C.c_uint64), ("backward_hits", C.c_uint64),
+                ("backward_hit_entries", C.c_uint64),
                 ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
 
 GraphP = C.c_void_p

@@ -275,10 +275,14 @@ typedef struct gt_bc_stats {
     uint64_t sources;          /* pivots with at least one out-edge (the others add nothing) */
     uint64_t batches;          /* groups of 64 sources run together */
     uint64_t levels;           /* BFS levels over all batches */
-    uint64_t entries_scanned;  /* adjacency entries read by the level kernels */
-    uint64_t lane_hits;        /* (entry, source) pairs on a shortest-path DAG, both phases */
-    double forward_ms;         /* device time of the forward phases (profiling only) */
-    double backward_ms;        /* device time of the backward phases (profiling only) */
+    uint64_t forward_entries;  /* adjacency entries the forward level kernels read */
+    uint64_t forward_hits;     /* (entry, source) pairs that added a path count */
+    uint64_t forward_hit_entries;   /* entries with at least one such source */
+    uint64_t backward_entries; /* adjacency entries the backward level kernels read */
+    uint64_t backward_hits;    /* (entry, source) pairs that added a dependency */
+    uint64_t backward_hit_entries;  /* entries with at least one such source */
+    double forward_ms;         /* device time of the forward launches (profiling only) */
+    double backward_ms;        /* device time of the backward launches (profiling only) */
 } gt_bc_stats;
 
 /* betweenness (crates/gt-algo/src/centrality.rs:614-689): Brandes over the
