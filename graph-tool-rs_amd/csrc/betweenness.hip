@@ -21,7 +21,10 @@
 //     gathers u's level-(L+1) mask; a DAG entry adds its factor to dep[v] on
 //     those lanes, and the 64-lane sum of the factors to the entry's edge
 //     accumulator.  Every entry and every vertex has one owner per launch, so
-//     no atomics and a fixed summation order.
+//     no atomics and a fixed summation order.  A finished vertex stores
+//     g = (1 + dep) / sigma instead of dep, so a factor is one product,
+//     sigma[v] * g[u], rather than the reference's (sigma[v] / sigma[u]) *
+//     (1 + dep[u]) with its division: the same value up to two roundings.
 // The wavefront gathers one mask per lane for 64 entries at a time, then walks
 // the entries that hit with all 64 lanes, four at a time so that their row
 // reads overlap.  Pieces of long rows leave partial rows that a fix-up kernel
@@ -41,7 +44,7 @@ constexpr uint32_t kBcUnreached = 0xffffffffu;
 // Entries and rows per row tile.  Rows with more than kBcPieceNnz entries
 // are cut into pieces of that many (the hits of a row are walked by one
 // wavefront, so no wavefront walks more than a piece of one row).
-constexpr uint32_t kBcTileNnz = 256;
+constexpr uint32_t kBcTileNnz = 128;
 constexpr uint32_t kBcTileRows = 64;
 constexpr uint32_t kBcPieceNnz = 128;
 // Forward levels launched between two reads of the level counter.
@@ -62,12 +65,15 @@ struct BcArgs {
     uint64_t *mask[2];      // [n] each: level masks, level L in mask[L & 1]
     uint32_t *lo, *hi;      // [n]: first and last level of the vertex over the lanes
     uint32_t *dist;         // [n][64]
-    double *sigma, *dep;    // [n][64]
+    double *sigma;          // [n][64]: shortest-path counts
+    double *gq;             // [n][64]: (1 + dependency) / sigma, once the level is done
     double *part;           // [npieces][64]
     uint64_t *pmask;        // [npieces]
     double *vacc;           // [n]
     double *eacc;           // [entries]: per entry position
     uint32_t *level;        // deepest level reached so far (device counter)
+    uint32_t *status;       // this batch's slot: deepest level + 1, or 0 if its
+                            // forward needed more levels than were launched
     // Work counters, one slot per wavefront of a launch (no shared atomics):
     // [wave][3] = entries, lane hits, entry hits.
     unsigned long long *work;
@@ -430,18 +436,13 @@ __device__ __forceinline__ void bwd_group(const BcArgs &a, uint64_t &hits, uint3
                                           bool one_row, double &emine, BwdGroup &G,
                                           uint32_t &lh, int lane) {
 #pragma clang fp contract(off)
-    double sv[kBcGroup], su[kBcGroup], du[kBcGroup];
-    bool on[kBcGroup];
     int kk[kBcGroup];
     G.cnt = 0;
 #pragma unroll
     for (int i = 0; i < kBcGroup; ++i) {
-        on[i] = false;
         kk[i] = -1;
         G.rk[i] = -1;
-        sv[i] = 1.0;
-        su[i] = 1.0;
-        du[i] = 0.0;
+        G.f[i] = 0.0;
         if (hits) {
             const int k = pop_hit(hits);
             kk[i] = k;
@@ -449,36 +450,42 @@ __device__ __forceinline__ void bwd_group(const BcArgs &a, uint64_t &hits, uint3
             const uint32_t uk = readlane32(u, k);
             const uint64_t Sk = readlane64(S, k);
             G.rk[i] = (int)readlane32(slot, k);
-            on[i] = (Sk >> lane) & 1;
-            if (on[i]) {
-                const size_t q = (size_t)uk * kWave + lane;
-                su[i] = a.sigma[q];
-                du[i] = a.dep[q];
-                sv[i] = one_row ? sv_row : a.sigma[(size_t)(rb + G.rk[i]) * kWave + lane];
+            if ((Sk >> lane) & 1) {
+                const double sv =
+                    one_row ? sv_row : a.sigma[(size_t)(rb + G.rk[i]) * kWave + lane];
+                G.f[i] = sv * a.gq[(size_t)uk * kWave + lane];
                 ++lh;
             }
         }
     }
-#pragma unroll
-    for (int i = 0; i < kBcGroup; ++i) G.f[i] = on[i] ? (sv[i] / su[i]) * (1.0 + du[i]) : 0.0;
     wave_sum4(G.f, kk, emine, lane);
 }
 
-// A finished row v with lanes m at level L: dependencies out, and the row's
-// vertex sum into the register of lane `slot` (level 0 is the source itself
-// and adds nothing).
+// A finished row v with lanes m at level L and dependencies dacc: the
+// row's (1 + dep) / sigma for its predecessors, and its vertex sum into the
+// register of lane `slot` (level 0 is the source itself and adds nothing).
 __device__ __forceinline__ void bwd_store(const BcArgs &a, uint32_t v, uint64_t m, double dacc,
                                           uint32_t L, int lane, int slot, double &vrow) {
+#pragma clang fp contract(off)
     const bool on = (m >> lane) & 1;
-    if (on) a.dep[(size_t)v * kWave + lane] = dacc;
+    if (on) {
+        const size_t q = (size_t)v * kWave + lane;
+        a.gq[q] = (1.0 + dacc) / a.sigma[q];
+    }
     if (L > 0) {
         const double t = wave_sum(on ? dacc : 0.0);
         if (lane == slot) vrow = t;
     }
 }
 
-// Level L (top = the deepest level, which has no successors).
-__global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, uint32_t L, uint32_t top) {
+// Level L.  The deepest level, top, has no successors; it is read from the
+// device counter.  `launched` is the number of forward levels launched: if
+// the counter reached it, the forward may not be complete and the whole
+// backward does nothing (the batch is run again).
+__global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, uint32_t L, uint32_t launched) {
+    const uint32_t top = __hip_atomic_load(a.level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (L == 0 && blockIdx.x == 0 && threadIdx.x == 0) *a.status = top < launched ? top + 1 : 0;
+    if (top >= launched || L > top) return;
     const int lane = threadIdx.x & (kWave - 1);
     const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * (kBlock / kWave);
@@ -606,7 +613,10 @@ __global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, uint32_t L, uint
         uint64_t leaves = live & ~done;
         while (leaves) {
             const int r = pop_hit(leaves);
-            if ((readlane64(m_r, r) >> lane) & 1) a.dep[(size_t)(rb + r) * kWave + lane] = 0.0;
+            if ((readlane64(m_r, r) >> lane) & 1) {
+                const size_t q = (size_t)(rb + r) * kWave + lane;
+                a.gq[q] = 1.0 / a.sigma[q];
+            }
         }
         if (rl && ((done >> lane) & 1) && L > 0) a.vacc[row] += vrow;
     }
@@ -614,7 +624,9 @@ __global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, uint32_t L, uint
 }
 
 // Finishes the long rows of level L from their pieces (one workgroup per row).
-__global__ __launch_bounds__(kBlock) void bc_backward_fix(BcArgs a, uint32_t L) {
+__global__ __launch_bounds__(kBlock) void bc_backward_fix(BcArgs a, uint32_t L, uint32_t launched) {
+    const uint32_t top = __hip_atomic_load(a.level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (top >= launched || L > top) return;
     __shared__ double red[kWavesPerBlock][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t *curm = a.mask[L & 1];
@@ -988,7 +1000,7 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     const size_t np = std::max<size_t>(std::max(bv.fwd->bcs.npieces, bv.bwd->bcs.npieces), 1);
     DevBuf *B = ws.btw;
     GT_TRY(B[0].reserve(rows * kWave * sizeof(double)));       // sigma
-    GT_TRY(B[1].reserve(rows * kWave * sizeof(double)));       // dep
+    GT_TRY(B[1].reserve(rows * kWave * sizeof(double)));       // gq
     GT_TRY(B[2].reserve(rows * kWave * sizeof(uint32_t)));     // dist
     GT_TRY(B[3].reserve(rows * 3 * sizeof(uint64_t) + 64));    // seen, mask[2], level
     GT_TRY(B[8].reserve(2 * 3 * kBcMaxWaves * sizeof(unsigned long long)));  // work slots
@@ -996,12 +1008,12 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     GT_TRY(B[5].reserve(rows * sizeof(double)));               // vacc
     GT_TRY(B[6].reserve(std::max<uint64_t>(bentries, 1) * sizeof(double)));  // eacc
     GT_TRY(B[7].reserve(np * kWave * sizeof(double) + np * sizeof(uint64_t)));
-    GT_TRY(B[9].reserve(std::max<size_t>(src.size(), 1) * sizeof(uint32_t)));
+    GT_TRY(B[9].reserve((src.size() + src.size() / kWave + 2) * sizeof(uint32_t)));  // src, status
 
     BcArgs base{};
     base.n = n;
     base.sigma = B[0].as<double>();
-    base.dep = B[1].as<double>();
+    base.gq = B[1].as<double>();
     base.dist = B[2].as<uint32_t>();
     base.seen = B[3].as<uint64_t>();
     base.mask[0] = base.seen + rows;
@@ -1041,49 +1053,50 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     const bool prof = g->profiling;
     if (prof)
         for (auto &e : ev) GT_HIP(hipEventCreate(&e));
-    uint64_t levels_total = 0, launches_f = 0, launches_b = 0;
+    uint64_t launches_f = 0, launches_b = 0;
     double fms = 0.0, bms = 0.0;
-    uint32_t probe = kBcLevelsPerCheck;
-    for (size_t first = 0; first < src.size(); first += kWave) {
+    const size_t nbatch = (src.size() + kWave - 1) / kWave;
+    uint32_t *dstatus = dsrc + src.size();
+    if (nbatch) GT_HIP(hipMemsetAsync(dstatus, 0, nbatch * sizeof(uint32_t), s));
+    std::vector<uint32_t> status(nbatch, 0);
+
+    auto setup = [&](size_t b) -> gt_status {
+        const size_t first = b * kWave;
         const uint32_t cnt = (uint32_t)std::min<size_t>(kWave, src.size() - first);
-        const uint64_t act = cnt == kWave ? ~0ull : ((1ull << cnt) - 1);
-        fa.act = ba.act = act;
+        fa.act = ba.act = cnt == kWave ? ~0ull : ((1ull << cnt) - 1);
+        ba.status = dstatus + b;
         hipLaunchKernelGGL(bc_reset, dim3(lgrid(n)), dim3(kBlock), 0, s, fa);
         hipLaunchKernelGGL(bc_seed, dim3(1), dim3(kWave), 0, s, fa, dsrc + first, cnt);
         GT_HIP(hipGetLastError());
-        uint32_t L = 0, top = 0;
-        for (;;) {
-            if (prof) GT_HIP(hipEventRecord(ev[0], s));
-            for (uint32_t k = 0; k < probe; ++k, ++L) {
-                hipLaunchKernelGGL(bc_forward, dim3(fgrid), dim3(kBlock), 0, s, fa, L);
-                if (fa.nlong)
-                    hipLaunchKernelGGL(bc_forward_fix, dim3(fixf), dim3(kBlock), 0, s, fa, L);
-                ++launches_f;
-            }
-            GT_HIP(hipGetLastError());
-            if (prof) GT_HIP(hipEventRecord(ev[1], s));
-            GT_HIP(hipMemcpyAsync(&g->h_state->counter, base.level, sizeof(uint32_t),
-                                  hipMemcpyDeviceToHost, s));
-            GT_HIP(hipStreamSynchronize(s));
-            if (prof) {
-                float ms = 0.f;
-                GT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-                fms += ms;
-            }
-            top = g->h_state->counter;
-            if (top < L) break;  // a level came out empty
+        return GT_OK;
+    };
+    auto forward = [&](uint32_t L0, uint32_t count) -> gt_status {
+        if (prof) GT_HIP(hipEventRecord(ev[0], s));
+        for (uint32_t L = L0; L < L0 + count; ++L) {
+            hipLaunchKernelGGL(bc_forward, dim3(fgrid), dim3(kBlock), 0, s, fa, L);
+            if (fa.nlong) hipLaunchKernelGGL(bc_forward_fix, dim3(fixf), dim3(kBlock), 0, s, fa, L);
+            ++launches_f;
         }
-        // Launch enough levels in one go next time.
-        probe = std::max<uint32_t>(kBcLevelsPerCheck, top + 2);
+        GT_HIP(hipGetLastError());
+        if (prof) {
+            GT_HIP(hipEventRecord(ev[1], s));
+            GT_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            GT_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            fms += ms;
+        }
+        return GT_OK;
+    };
+    auto backward = [&](uint32_t from, uint32_t launched) -> gt_status {
         if (prof) GT_HIP(hipEventRecord(ev[2], s));
-        for (uint32_t l = top + 1; l-- > 0;) {
-            hipLaunchKernelGGL(bc_backward, dim3(bgrid), dim3(kBlock), 0, s, ba, l, top);
+        for (uint32_t l = from + 1; l-- > 0;) {
+            hipLaunchKernelGGL(bc_backward, dim3(bgrid), dim3(kBlock), 0, s, ba, l, launched);
             if (ba.nlong)
-                hipLaunchKernelGGL(bc_backward_fix, dim3(fixb), dim3(kBlock), 0, s, ba, l);
+                hipLaunchKernelGGL(bc_backward_fix, dim3(fixb), dim3(kBlock), 0, s, ba, l,
+                                   launched);
             ++launches_b;
         }
         GT_HIP(hipGetLastError());
-        levels_total += top + 1;
         if (prof) {
             GT_HIP(hipEventRecord(ev[3], s));
             GT_HIP(hipEventSynchronize(ev[3]));
@@ -1091,7 +1104,54 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
             GT_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
             bms += ms;
         }
+        return GT_OK;
+    };
+    // A batch with the host in the loop: forward levels in groups until one
+    // comes out empty, then exactly the backward levels needed.
+    auto run_sync = [&](size_t b, uint32_t *probe) -> gt_status {
+        GT_TRY(setup(b));
+        uint32_t L = 0, top = 0;
+        for (;;) {
+            GT_TRY(forward(L, *probe));
+            L += *probe;
+            GT_HIP(hipMemcpyAsync(&g->h_state->counter, base.level, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            GT_HIP(hipStreamSynchronize(s));
+            top = g->h_state->counter;
+            if (top < L) break;  // a level came out empty
+            *probe *= 2;
+        }
+        GT_TRY(backward(top, 0xffffffffu));
+        status[b] = top + 1;
+        *probe = std::max<uint32_t>(kBcLevelsPerCheck, top + 2);
+        return GT_OK;
+    };
+    // The other batches run without the host: `depth` forward levels (the
+    // launches past the last level return at once), then depth + 1 backward
+    // levels that read the depth from the device.  A batch whose BFS went
+    // deeper leaves status 0 and is run again with the host in the loop.
+    uint32_t probe = kBcLevelsPerCheck;
+    for (size_t b = 0; b < nbatch; ++b) {
+        if (b == 0) {
+            GT_TRY(run_sync(b, &probe));
+            continue;
+        }
+        GT_TRY(setup(b));
+        GT_TRY(forward(0, probe));
+        GT_TRY(backward(probe, probe));
     }
+    if (nbatch > 1) {
+        std::vector<uint32_t> dev_status(nbatch);
+        GT_HIP(hipMemcpyAsync(dev_status.data(), dstatus, nbatch * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        for (size_t b = 1; b < nbatch; ++b) {
+            status[b] = dev_status[b];
+            if (status[b] == 0) GT_TRY(run_sync(b, &probe));
+        }
+    }
+    uint64_t levels_total = 0;
+    for (uint32_t st : status) levels_total += st;
     if (prof) {
         for (auto &e : ev) (void)hipEventDestroy(e);
         g->prof_ms[GT_SWEEP_BC_FORWARD] += fms;
