@@ -1131,6 +1131,7 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     // levels that read the depth from the device.  A batch whose BFS went
     // deeper leaves status 0 and is run again with the host in the loop.
     uint32_t probe = kBcLevelsPerCheck;
+    uint64_t rerun = 0;
     for (size_t b = 0; b < nbatch; ++b) {
         if (b == 0) {
             GT_TRY(run_sync(b, &probe));
@@ -1147,7 +1148,10 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
         GT_HIP(hipStreamSynchronize(s));
         for (size_t b = 1; b < nbatch; ++b) {
             status[b] = dev_status[b];
-            if (status[b] == 0) GT_TRY(run_sync(b, &probe));
+            if (status[b] == 0) {
+                ++rerun;
+                GT_TRY(run_sync(b, &probe));
+            }
         }
     }
     uint64_t levels_total = 0;
@@ -1200,7 +1204,8 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
             for (int c = 0; c < 3; ++c) work[3 * p + c] += slots[(p * kBcMaxWaves + w) * 3 + c];
     if (stats) {
         stats->sources = src.size();
-        stats->batches = (src.size() + kWave - 1) / kWave;
+        stats->batches = nbatch;
+        stats->batches_rerun = rerun;
         stats->levels = levels_total;
         stats->forward_entries = work[0];
         stats->forward_hits = work[1];

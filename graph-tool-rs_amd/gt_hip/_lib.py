@@ -72,7 +72,8 @@ class GtBcOut(C.Structure):
 
 
 class GtBcStats(C.Structure):
-    _fields_ = [("sources", C.c_uint64), ("batches", C.c_uint64), ("levels", C.c_uint64),
+    _fields_ = [("sources", C.c_uint64), ("batches", C.c_uint64),
+                ("batches_rerun", C.c_uint64), ("levels", C.c_uint64),
                 ("forward_entries", C.c_uint64), ("forward_hits", C.c_uint64),
                 ("forward_hit_entries", C.c_uint64),
                 ("backward_entries", C.c_uint64), ("backward_hits", C.c_uint64),

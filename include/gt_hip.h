@@ -274,6 +274,8 @@ typedef struct gt_bc_out {
 typedef struct gt_bc_stats {
     uint64_t sources;          /* pivots with at least one out-edge (the others add nothing) */
     uint64_t batches;          /* groups of 64 sources run together */
+    uint64_t batches_rerun;    /* batches whose BFS went deeper than the levels launched
+                                  for them, run again with the host in the loop */
     uint64_t levels;           /* BFS levels over all batches */
     uint64_t forward_entries;  /* adjacency entries the forward level kernels read */
     uint64_t forward_hits;     /* (entry, source) pairs that added a path count */
