@@ -285,9 +285,10 @@ static void adj_launch_fold(const AdjArgs &a, int sg, hipStream_t s) {
                        (uint32_t)sg);
 }
 
+// Grid of the vertex passes that end in a ticket (adj_init, grad_sums).
 static unsigned vgrid(uint64_t n) {
     uint64_t b = (n + kBlock - 1) / kBlock;
-    if (b > 2048) b = 2048;
+    if (b > kTicketGrid) b = kTicketGrid;
     return b ? (unsigned)b : 1u;
 }
 
@@ -423,7 +424,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         else adj_launch_fold<V, false, false>(b, sg, s);
     };
     uint64_t launched = 0;
-    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched));
+    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched,
+                 GT_SWEEP_ADJOINT));
     const uint64_t sweeps = g->h_state->iter;
     const V *lf = lam[sweeps & 1];
 

@@ -436,7 +436,7 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
         };
         uint64_t launched = 0;
         GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), params->epsilon, params->max_iter,
-                     sweep, fold, &launched));
+                     sweep, fold, &launched, GT_SWEEP_BATCHED));
         const uint64_t sweeps = g->h_state->iter;
         total += sweeps;
         uint64_t hit[kCols];

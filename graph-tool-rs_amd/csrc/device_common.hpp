@@ -11,6 +11,10 @@ namespace gt {
 
 constexpr int kWave = 64;
 
+// Most workgroups of a kernel that ends in a last-workgroup ticket: each
+// arrival is an atomic on one counter, and thousands of them serialise.
+constexpr unsigned kTicketGrid = 512;
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);

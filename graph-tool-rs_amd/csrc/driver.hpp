@@ -13,18 +13,21 @@ gt_status prof_begin(gt_graph *g, uint64_t k);
 gt_status prof_end(gt_graph *g, uint64_t k);
 gt_status prof_flush(gt_graph *g, uint64_t launched);
 
-// Launch sweeps until the device reports done.  Batches are sized from the
-// observed geometric decay of delta, so the host waits on the device a few
+// Launch sweeps until the device reports done.  The first batch is the
+// sweep count the same kind of loop (`hint_slot`) took last time on this
+// graph, so a repeated solve waits on the device once; later batches are
+// sized from the observed geometric decay of delta, so the host waits a few
 // times per solve and few launches find the loop already finished.
 // `sweep(k)` issues the SpMV kernel of sweep k of this run (k = 0, 1, ...),
 // timed when profiling; `fold(k)` then issues what completes the sweep (the
 // split rows and the convergence decision).
 template <typename Sweep, typename Fold>
 gt_status drive(gt_graph *g, SweepState *dst, double eps_pred, uint64_t cap, Sweep sweep,
-                Fold fold, uint64_t *launched_out) {
+                Fold fold, uint64_t *launched_out, int hint_slot) {
     hipStream_t s = g->stream;
     uint64_t launched = 0;
-    uint64_t batch = 2;
+    const uint64_t hint = g->sweep_hint[hint_slot];
+    uint64_t batch = (hint >= 2 && hint <= 64) ? hint : 2;
     double prev_delta = -1.0;
     uint64_t prev_iter = 0;
     for (;;) {
@@ -39,7 +42,10 @@ gt_status drive(gt_graph *g, SweepState *dst, double eps_pred, uint64_t cap, Swe
         GT_HIP(hipMemcpyAsync(g->h_state, dst, sizeof(SweepState), hipMemcpyDeviceToHost, s));
         GT_HIP(hipStreamSynchronize(s));
         const SweepState h = *g->h_state;
-        if (h.done || (cap > 0 && launched >= cap)) break;
+        if (h.done || (cap > 0 && launched >= cap)) {
+            g->sweep_hint[hint_slot] = h.launches;
+            break;
+        }
         uint64_t next = 4;
         if (prev_delta > 0.0 && h.delta > 0.0 && h.iter > prev_iter && h.delta < prev_delta) {
             const double rho =

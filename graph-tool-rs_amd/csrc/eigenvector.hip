@@ -348,7 +348,8 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         g->prof_kind = GT_SWEEP_EIGENVECTOR;
         // One launch beyond the answer is needed to learn its delta.
         const uint64_t cap = params->max_iter;
-        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, cap, sweep, fold, &launched));
+        GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, cap, sweep, fold, &launched,
+                     GT_SWEEP_EIGENVECTOR));
         EvState h;
         GT_HIP(hipMemcpy(&h, st, sizeof(EvState), hipMemcpyDeviceToHost));
         T = h.final_iter;

@@ -240,6 +240,10 @@ struct gt_graph {
     uint64_t prof_launches[GT_SWEEP_KINDS] = {};
     int prof_kind = 0;
     uint64_t last_sweeps = 0;
+    // Working sweeps the last loop of each kind took (first batch of the
+    // next): [0, GT_SWEEP_KINDS) by gt_sweep_kind, then the FP64 refinement
+    // phase of the FP32 form.
+    uint64_t sweep_hint[GT_SWEEP_KINDS + 1] = {};
     std::vector<hipEvent_t> ev;  // profiling events
     SweepState *h_state = nullptr;  // pinned mirror of the device state
 };

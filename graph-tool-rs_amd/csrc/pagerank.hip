@@ -43,6 +43,8 @@ struct PrArgs {
     const uint8_t *active;    // masked views only
     const uint32_t *iperm;    // sweep label -> caller's label (null: identity)
     const double *pers;       // dense personalisation, in sweep labels
+    const double *pers_c;     // the same in the caller's labels (seeding the map)
+    const float *x_src;       // pr_init: start from this FP32 iterate (sweep labels)
     double pval;              // constant personalisation
     double damping;
     double eps_abs;           // stop when delta < max(eps_abs, eps_rel * mass)
@@ -166,15 +168,18 @@ __global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, 
     V *x0 = static_cast<V *>(a.x_next);
     V *c0 = static_cast<V *>(a.c_next);
     double ssum = 0.0, mass = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        // Row i of the sweep is slot o of the caller's map.
-        const uint32_t o = a.iperm ? a.iperm[i] : i;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        // Row i of the sweep is slot iperm[i] of the caller's map; a cold
+        // start (had == 0, the usual case) reads no map and needs no label.
         double xi;
-        if (o >= had) {
+        if (a.x_src) {  // the FP64 refinement of the FP32 form
+            xi = static_cast<double>(a.x_src[i]);
+        } else if (had == 0) {
             xi = DPERS ? a.pers[i] : a.pval;
-            rank[o] = xi;
         } else {
-            xi = rank[o];
+            const uint32_t o = a.iperm ? a.iperm[i] : i;
+            xi = o >= had ? (DPERS ? a.pers[i] : a.pval) : rank[o];
         }
         const V xv = static_cast<V>(xi);
         x0[i] = xv;
@@ -192,6 +197,9 @@ __global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, 
             ssum += static_cast<double>(xv);
         }
     }
+    // Seed the caller's map: slots [had, n) take the personalisation.
+    for (uint64_t o = had + blockIdx.x * kBlock + threadIdx.x; o < n; o += stride)
+        rank[o] = DPERS ? a.pers_c[o] : a.pval;
     block_sum2(ssum, mass, red);
     if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = ssum;
@@ -255,15 +263,15 @@ int launch_sweep(const PrArgs &a, int per_cu, hipStream_t s) {
 }
 template <typename V, bool W, bool P, bool M>
 int launch_fold(const PrArgs &a, int sg, hipStream_t s) {
-    hipLaunchKernelGGL((pr_fold<V, W, P, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s, a,
-                       (uint32_t)sg);
+    hipLaunchKernelGGL((pr_fold<V, W, P, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s,
+                       a, (uint32_t)sg);
     return sg;
 }
 template <typename V, bool W, bool P, bool M>
 void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64_t iter0,
                  hipStream_t s) {
     uint32_t blocks = (n + kBlock - 1) / kBlock;
-    if (blocks > 2048) blocks = 2048;  // partials hold 2 x 2048 slots
+    if (blocks > kTicketGrid) blocks = kTicketGrid;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL((pr_init<V, W, P, M>), dim3(blocks), dim3(kBlock), 0, s, a, rank, had, n,
                        iter0);
@@ -413,6 +421,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     }
     a.iperm = S.iperm;
     GT_TRY(stage_vertex_prop(g, pers, g->ws.pers, &a.pers, &a.pval, &S.plan.dpers, uniform));
+    a.pers_c = a.pers;
     if (S.plan.dpers && S.iperm) {
         GT_TRY(g->ws.pers2.reserve(sizeof(double) * n));
         hipLaunchKernelGGL(permute_f64, dim3(vblocks(n)), dim3(kBlock), 0, s, a.pers, S.iperm, n,
@@ -497,7 +506,20 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
         sg = dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
     };
     auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };
-    GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched));
+    if (cap == 1) {
+        // Exactly one sweep (the FP64 refinement of the FP32 form): nothing
+        // to decide, so no wait on the device here.
+        GT_TRY(prof_begin(g, 0));
+        sweep(0);
+        GT_TRY(prof_end(g, 0));
+        fold(0);
+        GT_HIP(hipGetLastError());
+        GT_TRY(prof_flush(g, 1));
+        *iter_out = iter0 + 1;
+        return GT_OK;
+    }
+    GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched,
+                 iter0 > 0 ? GT_SWEEP_KINDS : g->prof_kind));
     *iter_out = g->h_state->iter;
     return GT_OK;
 }
@@ -612,13 +634,14 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     uint64_t it32 = 0;
     GT_TRY(pr_phase<float>(S, a32, x32, c32, rank->had, 0,
                            std::max(params->epsilon, std::ldexp(1.0, -22)), cap32, &it32));
-    // Widen the FP32 iterate into the caller's f64 map (and labels).
-    GT_TRY(pr_unpermute<float>(S, x32[it32 & 1]));
     (void)s;
     uint64_t total = it32;
     const double *latest = S.rank64;
     const bool capped = params->max_iter > 0 && it32 >= params->max_iter;
-    if (refine > 0 && !capped) {
+    if (refine == 0 || capped) {
+        // Widen the FP32 iterate into the caller's f64 map (and labels).
+        GT_TRY(pr_unpermute<float>(S, x32[it32 & 1]));
+    } else {
         GT_TRY(ws.x[1].reserve(sizeof(double) * n));
         GT_TRY(ws.c[0].reserve(sizeof(double) * n));
         GT_TRY(ws.c[1].reserve(sizeof(double) * n));
@@ -630,6 +653,7 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         uint64_t cap64 = refine;
         if (params->max_iter > 0) cap64 = std::min<uint64_t>(refine, params->max_iter - it32);
         a64.max_iter = it32 + cap64;
+        a64.x_src = x32[it32 & 1];  // FP64 start = the FP32 iterate, in sweep labels
         GT_TRY(ws.x[0].reserve(sizeof(double) * n));
         double *x64[2] = {S.iperm ? ws.x[0].as<double>() : S.rank64, ws.x[1].as<double>()};
         double *c64[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};

@@ -148,8 +148,18 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 // Sum of a split row's partial slots, in chunk order.
 __device__ __forceinline__ double long_row_sum(const double *long_part, const uint32_t *long_slot,
                                                uint32_t k) {
+    // Slot order, eight loads in flight: a hub has hundreds of slots.
+    const uint32_t e = long_slot[k + 1];
+    uint32_t q = long_slot[k];
     double acc = 0.0;
-    for (uint32_t q = long_slot[k]; q < long_slot[k + 1]; ++q) acc += long_part[q];
+    for (; q + 8 <= e; q += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = long_part[q + i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc += v[i];
+    }
+    for (; q < e; ++q) acc += long_part[q];
     return acc;
 }
 
