@@ -224,6 +224,9 @@ struct gt_graph {
     gt::Orient out;  // CSR: out-halves
     gt::Relabeled rel[2];  // [0]: pull = in-halves, [1]: pull = out-halves
     gt::Orient und;        // whole blocks (out-half then in-half), built on first use
+    // Vertex offsets of the last uneven batch found to have no edge joining
+    // two of its graphs.
+    std::vector<uint32_t> batch_checked;
     bool und_ready = false;
     // Edge list by edge id (source, target), kNoEdge where an id has no edge;
     // built on first use from the out-halves.

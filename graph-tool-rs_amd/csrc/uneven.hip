@@ -506,17 +506,21 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     a.long_part = ws.long_part.as<double>();
     hipLaunchKernelGGL(u_vgraph, dim3(ugrid(n)), dim3(kBlock), 0, s, a.voff, G, n,
                        U[kUVg].as<uint32_t>());
-    // Reject an edge that joins two graphs.
-    GT_HIP(hipMemsetAsync(U[kUBad].ptr, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(u_cross, dim3(ugrid((uint64_t)n * kWave)), dim3(kBlock), 0, s,
-                       g->out.off.as<uint32_t>(), g->out.idx.as<uint32_t>(),
-                       U[kUVg].as<uint32_t>(), n, U[kUBad].as<uint32_t>());
-    uint32_t bad = 0;
-    GT_HIP(hipMemcpyAsync(&g->h_state->counter, U[kUBad].ptr, sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    GT_HIP(hipStreamSynchronize(s));
-    bad = g->h_state->counter;
-    if (bad) return fail(GT_ERR_INVALID_ARG, "an edge joins two graphs of the batch");
+    // Reject an edge that joins two graphs (checked once per set of offsets:
+    // the graph's edges do not change).
+    if (g->batch_checked != bounds) {
+        GT_HIP(hipMemsetAsync(U[kUBad].ptr, 0, sizeof(uint32_t), s));
+        hipLaunchKernelGGL(u_cross, dim3(ugrid((uint64_t)n * kWave)), dim3(kBlock), 0, s,
+                           g->out.off.as<uint32_t>(), g->out.idx.as<uint32_t>(),
+                           U[kUVg].as<uint32_t>(), n, U[kUBad].as<uint32_t>());
+        uint32_t bad = 0;
+        GT_HIP(hipMemcpyAsync(&g->h_state->counter, U[kUBad].ptr, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        bad = g->h_state->counter;
+        if (bad) return fail(GT_ERR_INVALID_ARG, "an edge joins two graphs of the batch");
+        g->batch_checked = bounds;
+    }
     hipLaunchKernelGGL(u_reset, dim3(ugrid(G)), dim3(kBlock), 0, s, a);
     uint32_t live = 0;
     for (uint32_t k = 0; k < G; ++k) live += voff[k + 1] > voff[k];
