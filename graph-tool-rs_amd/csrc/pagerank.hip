@@ -44,7 +44,6 @@ struct PrArgs {
     const uint32_t *iperm;    // sweep label -> caller's label (null: identity)
     const double *pers;       // dense personalisation, in sweep labels
     const double *pers_c;     // the same in the caller's labels (seeding the map)
-    const float *x_src;       // pr_init: start from this FP32 iterate (sweep labels)
     double pval;              // constant personalisation
     double damping;
     double eps_abs;           // stop when delta < max(eps_abs, eps_rel * mass)
@@ -63,6 +62,7 @@ struct PrArgs {
 
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
 struct PrFin : TileAll {
+    static constexpr bool kWeighted = WEIGHTED;
     const PrArgs &a;
     double p_sink;
     double dsum = 0.0;
@@ -93,17 +93,44 @@ struct PrFin : TileAll {
     }
 };
 
+// The last sweep of the FP32 form, in FP64: the FP32 phase's final
+// contributions c = x / deg (FP32) are gathered, widened and summed in FP64,
+// the teleport update is FP64 and the new iterate FP64; delta is taken
+// against the FP32 iterate.  No sweep follows, so no contribution or dangling
+// mass is written.
+template <bool WEIGHTED, bool DPERS, bool MASK>
+struct RefineFin : TileAll {
+    static constexpr bool kWeighted = WEIGHTED;
+    const PrArgs &a;
+    double p_sink;
+    double dsum = 0.0;
+    double ssum = 0.0;
+    __device__ RefineFin(const PrArgs &args, double ps) : a(args), p_sink(ps) {}
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
+        const float *xc = static_cast<const float *>(a.x_cur);
+        double *xn = static_cast<double *>(a.x_next);
+        if (MASK && !a.active[i]) {
+            xn[i] = static_cast<double>(xc[i]);
+            return;
+        }
+        const double pv = DPERS ? a.pers[i] : a.pval;
+        const double nv = (1.0 - a.damping) * pv + a.damping * (p_sink * pv + acc);
+        dsum += fabs(nv - static_cast<double>(xc[i]));
+        xn[i] = nv;
+    }
+};
+
 // The SpMV of one sweep: every row but the split ones, with per-workgroup
-// delta and dangling-mass partials.  No fence and no ticket: pr_fold, the
-// next kernel on the stream, completes the sweep.
-template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
-__global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
-    if (a.st->done) return;
+// delta and dangling-mass partials.  No fence and no ticket: the fold, the
+// next kernel on the stream, completes the sweep.  S is the type of the
+// gathered contributions, V that of the sums.
+template <typename V, typename S, typename Fin>
+__device__ __forceinline__ void sweep_body(const PrArgs &a) {
     __shared__ WaveLds<V> L;
-    PrFin<V, WEIGHTED, DPERS, MASK> fin(a, a.st->p_sink);
-    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits,
-                            static_cast<const V *>(a.c_cur), static_cast<const V *>(a.w),
-                            a.long_part, a.short_row, L, fin);
+    Fin fin(a, a.st->p_sink);
+    spmv_waves<V, Fin::kWeighted>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits,
+                                  static_cast<const S *>(a.c_cur), static_cast<const V *>(a.w),
+                                  a.long_part, a.short_row, L, fin);
     double dsum = fin.dsum, ssum = fin.ssum;
     block_sum2(dsum, ssum, L.red);
     if (threadIdx.x == 0) {
@@ -112,17 +139,16 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     }
 }
 
-// Completes the sweep of the preceding pr_sweep launch (`sgrid` workgroups):
+// Completes the sweep of the preceding sweep launch (`sgrid` workgroups):
 // the split rows, one thread each, then -- in the last workgroup to arrive --
 // the fold of the sweep's and this kernel's partials in index order and the
 // convergence decision.
-template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
-__global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid) {
-    if (a.st->done) return;
+template <typename Fin>
+__device__ __forceinline__ void fold_body(const PrArgs &a, uint32_t sgrid) {
     __shared__ double red[8];
     __shared__ uint32_t flag;
     const int tid = threadIdx.x;
-    PrFin<V, WEIGHTED, DPERS, MASK> lf(a, a.st->p_sink);
+    Fin lf(a, a.st->p_sink);
     for (uint32_t k = blockIdx.x * kBlock + tid; k < a.nlong; k += gridDim.x * kBlock)
         lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
     double d = lf.dsum, s = lf.ssum;
@@ -157,6 +183,30 @@ __global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid
     }
 }
 
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
+    if (a.st->done) return;
+    sweep_body<V, V, PrFin<V, WEIGHTED, DPERS, MASK>>(a);
+}
+
+template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid) {
+    if (a.st->done) return;
+    fold_body<PrFin<V, WEIGHTED, DPERS, MASK>>(a, sgrid);
+}
+
+// The refinement runs once, after the FP32 loop finished (its state says
+// done), so neither kernel looks at the done flag.
+template <bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_refine(const PrArgs a) {
+    sweep_body<double, float, RefineFin<WEIGHTED, DPERS, MASK>>(a);
+}
+
+template <bool WEIGHTED, bool DPERS, bool MASK>
+__global__ __launch_bounds__(kBlock) void pr_refine_fold(const PrArgs a, uint32_t sgrid) {
+    fold_body<RefineFin<WEIGHTED, DPERS, MASK>>(a, sgrid);
+}
+
 // Seed slots [had, n) of the f64 map with pers, load the initial iterate into
 // the value type, compute c0 = x0/deg, the first dangling mass and the total
 // mass; resets the loop state (iter = `iter0`).
@@ -173,9 +223,7 @@ __global__ __launch_bounds__(kBlock) void pr_init(const PrArgs a, double *rank, 
         // Row i of the sweep is slot iperm[i] of the caller's map; a cold
         // start (had == 0, the usual case) reads no map and needs no label.
         double xi;
-        if (a.x_src) {  // the FP64 refinement of the FP32 form
-            xi = static_cast<double>(a.x_src[i]);
-        } else if (had == 0) {
+        if (had == 0) {
             xi = DPERS ? a.pers[i] : a.pval;
         } else {
             const uint32_t o = a.iperm ? a.iperm[i] : i;
@@ -294,6 +342,27 @@ void launch_init(const PrArgs &a, double *rank, uint64_t had, uint32_t n, uint64
 GT_PR_DISPATCH(sweep)
 GT_PR_DISPATCH(fold)
 #undef GT_PR_DISPATCH
+
+// The FP64 refinement sweep and its fold (one sweep, then the loop is done).
+template <bool W, bool P, bool M>
+void launch_refine(const PrArgs &a, int per_cu, hipStream_t s) {
+    const int sg = wave_grid(occupancy_cap((const void *)pr_refine<W, P, M>, per_cu), a.ntiles);
+    hipLaunchKernelGGL((pr_refine<W, P, M>), dim3(sg), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((pr_refine_fold<W, P, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s,
+                       a, (uint32_t)sg);
+}
+void dispatch_refine(const PrPlan &p, const PrArgs &a, int per_cu, hipStream_t s) {
+    switch ((p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0)) {
+        case 0: launch_refine<false, false, false>(a, per_cu, s); break;
+        case 1: launch_refine<false, false, true>(a, per_cu, s); break;
+        case 2: launch_refine<false, true, false>(a, per_cu, s); break;
+        case 3: launch_refine<false, true, true>(a, per_cu, s); break;
+        case 4: launch_refine<true, false, false>(a, per_cu, s); break;
+        case 5: launch_refine<true, false, true>(a, per_cu, s); break;
+        case 6: launch_refine<true, true, false>(a, per_cu, s); break;
+        default: launch_refine<true, true, true>(a, per_cu, s); break;
+    }
+}
 
 template <typename V>
 void dispatch_init(const PrPlan &p, const PrArgs &a, double *rank, uint64_t had, uint32_t n,
@@ -594,8 +663,9 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
 
 // FP32 sweeps (values, weights, SpMV accumulation) with FP64 dangling mass,
 // teleport update and L1 test, until delta < max(eps, 2^-22 * mass) -- below
-// that the L1 change of an FP32 iterate is rounding noise -- then `refine`
-// FP64 sweeps from the widened FP32 iterate.
+// that the L1 change of an FP32 iterate is rounding noise -- then, when
+// `refine` is non-zero, one FP64 sweep over the FP32 phase's last
+// contributions (FP64 sums, teleport and output).
 gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *weight,
                               const gt_prop *pers, double damping,
                               const gt_power_iteration *params, gt_vertex_out *rank,
@@ -642,25 +712,28 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         // Widen the FP32 iterate into the caller's f64 map (and labels).
         GT_TRY(pr_unpermute<float>(S, x32[it32 & 1]));
     } else {
-        GT_TRY(ws.x[1].reserve(sizeof(double) * n));
-        GT_TRY(ws.c[0].reserve(sizeof(double) * n));
-        GT_TRY(ws.c[1].reserve(sizeof(double) * n));
-        PrArgs a64 = S.base;
+        // One FP64 sweep from the FP32 phase's last contributions (already in
+        // HBM): no init kernel, 4-byte gathers, FP64 sums and output.
+        PrArgs ar = S.base;
         if (S.plan.weighted) {
             GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
-            a64.w = ws.wcsc64.ptr;
+            ar.w = ws.wcsc64.ptr;
         }
-        uint64_t cap64 = refine;
-        if (params->max_iter > 0) cap64 = std::min<uint64_t>(refine, params->max_iter - it32);
-        a64.max_iter = it32 + cap64;
-        a64.x_src = x32[it32 & 1];  // FP64 start = the FP32 iterate, in sweep labels
         GT_TRY(ws.x[0].reserve(sizeof(double) * n));
-        double *x64[2] = {S.iperm ? ws.x[0].as<double>() : S.rank64, ws.x[1].as<double>()};
-        double *c64[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
-        uint64_t it = 0;
-        GT_TRY(pr_phase<double>(S, a64, x64, c64, n, it32, params->epsilon, cap64, &it));
-        total = it;
-        latest = x64[(it - it32) & 1];
+        double *x64 = S.iperm ? ws.x[0].as<double>() : S.rank64;
+        ar.x_cur = x32[it32 & 1];
+        ar.c_cur = c32[it32 & 1];
+        ar.x_next = x64;
+        ar.max_iter = it32 + 1;
+        g->prof_kind = GT_SWEEP_PAGERANK_F64;
+        GT_TRY(prof_prepare(g));
+        GT_TRY(prof_begin(g, 0));
+        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, s);
+        GT_TRY(prof_end(g, 0));
+        GT_HIP(hipGetLastError());
+        GT_TRY(prof_flush(g, 1));
+        total = it32 + 1;
+        latest = x64;
         if (S.iperm) {
             GT_TRY(pr_unpermute<double>(S, latest));
             latest = S.rank64;

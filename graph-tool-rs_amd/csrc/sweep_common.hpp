@@ -58,11 +58,13 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename V, bool WEIGHTED, typename Fin>
+// V is the value type of the sums (and of the LDS tile); the gathered source
+// vector may be narrower (S), each value widened as it arrives.
+template <typename V, bool WEIGHTED, typename S, typename Fin>
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
-                                           const V *__restrict__ src, const V *__restrict__ w,
+                                           const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
                                            WaveLds<V> &L, Fin &fin) {
     constexpr int K = kWaveTileNnz / kWave;
@@ -86,7 +88,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t j = lane + k * kWave;
-            v[k] = j < cnt ? src[p[k] >> bits] : V(0);
+            v[k] = j < cnt ? static_cast<V>(src[p[k] >> bits]) : V(0);
         }
         if (WEIGHTED) {
 #pragma unroll

@@ -178,7 +178,8 @@ gt_status gt_pagerank_f64(gt_graph *g, const gt_view *view,
 
 /* Extension.  Same call, computed with FP32 values, weights and SpMV
  * accumulation while the dangling-mass sum, the teleport update and the L1
- * convergence test stay FP64, followed by FP64 refinement sweeps; rank is FP64.
+ * convergence test stay FP64, followed by one FP64 sweep over the last FP32
+ * contributions (FP64 sums, teleport update and output); rank is FP64.
  * *iters counts every sweep performed (FP32 and FP64). */
 gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view,
                             const gt_prop *weight, const gt_prop *pers,
