@@ -105,7 +105,8 @@ def _tune(args, dev, workload: str, weight) -> dict:
     from gt_hip import autotune
     t0 = time.perf_counter()
     rec = autotune.autotune(dev, workload, weight=weight, cache=None)
-    log(f"[bench] autotune ({workload}): {rec['tuning']} -> {rec['sweep_us']:.1f} us/sweep "
+    what = "FP32 + FP64 sweep" if workload == "pagerank" else "SpMM sweep"
+    log(f"[bench] autotune ({workload}): {rec['tuning']} -> {rec['sweep_us']:.1f} us {what} "
         f"(defaults {rec['default_sweep_us']:.1f}), {len(rec['trials'])} trials in "
         f"{time.perf_counter()-t0:.1f}s")
     return {"autotuned": True, "params": dev.tuning(), "sweep_us": rec["sweep_us"],
@@ -675,7 +676,9 @@ def main() -> int:
     ach64 = b64 * sw64[0] * args.steps / max(la64, 1) / avg64 / 1e9 if la64 else 0.0
 
     workload = f"pagerank_rmat{args.scale}_ef{args.edgefactor}"
-    traffic32, traffic_src = profiled_traffic(workload, "pr_sweep<float>")
+    hot = dev.tuning()["sweep_hot_kb"] > 0
+    traffic32, traffic_src = profiled_traffic(workload,
+                                              "pr_sweep_hot" if hot else "pr_sweep<float>")
     result = {
         "metric": METRIC,
         "value": round(gteps32, 3),
@@ -698,7 +701,8 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
-            "kernel": "pr_sweep<float> (FP32 PageRank sweep)",
+            "kernel": ("pr_sweep_hot (FP32 PageRank sweep, hot contributions in LDS)" if hot
+                       else "pr_sweep<float> (FP32 PageRank sweep)"),
             "bytes_per_sweep": b32, "avg_launch_us": round(avg32 * 1e6, 2),
             "traffic_source": traffic_src, "copy_calibration": copy,
         },

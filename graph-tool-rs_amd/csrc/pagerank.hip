@@ -189,6 +189,37 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     sweep_body<V, V, PrFin<V, WEIGHTED, DPERS, MASK>>(a);
 }
 
+// The FP32 sweep over a relabelled graph with the hot slice: one workgroup of
+// kHotWaves wavefronts per compute unit (the wavefronts per CU of the default
+// sweep), whose LDS holds the wave tiles and the first `nhot` contributions --
+// the most gathered vertices -- copied in at the start of the sweep.  nhot is
+// a multiple of 4, at most kHotMax.
+constexpr int kHotWaves = 12;
+constexpr int kHotBlock = kHotWaves * kWave;
+constexpr uint32_t kHotMax = 25344;  // 99 KB: with the tiles, 163,008 B of 163,840
+static_assert(sizeof(WaveLds<float, kHotWaves>) + 4 * kHotMax <= 163840, "LDS of one CU");
+
+template <bool WEIGHTED, bool DPERS>
+__global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32_t nhot) {
+    if (a.st->done) return;
+    __shared__ WaveLds<float, kHotWaves> L;
+    __shared__ float hot[kHotMax];
+    const float *c = static_cast<const float *>(a.c_cur);
+    for (uint32_t i = 4 * threadIdx.x; i < nhot; i += 4 * kHotBlock)
+        *reinterpret_cast<float4 *>(hot + i) = *reinterpret_cast<const float4 *>(c + i);
+    __syncthreads();
+    PrFin<float, WEIGHTED, DPERS, false> fin(a, a.st->p_sink);
+    spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
+                                      static_cast<const float *>(a.w), a.long_part, a.short_row,
+                                      L, fin, static_cast<const float *>(hot), nhot);
+    double dsum = fin.dsum, ssum = fin.ssum;
+    block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = dsum;
+        a.partials[gridDim.x + blockIdx.x] = ssum;
+    }
+}
+
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
 __global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid) {
     if (a.st->done) return;
@@ -308,6 +339,23 @@ int launch_sweep(const PrArgs &a, int per_cu, hipStream_t s) {
     const int sg = wave_grid(occupancy_cap((const void *)pr_sweep<V, W, P, M>, per_cu), a.ntiles);
     hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
+}
+// The hot-slice FP32 sweep over `nhot` contributions; returns its workgroup
+// count (one per compute unit, fewer for a small graph).
+template <bool W, bool P>
+int launch_sweep_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
+    const int sg = fit_grid(occupancy_cap((const void *)pr_sweep_hot<W, P>, 1),
+                            (a.ntiles + kHotWaves - 1) / kHotWaves);
+    hipLaunchKernelGGL((pr_sweep_hot<W, P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
+    return sg;
+}
+int dispatch_sweep_hot(const PrPlan &p, const PrArgs &a, uint32_t nhot, hipStream_t s) {
+    switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
+        case 0: return launch_sweep_hot<false, false>(a, nhot, s);
+        case 1: return launch_sweep_hot<false, true>(a, nhot, s);
+        case 2: return launch_sweep_hot<true, false>(a, nhot, s);
+        default: return launch_sweep_hot<true, true>(a, nhot, s);
+    }
 }
 template <typename V, bool W, bool P, bool M>
 int launch_fold(const PrArgs &a, int sg, hipStream_t s) {
@@ -566,13 +614,21 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     uint64_t launched = 0;
     PrArgs b = a;
     int sg = 0;
+    // The hot slice serves FP32 sweeps in relabelled order (the most gathered
+    // vertices first); masked views keep the caller's labels.
+    uint32_t nhot = 0;
+    if (std::is_same<V, float>::value && S.iperm && !S.plan.masked && g->tune.sweep_hot_kb > 0) {
+        const uint64_t want = (uint64_t)g->tune.sweep_hot_kb * 256;  // floats
+        nhot = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, kHotMax), g->n) & ~3u;
+    }
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
         b.x_cur = x[par];
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        sg = dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
+        sg = nhot ? dispatch_sweep_hot(S.plan, b, nhot, s)
+                  : dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
     };
     auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };
     if (cap == 1) {

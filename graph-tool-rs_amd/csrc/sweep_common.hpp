@@ -28,7 +28,15 @@
 // brackets every tile: fin.tile(t) (wave-uniform) before it -- returning false
 // skips the tile, loads included -- and fin.tile_end(t) after it.  TileAll is
 // the pair for finishers that need neither.
+//
+// With HOT, the first `nhot` entries of the source vector are also in LDS
+// (`hot`, filled by the caller before the walk): a gather below that label
+// reads LDS and sends no request to L2.  After relabelling by gather
+// frequency those are the most gathered vertices.  The values are the same,
+// so the sums are too.
 #pragma once
+
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "gt_internal.hpp"
@@ -37,14 +45,40 @@ namespace gt {
 
 static_assert(kWavesPerBlock * kWave == kBlock, "workgroups are whole wavefronts");
 
-template <typename V>
+// Per-wavefront tiles of a workgroup of NW wavefronts, and the workgroup's
+// reduction slots (two per wavefront).
+template <typename V, int NW = kWavesPerBlock>
 struct WaveLds {
-    V vals[kWavesPerBlock][kWaveTileNnz];
-    V rowsum[kWavesPerBlock][kWaveTileRows];
-    double red[8];
+    V vals[NW][kWaveTileNnz];
+    V rowsum[NW][kWaveTileRows];
+    double red[2 * NW];
 };
 
+// Two workgroup sums at once over NW wavefronts, in wavefront order.
+template <int NW>
+__device__ __forceinline__ void block_sum2_waves(double &a, double &b, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) {
+        red[wid] = a;
+        red[NW + wid] = b;
+    }
+    __syncthreads();
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        a += red[w];
+        b += red[NW + w];
+    }
+    __syncthreads();
+}
+
 // Default tile hooks: every tile is processed, nothing is done at its end.
+// A finisher deriving from TileAll keeps tile() as it is (the engine then
+// loads a wavefront's next tile before it is done with the current one).
 struct TileAll {
     __device__ __forceinline__ bool tile(uint32_t) const { return true; }
     __device__ __forceinline__ void tile_end(uint32_t) const {}
@@ -58,37 +92,100 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// What a wavefront loads for a tile before its gathers: the descriptor, the
+// packed words and, for a tile of whole rows, each row's offsets (lane q of
+// step i holds row 64 i + q).
+struct WavePre {
+    static constexpr int K = kWaveTileNnz / kWave;
+    static constexpr int R = kWaveTileRows / kWave;
+    Tile t;
+    uint32_t p[K];
+    uint32_t lo[R], hi[R];
+};
+
+__device__ __forceinline__ void wave_fetch(const Tile *__restrict__ tiles, uint32_t ti,
+                                           const uint32_t *__restrict__ off,
+                                           const uint32_t *__restrict__ packed, int lane,
+                                           WavePre &f) {
+    f.t = tiles[ti];
+    const uint32_t nb = f.t.nnz_begin, cnt = f.t.nnz_end - f.t.nnz_begin;
+#pragma unroll
+    for (int k = 0; k < WavePre::K; ++k) {
+        const uint32_t j = lane + k * kWave;
+        f.p[k] = j < cnt ? __builtin_nontemporal_load(packed + nb + j) : 0u;
+    }
+    const uint32_t rb = f.t.row_begin;
+    const uint32_t rows = f.t.row_end == kSplit ? 0u : f.t.row_end - rb;
+#pragma unroll
+    for (int i = 0; i < WavePre::R; ++i) {
+        const uint32_t q = lane + i * kWave;
+        f.lo[i] = q < rows ? off[rb + q] : 0u;
+        f.hi[i] = q < rows ? off[rb + q + 1] : 0u;
+    }
+}
+
 // V is the value type of the sums (and of the LDS tile); the gathered source
-// vector may be narrower (S), each value widened as it arrives.
-template <typename V, bool WEIGHTED, typename S, typename Fin>
+// vector may be narrower (S), each value widened as it arrives.  With a
+// finisher that takes every tile (one deriving from TileAll, which must not
+// override tile()), a wavefront loads its next tile's packed words and row
+// offsets right after the current tile's gathers, so their latency overlaps
+// the current tile's sums.
+template <typename V, bool WEIGHTED, bool HOT = false, int NW, typename S, typename Fin>
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
                                            const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
-                                           WaveLds<V> &L, Fin &fin) {
-    constexpr int K = kWaveTileNnz / kWave;
+                                           WaveLds<V, NW> &L, Fin &fin,
+                                           const S *hot = nullptr, uint32_t nhot = 0) {
+    constexpr int K = WavePre::K;
+    constexpr bool PIPE = std::is_base_of<TileAll, Fin>::value;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t slot_mask = (1u << bits) - 1u;
     V *vals = L.vals[wid];
     V *rowsum = L.rowsum[wid];
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    for (uint32_t ti = blockIdx.x * kWavesPerBlock + wid; ti < ntiles; ti += nwaves) {
-        if (!fin.tile(ti)) continue;
-        const Tile t = tiles[ti];
-        const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
-        uint32_t p[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t j = lane + k * kWave;
-            p[k] = j < cnt ? __builtin_nontemporal_load(packed + nb + j) : 0u;
+    const uint32_t nwaves = gridDim.x * NW;
+    WavePre cur, nxt;
+    uint32_t ti = blockIdx.x * NW + wid;
+    if (PIPE && ti < ntiles) wave_fetch(tiles, ti, off, packed, lane, cur);
+    for (; ti < ntiles; ti += nwaves) {
+        if (!PIPE) {
+            if (!fin.tile(ti)) continue;
+            wave_fetch(tiles, ti, off, packed, lane, cur);
         }
+        const Tile t = cur.t;
+        const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
         V v[K];
+        if (HOT) {
+            // Global gathers for the lanes above the slice only, LDS reads
+            // for every lane (a lane above it reads slot 0), then a select:
+            // separate registers keep all K gathers in flight at once.
+            S g[K], h[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t j = lane + k * kWave;
-            v[k] = j < cnt ? static_cast<V>(src[p[k] >> bits]) : V(0);
+            for (int k = 0; k < K; ++k) {
+                const uint32_t j = lane + k * kWave;
+                const uint32_t sx = cur.p[k] >> bits;
+                g[k] = S(0);
+                if (j < cnt && sx >= nhot) g[k] = src[sx];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t sx = cur.p[k] >> bits;
+                h[k] = hot[sx < nhot ? sx : 0u];
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t j = lane + k * kWave;
+                const uint32_t sx = cur.p[k] >> bits;
+                v[k] = j < cnt ? static_cast<V>(sx < nhot ? h[k] : g[k]) : V(0);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t j = lane + k * kWave;
+                v[k] = j < cnt ? static_cast<V>(src[cur.p[k] >> bits]) : V(0);
+            }
         }
         if (WEIGHTED) {
 #pragma unroll
@@ -97,6 +194,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                 if (j < cnt) v[k] *= __builtin_nontemporal_load(w + nb + j);
             }
         }
+        if (PIPE && ti + nwaves < ntiles) wave_fetch(tiles, ti + nwaves, off, packed, lane, nxt);
         if (t.row_end == kSplit) {
             V acc = V(0);
 #pragma unroll
@@ -104,46 +202,50 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             const double tot = wave_sum(static_cast<double>(acc));
             if (lane == 0) long_part[t.row_begin] = tot;
             fin.tile_end(ti);
-            continue;
-        }
+        } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t j = lane + k * kWave;
-            if (j < cnt) vals[bits ? (p[k] & slot_mask) : j] = v[k];
-        }
-        wave_sync();
-        const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
-        for (uint32_t q0 = 0; q0 < rows; q0 += kWave) {
-            const uint32_t q = q0 + lane;
-            uint32_t lo = 0, hi = 0;
-            bool longrow = false;
-            if (q < rows) {
-                lo = off[rb + q] - nb;
-                hi = off[rb + q + 1] - nb;
-                if (hi - lo < short_row) {
+            for (int k = 0; k < K; ++k) {
+                const uint32_t j = lane + k * kWave;
+                if (j < cnt) vals[bits ? (cur.p[k] & slot_mask) : j] = v[k];
+            }
+            wave_sync();
+            const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
+#pragma unroll
+            for (int i = 0; i < WavePre::R; ++i) {
+                const uint32_t q0 = i * kWave;
+                if (q0 >= rows) break;
+                const uint32_t q = q0 + lane;
+                uint32_t lo = 0, hi = 0;
+                bool longrow = false;
+                if (q < rows) {
+                    lo = cur.lo[i] - nb;
+                    hi = cur.hi[i] - nb;
+                    if (hi - lo < short_row) {
+                        V acc = V(0);
+                        for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
+                        rowsum[q] = acc;
+                    } else {
+                        longrow = true;
+                    }
+                }
+                uint64_t lmask = __ballot(longrow);
+                while (lmask) {
+                    const int r = __builtin_ctzll(lmask);
+                    lmask &= lmask - 1;
+                    const uint32_t rlo = __shfl(lo, r, kWave), rhi = __shfl(hi, r, kWave);
                     V acc = V(0);
-                    for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
-                    rowsum[q] = acc;
-                } else {
-                    longrow = true;
+                    for (uint32_t k = rlo + lane; k < rhi; k += kWave) acc += vals[k];
+                    acc = wave_sum(acc);
+                    if (lane == 0) rowsum[q0 + r] = acc;
                 }
             }
-            uint64_t lmask = __ballot(longrow);
-            while (lmask) {
-                const int r = __builtin_ctzll(lmask);
-                lmask &= lmask - 1;
-                const uint32_t rlo = __shfl(lo, r, kWave), rhi = __shfl(hi, r, kWave);
-                V acc = V(0);
-                for (uint32_t k = rlo + lane; k < rhi; k += kWave) acc += vals[k];
-                acc = wave_sum(acc);
-                if (lane == 0) rowsum[q0 + r] = acc;
-            }
+            wave_sync();
+            for (uint32_t q = lane; q < rows; q += kWave)
+                fin(rb + q, static_cast<double>(rowsum[q]));
+            fin.tile_end(ti);
+            wave_sync();  // the LDS tile is reused by this wavefront's next tile
         }
-        wave_sync();
-        for (uint32_t q = lane; q < rows; q += kWave)
-            fin(rb + q, static_cast<double>(rowsum[q]));
-        fin.tile_end(ti);
-        wave_sync();  // the LDS tile is reused by this wavefront's next tile
+        if (PIPE) cur = nxt;
     }
 }
 

@@ -53,7 +53,8 @@ class GtVertexOut(C.Structure):
 class GtTuning(C.Structure):
     _fields_ = [("sweep_blocks_per_cu", C.c_int32), ("tile_rows", C.c_int32),
                 ("short_row", C.c_int32), ("mm_tile_nnz", C.c_int32),
-                ("mm_blocks_per_cu", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("mm_blocks_per_cu", C.c_int32), ("sweep_hot_kb", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class GtView(C.Structure):

@@ -3,7 +3,10 @@
 The pull sweeps (PageRank, eigenvector, adjoint, uneven batch) depend on
 three parameters: the resident workgroups per compute unit, the rows per wave
 tile, and the row length below which one lane sums a row (above it the whole
-wavefront does).  The batched SpMM depends on its tile size and its resident
+wavefront does).  The FP32 PageRank sweep on a relabelled graph runs one
+workgroup per compute unit and also depends on how many of the most gathered
+contributions it keeps in LDS; the PageRank probe therefore times an FP32
+sweep and an FP64 sweep, and minimises their sum.  The batched SpMM depends on its tile size and its resident
 workgroups per compute unit.  ``autotune`` times candidates on the caller's
 own graph -- the sweep kernels alone, measured with the HIP events the library
 records on its stream (gt_graph_set_profiling / gt_sweep_stats) -- walks each
@@ -32,6 +35,7 @@ SWEEP_SPACE = {
     "sweep_blocks_per_cu": (2, 3, 4, 6, 8),
     "tile_rows": (64, 128, 256),
     "short_row": (16, 32, 64),
+    "sweep_hot_kb": (99, 64, -1),
 }
 BATCHED_SPACE = {
     "mm_tile_nnz": (512, 1024, 2048, 4096),
@@ -72,8 +76,8 @@ def _store(path: Path, key: str, rec: dict) -> None:
 
 
 class _PagerankProbe:
-    """Runs capped FP32-form PageRank solves and returns the mean FP32 sweep
-    kernel time (seconds)."""
+    """Runs capped FP32-form and FP64 PageRank solves and returns the mean FP32
+    sweep kernel time plus the mean FP64 sweep kernel time (seconds)."""
 
     def __init__(self, dev, weight, sweeps: int, reps: int):
         from . import _prop, Unity
@@ -88,19 +92,22 @@ class _PagerankProbe:
                                     dev.graph_id)
         self.iters = C.c_uint64()
 
-    def _solve(self):
+    def _solve(self, fn: str):
         self.out.had = 0
-        call("gt_pagerank_mixed", self.dev.handle, C.byref(self.view), C.byref(self.wp), None,
+        call(fn, self.dev.handle, C.byref(self.view), C.byref(self.wp), None,
              0.85, C.byref(self.params), C.byref(self.out), C.byref(self.iters))
 
     def __call__(self) -> float:
-        self._solve()  # warm-up: rebuilds the schedule after a tile change
+        self._solve("gt_pagerank_mixed")  # warm-up: rebuilds the schedule after a tile change
         self.dev.set_profiling(True)
         for _ in range(self.reps):
-            self._solve()
-        ms, launches, _ = self.dev.sweep_stats(_lib.GT_SWEEP_PAGERANK_F32)
+            self._solve("gt_pagerank_mixed")
+            self._solve("gt_pagerank_f64")
+        ms32, la32, _ = self.dev.sweep_stats(_lib.GT_SWEEP_PAGERANK_F32)
+        ms64, la64, _ = self.dev.sweep_stats(_lib.GT_SWEEP_PAGERANK_F64)
         self.dev.set_profiling(False)
-        return ms * 1e-3 / max(launches, 1)
+        # The mixed solve's FP64 refinement sweep counts among the FP64 launches.
+        return ms32 * 1e-3 / max(la32, 1) + ms64 * 1e-3 / max(la64, 1)
 
 
 class _BatchedProbe:
@@ -163,7 +170,8 @@ def autotune(dev, workload: str = "pagerank", weight=None, cache: Optional[Path]
              sweeps: int = 6, reps: int = 3) -> dict:
     """Tune ``dev`` for ``workload`` ("pagerank": the pull sweeps; "batched":
     the k = 64 SpMM), leave the best parameters set on it, cache them, and
-    return {"tuning", "sweep_us", "default_sweep_us", "trials"}."""
+    return {"tuning", "sweep_us", "default_sweep_us", "trials"}; "sweep_us" is the
+    probe's objective (for "pagerank" the FP32 plus the FP64 sweep time)."""
     weighted = weight is not None
     if workload == "pagerank":
         probe, space = _PagerankProbe(dev, weight, sweeps, reps), SWEEP_SPACE

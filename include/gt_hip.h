@@ -374,7 +374,10 @@ typedef struct gt_tuning {
     int32_t short_row;           /* rows with fewer entries are summed by one lane: 1..1024 (32) */
     int32_t mm_tile_nnz;         /* batched SpMM: entries per tile, 256..8192 (1024) */
     int32_t mm_blocks_per_cu;    /* batched SpMM: resident workgroups per CU: 1..8 (8) */
-    int32_t reserved[3];
+    int32_t sweep_hot_kb;        /* FP32 PageRank sweeps on a relabelled graph: KB of the most
+                                    gathered contributions held in LDS, one workgroup per CU;
+                                    -1 turns it off: -1, 1..99 (99) */
+    int32_t reserved[2];
 } gt_tuning;
 
 gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);

@@ -221,6 +221,48 @@ def test_mixed_precision_within_1e6(gpu):
         assert it >= 1
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("personalised", [False, True])
+def test_mixed_hot_slice(gpu, weighted, personalised):
+    """The FP32 sweep that keeps the most gathered contributions in LDS (the
+    default on unfiltered graphs) against the oracle, and against the same
+    solve with the slice off: graphs wholly inside the slice (scale 12), larger
+    than it (scale 16), with n not a multiple of 4 (1003), and a 1 KB slice."""
+    cases = [oracle.gen_rmat(12, 16 << 12, 0.57, 0.19, 0.19, 5) + (1 << 12, 0),
+             oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 0),
+             oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 1)]
+    r = Rng(31)
+    n = 1003
+    s = np.array([r.below(n) for _ in range(9000)], np.uint32)
+    t = np.array([r.below(n) for _ in range(9000)], np.uint32)
+    cases.append((s, t, n, 0))
+    for s, t, n, kb in cases:
+        rng = np.random.default_rng(n + kb)
+        w = rng.uniform(0.1, 1.0, len(s)) if weighted else None
+        # Normalised: an R-MAT graph has many sinks, whose mass the
+        # personalisation hands back each sweep.
+        p = rng.uniform(0.0, 2.0, n) if personalised else None
+        if p is not None:
+            p /= p.sum()
+        g = adj(n, s, t)
+        dev = g.device()
+        want, _ = oracle.Adj(n, s, t).pagerank(weight=w, pers=p, epsilon=1e-12, max_iter=1000,
+                                               threads=8)
+        got = {}
+        for hot in (kb, -1):
+            dev.set_tuning(sweep_hot_kb=hot)
+            wp = None if w is None else gt_hip.DenseProp.from_vec(g.graph_id, w)
+            pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
+            got[hot] = pr(g, 1e-9, 1000, weight=wp, pers=pp, fn=gt_hip.pagerank_mixed)
+        dev.set_tuning(sweep_hot_kb=0)
+        scale = max(1.0, np.abs(want).sum())
+        for hot, (x, it) in got.items():
+            assert np.abs(x - want).sum() <= 1e-6 * scale, (n, kb, hot)
+        # Only the order of the FP64 partial folds differs between the two.
+        assert abs(got[kb][1] - got[-1][1]) <= 1
+        assert np.abs(got[kb][0] - got[-1][0]).sum() <= 1e-7 * scale
+
+
 # ---------------------------------------------------------------- errors
 
 def test_errors(gpu):
