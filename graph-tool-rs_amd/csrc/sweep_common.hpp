@@ -92,9 +92,29 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A raw buffer resource over `bytes` bytes at `base`: a load at a byte offset
+// at or past `bytes` returns 0 (the hardware's range check), so a masked lane
+// needs no branch.  Loads without branches let the compiler count exactly how
+// many are in flight, and wait only for the ones a value comes from.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+constexpr int kNonTemporal = 2;  // cache policy of a buffer load: streamed once
+
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float *) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0,
+                                                                          kNonTemporal));
+}
+__device__ __forceinline__ double buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double *) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0,
+                                                                           kNonTemporal));
+}
+
 // What a wavefront loads for a tile before its gathers: the descriptor, the
 // packed words and, for a tile of whole rows, each row's offsets (lane q of
-// step i holds row 64 i + q).
+// step i holds row 64 i + q).  Every load is unconditional (lanes past the
+// tile read 0 through the buffer range check).
 struct WavePre {
     static constexpr int K = kWaveTileNnz / kWave;
     static constexpr int R = kWaveTileRows / kWave;
@@ -103,24 +123,41 @@ struct WavePre {
     uint32_t lo[R], hi[R];
 };
 
-__device__ __forceinline__ void wave_fetch(const Tile *__restrict__ tiles, uint32_t ti,
-                                           const uint32_t *__restrict__ off,
+// The tile table is read-only for a sweep: read through the constant address
+// space, a wave-uniform descriptor is a scalar load into SGPRs, and the buffer
+// resources built from it stay scalar too.
+typedef const __attribute__((address_space(4))) Tile ConstTile;
+
+__device__ __forceinline__ Tile wave_tile(const Tile *__restrict__ tiles, uint32_t ti) {
+    ConstTile *c = (ConstTile *)tiles + ti;
+    Tile t;
+    t.row_begin = c->row_begin;
+    t.row_end = c->row_end;
+    t.nnz_begin = c->nnz_begin;
+    t.nnz_end = c->nnz_end;
+    return t;
+}
+
+// Loads of tile `t` (its descriptor already read).
+__device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, int lane,
                                            WavePre &f) {
-    f.t = tiles[ti];
+    f.t = t;
     const uint32_t nb = f.t.nnz_begin, cnt = f.t.nnz_end - f.t.nnz_begin;
+    const __amdgpu_buffer_rsrc_t rp = buffer_rsrc(packed + nb, 4 * cnt);
 #pragma unroll
-    for (int k = 0; k < WavePre::K; ++k) {
-        const uint32_t j = lane + k * kWave;
-        f.p[k] = j < cnt ? __builtin_nontemporal_load(packed + nb + j) : 0u;
-    }
-    const uint32_t rb = f.t.row_begin;
-    const uint32_t rows = f.t.row_end == kSplit ? 0u : f.t.row_end - rb;
+    for (int k = 0; k < WavePre::K; ++k)
+        f.p[k] = __builtin_amdgcn_raw_buffer_load_b32(rp, 4 * (lane + k * kWave), 0, kNonTemporal);
+    // A split chunk's row_begin is a partial slot, not a row: no offsets.
+    const bool split = f.t.row_end == kSplit;
+    const uint32_t rb = split ? 0u : f.t.row_begin;
+    const uint32_t rows = split ? 0u : f.t.row_end - f.t.row_begin;
+    const __amdgpu_buffer_rsrc_t ro = buffer_rsrc(off + rb, split ? 0u : 4 * (rows + 1));
 #pragma unroll
     for (int i = 0; i < WavePre::R; ++i) {
         const uint32_t q = lane + i * kWave;
-        f.lo[i] = q < rows ? off[rb + q] : 0u;
-        f.hi[i] = q < rows ? off[rb + q + 1] : 0u;
+        f.lo[i] = __builtin_amdgcn_raw_buffer_load_b32(ro, 4 * q, 0, 0);
+        f.hi[i] = __builtin_amdgcn_raw_buffer_load_b32(ro, 4 * (q + 1), 0, 0);
     }
 }
 
@@ -148,53 +185,52 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     const uint32_t nwaves = gridDim.x * NW;
     WavePre cur, nxt;
     uint32_t ti = blockIdx.x * NW + wid;
-    if (PIPE && ti < ntiles) wave_fetch(tiles, ti, off, packed, lane, cur);
+    if (PIPE && ti < ntiles) wave_fetch(wave_tile(tiles, ti), off, packed, lane, cur);
     for (; ti < ntiles; ti += nwaves) {
         if (!PIPE) {
             if (!fin.tile(ti)) continue;
-            wave_fetch(tiles, ti, off, packed, lane, cur);
+            wave_fetch(wave_tile(tiles, ti), off, packed, lane, cur);
         }
+        // The next tile's descriptor (the last tile reads its own again).
+        const Tile tnext = wave_tile(tiles, ti + nwaves < ntiles ? ti + nwaves : ntiles - 1);
         const Tile t = cur.t;
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
-        V v[K];
-        if (HOT) {
-            // Global gathers for the lanes above the slice only, LDS reads
-            // for every lane (a lane above it reads slot 0), then a select:
-            // separate registers keep all K gathers in flight at once.
-            S g[K], h[K];
+        // 1. The gathers (global ones only above the hot slice).
+        S g[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t j = lane + k * kWave;
-                const uint32_t sx = cur.p[k] >> bits;
-                g[k] = S(0);
-                if (j < cnt && sx >= nhot) g[k] = src[sx];
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t sx = cur.p[k] >> bits;
-                h[k] = hot[sx < nhot ? sx : 0u];
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t j = lane + k * kWave;
-                const uint32_t sx = cur.p[k] >> bits;
-                v[k] = j < cnt ? static_cast<V>(sx < nhot ? h[k] : g[k]) : V(0);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t j = lane + k * kWave;
-                v[k] = j < cnt ? static_cast<V>(src[cur.p[k] >> bits]) : V(0);
-            }
+        for (int k = 0; k < K; ++k) {
+            const uint32_t j = lane + k * kWave;
+            const uint32_t sx = cur.p[k] >> bits;
+            g[k] = S(0);
+            if (j < cnt && (!HOT || sx >= nhot)) g[k] = src[sx];
         }
+        // 2. The weights, then the next tile's words and offsets (the last
+        //    tile loads itself again): both without branches, so waiting
+        //    for the gathers leaves them in flight.
+        V wt[K];
         if (WEIGHTED) {
+            const __amdgpu_buffer_rsrc_t rw = buffer_rsrc(w + nb, sizeof(V) * cnt);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t j = lane + k * kWave;
-                if (j < cnt) v[k] *= __builtin_nontemporal_load(w + nb + j);
-            }
+            for (int k = 0; k < K; ++k)
+                wt[k] = buffer_load(rw, sizeof(V) * (lane + k * kWave), (V *)nullptr);
         }
-        if (PIPE && ti + nwaves < ntiles) wave_fetch(tiles, ti + nwaves, off, packed, lane, nxt);
+        if (PIPE) wave_fetch(tnext, off, packed, lane, nxt);
+        // Nothing that reads a gathered value moves above the loads issued so far.
+        __builtin_amdgcn_sched_barrier(0);
+        // 3. The hot slice (every lane reads LDS; a lane above it reads slot 0).
+        V v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t j = lane + k * kWave;
+            const uint32_t sx = cur.p[k] >> bits;
+            S x = g[k];
+            if (HOT) {
+                const S h = hot[sx < nhot ? sx : 0u];
+                x = sx < nhot ? h : x;
+            }
+            v[k] = j < cnt ? static_cast<V>(x) : V(0);
+            if (WEIGHTED) v[k] *= wt[k];
+        }
         if (t.row_end == kSplit) {
             V acc = V(0);
 #pragma unroll
