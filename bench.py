@@ -294,7 +294,9 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
         "backward_ms": round(dtb / args.steps * 1e3, 4), "backward_sweeps": bo[-1],
         "roofline": {"bound": "hbm", "achieved": round(ach_b, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "adj_sweep<float, true, false> (adjoint sweep)",
+                     "kernel": ("adj_sweep_hot<true, false> (adjoint sweep, hot lambda in LDS)"
+                                if dev.tuning()["sweep_hot_kb"] > 0
+                                else "adj_sweep<float, true, false> (adjoint sweep)"),
                      "bytes_per_sweep": Bf,
                      "avg_launch_us": round(ms_b / max(la_b, 1) * 1e3, 2)},
         "grad_damping": gd.value,

@@ -8,7 +8,9 @@
 //   dL/dd   = lambda . (x - p) / d
 //
 // The transposed system is solved by the same power iteration, pulling over
-// the opposite orientation, until its L1 change drops below
+// the opposite orientation (relabelled like the forward sweep's, the most
+// gathered targets first; lambda and mu are mapped back to the caller's
+// labels for the gradient passes), until its L1 change drops below
 // epsilon * ||dL/dx||_1 (the system is linear in dL/dx, so the tolerance
 // scales with it; FP32 runs stop no later than 2^-22 * ||dL/dx||_1, below
 // which an FP32 iterate's change is rounding noise): row s of the out-halves gathers lambda at its
@@ -88,6 +90,27 @@ __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
                             a.long_part, a.short_row, L, fin);
     double d = fin.dsum, p = fin.psum;
     block_sum2(d, p, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = d;
+        a.partials[gridDim.x + blockIdx.x] = p;
+    }
+}
+
+// The FP32 transposed sweep with the hot slice of lambda in LDS (one
+// workgroup of kHotWaves wavefronts per compute unit).
+template <bool WEIGHTED, bool DPERS>
+__global__ __launch_bounds__(kHotBlock) void adj_sweep_hot(const AdjArgs a, uint32_t nhot) {
+    if (a.st->done) return;
+    __shared__ WaveLds<float, kHotWaves> L;
+    __shared__ float hot[kHotMax];
+    const float *lc = static_cast<const float *>(a.l_cur);
+    fill_hot(lc, nhot, hot);
+    AdjFin<float, WEIGHTED, DPERS> fin(a, a.st->p_sink);
+    spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, lc,
+                                      static_cast<const float *>(a.w), a.long_part, a.short_row,
+                                      L, fin, static_cast<const float *>(hot), nhot);
+    double d = fin.dsum, p = fin.psum;
+    block_sum2_waves<kHotWaves>(d, p, L.red);
     if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = d;
         a.partials[gridDim.x + blockIdx.x] = p;
@@ -268,6 +291,16 @@ __global__ __launch_bounds__(kBlock) void grad_pers_k(const V *lam, const SweepS
         gp[i] = f * static_cast<double>(lam[i]);
 }
 
+// out[i] = in[idx[i]]: with idx = iperm a caller-label vector into relabelled
+// order, with idx = perm back.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void adj_gather(const T *__restrict__ in,
+                                                     const uint32_t *__restrict__ idx, uint32_t n,
+                                                     T *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        out[i] = in[idx[i]];
+}
+
 __global__ __launch_bounds__(kBlock) void f32_to_f64_v(const float *a, uint32_t n, double *b) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
         b[i] = static_cast<double>(a[i]);
@@ -277,6 +310,13 @@ template <typename V, bool W, bool P>
 static int adj_launch(const AdjArgs &a, int per_cu, hipStream_t s) {
     const int sg = wave_grid(occupancy_cap((const void *)adj_sweep<V, W, P>, per_cu), a.ntiles);
     hipLaunchKernelGGL((adj_sweep<V, W, P>), dim3(sg), dim3(kBlock), 0, s, a);
+    return sg;
+}
+template <bool W, bool P>
+static int adj_launch_hot(const AdjArgs &a, uint32_t nhot, hipStream_t s) {
+    const int sg = fit_grid(occupancy_cap((const void *)adj_sweep_hot<W, P>, 1),
+                            (a.ntiles + kHotWaves - 1) / kHotWaves);
+    hipLaunchKernelGGL((adj_sweep_hot<W, P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
     return sg;
 }
 template <typename V, bool W, bool P>
@@ -365,24 +405,26 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         if (iters) *iters = 0;
         return GT_OK;
     }
-    GT_TRY(ensure_schedule(g, *push, n));
-    if (weighted) {
-        GT_TRY(stage_pull_weights<V>(g, weight, *push, wsrc, wdt, ws.wcsr));
-        a.w = ws.wcsr.ptr;
-    }
+    // The transposed system and mu run over the relabelled push orientation:
+    // targets renumbered by how often they are gathered (the relabelling of
+    // the opposite view's PageRank), rows in the same order.  dL/dx, the
+    // degrees and the personalisation are gathered into that order; lambda
+    // and mu come back to the caller's labels for the gradient passes.
+    Relabeled *R = nullptr;
+    GT_TRY(ensure_relabeled(g, reversed ? 0 : 1, &R));
+    Orient &rp = R->pull;
+    const uint32_t *perm = R->perm.as<uint32_t>();
+    const uint32_t *iperm = R->iperm.as<uint32_t>();
+    GT_TRY(ensure_schedule(g, rp, n));
     GT_TRY(prof_prepare(g));
     GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
-    GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, push->nslots)));
+    GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, rp.nslots)));
     GT_TRY(ws.state.reserve(sizeof(SweepState) * 2));
     GT_TRY(ws.lam[0].reserve(sizeof(V) * n));
     GT_TRY(ws.lam[1].reserve(sizeof(V) * n));
     V *lam[2] = {ws.lam[0].as<V>(), ws.lam[1].as<V>()};
-    a.tiles = push->tiles.as<Tile>();
-    a.ntiles = push->ntiles;
-    a.nlong = push->nlong;
+    // Caller-label arguments (the gradient passes).
     a.row_off = push->off.as<uint32_t>();
-    a.packed = push->packed.as<uint32_t>();
-    a.bits = push->bits;
     a.short_row = (uint32_t)g->tune.short_row;
     a.cvec = c;
     a.damping = damping;
@@ -391,12 +433,41 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     a.max_iter = params->max_iter;
     a.partials = ws.partials.as<double>();
     a.long_part = ws.long_part.as<double>();
-    a.long_rows = push->long_rows.as<uint32_t>();
-    a.long_slot = push->long_slot.as<uint32_t>();
     a.st = ws.state.as<SweepState>();
+    // Relabelled arguments (the sweeps and mu).
+    AdjArgs r = a;
+    r.tiles = rp.tiles.as<Tile>();
+    r.ntiles = rp.ntiles;
+    r.nlong = rp.nlong;
+    r.row_off = rp.off.as<uint32_t>();
+    r.packed = rp.packed.as<uint32_t>();
+    r.bits = rp.bits;
+    r.long_rows = rp.long_rows.as<uint32_t>();
+    r.long_slot = rp.long_slot.as<uint32_t>();
+    {
+        const dim3 vg(vgrid(n)), bd(kBlock);
+        GT_TRY(ws.arl[0].reserve(sizeof(double) * n));
+        hipLaunchKernelGGL(adj_gather<double>, vg, bd, 0, s, c, iperm, n, ws.arl[0].as<double>());
+        r.cvec = ws.arl[0].as<double>();
+        if (weighted) {
+            GT_TRY(ws.arl[1].reserve(sizeof(double) * n));
+            hipLaunchKernelGGL(adj_gather<double>, vg, bd, 0, s, a.deg, iperm, n,
+                               ws.arl[1].as<double>());
+            r.deg = ws.arl[1].as<double>();
+            GT_TRY(stage_pull_weights<V>(g, weight, rp, wsrc, wdt, ws.wcsr));
+            r.w = ws.wcsr.ptr;
+        }
+        if (dpers) {
+            GT_TRY(ws.arl[2].reserve(sizeof(double) * n));
+            hipLaunchKernelGGL(adj_gather<double>, vg, bd, 0, s, a.pers, iperm, n,
+                               ws.arl[2].as<double>());
+            r.pers = ws.arl[2].as<double>();
+        }
+        GT_HIP(hipGetLastError());
+    }
     GT_HIP(hipMemsetAsync(a.st, 0, sizeof(SweepState), s));
     {
-        AdjArgs b = a;
+        AdjArgs b = r;
         b.l_next = lam[0];
         if (dpers)
             hipLaunchKernelGGL((adj_init<V, true>), dim3(vgrid(n)), dim3(kBlock), 0, s, b, n);
@@ -405,17 +476,26 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         GT_HIP(hipGetLastError());
     }
     g->prof_kind = GT_SWEEP_ADJOINT;
-    AdjArgs b = a;
+    AdjArgs b = r;
     int sg = 0;
     const int pc = g->tune.sweep_blocks_per_cu;
+    // FP32 sweeps keep the most gathered entries of lambda in LDS.
+    const uint32_t nhot = std::is_same<V, float>::value ? hot_len(g->tune.sweep_hot_kb, n) : 0u;
     auto sweep = [&](uint64_t q) {
         const int par = (int)(q & 1);
         b.l_cur = lam[par];
         b.l_next = lam[par ^ 1];
-        if (weighted && dpers) sg = adj_launch<V, true, true>(b, pc, s);
-        else if (weighted) sg = adj_launch<V, true, false>(b, pc, s);
-        else if (dpers) sg = adj_launch<V, false, true>(b, pc, s);
-        else sg = adj_launch<V, false, false>(b, pc, s);
+        if (nhot) {
+            if (weighted && dpers) sg = adj_launch_hot<true, true>(b, nhot, s);
+            else if (weighted) sg = adj_launch_hot<true, false>(b, nhot, s);
+            else if (dpers) sg = adj_launch_hot<false, true>(b, nhot, s);
+            else sg = adj_launch_hot<false, false>(b, nhot, s);
+        } else {
+            if (weighted && dpers) sg = adj_launch<V, true, true>(b, pc, s);
+            else if (weighted) sg = adj_launch<V, true, false>(b, pc, s);
+            else if (dpers) sg = adj_launch<V, false, true>(b, pc, s);
+            else sg = adj_launch<V, false, false>(b, pc, s);
+        }
     };
     auto fold = [&](uint64_t) {
         if (weighted && dpers) adj_launch_fold<V, true, true>(b, sg, s);
@@ -427,7 +507,10 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched,
                  GT_SWEEP_ADJOINT));
     const uint64_t sweeps = g->h_state->iter;
-    const V *lf = lam[sweeps & 1];
+    const V *lr = lam[sweeps & 1];  // relabelled order
+    V *lf = lam[(sweeps & 1) ^ 1];  // the caller's labels
+    hipLaunchKernelGGL(adj_gather<V>, dim3(vgrid(n)), dim3(kBlock), 0, s, lr, perm, n, lf);
+    GT_HIP(hipGetLastError());
 
     // Edge gradients: mu = M^T lambda (one more engine pass), then one pass
     // over edge ids.
@@ -440,17 +523,19 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         }
         GT_TRY(ensure_edge_list(g));
         GT_TRY(ws.mu.reserve(sizeof(double) * n));
+        GT_TRY(ws.arl[3].reserve(sizeof(double) * n));
         double *mu = ws.mu.as<double>();
-        AdjArgs b = a;
-        const dim3 mg(wave_grid(occupancy_cap((const void *)adj_mu<V, true>, pc), a.ntiles)),
-            lg(fold_grid(a.nlong)), bd(kBlock);
+        double *mr = ws.arl[3].as<double>();
+        const dim3 mg(wave_grid(occupancy_cap((const void *)adj_mu<V, true>, pc), r.ntiles)),
+            lg(fold_grid(r.nlong)), bd(kBlock);
         if (weighted) {
-            hipLaunchKernelGGL((adj_mu<V, true>), mg, bd, 0, s, b, lf, mu);
-            hipLaunchKernelGGL((adj_mu_long<V, true>), lg, bd, 0, s, b, mu);
+            hipLaunchKernelGGL((adj_mu<V, true>), mg, bd, 0, s, r, lr, mr);
+            hipLaunchKernelGGL((adj_mu_long<V, true>), lg, bd, 0, s, r, mr);
         } else {
-            hipLaunchKernelGGL((adj_mu<V, false>), mg, bd, 0, s, b, lf, mu);
-            hipLaunchKernelGGL((adj_mu_long<V, false>), lg, bd, 0, s, b, mu);
+            hipLaunchKernelGGL((adj_mu<V, false>), mg, bd, 0, s, r, lr, mr);
+            hipLaunchKernelGGL((adj_mu_long<V, false>), lg, bd, 0, s, r, mr);
         }
+        hipLaunchKernelGGL(adj_gather<double>, dim3(vgrid(n)), bd, 0, s, mr, perm, n, mu);
         // Rows of the push orientation are sources for a directed view and
         // targets for a reversed one.
         const uint32_t *es = (reversed ? g->etgt : g->esrc).as<uint32_t>();

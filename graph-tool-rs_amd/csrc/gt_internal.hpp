@@ -183,6 +183,7 @@ struct Workspace {
     DevBuf ax, ac;      // adjoint: staged x* and dL/dx (f64)
     DevBuf wcsr;        // adjoint: weights in the push orientation's packed order
     DevBuf mu;          // adjoint: M^T lambda, f64[n]
+    DevBuf arl[4];      // adjoint, relabelled order: dL/dx, degree, personalisation, mu
     DevBuf gw, gp;      // adjoint: staged gradients for host outputs
     DevBuf bx[2];       // batched: [n][64] f32 iterates
     DevBuf bc[2];       // batched: [n][64] f32 contributions

@@ -189,25 +189,16 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     sweep_body<V, V, PrFin<V, WEIGHTED, DPERS, MASK>>(a);
 }
 
-// The FP32 sweep over a relabelled graph with the hot slice: one workgroup of
-// kHotWaves wavefronts per compute unit (the wavefronts per CU of the default
-// sweep), whose LDS holds the wave tiles and the first `nhot` contributions --
-// the most gathered vertices -- copied in at the start of the sweep.  nhot is
-// a multiple of 4, at most kHotMax.
-constexpr int kHotWaves = 12;
-constexpr int kHotBlock = kHotWaves * kWave;
-constexpr uint32_t kHotMax = 25344;  // 99 KB: with the tiles, 163,008 B of 163,840
-static_assert(sizeof(WaveLds<float, kHotWaves>) + 4 * kHotMax <= 163840, "LDS of one CU");
-
+// The FP32 sweep over a relabelled graph with the hot slice (kHotWaves
+// wavefronts, one workgroup per compute unit; the first `nhot` contributions
+// -- the most gathered vertices -- in LDS).
 template <bool WEIGHTED, bool DPERS>
 __global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32_t nhot) {
     if (a.st->done) return;
     __shared__ WaveLds<float, kHotWaves> L;
     __shared__ float hot[kHotMax];
     const float *c = static_cast<const float *>(a.c_cur);
-    for (uint32_t i = 4 * threadIdx.x; i < nhot; i += 4 * kHotBlock)
-        *reinterpret_cast<float4 *>(hot + i) = *reinterpret_cast<const float4 *>(c + i);
-    __syncthreads();
+    fill_hot(c, nhot, hot);
     PrFin<float, WEIGHTED, DPERS, false> fin(a, a.st->p_sink);
     spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
                                       static_cast<const float *>(a.w), a.long_part, a.short_row,
@@ -616,11 +607,9 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     int sg = 0;
     // The hot slice serves FP32 sweeps in relabelled order (the most gathered
     // vertices first); masked views keep the caller's labels.
-    uint32_t nhot = 0;
-    if (std::is_same<V, float>::value && S.iperm && !S.plan.masked && g->tune.sweep_hot_kb > 0) {
-        const uint64_t want = (uint64_t)g->tune.sweep_hot_kb * 256;  // floats
-        nhot = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, kHotMax), g->n) & ~3u;
-    }
+    const uint32_t nhot = std::is_same<V, float>::value && S.iperm && !S.plan.masked
+                              ? hot_len(g->tune.sweep_hot_kb, g->n)
+                              : 0u;
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
         b.x_cur = x[par];

@@ -36,6 +36,7 @@
 // so the sums are too.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -283,6 +284,31 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
         }
         if (PIPE) cur = nxt;
     }
+}
+
+// The hot-slice sweeps (FP32, relabelled graphs): one workgroup of kHotWaves
+// wavefronts per compute unit -- the wavefronts per CU of the default sweep --
+// whose LDS holds the wave tiles and the first `nhot` entries of the gathered
+// vector (the most gathered vertices), copied in at the start of the sweep.
+// nhot is a multiple of 4, at most kHotMax.
+constexpr int kHotWaves = 12;
+constexpr int kHotBlock = kHotWaves * kWave;
+constexpr uint32_t kHotMax = 25344;  // 99 KB: with the tiles, 163,008 B of 163,840
+static_assert(sizeof(WaveLds<float, kHotWaves>) + 4 * kHotMax <= 163840, "LDS of one CU");
+
+// Entries below `nhot` (a multiple of 4; `src` 16-byte aligned) into LDS,
+// then a workgroup barrier.
+__device__ __forceinline__ void fill_hot(const float *src, uint32_t nhot, float *hot) {
+    for (uint32_t i = 4 * threadIdx.x; i < nhot; i += 4 * blockDim.x)
+        *reinterpret_cast<float4 *>(hot + i) = *reinterpret_cast<const float4 *>(src + i);
+    __syncthreads();
+}
+
+// The hot-slice length for a tuning value in KB (<= 0: off) and n vertices.
+inline uint32_t hot_len(int kb, uint32_t n) {
+    if (kb <= 0) return 0;
+    const uint64_t want = (uint64_t)kb * 256;  // floats
+    return (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, kHotMax), n) & ~3u;
 }
 
 // Sum of a split row's partial slots, in chunk order.
