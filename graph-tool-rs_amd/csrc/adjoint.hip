@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
 template <bool WEIGHTED, bool DPERS>
 __global__ __launch_bounds__(kHotBlock) void adj_sweep_hot(const AdjArgs a, uint32_t nhot) {
     if (a.st->done) return;
-    __shared__ WaveLds<float, kHotWaves> L;
+    __shared__ WaveLds<float, kHotWaves, false> L;
     __shared__ float hot[kHotMax];
     const float *lc = static_cast<const float *>(a.l_cur);
     fill_hot(lc, nhot, hot);

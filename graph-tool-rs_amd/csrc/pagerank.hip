@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
 template <bool WEIGHTED, bool DPERS>
 __global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32_t nhot) {
     if (a.st->done) return;
-    __shared__ WaveLds<float, kHotWaves> L;
+    __shared__ WaveLds<float, kHotWaves, false> L;
     __shared__ float hot[kHotMax];
     const float *c = static_cast<const float *>(a.c_cur);
     fill_hot(c, nhot, hot);

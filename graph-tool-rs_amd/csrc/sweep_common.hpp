@@ -46,12 +46,13 @@ namespace gt {
 
 static_assert(kWavesPerBlock * kWave == kBlock, "workgroups are whole wavefronts");
 
-// Per-wavefront tiles of a workgroup of NW wavefronts, and the workgroup's
-// reduction slots (two per wavefront).
-template <typename V, int NW = kWavesPerBlock>
+// Per-wavefront tiles of a workgroup of NW wavefronts, each with its row sums
+// (ROWS; without, the row sums stay in registers and the LDS goes to a hot
+// slice), and the workgroup's reduction slots (two per wavefront).
+template <typename V, int NW = kWavesPerBlock, bool ROWS = true>
 struct WaveLds {
     V vals[NW][kWaveTileNnz];
-    V rowsum[NW][kWaveTileRows];
+    V rowsum[NW][ROWS ? kWaveTileRows : 1];
     double red[2 * NW];
 };
 
@@ -168,13 +169,14 @@ __device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__rest
 // override tile()), a wavefront loads its next tile's packed words and row
 // offsets right after the current tile's gathers, so their latency overlaps
 // the current tile's sums.
-template <typename V, bool WEIGHTED, bool HOT = false, int NW, typename S, typename Fin>
+template <typename V, bool WEIGHTED, bool HOT = false, int NW, bool ROWS, typename S,
+          typename Fin>
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
                                            const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
-                                           WaveLds<V, NW> &L, Fin &fin,
+                                           WaveLds<V, NW, ROWS> &L, Fin &fin,
                                            const S *hot = nullptr, uint32_t nhot = 0) {
     constexpr int K = WavePre::K;
     constexpr bool PIPE = std::is_base_of<TileAll, Fin>::value;
@@ -182,7 +184,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t slot_mask = (1u << bits) - 1u;
     V *vals = L.vals[wid];
-    V *rowsum = L.rowsum[wid];
+    V *rowsum = L.rowsum[wid];  // ROWS only
     const uint32_t nwaves = gridDim.x * NW;
     WavePre cur, nxt;
     uint32_t ti = blockIdx.x * NW + wid;
@@ -247,10 +249,13 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             }
             wave_sync();
             const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
+            // Row q's sum in rowsum[q] (ROWS), else row 64 i + lane's in rs[i].
+            V rs[WavePre::R];
 #pragma unroll
             for (int i = 0; i < WavePre::R; ++i) {
+                rs[i] = V(0);
                 const uint32_t q0 = i * kWave;
-                if (q0 >= rows) break;
+                if (q0 >= rows) continue;
                 const uint32_t q = q0 + lane;
                 uint32_t lo = 0, hi = 0;
                 bool longrow = false;
@@ -260,7 +265,10 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                     if (hi - lo < short_row) {
                         V acc = V(0);
                         for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
-                        rowsum[q] = acc;
+                        if (ROWS)
+                            rowsum[q] = acc;
+                        else
+                            rs[i] = acc;
                     } else {
                         longrow = true;
                     }
@@ -272,13 +280,27 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                     const uint32_t rlo = __shfl(lo, r, kWave), rhi = __shfl(hi, r, kWave);
                     V acc = V(0);
                     for (uint32_t k = rlo + lane; k < rhi; k += kWave) acc += vals[k];
-                    acc = wave_sum(acc);
-                    if (lane == 0) rowsum[q0 + r] = acc;
+                    acc = wave_sum(acc);  // every lane holds the same total
+                    if (ROWS) {
+                        if (lane == 0) rowsum[q0 + r] = acc;
+                    } else if (lane == r) {
+                        rs[i] = acc;
+                    }
                 }
             }
-            wave_sync();
-            for (uint32_t q = lane; q < rows; q += kWave)
-                fin(rb + q, static_cast<double>(rowsum[q]));
+            if (ROWS) {
+                wave_sync();
+                for (uint32_t q = lane; q < rows; q += kWave)
+                    fin(rb + q, static_cast<double>(rowsum[q]));
+            } else {
+                // One copy of the finisher: the sums rotate through rs[0].
+#pragma unroll 1
+                for (uint32_t q = lane; q < rows; q += kWave) {
+                    fin(rb + q, static_cast<double>(rs[0]));
+#pragma unroll
+                    for (int i = 0; i + 1 < WavePre::R; ++i) rs[i] = rs[i + 1];
+                }
+            }
             fin.tile_end(ti);
             wave_sync();  // the LDS tile is reused by this wavefront's next tile
         }
@@ -293,8 +315,8 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 // nhot is a multiple of 4, at most kHotMax.
 constexpr int kHotWaves = 12;
 constexpr int kHotBlock = kHotWaves * kWave;
-constexpr uint32_t kHotMax = 25344;  // 99 KB: with the tiles, 163,008 B of 163,840
-static_assert(sizeof(WaveLds<float, kHotWaves>) + 4 * kHotMax <= 163840, "LDS of one CU");
+constexpr uint32_t kHotMax = 28416;  // 111 KB: with the tiles, 163,008 B of 163,840
+static_assert(sizeof(WaveLds<float, kHotWaves, false>) + 4 * kHotMax <= 163840, "LDS of one CU");
 
 // Entries below `nhot` (a multiple of 4; `src` 16-byte aligned) into LDS,
 // then a workgroup barrier.
