@@ -480,7 +480,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     int sg = 0;
     const int pc = g->tune.sweep_blocks_per_cu;
     // FP32 sweeps keep the most gathered entries of lambda in LDS.
-    const uint32_t nhot = std::is_same<V, float>::value ? hot_len(g->tune.sweep_hot_kb, n) : 0u;
+    const uint32_t nhot =
+        std::is_same<V, float>::value ? hot_len<float>(g->tune.sweep_hot_kb, n) : 0u;
     auto sweep = [&](uint64_t q) {
         const int par = (int)(q & 1);
         b.l_cur = lam[par];

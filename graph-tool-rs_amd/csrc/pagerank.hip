@@ -189,20 +189,20 @@ __global__ __launch_bounds__(kBlock) void pr_sweep(const PrArgs a) {
     sweep_body<V, V, PrFin<V, WEIGHTED, DPERS, MASK>>(a);
 }
 
-// The FP32 sweep over a relabelled graph with the hot slice (kHotWaves
-// wavefronts, one workgroup per compute unit; the first `nhot` contributions
-// -- the most gathered vertices -- in LDS).
-template <bool WEIGHTED, bool DPERS>
+// The sweep over a relabelled graph with the hot slice (kHotWaves wavefronts,
+// one workgroup per compute unit; the first `nhot` contributions -- the most
+// gathered vertices -- in LDS).
+template <typename V, bool WEIGHTED, bool DPERS>
 __global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32_t nhot) {
     if (a.st->done) return;
-    __shared__ WaveLds<float, kHotWaves, false> L;
-    __shared__ float hot[kHotMax];
-    const float *c = static_cast<const float *>(a.c_cur);
+    __shared__ WaveLds<V, kHotWaves, false> L;
+    __shared__ V hot[hot_max<V>()];
+    const V *c = static_cast<const V *>(a.c_cur);
     fill_hot(c, nhot, hot);
-    PrFin<float, WEIGHTED, DPERS, false> fin(a, a.st->p_sink);
-    spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
-                                      static_cast<const float *>(a.w), a.long_part, a.short_row,
-                                      L, fin, static_cast<const float *>(hot), nhot);
+    PrFin<V, WEIGHTED, DPERS, false> fin(a, a.st->p_sink);
+    spmv_waves<V, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
+                                  static_cast<const V *>(a.w), a.long_part, a.short_row, L, fin,
+                                  static_cast<const V *>(hot), nhot);
     double dsum = fin.dsum, ssum = fin.ssum;
     block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
     if (threadIdx.x == 0) {
@@ -331,21 +331,28 @@ int launch_sweep(const PrArgs &a, int per_cu, hipStream_t s) {
     hipLaunchKernelGGL((pr_sweep<V, W, P, M>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
 }
-// The hot-slice FP32 sweep over `nhot` contributions; returns its workgroup
-// count (one per compute unit, fewer for a small graph).
-template <bool W, bool P>
+// The hot-slice sweep over `nhot` contributions; returns its workgroup count
+// (one per compute unit, fewer for a small graph).
+template <typename V, bool W, bool P>
 int launch_sweep_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
-    const int sg = fit_grid(occupancy_cap((const void *)pr_sweep_hot<W, P>, 1),
+    const int sg = fit_grid(occupancy_cap((const void *)pr_sweep_hot<V, W, P>, 1),
                             (a.ntiles + kHotWaves - 1) / kHotWaves);
-    hipLaunchKernelGGL((pr_sweep_hot<W, P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
+    hipLaunchKernelGGL((pr_sweep_hot<V, W, P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
     return sg;
 }
+template <typename V>
 int dispatch_sweep_hot(const PrPlan &p, const PrArgs &a, uint32_t nhot, hipStream_t s) {
-    switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
-        case 0: return launch_sweep_hot<false, false>(a, nhot, s);
-        case 1: return launch_sweep_hot<false, true>(a, nhot, s);
-        case 2: return launch_sweep_hot<true, false>(a, nhot, s);
-        default: return launch_sweep_hot<true, true>(a, nhot, s);
+    if constexpr (sizeof(V) == 8) {
+        // FP64 serves unweighted sweeps only (the caller checks).
+        return p.dpers ? launch_sweep_hot<V, false, true>(a, nhot, s)
+                       : launch_sweep_hot<V, false, false>(a, nhot, s);
+    } else {
+        switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
+            case 0: return launch_sweep_hot<V, false, false>(a, nhot, s);
+            case 1: return launch_sweep_hot<V, false, true>(a, nhot, s);
+            case 2: return launch_sweep_hot<V, true, false>(a, nhot, s);
+            default: return launch_sweep_hot<V, true, true>(a, nhot, s);
+        }
     }
 }
 template <typename V, bool W, bool P, bool M>
@@ -605,18 +612,19 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     uint64_t launched = 0;
     PrArgs b = a;
     int sg = 0;
-    // The hot slice serves FP32 sweeps in relabelled order (the most gathered
-    // vertices first); masked views keep the caller's labels.
-    const uint32_t nhot = std::is_same<V, float>::value && S.iperm && !S.plan.masked
-                              ? hot_len(g->tune.sweep_hot_kb, g->n)
-                              : 0u;
+    // The hot slice serves sweeps in relabelled order (the most gathered
+    // vertices first); masked views keep the caller's labels.  FP64 with
+    // weights is left out: its registers would spill at one workgroup of 12
+    // wavefronts per CU.
+    const bool hot_ok = S.iperm && !S.plan.masked && !(sizeof(V) == 8 && S.plan.weighted);
+    const uint32_t nhot = hot_ok ? hot_len<V>(g->tune.sweep_hot_kb, g->n) : 0u;
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
         b.x_cur = x[par];
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        sg = nhot ? dispatch_sweep_hot(S.plan, b, nhot, s)
+        sg = nhot ? dispatch_sweep_hot<V>(S.plan, b, nhot, s)
                   : dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
     };
     auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };

@@ -308,29 +308,40 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     }
 }
 
-// The hot-slice sweeps (FP32, relabelled graphs): one workgroup of kHotWaves
+// The hot-slice sweeps (relabelled graphs): one workgroup of kHotWaves
 // wavefronts per compute unit -- the wavefronts per CU of the default sweep --
 // whose LDS holds the wave tiles and the first `nhot` entries of the gathered
 // vector (the most gathered vertices), copied in at the start of the sweep.
-// nhot is a multiple of 4, at most kHotMax.
+// nhot is a multiple of 4, at most hot_max<V>().
 constexpr int kHotWaves = 12;
 constexpr int kHotBlock = kHotWaves * kWave;
-constexpr uint32_t kHotMax = 28416;  // 111 KB: with the tiles, 163,008 B of 163,840
+constexpr uint32_t kHotMax = 28416;   // FP32: 111 KB; with the tiles, 163,056 B of 163,840
+constexpr uint32_t kHotMax64 = 8128;  // FP64: 63.5 KB; with the tiles, 163,616 B
 static_assert(sizeof(WaveLds<float, kHotWaves, false>) + 4 * kHotMax <= 163840, "LDS of one CU");
+static_assert(sizeof(WaveLds<double, kHotWaves, false>) + 8 * kHotMax64 <= 163840,
+              "LDS of one CU");
+template <typename V>
+constexpr uint32_t hot_max() {
+    return sizeof(V) == 4 ? kHotMax : kHotMax64;
+}
 
 // Entries below `nhot` (a multiple of 4; `src` 16-byte aligned) into LDS,
-// then a workgroup barrier.
-__device__ __forceinline__ void fill_hot(const float *src, uint32_t nhot, float *hot) {
-    for (uint32_t i = 4 * threadIdx.x; i < nhot; i += 4 * blockDim.x)
+// 16 bytes per thread and step, then a workgroup barrier.
+template <typename T>
+__device__ __forceinline__ void fill_hot(const T *src, uint32_t nhot, T *hot) {
+    constexpr uint32_t per = 16 / sizeof(T);
+    for (uint32_t i = per * threadIdx.x; i < nhot; i += per * blockDim.x)
         *reinterpret_cast<float4 *>(hot + i) = *reinterpret_cast<const float4 *>(src + i);
     __syncthreads();
 }
 
-// The hot-slice length for a tuning value in KB (<= 0: off) and n vertices.
+// The hot-slice length for a tuning value in KB (<= 0: off), n vertices and
+// value type V.
+template <typename V>
 inline uint32_t hot_len(int kb, uint32_t n) {
     if (kb <= 0) return 0;
-    const uint64_t want = (uint64_t)kb * 256;  // floats
-    return (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, kHotMax), n) & ~3u;
+    const uint64_t want = (uint64_t)kb * 1024 / sizeof(V);
+    return (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, hot_max<V>()), n) & ~3u;
 }
 
 // Sum of a split row's partial slots, in chunk order.

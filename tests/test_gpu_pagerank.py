@@ -263,6 +263,34 @@ def test_mixed_hot_slice(gpu, weighted, personalised):
         assert np.abs(got[kb][0] - got[-1][0]).sum() <= 1e-7 * scale
 
 
+@pytest.mark.parametrize("personalised", [False, True])
+def test_fp64_hot_slice(gpu, personalised):
+    """The FP64 drop-in with its hot slice (unweighted, unfiltered graphs: on by
+    default) and without, against the oracle at the drop-in's bar: graphs
+    inside the slice, larger than it, a 1 KB slice, n not a multiple of 4."""
+    r = Rng(37)
+    n = 1003
+    cases = [oracle.gen_rmat(12, 16 << 12, 0.57, 0.19, 0.19, 8) + (1 << 12, 0),
+             oracle.gen_rmat(15, 16 << 15, 0.57, 0.19, 0.19, 9) + (1 << 15, 0),
+             oracle.gen_rmat(15, 16 << 15, 0.57, 0.19, 0.19, 9) + (1 << 15, 1),
+             (np.array([r.below(n) for _ in range(9000)], np.uint32),
+              np.array([r.below(n) for _ in range(9000)], np.uint32), n, 0)]
+    for s, t, n, kb in cases:
+        p = np.random.default_rng(n).uniform(0.0, 2.0, n) if personalised else None
+        if p is not None:
+            p /= p.sum()
+        g = adj(n, s, t)
+        dev = g.device()
+        want, it_o = oracle.Adj(n, s, t).pagerank(pers=p, epsilon=1e-12, max_iter=1000,
+                                                  threads=8)
+        for hot in (kb, -1):
+            dev.set_tuning(sweep_hot_kb=hot)
+            pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
+            got, it = pr(g, 1e-12, 1000, pers=pp)
+            parity(got, it, want, it_o, 1e-12)
+        dev.set_tuning(sweep_hot_kb=0)
+
+
 # ---------------------------------------------------------------- errors
 
 def test_errors(gpu):
