@@ -102,6 +102,26 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     }
 }
 
+// The unweighted sweep over a relabelled graph with the hot slice of y in LDS
+// (kHotWaves wavefronts, one workgroup per compute unit).
+__global__ __launch_bounds__(kHotBlock) void ev_sweep_hot(const EvArgs a, uint32_t nhot) {
+    if (a.st->done) return;
+    __shared__ WaveLds<double, kHotWaves, false> L;
+    __shared__ double hot[kHotMax64];
+    fill_hot(a.y_cur, nhot, hot);
+    const bool first = a.st->launches == 0;
+    EvFin<false> fin(a, a.st->sc_cur, a.st->sc_prev, first);
+    spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur, a.w,
+                                    a.long_part, a.short_row, L, fin,
+                                    static_cast<const double *>(hot), nhot);
+    double sq = fin.sq, dl = fin.dl;
+    block_sum2_waves<kHotWaves>(sq, dl, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = sq;
+        a.partials[gridDim.x + blockIdx.x] = dl;
+    }
+}
+
 // Completes the preceding ev_sweep launch (`sgrid` workgroups): split rows,
 // then in the last workgroup the index-order fold and the decision.
 template <bool WEIGHTED, bool MASK>
@@ -208,6 +228,12 @@ template <bool W, bool M>
 static int ev_launch(const EvArgs &a, int per_cu, hipStream_t s) {
     const int sg = wave_grid(occupancy_cap((const void *)ev_sweep<W, M>, per_cu), a.ntiles);
     hipLaunchKernelGGL((ev_sweep<W, M>), dim3(sg), dim3(kBlock), 0, s, a);
+    return sg;
+}
+static int ev_launch_hot(const EvArgs &a, uint32_t nhot, hipStream_t s) {
+    const int sg = fit_grid(occupancy_cap((const void *)ev_sweep_hot, 1),
+                            (a.ntiles + kHotWaves - 1) / kHotWaves);
+    hipLaunchKernelGGL(ev_sweep_hot, dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
     return sg;
 }
 template <bool W, bool M>
@@ -321,13 +347,18 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         EvArgs b = a;
         int sg = 0;
         const int pc = g->tune.sweep_blocks_per_cu;
+        // Unweighted sweeps in relabelled order keep the most gathered
+        // entries of y in LDS.
+        const uint32_t nhot =
+            iperm && !weighted && !masked ? hot_len<double>(g->tune.sweep_hot_kb, n) : 0u;
         auto sweep = [&](uint64_t k) {
             // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
             const uint64_t L = k + 1;
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            if (weighted && masked) sg = ev_launch<true, true>(b, pc, s);
+            if (nhot) sg = ev_launch_hot(b, nhot, s);
+            else if (weighted && masked) sg = ev_launch<true, true>(b, pc, s);
             else if (weighted) sg = ev_launch<true, false>(b, pc, s);
             else if (masked) sg = ev_launch<false, true>(b, pc, s);
             else sg = ev_launch<false, false>(b, pc, s);

@@ -38,6 +38,8 @@ def test_tuning_round_trip_and_defaults(gpu):
     {"tile_rows": 16, "short_row": 1, "sweep_blocks_per_cu": 1},
     {"tile_rows": 64, "short_row": 1024, "sweep_blocks_per_cu": 8},
     {"tile_rows": 128, "short_row": 16, "sweep_blocks_per_cu": 4},
+    {"sweep_hot_kb": -1},
+    {"sweep_hot_kb": 1, "tile_rows": 64},
 ])
 def test_every_setting_matches_the_oracle(gpu, tune):
     n, s, t = _graph()
