@@ -224,6 +224,26 @@ __global__ __launch_bounds__(kBlock) void pr_refine(const PrArgs a) {
     sweep_body<double, float, RefineFin<WEIGHTED, DPERS, MASK>>(a);
 }
 
+// The refinement over a relabelled, unweighted graph: FP32 tiles and hot
+// slice (the gathered values are FP32), FP64 row sums in registers.
+template <bool DPERS>
+__global__ __launch_bounds__(kHotBlock) void pr_refine_hot(const PrArgs a, uint32_t nhot) {
+    __shared__ WaveLds<float, kHotWaves, false> L;
+    __shared__ float hot[kHotMax];
+    const float *c = static_cast<const float *>(a.c_cur);
+    fill_hot(c, nhot, hot);
+    RefineFin<false, DPERS, false> fin(a, a.st->p_sink);
+    spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
+                                    static_cast<const double *>(a.w), a.long_part, a.short_row,
+                                    L, fin, static_cast<const float *>(hot), nhot);
+    double dsum = fin.dsum, ssum = fin.ssum;
+    block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = dsum;
+        a.partials[gridDim.x + blockIdx.x] = ssum;
+    }
+}
+
 template <bool WEIGHTED, bool DPERS, bool MASK>
 __global__ __launch_bounds__(kBlock) void pr_refine_fold(const PrArgs a, uint32_t sgrid) {
     fold_body<RefineFin<WEIGHTED, DPERS, MASK>>(a, sgrid);
@@ -397,7 +417,24 @@ void launch_refine(const PrArgs &a, int per_cu, hipStream_t s) {
     hipLaunchKernelGGL((pr_refine_fold<W, P, M>), dim3(fold_grid(a.nlong)), dim3(kBlock), 0, s,
                        a, (uint32_t)sg);
 }
-void dispatch_refine(const PrPlan &p, const PrArgs &a, int per_cu, hipStream_t s) {
+template <bool P>
+void launch_refine_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
+    const int sg = fit_grid(occupancy_cap((const void *)pr_refine_hot<P>, 1),
+                            (a.ntiles + kHotWaves - 1) / kHotWaves);
+    hipLaunchKernelGGL((pr_refine_hot<P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
+    hipLaunchKernelGGL((pr_refine_fold<false, P, false>), dim3(fold_grid(a.nlong)), dim3(kBlock),
+                       0, s, a, (uint32_t)sg);
+}
+// `nhot` > 0: the hot-slice refinement (unweighted, unfiltered, relabelled).
+void dispatch_refine(const PrPlan &p, const PrArgs &a, int per_cu, uint32_t nhot,
+                     hipStream_t s) {
+    if (nhot) {
+        if (p.dpers)
+            launch_refine_hot<true>(a, nhot, s);
+        else
+            launch_refine_hot<false>(a, nhot, s);
+        return;
+    }
     switch ((p.weighted ? 4 : 0) | (p.dpers ? 2 : 0) | (p.masked ? 1 : 0)) {
         case 0: launch_refine<false, false, false>(a, per_cu, s); break;
         case 1: launch_refine<false, false, true>(a, per_cu, s); break;
@@ -781,7 +818,10 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         g->prof_kind = GT_SWEEP_PAGERANK_F64;
         GT_TRY(prof_prepare(g));
         GT_TRY(prof_begin(g, 0));
-        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, s);
+        const uint32_t nhot = S.iperm && !S.plan.masked && !S.plan.weighted
+                                  ? hot_len<float>(g->tune.sweep_hot_kb, n)
+                                  : 0u;
+        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, nhot, s);
         GT_TRY(prof_end(g, 0));
         GT_HIP(hipGetLastError());
         GT_TRY(prof_flush(g, 1));

@@ -169,22 +169,26 @@ __device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__rest
 // override tile()), a wavefront loads its next tile's packed words and row
 // offsets right after the current tile's gathers, so their latency overlaps
 // the current tile's sums.
-template <typename V, bool WEIGHTED, bool HOT = false, int NW, bool ROWS, typename S,
-          typename Fin>
+template <typename V, bool WEIGHTED, bool HOT = false, typename TT, int NW, bool ROWS,
+          typename S, typename Fin>
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
                                            const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
-                                           WaveLds<V, NW, ROWS> &L, Fin &fin,
+                                           WaveLds<TT, NW, ROWS> &L, Fin &fin,
                                            const S *hot = nullptr, uint32_t nhot = 0) {
+    // The LDS tile holds TT: V, or -- unweighted, row sums in registers -- the
+    // narrower gathered type S, widened exactly when a row is summed.
+    static_assert(std::is_same<TT, V>::value || (std::is_same<TT, S>::value && !WEIGHTED && !ROWS),
+                  "a narrower tile holds unweighted gathered values only");
     constexpr int K = WavePre::K;
     constexpr bool PIPE = std::is_base_of<TileAll, Fin>::value;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t slot_mask = (1u << bits) - 1u;
-    V *vals = L.vals[wid];
-    V *rowsum = L.rowsum[wid];  // ROWS only
+    TT *vals = L.vals[wid];
+    TT *rowsum = L.rowsum[wid];  // ROWS only (TT == V)
     const uint32_t nwaves = gridDim.x * NW;
     WavePre cur, nxt;
     uint32_t ti = blockIdx.x * NW + wid;
@@ -245,7 +249,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const uint32_t j = lane + k * kWave;
-                if (j < cnt) vals[bits ? (cur.p[k] & slot_mask) : j] = v[k];
+                if (j < cnt) vals[bits ? (cur.p[k] & slot_mask) : j] = static_cast<TT>(v[k]);
             }
             wave_sync();
             const uint32_t rb = t.row_begin, rows = t.row_end - t.row_begin;
@@ -264,7 +268,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                     hi = cur.hi[i] - nb;
                     if (hi - lo < short_row) {
                         V acc = V(0);
-                        for (uint32_t k = lo; k < hi; ++k) acc += vals[k];
+                        for (uint32_t k = lo; k < hi; ++k) acc += static_cast<V>(vals[k]);
                         if (ROWS)
                             rowsum[q] = acc;
                         else
@@ -279,7 +283,8 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                     lmask &= lmask - 1;
                     const uint32_t rlo = __shfl(lo, r, kWave), rhi = __shfl(hi, r, kWave);
                     V acc = V(0);
-                    for (uint32_t k = rlo + lane; k < rhi; k += kWave) acc += vals[k];
+                    for (uint32_t k = rlo + lane; k < rhi; k += kWave)
+                        acc += static_cast<V>(vals[k]);
                     acc = wave_sum(acc);  // every lane holds the same total
                     if (ROWS) {
                         if (lane == 0) rowsum[q0 + r] = acc;
