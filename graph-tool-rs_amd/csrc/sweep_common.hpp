@@ -352,10 +352,18 @@ inline uint32_t hot_len(int kb, uint32_t n) {
 // Sum of a split row's partial slots, in chunk order.
 __device__ __forceinline__ double long_row_sum(const double *long_part, const uint32_t *long_slot,
                                                uint32_t k) {
-    // Slot order, eight loads in flight: a hub has hundreds of slots.
+    // Slot order, 32 loads in flight, then eight: a hub has hundreds of slots
+    // and sets the fold's critical path (the sum's order is the slot order).
     const uint32_t e = long_slot[k + 1];
     uint32_t q = long_slot[k];
     double acc = 0.0;
+    for (; q + 32 <= e; q += 32) {
+        double v[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) v[i] = long_part[q + i];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) acc += v[i];
+    }
     for (; q + 8 <= e; q += 8) {
         double v[8];
 #pragma unroll
