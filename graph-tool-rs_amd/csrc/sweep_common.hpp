@@ -1,4 +1,6 @@
-// The pull-SpMV engine shared by the PageRank, eigenvector and adjoint sweeps.
+// The pull-SpMV engine shared by the PageRank sweeps (and the FP32 form's FP64
+// refinement), the eigenvector sweep, the adjoint's transposed sweep and mu
+// pass, and the uneven batch's sweeps.
 //
 // Work is cut into wave tiles of <= kWaveTileNnz in-entries and <= kWaveTileRows
 // rows; a row longer than kWaveTileNnz is cut into kWaveTileNnz-entry chunks
@@ -16,7 +18,9 @@
 // w + W, ...) with no workgroup barrier:
 //   phase 1  kWaveTileNnz / 64 packed words per lane in flight (coalesced),
 //            as many gathers, each value times the packed-order weight when
-//            weighted, scattered into the wavefront's LDS tile;
+//            weighted, scattered into the wavefront's LDS tile; the next
+//            tile's packed words and row offsets are loaded while the
+//            gathers are in flight (see spmv_waves);
 //   phase 2  one lane per row when the row has fewer than `short_row` entries
 //            (a tuning parameter), the whole wavefront (shuffle tree) for
 //            longer rows;
@@ -48,7 +52,9 @@ static_assert(kWavesPerBlock * kWave == kBlock, "workgroups are whole wavefronts
 
 // Per-wavefront tiles of a workgroup of NW wavefronts, each with its row sums
 // (ROWS; without, the row sums stay in registers and the LDS goes to a hot
-// slice), and the workgroup's reduction slots (two per wavefront).
+// slice), and the workgroup's reduction slots (two per wavefront).  V is the
+// tile's element type: the sums' type, or, for an unweighted sweep with
+// register row sums, the narrower gathered type.
 template <typename V, int NW = kWavesPerBlock, bool ROWS = true>
 struct WaveLds {
     V vals[NW][kWaveTileNnz];
@@ -163,8 +169,9 @@ __device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__rest
     }
 }
 
-// V is the value type of the sums (and of the LDS tile); the gathered source
-// vector may be narrower (S), each value widened as it arrives.  With a
+// V is the value type of the sums; the gathered source vector may be narrower
+// (S), each value widened as it is summed, and the LDS tile holds TT (V, or S
+// for an unweighted sweep with register row sums).  With a
 // finisher that takes every tile (one deriving from TileAll, which must not
 // override tile()), a wavefront loads its next tile's packed words and row
 // offsets right after the current tile's gathers, so their latency overlaps
