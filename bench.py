@@ -783,10 +783,16 @@ def main() -> int:
             dtc = time.perf_counter() - t0
             if dtc >= 10.0 or reps >= 64:
                 break
+        # The same path on one host thread: a solve capped at 6 sweeps (its
+        # setup is spread over fewer sweeps, so this rate is slightly low).
+        t0 = time.perf_counter()
+        _, it1 = o.pagerank(epsilon=args.epsilon, max_iter=6, grain=4096, threads=1)
+        dt1 = time.perf_counter() - t0
         del o
         cpu_gteps = m * it_o * reps / dtc / 1e9
+        cpu1 = m * it1 / dt1 / 1e9
         log(f"[bench] oracle pagerank: {reps} solves in {dtc:.1f}s, {it_o} sweeps each, "
-            f"{cpu_gteps:.3f} GTEPS on {threads} threads")
+            f"{cpu_gteps:.3f} GTEPS on {threads} threads; {cpu1:.3f} GTEPS on 1 thread")
         nz = want != 0
 
         def errs(x):
@@ -798,6 +804,8 @@ def main() -> int:
             "sample": f"{reps} cold-start pagerank_with solves (oracle C restatement of the "
                       f"reference's rayon path, Plan grain 4096, {it_o} sweeps each) on the same "
                       f"R-MAT-{args.scale} graph, {dtc:.1f}s",
+            "one_thread": {"value": round(cpu1, 4), "unit": "GTEPS", "cores": 1,
+                           "sample": f"one solve capped at {it1} sweeps, {dt1:.1f}s"},
         }
         result["accuracy_vs_fp64_oracle"] = {
             "mixed": errs(x32), "fp64_dropin": errs(x64),
