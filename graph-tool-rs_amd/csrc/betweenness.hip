@@ -358,19 +358,22 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a, uint32_t L) {
 // Sum of the partial rows [p0, p1) for this thread's lane, in a fixed order:
 // wavefront q of the workgroup sums pieces p0 + q, p0 + q + 4, ..., then the
 // four wavefront sums are added in wavefront order.  Block-uniform call.
+// Sixteen pieces are loaded at a time: a hub of ~200 pieces sets the fix-up
+// kernel's critical path.
 __device__ __forceinline__ double sum_pieces(const double *part, uint32_t p0, uint32_t p1,
                                              double (*red)[kWave]) {
+    constexpr int B = 16;
     const int lane = threadIdx.x & (kWave - 1);
     const int q = threadIdx.x / kWave;
     constexpr uint32_t S = kWavesPerBlock;
     double acc = 0.0;
     uint32_t p = p0 + q;
-    for (; p + (kBcGroup - 1) * S < p1; p += kBcGroup * S) {
-        double v[kBcGroup];
+    for (; p + (B - 1) * S < p1; p += B * S) {
+        double v[B];
 #pragma unroll
-        for (int i = 0; i < kBcGroup; ++i) v[i] = part[(size_t)(p + i * S) * kWave + lane];
+        for (int i = 0; i < B; ++i) v[i] = part[(size_t)(p + i * S) * kWave + lane];
 #pragma unroll
-        for (int i = 0; i < kBcGroup; ++i) acc += v[i];
+        for (int i = 0; i < B; ++i) acc += v[i];
     }
     for (; p < p1; p += S) acc += part[(size_t)p * kWave + lane];
     red[q][lane] = acc;
