@@ -562,8 +562,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path "
+              "(a row-sharded design exists on paper only)", file=sys.stderr)
         return 2
 
     import torch

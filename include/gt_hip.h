@@ -49,7 +49,7 @@ typedef enum gt_status {
     GT_ERR_NO_SUCH_VERTEX = 5,/* edge endpoint >= vertex count (AdjList::add_edge) */
     GT_ERR_HIP = 6,           /* a HIP runtime call failed */
     GT_ERR_OOM = 7,           /* device allocation failed */
-    GT_ERR_UNSUPPORTED = 8    /* valid request this build does not serve */
+    GT_ERR_UNSUPPORTED = 8    /* valid request the library does not serve */
 } gt_status;
 
 /* Where a buffer argument lives. */
