@@ -337,13 +337,27 @@ constexpr uint32_t hot_max() {
     return sizeof(V) == 4 ? kHotMax : kHotMax64;
 }
 
-// Entries below `nhot` (a multiple of 4; `src` 16-byte aligned) into LDS,
-// 16 bytes per thread and step, then a workgroup barrier.
+// Entries below `nhot` (a multiple of 4, at most hot_max<T>(); `src` 16-byte
+// aligned) into LDS by a kHotBlock-thread workgroup, then a workgroup
+// barrier.  Every thread issues all its 16-byte loads before the first LDS
+// write, so the fill costs one memory round trip, not one per step.
 template <typename T>
 __device__ __forceinline__ void fill_hot(const T *src, uint32_t nhot, T *hot) {
     constexpr uint32_t per = 16 / sizeof(T);
-    for (uint32_t i = per * threadIdx.x; i < nhot; i += per * blockDim.x)
-        *reinterpret_cast<float4 *>(hot + i) = *reinterpret_cast<const float4 *>(src + i);
+    constexpr int kSteps = (int)((hot_max<T>() / per + kHotBlock - 1) / kHotBlock);
+    // No branches: a step past nhot reads entries 0..3 again and writes them
+    // back unchanged, so all loads are in flight before the first wait.
+    float4 v[kSteps];
+#pragma unroll
+    for (int r = 0; r < kSteps; ++r) {
+        const uint32_t i = per * (threadIdx.x + (uint32_t)r * kHotBlock);
+        v[r] = *reinterpret_cast<const float4 *>(src + (i < nhot ? i : 0u));
+    }
+#pragma unroll
+    for (int r = 0; r < kSteps; ++r) {
+        const uint32_t i = per * (threadIdx.x + (uint32_t)r * kHotBlock);
+        *reinterpret_cast<float4 *>(hot + (i < nhot ? i : 0u)) = v[r];
+    }
     __syncthreads();
 }
 
