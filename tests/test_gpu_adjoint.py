@@ -64,6 +64,41 @@ def test_backward_on_reversed_view_matches_oracle(gpu):
     assert abs(gd - gd_o) <= 1e-9 * max(1.0, abs(gd_o))
 
 
+@pytest.mark.parametrize("reversed_", [False, True])
+def test_f32_hot_slice_on_and_off(gpu, reversed_):
+    """The FP32 transposed sweep runs over the relabelled orientation and keeps
+    the most gathered entries of lambda in LDS (the default).  With the slice
+    off the gradients agree with it, and both with the oracle adjoint at the
+    FP32 bar: weighted and personalised, on a graph inside the slice (scale
+    10) and one larger than it (scale 16)."""
+    for scale, seed in ((10, 21), (16, 22)):
+        n, s, t, w, c = _problem(scale, seed)
+        p = np.random.default_rng(seed).uniform(0.0, 2.0, n)
+        p /= p.sum()
+        g = gt_hip.AdjList.with_vertices(n)
+        g.add_edges(s, t)
+        o = oracle.Adj(n, s, t)
+        x, _ = o.pagerank(weight=w, pers=p, epsilon=1e-15, max_iter=10_000, reversed=reversed_)
+        gw_o, gp_o, gd_o, _, _ = o.pagerank_grad(x, c, weight=w, pers=p, epsilon=1e-13,
+                                                 reversed=reversed_)
+        wm = gt_hip.DenseProp.from_vec(g.graph_id, w)
+        pm = gt_hip.DenseProp.from_vec(g.graph_id, p)
+        view = g.reversed() if reversed_ else g
+        got = {}
+        for hot in (0, -1):
+            g.device().set_tuning(sweep_hot_kb=hot)
+            got[hot] = gt_hip.pagerank_grad(view, wm, pm, 0.85, _params(n, 1e-13), x, c,
+                                            precision="f32")
+        g.device().set_tuning(sweep_hot_kb=0)
+        sw, sp = np.abs(gw_o).max(), np.abs(gp_o).max()
+        for gw, gp, gd, _ in got.values():
+            assert np.abs(gw - gw_o).max() <= 2e-4 * sw, (scale, reversed_)
+            assert np.abs(gp - gp_o).max() <= 2e-4 * sp
+            assert abs(gd - gd_o) <= 2e-4 * max(1.0, abs(gd_o))
+        # Only the order of the FP64 partial folds differs between the two.
+        assert np.abs(got[0][0] - got[-1][0]).max() <= 1e-5 * sw
+
+
 def test_backward_matches_finite_differences_on_gpu(gpu):
     n, s, t, w, c = _problem(10, 11)
     g = gt_hip.AdjList.with_vertices(n)
