@@ -102,8 +102,9 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     }
 }
 
-// The unweighted sweep over a relabelled graph with the hot slice of y in LDS
+// The sweep over a relabelled graph with the hot slice of y in LDS
 // (kHotWaves wavefronts, one workgroup per compute unit).
+template <bool WEIGHTED>
 __global__ __launch_bounds__(kHotBlock) void ev_sweep_hot(const EvArgs a, uint32_t nhot) {
     if (a.st->done) return;
     __shared__ WaveLds<double, kHotWaves, false> L;
@@ -111,9 +112,9 @@ __global__ __launch_bounds__(kHotBlock) void ev_sweep_hot(const EvArgs a, uint32
     fill_hot(a.y_cur, nhot, hot);
     const bool first = a.st->launches == 0;
     EvFin<false> fin(a, a.st->sc_cur, a.st->sc_prev, first);
-    spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur, a.w,
-                                    a.long_part, a.short_row, L, fin,
-                                    static_cast<const double *>(hot), nhot);
+    spmv_waves<double, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur,
+                                       a.w, a.long_part, a.short_row, L, fin,
+                                       static_cast<const double *>(hot), nhot);
     double sq = fin.sq, dl = fin.dl;
     block_sum2_waves<kHotWaves>(sq, dl, L.red);
     if (threadIdx.x == 0) {
@@ -230,10 +231,11 @@ static int ev_launch(const EvArgs &a, int per_cu, hipStream_t s) {
     hipLaunchKernelGGL((ev_sweep<W, M>), dim3(sg), dim3(kBlock), 0, s, a);
     return sg;
 }
+template <bool W>
 static int ev_launch_hot(const EvArgs &a, uint32_t nhot, hipStream_t s) {
-    const int sg = fit_grid(occupancy_cap((const void *)ev_sweep_hot, 1),
+    const int sg = fit_grid(occupancy_cap((const void *)ev_sweep_hot<W>, 1),
                             (a.ntiles + kHotWaves - 1) / kHotWaves);
-    hipLaunchKernelGGL(ev_sweep_hot, dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
+    hipLaunchKernelGGL((ev_sweep_hot<W>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
     return sg;
 }
 template <bool W, bool M>
@@ -347,17 +349,17 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         EvArgs b = a;
         int sg = 0;
         const int pc = g->tune.sweep_blocks_per_cu;
-        // Unweighted sweeps in relabelled order keep the most gathered
-        // entries of y in LDS.
-        const uint32_t nhot =
-            iperm && !weighted && !masked ? hot_len<double>(g->tune.sweep_hot_kb, n) : 0u;
+        // Sweeps in relabelled order keep the most gathered entries of y in
+        // LDS.
+        const uint32_t nhot = iperm && !masked ? hot_len<double>(g->tune.sweep_hot_kb, n) : 0u;
         auto sweep = [&](uint64_t k) {
             // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
             const uint64_t L = k + 1;
             b.y_cur = y[(L - 1) % 3];
             b.y_prev = y[(L + 1) % 3];
             b.y_next = y[L % 3];
-            if (nhot) sg = ev_launch_hot(b, nhot, s);
+            if (nhot)
+                sg = weighted ? ev_launch_hot<true>(b, nhot, s) : ev_launch_hot<false>(b, nhot, s);
             else if (weighted && masked) sg = ev_launch<true, true>(b, pc, s);
             else if (weighted) sg = ev_launch<true, false>(b, pc, s);
             else if (masked) sg = ev_launch<false, true>(b, pc, s);

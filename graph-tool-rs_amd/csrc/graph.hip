@@ -81,7 +81,7 @@ gt_tuning resolve_tuning(const gt_tuning &t) {
     r.mm_tile_nnz = t.mm_tile_nnz > 0 ? std::max(256, std::min(t.mm_tile_nnz, 8192))
                                       : (int)kMmTileNnz;
     r.mm_blocks_per_cu = t.mm_blocks_per_cu > 0 ? std::min(t.mm_blocks_per_cu, 8) : 8;
-    r.sweep_hot_kb = t.sweep_hot_kb < 0 ? -1 : t.sweep_hot_kb > 0 ? std::min(t.sweep_hot_kb, 111) : 111;
+    r.sweep_hot_kb = t.sweep_hot_kb < 0 ? -1 : t.sweep_hot_kb > 0 ? std::min(t.sweep_hot_kb, 128) : 128;
     return r;
 }
 

@@ -362,17 +362,11 @@ int launch_sweep_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
 }
 template <typename V>
 int dispatch_sweep_hot(const PrPlan &p, const PrArgs &a, uint32_t nhot, hipStream_t s) {
-    if constexpr (sizeof(V) == 8) {
-        // FP64 serves unweighted sweeps only (the caller checks).
-        return p.dpers ? launch_sweep_hot<V, false, true>(a, nhot, s)
-                       : launch_sweep_hot<V, false, false>(a, nhot, s);
-    } else {
-        switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
-            case 0: return launch_sweep_hot<V, false, false>(a, nhot, s);
-            case 1: return launch_sweep_hot<V, false, true>(a, nhot, s);
-            case 2: return launch_sweep_hot<V, true, false>(a, nhot, s);
-            default: return launch_sweep_hot<V, true, true>(a, nhot, s);
-        }
+    switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
+        case 0: return launch_sweep_hot<V, false, false>(a, nhot, s);
+        case 1: return launch_sweep_hot<V, false, true>(a, nhot, s);
+        case 2: return launch_sweep_hot<V, true, false>(a, nhot, s);
+        default: return launch_sweep_hot<V, true, true>(a, nhot, s);
     }
 }
 template <typename V, bool W, bool P, bool M>
@@ -650,11 +644,9 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     PrArgs b = a;
     int sg = 0;
     // The hot slice serves sweeps in relabelled order (the most gathered
-    // vertices first); masked views keep the caller's labels.  FP64 with
-    // weights is left out: its registers would spill at one workgroup of 12
-    // wavefronts per CU.
-    const bool hot_ok = S.iperm && !S.plan.masked && !(sizeof(V) == 8 && S.plan.weighted);
-    const uint32_t nhot = hot_ok ? hot_len<V>(g->tune.sweep_hot_kb, g->n) : 0u;
+    // vertices first); masked views keep the caller's labels.
+    const uint32_t nhot =
+        S.iperm && !S.plan.masked ? hot_len<V>(g->tune.sweep_hot_kb, g->n) : 0u;
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
         b.x_cur = x[par];

@@ -321,14 +321,15 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 }
 
 // The hot-slice sweeps (relabelled graphs): one workgroup of kHotWaves
-// wavefronts per compute unit -- the wavefronts per CU of the default sweep --
-// whose LDS holds the wave tiles and the first `nhot` entries of the gathered
-// vector (the most gathered vertices), copied in at the start of the sweep.
-// nhot is a multiple of 4, at most hot_max<V>().
-constexpr int kHotWaves = 12;
+// wavefronts per compute unit (two per SIMD: with 12 or 16 per CU and the
+// smaller slice that leaves, the R-MAT 22 sweeps were slower) whose LDS
+// holds the wave tiles and the first `nhot` entries of the gathered vector
+// (the most gathered vertices), copied in at the start of the sweep.  nhot
+// is a multiple of 4, at most hot_max<V>().
+constexpr int kHotWaves = 8;
 constexpr int kHotBlock = kHotWaves * kWave;
-constexpr uint32_t kHotMax = 28416;   // FP32: 111 KB; with the tiles, 163,056 B of 163,840
-constexpr uint32_t kHotMax64 = 8128;  // FP64: 63.5 KB; with the tiles, 163,616 B
+constexpr uint32_t kHotMax = 32704;    // FP32: 127.75 KB; with the tiles, 163,744 B of 163,840
+constexpr uint32_t kHotMax64 = 12224;  // FP64: 95.5 KB; with the tiles, 163,520 B
 static_assert(sizeof(WaveLds<float, kHotWaves, false>) + 4 * kHotMax <= 163840, "LDS of one CU");
 static_assert(sizeof(WaveLds<double, kHotWaves, false>) + 8 * kHotMax64 <= 163840,
               "LDS of one CU");

@@ -35,7 +35,7 @@ SWEEP_SPACE = {
     "sweep_blocks_per_cu": (2, 3, 4, 6, 8),
     "tile_rows": (64, 128, 256),
     "short_row": (16, 32, 64),
-    "sweep_hot_kb": (111, 64, -1),
+    "sweep_hot_kb": (128, 64, -1),
 }
 BATCHED_SPACE = {
     "mm_tile_nnz": (512, 1024, 2048, 4096),

@@ -376,7 +376,7 @@ typedef struct gt_tuning {
     int32_t mm_blocks_per_cu;    /* batched SpMM: resident workgroups per CU: 1..8 (8) */
     int32_t sweep_hot_kb;        /* FP32 PageRank sweeps on a relabelled graph: KB of the most
                                     gathered contributions held in LDS, one workgroup per CU;
-                                    -1 turns it off: -1, 1..111 (111) */
+                                    -1 turns it off: -1, 1..128 (128) */
     int32_t reserved[2];
 } gt_tuning;
 

@@ -23,12 +23,12 @@ def test_tuning_round_trip_and_defaults(gpu):
     dev = gt_hip.DeviceGraph.from_edges(n, s, t)
     d = dev.tuning()
     assert d == {"sweep_blocks_per_cu": 3, "tile_rows": 256, "short_row": 32,
-                 "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 111}
+                 "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 128}
     got = dev.set_tuning(tile_rows=1000, short_row=8, sweep_blocks_per_cu=99, mm_tile_nnz=10)
     assert got["tile_rows"] == 256 and got["short_row"] == 8
     assert got["sweep_blocks_per_cu"] == 8 and got["mm_tile_nnz"] == 256
     assert dev.set_tuning(tile_rows=0, short_row=0)["tile_rows"] == 256
-    assert dev.set_tuning(sweep_hot_kb=500)["sweep_hot_kb"] == 111
+    assert dev.set_tuning(sweep_hot_kb=500)["sweep_hot_kb"] == 128
     assert dev.set_tuning(sweep_hot_kb=-7)["sweep_hot_kb"] == -1
     with pytest.raises(ValueError):
         dev.set_tuning(bogus=1)
