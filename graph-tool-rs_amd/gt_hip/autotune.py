@@ -34,7 +34,7 @@ DEFAULT_CACHE = Path(os.environ.get("GT_HIP_TUNING_CACHE",
 SWEEP_SPACE = {
     "sweep_blocks_per_cu": (2, 3, 4, 6, 8),
     "tile_rows": (64, 128, 256),
-    "short_row": (16, 32, 64),
+    "short_row": (32, 64, 256),
     "sweep_hot_kb": (128, 64, -1),
 }
 BATCHED_SPACE = {

@@ -371,7 +371,7 @@ gt_status gt_graph_from_doc(const gt_doc *d, uint64_t graph_id, gt_graph **out);
 typedef struct gt_tuning {
     int32_t sweep_blocks_per_cu; /* resident workgroups per CU, pull sweeps: 1..8 (3) */
     int32_t tile_rows;           /* rows per wave tile: 16..256 (256) */
-    int32_t short_row;           /* rows with fewer entries are summed by one lane: 1..1024 (32) */
+    int32_t short_row;           /* rows with fewer entries are summed by one lane: 1..1024 (64) */
     int32_t mm_tile_nnz;         /* batched SpMM: entries per tile, 256..8192 (1024) */
     int32_t mm_blocks_per_cu;    /* batched SpMM: resident workgroups per CU: 1..8 (8) */
     int32_t sweep_hot_kb;        /* FP32 PageRank sweeps on a relabelled graph: KB of the most

@@ -22,7 +22,7 @@ def test_tuning_round_trip_and_defaults(gpu):
     n, s, t = _graph(8)
     dev = gt_hip.DeviceGraph.from_edges(n, s, t)
     d = dev.tuning()
-    assert d == {"sweep_blocks_per_cu": 3, "tile_rows": 256, "short_row": 32,
+    assert d == {"sweep_blocks_per_cu": 3, "tile_rows": 256, "short_row": 64,
                  "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 128}
     got = dev.set_tuning(tile_rows=1000, short_row=8, sweep_blocks_per_cu=99, mm_tile_nnz=10)
     assert got["tile_rows"] == 256 and got["short_row"] == 8
