@@ -31,7 +31,9 @@
 // The finisher `fin` is called as fin(row, sum) for every whole row, and
 // brackets every tile: fin.tile(t) (wave-uniform) before it -- returning false
 // skips the tile, loads included -- and fin.tile_end(t) after it.  TileAll is
-// the pair for finishers that need neither.
+// the pair for finishers that need neither.  A finisher that skips tiles may
+// also be given a list of tiles to walk (`tlist`, ntiles entries) in place of
+// tiles 0 .. ntiles - 1.
 //
 // With HOT, the first `nhot` entries of the source vector are also in LDS
 // (`hot`, filled by the caller before the walk): a gather below that label
@@ -146,6 +148,11 @@ __device__ __forceinline__ Tile wave_tile(const Tile *__restrict__ tiles, uint32
     return t;
 }
 
+// Entry i of a device list, as a scalar load.
+__device__ __forceinline__ uint32_t list_entry(const uint32_t *list, uint32_t i) {
+    return ((const __attribute__((address_space(4))) uint32_t *)list)[i];
+}
+
 // Loads of tile `t` (its descriptor already read).
 __device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, int lane,
@@ -184,7 +191,8 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                                            const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
                                            WaveLds<TT, NW, ROWS> &L, Fin &fin,
-                                           const S *hot = nullptr, uint32_t nhot = 0) {
+                                           const S *hot = nullptr, uint32_t nhot = 0,
+                                           const uint32_t *tlist = nullptr) {
     // The LDS tile holds TT: V, or -- unweighted, row sums in registers -- the
     // narrower gathered type S, widened exactly when a row is summed.
     static_assert(std::is_same<TT, V>::value || (std::is_same<TT, S>::value && !WEIGHTED && !ROWS),
@@ -201,9 +209,12 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     uint32_t ti = blockIdx.x * NW + wid;
     if (PIPE && ti < ntiles) wave_fetch(wave_tile(tiles, ti), off, packed, lane, cur);
     for (; ti < ntiles; ti += nwaves) {
+        // The tile: the list's ti-th entry when walking a list (finishers
+        // that skip tiles only), else ti.
+        const uint32_t tid = !PIPE && tlist ? list_entry(tlist, ti) : ti;
         if (!PIPE) {
-            if (!fin.tile(ti)) continue;
-            wave_fetch(wave_tile(tiles, ti), off, packed, lane, cur);
+            if (!fin.tile(tid)) continue;
+            wave_fetch(wave_tile(tiles, tid), off, packed, lane, cur);
         }
         // The next tile's descriptor (the last tile reads its own again).
         const Tile tnext = wave_tile(tiles, ti + nwaves < ntiles ? ti + nwaves : ntiles - 1);
@@ -251,7 +262,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             for (int k = 0; k < K; ++k) acc += v[k];
             const double tot = wave_sum(static_cast<double>(acc));
             if (lane == 0) long_part[t.row_begin] = tot;
-            fin.tile_end(ti);
+            fin.tile_end(tid);
         } else {
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -313,7 +324,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                     for (int i = 0; i + 1 < WavePre::R; ++i) rs[i] = rs[i + 1];
                 }
             }
-            fin.tile_end(ti);
+            fin.tile_end(tid);
             wave_sync();  // the LDS tile is reused by this wavefront's next tile
         }
         if (PIPE) cur = nxt;

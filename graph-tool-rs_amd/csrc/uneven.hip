@@ -23,6 +23,12 @@
 // fold keeps on the device), so the host issues one launch per
 // kUnevenCapture sweeps.  No memset node is recorded: every buffer a replay
 // needs cleared is cleared by a kernel.
+//
+// A sweep walks a list of the tiles of the graphs still iterating, which the
+// fold before it writes (one atomic add per graph for its place in the list),
+// so converged graphs cost nothing.  Each tile is still summed by one
+// wavefront in a fixed order and the fold reads the partials in tile order:
+// the list's order changes no result.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -76,6 +82,10 @@ struct UArgs {
     double *tpart;              // [2 * ntiles]
     UState st;
     uint32_t *active;           // graphs still iterating
+    uint32_t *list;             // [ntiles]: tiles of the graphs still iterating
+    uint32_t *list_count;       // [2]: list lengths, by sweep parity
+    uint32_t *list_n;           // this sweep's list length (an entry of list_count)
+    uint32_t *list_n_next;      // the next sweep's (the other entry)
 };
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
@@ -168,10 +178,24 @@ struct UAdjFin {
 template <typename Fin, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void u_sweep(const UArgs a) {
     if (*a.active == 0) return;
+    // The listed tiles; the fold after this sweep counts the next list from 0.
+    const uint32_t nt = uniform_u32(*a.list_n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.list_n_next = 0u;
     __shared__ WaveLds<float> L;
     Fin fin(a);
-    spmv_waves<float, WEIGHTED>(a.tiles, a.ntiles, a.off, a.packed, a.bits, a.src, a.w,
-                                a.long_part, a.short_row, L, fin);
+    spmv_waves<float, WEIGHTED>(a.tiles, nt, a.off, a.packed, a.bits, a.src, a.w, a.long_part,
+                                a.short_row, L, fin, static_cast<const float *>(nullptr), 0u,
+                                a.list);
+}
+
+// Before the first sweep of a call: every tile listed.
+__global__ __launch_bounds__(kBlock) void u_list_init(UArgs a) {
+    const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
+    for (uint32_t i = i0; i < a.ntiles; i += gridDim.x * kBlock) a.list[i] = i;
+    if (i0 == 0) {
+        a.list_count[0] = a.ntiles;
+        a.list_count[1] = 0u;
+    }
 }
 
 // One wavefront per graph: the graph's tile partials in tile order, its split
@@ -195,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void u_fold(const UArgs a) {
         fin(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
     d = wave_sum(d + fin.dsum);
     s = wave_sum(s + fin.second());
+    bool more = false;
     if (lane == 0) {
         const uint64_t it = a.st.iter[g] + 1;
         const double thr = ADJ ? fmax(a.eps, a.eps_rel) * (a.st.scale[g] > 0.0 ? a.st.scale[g] : 1.0)
@@ -205,7 +230,17 @@ __global__ __launch_bounds__(kBlock) void u_fold(const UArgs a) {
         if (!(d >= thr) || (a.max_iter > 0 && it >= a.max_iter)) {
             a.st.done[g] = 1u;
             atomicSub(a.active, 1u);
+        } else {
+            more = true;
         }
+    }
+    // A graph still iterating lists its tiles for the next sweep.
+    if (__ballot(more)) {
+        const uint32_t t0 = a.tb[g], cnt = a.tb[g + 1] - t0;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.list_n_next, cnt);
+        base = __shfl(base, 0, kWave);
+        for (uint32_t i = lane; i < cnt; i += kWave) a.list[base + i] = t0 + i;
     }
 }
 
@@ -408,7 +443,9 @@ unsigned ugrid(uint64_t work) {
 }
 
 // Workspace slots of the batch arrays.
-enum { kUVoff, kUDamp, kUVg, kUTg, kUTb, kULb, kUState, kUTpart, kUBad, kUSums, kUCount };
+enum { kUVoff, kUDamp, kUVg, kUTg, kUTb, kULb, kUState, kUTpart, kUBad, kUSums, kUList, kUListN,
+       kUCount };
+static_assert(kUCount <= (int)(sizeof(Workspace::ub) / sizeof(DevBuf)), "workspace slots");
 
 // The device description of a batch over one orientation `o` (pull for the
 // forward, push for the adjoint): validation, per-vertex graph ids, the
@@ -471,6 +508,8 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     GT_TRY(U[kUTpart].reserve(sizeof(double) * 2 * std::max<size_t>(1, T.size())));
     GT_TRY(U[kUBad].reserve(sizeof(uint32_t)));
     GT_TRY(U[kUSums].reserve(sizeof(double) * 2 * (size_t)G));
+    GT_TRY(U[kUList].reserve(sizeof(uint32_t) * std::max<size_t>(1, T.size())));
+    GT_TRY(U[kUListN].reserve(sizeof(uint32_t) * 2));
     GT_HIP(hipMemcpyAsync(U[kUVoff].ptr, voff, sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, s));
     GT_HIP(hipMemcpyAsync(U[kUDamp].ptr, damping, sizeof(double) * G, hipMemcpyHostToDevice, s));
     GT_HIP(hipMemcpyAsync(U[kUTg].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
@@ -494,6 +533,8 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     a.tb = U[kUTb].as<uint32_t>();
     a.lb = U[kULb].as<uint32_t>();
     a.tpart = U[kUTpart].as<double>();
+    a.list = U[kUList].as<uint32_t>();
+    a.list_count = U[kUListN].as<uint32_t>();
     a.tiles = o.tiles.as<Tile>();
     a.ntiles = o.ntiles;
     a.off = o.off.as<uint32_t>();
@@ -544,10 +585,16 @@ gt_status u_iterate(gt_graph *g, UArgs a, const float *xs[2], float *xn[2], floa
     if (g->stream != cs) GT_HIP(hipStreamSynchronize(g->stream));
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipLaunchKernelGGL(u_list_init, dim3(ugrid(std::max<uint32_t>(1, a.ntiles))), dim3(kBlock), 0,
+                       cs, a);
+    GT_HIP(hipGetLastError());
     GT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    static_assert(kUnevenCapture % 2 == 0, "a replay starts at parity 0");
     for (int k = 0; k < kUnevenCapture; ++k) {
         const int par = k & 1;
         UArgs b = a;
+        b.list_n = a.list_count + par;
+        b.list_n_next = a.list_count + (par ^ 1);
         b.x_cur = xs[par];
         b.x_next = xn[par ^ 1];
         b.c_next = cn ? cn[par ^ 1] : nullptr;
