@@ -83,3 +83,67 @@ def test_uneven_backward_matches_each_graph(gpu):
         assert np.abs(gpk - gp_o).max() <= 2e-4 * np.abs(gp_o).max(), k
         assert abs(gd[k] - gd_o) <= 2e-4 * max(1.0, abs(gd_o)), k
     assert np.all(it >= 1)
+
+
+def _edge_case_batch():
+    """Six graphs for the sweep's list of still-iterating tiles: empty graphs
+    first, between the others and last; a hub whose in-row of 3,000 entries is
+    split into chunks, with damping 0.95 so that it iterates far longer than
+    the rest; a graph with damping 0.3 that stops after a few sweeps."""
+    rng = np.random.default_rng(17)
+    nh = 3001
+    hub_s = np.concatenate([np.arange(1, nh), np.zeros(40, np.int64)])
+    hub_t = np.concatenate([np.zeros(nh - 1, np.int64), rng.integers(1, nh, 40)])
+    parts = [(0, np.zeros(0, np.int64), np.zeros(0, np.int64), 0.85),
+             (nh, hub_s, hub_t, 0.95),
+             (0, np.zeros(0, np.int64), np.zeros(0, np.int64), 0.5),
+             (200, rng.integers(0, 200, 900), rng.integers(0, 200, 900), 0.3),
+             (700, rng.integers(0, 700, 5000), rng.integers(0, 700, 5000), 0.85),
+             (0, np.zeros(0, np.int64), np.zeros(0, np.int64), 0.7)]
+    voff = np.zeros(len(parts) + 1, np.uint32)
+    eoff = np.zeros(len(parts) + 1, np.int64)
+    s, t = [], []
+    for k, (nk, sk, tk, _) in enumerate(parts):
+        s.append(sk + voff[k])
+        t.append(tk + voff[k])
+        voff[k + 1] = voff[k] + nk
+        eoff[k + 1] = eoff[k] + len(sk)
+    s = np.concatenate(s).astype(np.uint32)
+    t = np.concatenate(t).astype(np.uint32)
+    w = (1.0 - 0.9 * rng.random(len(s))).astype(np.float32)
+    d = np.array([p[3] for p in parts])
+    n = int(voff[-1])
+    return voff, eoff, d, s, t, w, n, gt_hip.DeviceGraph.from_edges(n, s, t)
+
+
+def test_uneven_list_edge_cases(gpu):
+    """Empty graphs, a split hub row and sweep counts from a few to hundreds,
+    forward and adjoint, each non-empty graph against the oracle alone."""
+    voff, eoff, d, s, t, w, n, dev = _edge_case_batch()
+    G = len(d)
+    live = [k for k in range(G) if voff[k + 1] > voff[k]]
+    x, it = gt_hip.pagerank_uneven(dev, voff, d, weight=w, epsilon=1e-6)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(n)
+    xs = np.zeros(n)
+    grads = {}
+    for k in live:
+        nk, sk, tk, wk = _graph(k, voff, eoff, s, t, w)
+        o = oracle.Adj(nk, sk, tk)
+        want, ito = o.pagerank(weight=wk, damping=float(d[k]), epsilon=1e-6, grain=64)
+        got = x[voff[k]:voff[k + 1]]
+        assert np.abs(got - want).sum() <= 1e-5, k
+        assert abs(int(it[k]) - ito) <= 1, (k, int(it[k]), ito)
+        xk, _ = o.pagerank(weight=wk, damping=float(d[k]), epsilon=1e-15, max_iter=100_000)
+        xs[voff[k]:voff[k + 1]] = xk
+        grads[k] = o.pagerank_grad(xk, c[voff[k]:voff[k + 1]], weight=wk, damping=float(d[k]),
+                                   epsilon=1e-12)
+    # The hub graph iterates long after the damping-0.3 graph has stopped.
+    assert int(it[1]) > 10 * int(it[3]), (int(it[1]), int(it[3]))
+    gw, gp, gd, _ = gt_hip.pagerank_uneven_grad(dev, voff, d, xs, c, weight=w, epsilon=1e-9)
+    for k in live:
+        gw_o, gp_o, gd_o, _, _ = grads[k]
+        lo, hi = int(eoff[k]), int(eoff[k + 1])
+        assert np.abs(gw[lo:hi] - gw_o).max() <= 2e-4 * np.abs(gw_o).max(), k
+        assert np.abs(gp[voff[k]:voff[k + 1]] - gp_o).max() <= 2e-4 * np.abs(gp_o).max(), k
+        assert abs(gd[k] - gd_o) <= 2e-4 * max(1.0, abs(gd_o)), k
