@@ -471,8 +471,9 @@ def bench_betweenness(args, torch, gt_hip, _lib) -> int:
         solve()
     torch.cuda.synchronize()
     t = time.perf_counter()
+    timed = {"streams": 0}
     for _ in range(steps):
-        solve()
+        timed = solve()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
     dev.set_profiling(True)
@@ -499,8 +500,11 @@ def bench_betweenness(args, torch, gt_hip, _lib) -> int:
                    "sources": sources, "rmat": [0.57, 0.19, 0.19, 0.05], "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "bc_forward + bc_backward level launches (one call)",
+                     "kernel": "bc_forward + bc_backward level launches (one call, timed "
+                               "per level group on one stream)",
                      "bytes_per_call": int(B), "kernel_ms_per_call": round(kern_ms, 3),
+                     "achieved_over_call_wall": round(B / (dt / steps) / 1e9, 1),
+                     "streams_per_timed_call": int(timed["streams"]),
                      "copy_calibration": copy},
         "work": prof,
     }

@@ -83,6 +83,13 @@ struct BcArgs {
 // workgroups, at most kMaxSweepGrid).
 constexpr uint32_t kBcMaxWaves = kMaxSweepGrid * kWavesPerBlock;
 
+// Streams a call runs batches on (1, 2 or 4), and the most device memory
+// the other streams' batch state may take together (bytes).  At R-MAT 16,
+// all pivots: 460.6 ms per call on one stream, 321.8 on two, 342.7 on four.
+constexpr int kBcLanes = 2;
+static_assert(kBcLanes == 1 || kBcLanes == 2 || kBcLanes == 4, "lanes");
+constexpr double kBcLaneBudget = 16.0 * (1ull << 30);
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int k) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, k);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), k);
@@ -716,6 +723,13 @@ __global__ __launch_bounds__(kBlock) void bc_emit_vertices(const double *vacc, u
         out[v] = halve ? vacc[v] / 2.0 : vacc[v];
 }
 
+// acc += other, entry by entry (the second stream's sums into the first's).
+__global__ __launch_bounds__(kBlock) void bc_add(double *acc, const double *other, uint64_t len) {
+    for (uint64_t e = blockIdx.x * (uint64_t)kBlock + threadIdx.x; e < len;
+         e += (uint64_t)gridDim.x * kBlock)
+        acc[e] += other[e];
+}
+
 __global__ __launch_bounds__(kBlock) void bc_fill(double *out, uint64_t len) {
     for (uint64_t e = blockIdx.x * (uint64_t)kBlock + threadIdx.x; e < len;
          e += (uint64_t)gridDim.x * kBlock)
@@ -1001,83 +1015,133 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     const size_t rows = (size_t)std::max<uint32_t>(n, 1);
     const uint64_t bentries = bo[n];
     const size_t np = std::max<size_t>(std::max(bv.fwd->bcs.npieces, bv.bwd->bcs.npieces), 1);
-    DevBuf *B = ws.btw;
-    GT_TRY(B[0].reserve(rows * kWave * sizeof(double)));       // sigma
-    GT_TRY(B[1].reserve(rows * kWave * sizeof(double)));       // gq
-    GT_TRY(B[2].reserve(rows * kWave * sizeof(uint32_t)));     // dist
-    GT_TRY(B[3].reserve(rows * 3 * sizeof(uint64_t) + 64));    // seen, mask[2], level
-    GT_TRY(B[8].reserve(2 * 3 * kBcMaxWaves * sizeof(unsigned long long)));  // work slots
-    GT_TRY(B[4].reserve(rows * 2 * sizeof(uint32_t)));         // lo, hi
-    GT_TRY(B[5].reserve(rows * sizeof(double)));               // vacc
-    GT_TRY(B[6].reserve(std::max<uint64_t>(bentries, 1) * sizeof(double)));  // eacc
-    GT_TRY(B[7].reserve(np * kWave * sizeof(double) + np * sizeof(uint64_t)));
-    GT_TRY(B[9].reserve((src.size() + src.size() / kWave + 2) * sizeof(uint32_t)));  // src, status
-
+    // A batch's state (rows, masks, pieces, work slots) and the accumulators
+    // in B[0..8]; the sources and per-batch status in ws.btw[9].
+    auto state = [&](DevBuf *B, BcArgs &st, unsigned long long **slots) -> gt_status {
+        GT_TRY(B[0].reserve(rows * kWave * sizeof(double)));       // sigma
+        GT_TRY(B[1].reserve(rows * kWave * sizeof(double)));       // gq
+        GT_TRY(B[2].reserve(rows * kWave * sizeof(uint32_t)));     // dist
+        GT_TRY(B[3].reserve(rows * 3 * sizeof(uint64_t) + 64));    // seen, mask[2], level
+        GT_TRY(B[8].reserve(2 * 3 * kBcMaxWaves * sizeof(unsigned long long)));  // work slots
+        GT_TRY(B[4].reserve(rows * 2 * sizeof(uint32_t)));         // lo, hi
+        GT_TRY(B[5].reserve(rows * sizeof(double)));               // vacc
+        GT_TRY(B[6].reserve(std::max<uint64_t>(bentries, 1) * sizeof(double)));  // eacc
+        GT_TRY(B[7].reserve(np * kWave * sizeof(double) + np * sizeof(uint64_t)));
+        st = BcArgs{};
+        st.n = n;
+        st.sigma = B[0].as<double>();
+        st.gq = B[1].as<double>();
+        st.dist = B[2].as<uint32_t>();
+        st.seen = B[3].as<uint64_t>();
+        st.mask[0] = st.seen + rows;
+        st.mask[1] = st.mask[0] + rows;
+        st.level = reinterpret_cast<uint32_t *>(st.mask[1] + rows);
+        *slots = B[8].as<unsigned long long>();
+        st.lo = B[4].as<uint32_t>();
+        st.hi = st.lo + rows;
+        st.vacc = B[5].as<double>();
+        st.eacc = B[6].as<double>();
+        st.part = B[7].as<double>();
+        st.pmask = reinterpret_cast<uint64_t *>(st.part + np * kWave);
+        return GT_OK;
+    };
+    // Lane 0 runs on the call's stream; lanes 1 .. kBcLanes - 1 on streams of
+    // their own, each with its own state and accumulators.  Batch b runs on
+    // lane b mod kBcLanes, so several batches' level launches overlap; the
+    // other lanes' sums are added to lane 0's at the end, in lane order.
+    // Calls that time their level groups with events, or have too few
+    // batches, use lane 0 only.
+    struct Lane {
+        hipStream_t st;
+        BcArgs fa, ba;
+        unsigned long long *work;
+    };
+    const size_t nbatch = (src.size() + kWave - 1) / kWave;
+    const bool prof = g->profiling;
+    const double lane_bytes = (double)rows * (kWave * 20.0 + 48.0) + 8.0 * (double)bentries +
+                              (double)np * (kWave * 8.0 + 8.0);
+    int nlanes = 1;
+    if (!prof)
+        while (nlanes < kBcLanes && nbatch >= 2 * (size_t)nlanes + 1 &&
+               lane_bytes * nlanes <= kBcLaneBudget)
+            nlanes *= 2;
+    Lane lanes[4];
     BcArgs base{};
-    base.n = n;
-    base.sigma = B[0].as<double>();
-    base.gq = B[1].as<double>();
-    base.dist = B[2].as<uint32_t>();
-    base.seen = B[3].as<uint64_t>();
-    base.mask[0] = base.seen + rows;
-    base.mask[1] = base.mask[0] + rows;
-    base.level = reinterpret_cast<uint32_t *>(base.mask[1] + rows);
-    unsigned long long *work_slots = B[8].as<unsigned long long>();
-    base.lo = B[4].as<uint32_t>();
-    base.hi = base.lo + rows;
-    base.vacc = B[5].as<double>();
-    base.eacc = B[6].as<double>();
-    base.part = B[7].as<double>();
-    base.pmask = reinterpret_cast<uint64_t *>(base.part + np * kWave);
-    uint32_t *dsrc = B[9].as<uint32_t>();
+    unsigned long long *work_slots = nullptr;
+    GT_TRY(state(ws.btw, base, &work_slots));
+    GT_TRY(ws.btw[9].reserve((src.size() + src.size() / kWave + 2) * sizeof(uint32_t)));
+    uint32_t *dsrc = ws.btw[9].as<uint32_t>();
     if (!src.empty())
         GT_HIP(hipMemcpyAsync(dsrc, src.data(), src.size() * sizeof(uint32_t),
                               hipMemcpyHostToDevice, s));
-    BcArgs fa = base, ba = base;
-    fill_sched(fa, *bv.fwd);
-    fill_sched(ba, *bv.bwd);
-    fa.work = work_slots;
-    ba.work = work_slots + 3 * (size_t)kBcMaxWaves;
-
-    GT_HIP(hipMemsetAsync(base.vacc, 0, rows * sizeof(double), s));
-    GT_HIP(hipMemsetAsync(base.eacc, 0, std::max<uint64_t>(bentries, 1) * sizeof(double), s));
-    GT_HIP(hipMemsetAsync(work_slots, 0, 2 * 3 * kBcMaxWaves * sizeof(unsigned long long), s));
+    for (int k = 0; k < nlanes; ++k) {
+        Lane &ln = lanes[k];
+        BcArgs st = base;
+        ln.work = work_slots;
+        ln.st = s;
+        if (k > 0) {
+            hipStream_t &side = g->side_stream[k - 1];
+            if (!side) GT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+            ln.st = side;
+            GT_TRY(state(ws.btwx[k - 1], st, &ln.work));
+        }
+        ln.fa = st;
+        ln.ba = st;
+        fill_sched(ln.fa, *bv.fwd);
+        fill_sched(ln.ba, *bv.bwd);
+        ln.fa.work = ln.work;
+        ln.ba.work = ln.work + 3 * (size_t)kBcMaxWaves;
+        GT_HIP(hipMemsetAsync(st.vacc, 0, rows * sizeof(double), s));
+        GT_HIP(hipMemsetAsync(st.eacc, 0, std::max<uint64_t>(bentries, 1) * sizeof(double), s));
+        GT_HIP(hipMemsetAsync(ln.work, 0, 2 * 3 * kBcMaxWaves * sizeof(unsigned long long), s));
+    }
 
     const int fcap = occupancy_cap((const void *)bc_forward, 8);
     const int bcap = occupancy_cap((const void *)bc_backward, 8);
+    const BcArgs &fa0 = lanes[0].fa, &ba0 = lanes[0].ba;
     const unsigned fgrid = (unsigned)std::max<uint64_t>(
-        1, std::min<uint64_t>(fcap, ((uint64_t)fa.ntiles + kWavesPerBlock - 1) / kWavesPerBlock));
+        1, std::min<uint64_t>(fcap, ((uint64_t)fa0.ntiles + kWavesPerBlock - 1) / kWavesPerBlock));
     const unsigned bgrid = (unsigned)std::max<uint64_t>(
-        1, std::min<uint64_t>(bcap, ((uint64_t)ba.ntiles + kWavesPerBlock - 1) / kWavesPerBlock));
-    const unsigned fixf = std::max<uint32_t>(1, std::min<uint32_t>(fa.nlong, 4096));
-    const unsigned fixb = std::max<uint32_t>(1, std::min<uint32_t>(ba.nlong, 4096));
+        1, std::min<uint64_t>(bcap, ((uint64_t)ba0.ntiles + kWavesPerBlock - 1) / kWavesPerBlock));
+    const unsigned fixf = std::max<uint32_t>(1, std::min<uint32_t>(fa0.nlong, 4096));
+    const unsigned fixb = std::max<uint32_t>(1, std::min<uint32_t>(ba0.nlong, 4096));
 
     hipEvent_t ev[4] = {};
-    const bool prof = g->profiling;
     if (prof)
         for (auto &e : ev) GT_HIP(hipEventCreate(&e));
     uint64_t launches_f = 0, launches_b = 0;
     double fms = 0.0, bms = 0.0;
-    const size_t nbatch = (src.size() + kWave - 1) / kWave;
     uint32_t *dstatus = dsrc + src.size();
     if (nbatch) GT_HIP(hipMemsetAsync(dstatus, 0, nbatch * sizeof(uint32_t), s));
     std::vector<uint32_t> status(nbatch, 0);
+    // The other lanes start after the set-up above; the call's stream waits
+    // for them before the outputs.
+    hipEvent_t fork = nullptr, join[3] = {};
+    if (nlanes > 1) {
+        GT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        GT_HIP(hipEventRecord(fork, s));
+        for (int k = 1; k < nlanes; ++k) {
+            GT_HIP(hipEventCreateWithFlags(&join[k - 1], hipEventDisableTiming));
+            GT_HIP(hipStreamWaitEvent(lanes[k].st, fork, 0));
+        }
+    }
 
-    auto setup = [&](size_t b) -> gt_status {
+    auto setup = [&](Lane &ln, size_t b) -> gt_status {
         const size_t first = b * kWave;
         const uint32_t cnt = (uint32_t)std::min<size_t>(kWave, src.size() - first);
-        fa.act = ba.act = cnt == kWave ? ~0ull : ((1ull << cnt) - 1);
-        ba.status = dstatus + b;
-        hipLaunchKernelGGL(bc_reset, dim3(lgrid(n)), dim3(kBlock), 0, s, fa);
-        hipLaunchKernelGGL(bc_seed, dim3(1), dim3(kWave), 0, s, fa, dsrc + first, cnt);
+        ln.fa.act = ln.ba.act = cnt == kWave ? ~0ull : ((1ull << cnt) - 1);
+        ln.ba.status = dstatus + b;
+        hipLaunchKernelGGL(bc_reset, dim3(lgrid(n)), dim3(kBlock), 0, ln.st, ln.fa);
+        hipLaunchKernelGGL(bc_seed, dim3(1), dim3(kWave), 0, ln.st, ln.fa, dsrc + first, cnt);
         GT_HIP(hipGetLastError());
         return GT_OK;
     };
-    auto forward = [&](uint32_t L0, uint32_t count) -> gt_status {
+    auto forward = [&](Lane &ln, uint32_t L0, uint32_t count) -> gt_status {
         if (prof) GT_HIP(hipEventRecord(ev[0], s));
         for (uint32_t L = L0; L < L0 + count; ++L) {
-            hipLaunchKernelGGL(bc_forward, dim3(fgrid), dim3(kBlock), 0, s, fa, L);
-            if (fa.nlong) hipLaunchKernelGGL(bc_forward_fix, dim3(fixf), dim3(kBlock), 0, s, fa, L);
+            hipLaunchKernelGGL(bc_forward, dim3(fgrid), dim3(kBlock), 0, ln.st, ln.fa, L);
+            if (ln.fa.nlong)
+                hipLaunchKernelGGL(bc_forward_fix, dim3(fixf), dim3(kBlock), 0, ln.st, ln.fa, L);
             ++launches_f;
         }
         GT_HIP(hipGetLastError());
@@ -1090,12 +1154,13 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
         }
         return GT_OK;
     };
-    auto backward = [&](uint32_t from, uint32_t launched) -> gt_status {
+    auto backward = [&](Lane &ln, uint32_t from, uint32_t launched) -> gt_status {
         if (prof) GT_HIP(hipEventRecord(ev[2], s));
         for (uint32_t l = from + 1; l-- > 0;) {
-            hipLaunchKernelGGL(bc_backward, dim3(bgrid), dim3(kBlock), 0, s, ba, l, launched);
-            if (ba.nlong)
-                hipLaunchKernelGGL(bc_backward_fix, dim3(fixb), dim3(kBlock), 0, s, ba, l,
+            hipLaunchKernelGGL(bc_backward, dim3(bgrid), dim3(kBlock), 0, ln.st, ln.ba, l,
+                               launched);
+            if (ln.ba.nlong)
+                hipLaunchKernelGGL(bc_backward_fix, dim3(fixb), dim3(kBlock), 0, ln.st, ln.ba, l,
                                    launched);
             ++launches_b;
         }
@@ -1112,19 +1177,20 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     // A batch with the host in the loop: forward levels in groups until one
     // comes out empty, then exactly the backward levels needed.
     auto run_sync = [&](size_t b, uint32_t *probe) -> gt_status {
-        GT_TRY(setup(b));
+        Lane &ln = lanes[0];
+        GT_TRY(setup(ln, b));
         uint32_t L = 0, top = 0;
         for (;;) {
-            GT_TRY(forward(L, *probe));
+            GT_TRY(forward(ln, L, *probe));
             L += *probe;
-            GT_HIP(hipMemcpyAsync(&g->h_state->counter, base.level, sizeof(uint32_t),
+            GT_HIP(hipMemcpyAsync(&g->h_state->counter, ln.fa.level, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, s));
             GT_HIP(hipStreamSynchronize(s));
             top = g->h_state->counter;
             if (top < L) break;  // a level came out empty
             *probe *= 2;
         }
-        GT_TRY(backward(top, 0xffffffffu));
+        GT_TRY(backward(ln, top, 0xffffffffu));
         status[b] = top + 1;
         *probe = std::max<uint32_t>(kBcLevelsPerCheck, top + 2);
         return GT_OK;
@@ -1140,9 +1206,14 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
             GT_TRY(run_sync(b, &probe));
             continue;
         }
-        GT_TRY(setup(b));
-        GT_TRY(forward(0, probe));
-        GT_TRY(backward(probe, probe));
+        Lane &ln = lanes[b % (size_t)nlanes];
+        GT_TRY(setup(ln, b));
+        GT_TRY(forward(ln, 0, probe));
+        GT_TRY(backward(ln, probe, probe));
+    }
+    for (int k = 1; k < nlanes; ++k) {
+        GT_HIP(hipEventRecord(join[k - 1], lanes[k].st));
+        GT_HIP(hipStreamWaitEvent(s, join[k - 1], 0));
     }
     if (nbatch > 1) {
         std::vector<uint32_t> dev_status(nbatch);
@@ -1165,6 +1236,15 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
         g->prof_launches[GT_SWEEP_BC_FORWARD] += launches_f;
         g->prof_ms[GT_SWEEP_BC_BACKWARD] += bms;
         g->prof_launches[GT_SWEEP_BC_BACKWARD] += launches_b;
+    }
+
+    for (int k = 1; k < nlanes; ++k) {
+        const BcArgs &o = lanes[k].fa;
+        hipLaunchKernelGGL(bc_add, dim3(lgrid(n)), dim3(kBlock), 0, s, base.vacc, o.vacc,
+                           (uint64_t)n);
+        hipLaunchKernelGGL(bc_add, dim3(lgrid(bentries)), dim3(kBlock), 0, s, base.eacc, o.eacc,
+                           bentries);
+        GT_HIP(hipGetLastError());
     }
 
     // Outputs: vertex sums, edge sums scattered from entry positions.
@@ -1197,14 +1277,22 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     GT_HIP(hipGetLastError());
     if (out->loc == GT_HOST && n)
         GT_HIP(hipMemcpyAsync(out->vertex, vdst, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> slots(2 * 3 * (size_t)kBcMaxWaves);
-    GT_HIP(hipMemcpyAsync(slots.data(), work_slots, slots.size() * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> slots(2 * 3 * (size_t)kBcMaxWaves * nlanes);
+    for (int k = 0; k < nlanes; ++k)
+        GT_HIP(hipMemcpyAsync(slots.data() + (size_t)k * 2 * 3 * kBcMaxWaves, lanes[k].work,
+                              2 * 3 * kBcMaxWaves * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : join)
+        if (e) (void)hipEventDestroy(e);
     unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
-    for (size_t w = 0; w < kBcMaxWaves; ++w)
-        for (int p = 0; p < 2; ++p)
-            for (int c = 0; c < 3; ++c) work[3 * p + c] += slots[(p * kBcMaxWaves + w) * 3 + c];
+    for (int k = 0; k < nlanes; ++k)
+        for (size_t w = 0; w < kBcMaxWaves; ++w)
+            for (int p = 0; p < 2; ++p)
+                for (int c = 0; c < 3; ++c)
+                    work[3 * p + c] +=
+                        slots[(size_t)k * 2 * 3 * kBcMaxWaves + (p * kBcMaxWaves + w) * 3 + c];
     if (stats) {
         stats->sources = src.size();
         stats->batches = nbatch;
@@ -1218,6 +1306,7 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
         stats->backward_hit_entries = work[5];
         stats->forward_ms = fms;
         stats->backward_ms = bms;
+        stats->streams = (uint64_t)nlanes;
     }
     return GT_OK;
 }

@@ -479,6 +479,11 @@ gt_status gt_graph_free(gt_graph *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     for (hipEvent_t e : g->ev) (void)hipEventDestroy(e);
     if (g->h_state) (void)hipHostFree(g->h_state);
+    for (hipStream_t st : g->side_stream)
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
     if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
     delete g;
     return GT_OK;

@@ -193,6 +193,7 @@ struct Workspace {
     DevBuf masks;       // staged vertex + edge masks
     DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
     DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces
+    DevBuf btwx[3][9];  // betweenness: the other streams' batch state and accumulators
     Orient fpull;       // filtered view: compacted pull orientation
     Orient fpush;       // filtered view: compacted push orientation
     Orient fund;        // undirected filtered view: merged halves
@@ -234,6 +235,7 @@ struct gt_graph {
     gt::DevBuf esrc, etgt;
     bool edges_ready = false;
     hipStream_t own_stream = nullptr;
+    hipStream_t side_stream[3] = {};  // betweenness: the other streams, made on first use
     hipStream_t stream = nullptr;
     gt::Workspace ws;
     gt_tuning tune{};       // effective launch / schedule parameters

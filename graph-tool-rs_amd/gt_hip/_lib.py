@@ -79,7 +79,8 @@ class GtBcStats(C.Structure):
                 ("forward_hit_entries", C.c_uint64),
                 ("backward_entries", C.c_uint64), ("backward_hits", C.c_uint64),
                 ("backward_hit_entries", C.c_uint64),
-                ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double),
+                ("streams", C.c_uint64)]
 
 GraphP = C.c_void_p
 

@@ -286,6 +286,7 @@ typedef struct gt_bc_stats {
     uint64_t backward_hit_entries;  /* entries with at least one such source */
     double forward_ms;         /* device time of the forward launches (profiling only) */
     double backward_ms;        /* device time of the backward launches (profiling only) */
+    uint64_t streams;          /* streams the batches ran on (1 when profiling) */
 } gt_bc_stats;
 
 /* betweenness (crates/gt-algo/src/centrality.rs:614-689): Brandes over the
