@@ -111,6 +111,11 @@ def test_rmat_all_pivots(gpu, kind):
                        threads=8)
     check((gv, ge), (wv, we))
     assert st["batches"] == -(-st["sources"] // 64)
+    # The 16 batches alternate between two streams; the lanes' sums are
+    # added in a fixed order, so a second call is bit-identical.
+    assert st["streams"] == 2
+    gv2, ge2, _ = device_bc(view, g)
+    assert np.array_equal(gv, gv2) and np.array_equal(ge, ge2)
 
 
 def test_hubs_are_cut_into_pieces(gpu):
