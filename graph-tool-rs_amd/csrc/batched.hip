@@ -32,7 +32,7 @@ struct BatchState {
     double delta[kCols];
     uint64_t iters[kCols];
     uint64_t active;         // bit c: column c still iterating
-    uint64_t pad;
+    uint64_t was_active;     // active before the last fold
 };
 
 struct BArgs {
@@ -60,10 +60,11 @@ struct BArgs {
 };
 
 // Sum over entries lo, lo+stride, ... < hi of w(j) * c[src(j)][lane], in
-// entry order.  Eight entries are in flight per wavefront.
+// entry order.  Eight entries are in flight per wavefront.  A converged
+// column's lane loads nothing (its sum is not used).
 template <bool WEIGHTED>
 __device__ __forceinline__ float row_accumulate(const BArgs &a, uint32_t lo, uint32_t hi,
-                                                uint32_t stride, int lane) {
+                                                uint32_t stride, int lane, bool col_on) {
     float acc = 0.f;
     for (uint32_t j0 = lo; j0 < hi; j0 += 8 * stride) {
         const uint32_t mj = j0 + (uint32_t)lane * stride;
@@ -75,7 +76,7 @@ __device__ __forceinline__ float row_accumulate(const BArgs &a, uint32_t lo, uin
         for (int u = 0; u < 8; ++u) {
             const uint32_t s = __shfl(my_idx, u, kWave);
             wv[u] = WEIGHTED ? __shfl(my_w, u, kWave) : 1.f;
-            v[u] = (j0 + u * stride < hi) ? a.c_cur[(size_t)s * kCols + lane] : 0.f;
+            v[u] = (col_on && j0 + u * stride < hi) ? a.c_cur[(size_t)s * kCols + lane] : 0.f;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -84,17 +85,22 @@ __device__ __forceinline__ float row_accumulate(const BArgs &a, uint32_t lo, uin
     return acc;
 }
 
+// A converged column's values are copied into the other buffers on the sweep
+// after it stopped (col_copy) and left alone afterwards: both buffers then
+// hold its last iterate.
 template <bool WEIGHTED>
 __device__ __forceinline__ void finish_row(const BArgs &a, uint32_t r, float acc, int lane,
-                                           bool col_on, uint32_t seed, double psink,
-                                           double &dsum, double &ssum) {
+                                           bool col_on, bool col_copy, uint32_t seed,
+                                           double psink, double &dsum, double &ssum) {
     const size_t o = (size_t)r * kCols + lane;
-    const float xo = a.x_cur[o];
     if (!col_on) {
-        a.x_next[o] = xo;
-        a.c_next[o] = a.c_cur[o];
+        if (col_copy) {
+            a.x_next[o] = a.x_cur[o];
+            a.c_next[o] = a.c_cur[o];
+        }
         return;
     }
+    const float xo = a.x_cur[o];
     const double pv = (r == seed) ? 1.0 : 0.0;
     const double nv = (1.0 - a.damping) * pv + a.damping * (psink * pv + (double)acc);
     const float nf = (float)nv;
@@ -119,6 +125,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
     const int wid = tid / kWave;
     const uint64_t active = a.st->active;
     const bool col_on = (active >> lane) & 1ull;
+    const bool col_copy = !col_on && ((a.st->was_active >> lane) & 1ull);
     const uint32_t seed = a.seeds[lane];
     const double psink = a.st->p_sink[lane];
     double dsum = 0.0, ssum = 0.0;
@@ -128,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
         if (t.row_end == kSplit) {
             // All four wavefronts share the chunk, entries strided by 4.
             s_part[wid][lane] =
-                row_accumulate<WEIGHTED>(a, t.nnz_begin + wid, t.nnz_end, kBlock / kWave, lane);
+                row_accumulate<WEIGHTED>(a, t.nnz_begin + wid, t.nnz_end, kBlock / kWave, lane,
+                                         col_on);
             __syncthreads();
             if (wid == 0)
                 a.long_part[(size_t)t.row_begin * kCols + lane] =
@@ -137,8 +145,9 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
             continue;
         }
         for (uint32_t r = t.row_begin + wid; r < t.row_end; r += kBlock / kWave) {
-            const float acc = row_accumulate<WEIGHTED>(a, a.in_off[r], a.in_off[r + 1], 1, lane);
-            finish_row<WEIGHTED>(a, r, acc, lane, col_on, seed, psink, dsum, ssum);
+            const float acc =
+                row_accumulate<WEIGHTED>(a, a.in_off[r], a.in_off[r + 1], 1, lane, col_on);
+            finish_row<WEIGHTED>(a, r, acc, lane, col_on, col_copy, seed, psink, dsum, ssum);
         }
     }
 
@@ -171,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
     const int wid = tid / kWave;
     const uint64_t active = a.st->active;
     const bool col_on = (active >> lane) & 1ull;
+    const bool col_copy = !col_on && ((a.st->was_active >> lane) & 1ull);
     const uint32_t seed = a.seeds[lane];
     const double psink = a.st->p_sink[lane];
     double d = 0.0, s = 0.0;
@@ -185,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
         float acc = 0.f;
         for (uint32_t q = a.long_slot[k]; q < a.long_slot[k + 1]; ++q)
             acc += a.long_part[(size_t)q * kCols + lane];
-        finish_row<WEIGHTED>(a, a.long_rows[k], acc, lane, col_on, seed, psink, d, s);
+        finish_row<WEIGHTED>(a, a.long_rows[k], acc, lane, col_on, col_copy, seed, psink, d, s);
     }
     s_red[0][wid][lane] = d;
     s_red[1][wid][lane] = s;
@@ -227,6 +237,7 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, kWave));
         if (lane == 0) {
+            st->was_active = active;
             st->active = mask;
             st->h.delta = dmax;
             st->h.iter += 1;
@@ -264,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void ppr_init(const BArgs a, uint32_t n, ui
         st->iters[lane] = 0;
         if (lane == 0) {
             st->active = active;
+            st->was_active = active;
             st->h = SweepState{};
         }
     }
