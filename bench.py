@@ -168,9 +168,15 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
     col_sweeps = int(it.sum())
     sweeps = int(it.max())
     gteps = m * col_sweeps * args.steps / dt / 1e9
-    B = 8 * m + 4 * (n + 1) + 16 * n * k + 4 * n
+    # Algorithmic bytes of one solve: every sweep streams the tiles' entries
+    # and weights (8m), the row offsets and out-degrees (8n); each (column,
+    # sweep) with the column still iterating reads and writes its x and c
+    # (16n); a column that stops before the last sweep is copied once (16n).
+    copied = int((it < sweeps).sum())
+    B_solve = sweeps * (8 * m + 4 * (n + 1) + 4 * n) + 16 * n * (col_sweeps + copied)
+    B = B_solve // max(sweeps, 1)
     avg = ms_k / max(launches, 1) * 1e-3
-    ach = B * sweeps * args.steps / (ms_k * 1e-3) / 1e9 if launches else 0.0
+    ach = B_solve * args.steps / (ms_k * 1e-3) / 1e9 if launches else 0.0
     log(f"[bench] batched: {dt/args.steps*1e3:.2f} ms/solve, sweeps {sweeps} (columns "
         f"{it.min()}..{it.max()}), {launches} launches ({avg*1e6:.1f} us avg)")
     result = {
@@ -184,7 +190,8 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                    "seeds": "64 distinct uniform", "damping": 0.85, "epsilon": eps},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "ppr_sweep<true> (batched SpMM sweep)", "bytes_per_sweep": B,
+                     "kernel": "ppr_sweep<true> (batched SpMM sweep)",
+                     "bytes_per_sweep": B, "bytes_per_solve": B_solve,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
         "column_sweeps": {"min": int(it.min()), "max": int(it.max()), "sum": col_sweeps},
         "tuning": tuning,

@@ -192,9 +192,20 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
     }
     const uint32_t waves = gridDim.x * (kBlock / kWave);
     for (uint32_t k = blockIdx.x * (kBlock / kWave) + wid; k < a.nlong; k += waves) {
+        // Chunk order, in rounds of 16 chunk rows in flight (a round's loads
+        // past the row read its last chunk again and are not added): a hub's
+        // hundreds of chunks set the fold's critical path.
         float acc = 0.f;
-        for (uint32_t q = a.long_slot[k]; q < a.long_slot[k + 1]; ++q)
-            acc += a.long_part[(size_t)q * kCols + lane];
+        const uint32_t e = a.long_slot[k + 1];
+        for (uint32_t q = a.long_slot[k]; q < e; q += 16) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                v[i] = a.long_part[(size_t)(q + i < e ? q + i : e - 1) * kCols + lane];
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (q + i < e) acc += v[i];
+        }
         finish_row<WEIGHTED>(a, a.long_rows[k], acc, lane, col_on, col_copy, seed, psink, d, s);
     }
     s_red[0][wid][lane] = d;
