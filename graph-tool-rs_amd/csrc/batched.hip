@@ -170,6 +170,30 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
 // sweep's per-column partials; the last workgroup to arrive folds the
 // workgroup sums in index order and updates the per-column state.  (Two
 // levels, so no workgroup walks all of the sweep's partials alone.)
+// Sums of rows b, b + stride, ... < e of two [rows][64] partial arrays (d
+// from pd, s from ps), column `lane`, in row order, in rounds of 16 rows in
+// flight (a round's loads past e read row b again and are not added).
+__device__ __forceinline__ void sum_rows2(const double *pd, const double *ps, uint32_t b,
+                                          uint32_t e, uint32_t stride, int lane, double &d,
+                                          double &s) {
+    for (; b < e; b += 16 * stride) {
+        double vd[16], vs[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t q = b + (uint32_t)i * stride;
+            const size_t o = (size_t)(q < e ? q : b) * kCols + lane;
+            vd[i] = pd[o];
+            vs[i] = ps[o];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (b + (uint32_t)i * stride < e) {
+                d += vd[i];
+                s += vs[i];
+            }
+    }
+}
+
 template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid) {
     if (a.st->h.done) return;
@@ -186,10 +210,8 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
     double d = 0.0, s = 0.0;
     const uint32_t b0 = (uint32_t)((uint64_t)sgrid * blockIdx.x / gridDim.x);
     const uint32_t b1 = (uint32_t)((uint64_t)sgrid * (blockIdx.x + 1) / gridDim.x);
-    for (uint32_t b = b0 + wid; b < b1; b += kBlock / kWave) {
-        d += a.partials[(size_t)b * kCols + lane];
-        s += a.partials[((size_t)sgrid + b) * kCols + lane];
-    }
+    sum_rows2(a.partials, a.partials + (size_t)sgrid * kCols, b0 + wid, b1, kBlock / kWave, lane,
+              d, s);
     const uint32_t waves = gridDim.x * (kBlock / kWave);
     for (uint32_t k = blockIdx.x * (kBlock / kWave) + wid; k < a.nlong; k += waves) {
         // Chunk order, in rounds of 16 chunk rows in flight (a round's loads
@@ -222,10 +244,7 @@ __global__ __launch_bounds__(kBlock) void ppr_fold(const BArgs a, uint32_t sgrid
 
     d = 0.0;
     s = 0.0;
-    for (uint32_t b = wid; b < gridDim.x; b += kBlock / kWave) {
-        d += fp[(size_t)b * kCols + lane];
-        s += fp[((size_t)gridDim.x + b) * kCols + lane];
-    }
+    sum_rows2(fp, fp + (size_t)gridDim.x * kCols, wid, gridDim.x, kBlock / kWave, lane, d, s);
     s_red[0][wid][lane] = d;
     s_red[1][wid][lane] = s;
     __syncthreads();
