@@ -66,11 +66,22 @@ __global__ __launch_bounds__(kBlock) void deg_mid(const uint32_t *__restrict__ o
         const uint32_t v = rows[k];
         const uint32_t lo = off[v], hi = off[v + 1];
         double acc = accumulate ? deg[v] : 0.0;
+        // Storage order, 64 weights at a time: the next 64 are loaded before
+        // this 64 is folded, and the fold reads each lane's weight with
+        // readlane (no LDS round trip in the chain of adds).
+        double x = lo + lane < hi ? read_w<S>(w, eid[lo + lane], kind, value) : 0.0;
         for (uint32_t j0 = lo; j0 < hi; j0 += kWave) {
-            const uint32_t j = j0 + lane;
-            const double x = j < hi ? read_w<S>(w, eid[j], kind, value) : 0.0;
+            const uint32_t jn = j0 + kWave + lane;
+            const double xn = jn < hi ? read_w<S>(w, eid[jn], kind, value) : 0.0;
             const uint32_t cnt = hi - j0 < (uint32_t)kWave ? hi - j0 : (uint32_t)kWave;
-            for (uint32_t u = 0; u < cnt; ++u) acc += __shfl(x, (int)u, kWave);
+            const uint64_t xb = __builtin_bit_cast(uint64_t, x);
+            const int xlo = (int)(uint32_t)xb, xhi = (int)(uint32_t)(xb >> 32);
+            for (uint32_t u = 0; u < cnt; ++u) {
+                const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(xlo, (int)u);
+                const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(xhi, (int)u);
+                acc += __builtin_bit_cast(double, ((uint64_t)h << 32) | l);
+            }
+            x = xn;
         }
         if (lane == 0) deg[v] = acc;
     }
