@@ -1116,13 +1116,29 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
     std::vector<uint32_t> status(nbatch, 0);
     // The other lanes start after the set-up above; the call's stream waits
     // for them before the outputs.
-    hipEvent_t fork = nullptr, join[3] = {};
+    // On every way out of the call the other lanes' streams are drained (an
+    // error part-way must not leave them writing the workspace the next call
+    // reuses) and the events released.
+    struct LaneScope {
+        hipStream_t side[3] = {};
+        hipEvent_t fork = nullptr, join[3] = {};
+        ~LaneScope() {
+            for (hipStream_t st : side)
+                if (st) (void)hipStreamSynchronize(st);
+            if (fork) (void)hipEventDestroy(fork);
+            for (hipEvent_t e : join)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } scope;
+    hipEvent_t &fork = scope.fork;
+    hipEvent_t *join = scope.join;
     if (nlanes > 1) {
         GT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         GT_HIP(hipEventRecord(fork, s));
         for (int k = 1; k < nlanes; ++k) {
             GT_HIP(hipEventCreateWithFlags(&join[k - 1], hipEventDisableTiming));
             GT_HIP(hipStreamWaitEvent(lanes[k].st, fork, 0));
+            scope.side[k - 1] = lanes[k].st;
         }
     }
 
@@ -1283,9 +1299,6 @@ gt_status betweenness_impl(gt_graph *g, const gt_view *view, const uint32_t *piv
                               2 * 3 * kBcMaxWaves * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, s));
     GT_HIP(hipStreamSynchronize(s));
-    if (fork) (void)hipEventDestroy(fork);
-    for (hipEvent_t e : join)
-        if (e) (void)hipEventDestroy(e);
     unsigned long long work[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < nlanes; ++k)
         for (size_t w = 0; w < kBcMaxWaves; ++w)
