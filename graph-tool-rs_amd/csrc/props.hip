@@ -97,8 +97,21 @@ __global__ __launch_bounds__(kBlock) void deg_long(const uint32_t *__restrict__ 
     __shared__ double red[8];
     const uint32_t v = rows[blockIdx.x];
     const uint32_t lo = off[v], hi = off[v + 1];
+    // Each thread's entries j, j + kBlock, ... in order, eight in flight (the
+    // hub rows are tens of thousands of entries long).
     double acc = 0.0;
-    for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) acc += read_w<S>(w, eid[j], kind, value);
+    uint32_t j = lo + threadIdx.x;
+    for (; j + 7 * kBlock < hi; j += 8 * kBlock) {
+        uint32_t e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) e[i] = eid[j + i * kBlock];
+        double x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = read_w<S>(w, e[i], kind, value);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc += x[i];
+    }
+    for (; j < hi; j += kBlock) acc += read_w<S>(w, eid[j], kind, value);
     acc = block_sum(acc, red);
     if (threadIdx.x == 0) deg[v] = accumulate ? deg[v] + acc : acc;
 }
