@@ -38,6 +38,7 @@ struct AdjArgs {
     const uint32_t *row_off;   // push orientation (rows = sources)
     const uint32_t *packed;    // packed targets of the push orientation's wave tiles
     uint32_t bits;             // slot bits of `packed`
+    const uint32_t *tbase;     // source base of each tile
     uint32_t short_row;        // rows with fewer entries are summed by one lane
     const void *w;             // weights in packed order (value type)
     const double *deg;         // weighted out-degree (weighted)
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void adj_sweep(const AdjArgs a) {
     if (a.st->done) return;
     __shared__ WaveLds<V> L;
     AdjFin<V, WEIGHTED, DPERS> fin(a, a.st->p_sink);
-    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits,
+    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, a.tbase,
                             static_cast<const V *>(a.l_cur), static_cast<const V *>(a.w),
                             a.long_part, a.short_row, L, fin);
     double d = fin.dsum, p = fin.psum;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kHotBlock) void adj_sweep_hot(const AdjArgs a, uint
     const float *lc = static_cast<const float *>(a.l_cur);
     fill_hot(lc, nhot, hot);
     AdjFin<float, WEIGHTED, DPERS> fin(a, a.st->p_sink);
-    spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, lc,
+    spmv_waves<float, WEIGHTED, true>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, a.tbase, lc,
                                       static_cast<const float *>(a.w), a.long_part, a.short_row,
                                       L, fin, static_cast<const float *>(hot), nhot);
     double d = fin.dsum, p = fin.psum;
@@ -212,7 +213,7 @@ template <typename V, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void adj_mu(const AdjArgs a, const V *lam, double *mu) {
     __shared__ WaveLds<V> L;
     MuFin<V, WEIGHTED> fin(a, mu);
-    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, lam,
+    spmv_waves<V, WEIGHTED>(a.tiles, a.ntiles, a.row_off, a.packed, a.bits, a.tbase, lam,
                             static_cast<const V *>(a.w), a.long_part, a.short_row, L, fin);
 }
 
@@ -416,6 +417,7 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     const uint32_t *perm = R->perm.as<uint32_t>();
     const uint32_t *iperm = R->iperm.as<uint32_t>();
     GT_TRY(ensure_schedule(g, rp, n));
+    note_layout(g, rp);
     GT_TRY(prof_prepare(g));
     GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, rp.nslots)));
@@ -442,6 +444,7 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
     r.row_off = rp.off.as<uint32_t>();
     r.packed = rp.packed.as<uint32_t>();
     r.bits = rp.bits;
+    r.tbase = rp.tbase.as<uint32_t>();
     r.long_rows = rp.long_rows.as<uint32_t>();
     r.long_slot = rp.long_slot.as<uint32_t>();
     {

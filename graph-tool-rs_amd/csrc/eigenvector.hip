@@ -49,6 +49,7 @@ struct EvArgs {
     const uint32_t *in_off;
     const uint32_t *packed;   // packed entry stream of the wave tiles
     uint32_t bits;            // slot bits of `packed`
+    const uint32_t *tbase;    // source base of each tile
     uint32_t short_row;        // rows with fewer entries are summed by one lane
     const double *w;          // weights in packed order
     const uint8_t *active;
@@ -92,8 +93,8 @@ __global__ __launch_bounds__(kBlock) void ev_sweep(const EvArgs a) {
     __shared__ WaveLds<double> L;
     const bool first = a.st->launches == 0;
     EvFin<MASK> fin(a, a.st->sc_cur, a.st->sc_prev, first);
-    spmv_waves<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur, a.w,
-                                 a.long_part, a.short_row, L, fin);
+    spmv_waves<double, WEIGHTED>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase,
+                                 a.y_cur, a.w, a.long_part, a.short_row, L, fin);
     double sq = fin.sq, dl = fin.dl;
     block_sum2(sq, dl, L.red);
     if (threadIdx.x == 0) {
@@ -112,8 +113,8 @@ __global__ __launch_bounds__(kHotBlock) void ev_sweep_hot(const EvArgs a, uint32
     fill_hot(a.y_cur, nhot, hot);
     const bool first = a.st->launches == 0;
     EvFin<false> fin(a, a.st->sc_cur, a.st->sc_prev, first);
-    spmv_waves<double, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.y_cur,
-                                       a.w, a.long_part, a.short_row, L, fin,
+    spmv_waves<double, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase,
+                                       a.y_cur, a.w, a.long_part, a.short_row, L, fin,
                                        static_cast<const double *>(hot), nhot);
     double sq = fin.sq, dl = fin.dl;
     block_sum2_waves<kHotWaves>(sq, dl, L.red);
@@ -295,6 +296,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         return GT_OK;
     }
     GT_TRY(ensure_schedule(g, *pull, n));
+    note_layout(g, *pull);
     if (weighted) {
         GT_TRY(stage_pull_weights<double>(g, weight, *pull, wsrc, wdt, ws.wcsc64));
         wp = ws.wcsc64.as<double>();
@@ -336,6 +338,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
         a.in_off = pull->off.as<uint32_t>();
         a.packed = pull->packed.as<uint32_t>();
         a.bits = pull->bits;
+        a.tbase = pull->tbase.as<uint32_t>();
         a.short_row = (uint32_t)g->tune.short_row;
         a.w = wp;
         a.active = active;

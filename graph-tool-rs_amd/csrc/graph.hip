@@ -5,6 +5,7 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "device_common.hpp"
 #include "gt_internal.hpp"
 
 namespace gt {
@@ -126,28 +127,70 @@ gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bou
     return GT_OK;
 }
 
+// Smallest source of each wave tile (its packing base) and, through one
+// atomic per tile, the widest span (largest - smallest source) of any tile.
+// One workgroup per tile.
+__global__ __launch_bounds__(kBlock) void tile_span(const Tile *__restrict__ tiles,
+                                                    const uint32_t *__restrict__ idx,
+                                                    uint32_t *__restrict__ tbase,
+                                                    uint32_t *__restrict__ span_max) {
+    __shared__ uint32_t lo[kWavesPerBlock], hi[kWavesPerBlock];
+    const Tile t = tiles[blockIdx.x];
+    uint32_t a = 0xffffffffu, b = 0u;
+    for (uint32_t j = t.nnz_begin + threadIdx.x; j < t.nnz_end; j += kBlock) {
+        const uint32_t v = idx[j];
+        a = v < a ? v : a;
+        b = v > b ? v : b;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t a2 = __shfl_xor(a, o, kWave), b2 = __shfl_xor(b, o, kWave);
+        a = a2 < a ? a2 : a;
+        b = b2 > b ? b2 : b;
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        lo[wid] = a;
+        hi[wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            a = lo[w] < a ? lo[w] : a;
+            b = hi[w] > b ? hi[w] : b;
+        }
+        if (t.nnz_end == t.nnz_begin) a = b = 0u;
+        tbase[blockIdx.x] = a;
+        atomicMax(span_max, b - a);
+    }
+}
+
 // The packed entry stream of one wave tile (one workgroup per tile).  A tile
 // of whole rows is sorted by packed word -- by source, ties by slot -- with a
 // bitonic network over kWaveTileNnz LDS keys (padding keys are all ones and
-// sort last); split chunks and unpacked orientations keep row order.
+// sort last); split chunks keep row order.  Packed sources are relative to
+// tbase[tile]; with bits == 0 the stream is the plain sources in row order
+// and every base is set to 0.
 __global__ __launch_bounds__(kBlock) void pack_tiles(const Tile *__restrict__ tiles,
                                                      const uint32_t *__restrict__ idx,
                                                      const uint32_t *__restrict__ eid,
-                                                     uint32_t bits, uint32_t *__restrict__ packed,
+                                                     uint32_t bits, uint32_t *__restrict__ tbase,
+                                                     uint32_t *__restrict__ packed,
                                                      uint32_t *__restrict__ peid) {
     __shared__ uint32_t key[kWaveTileNnz];
     const Tile t = tiles[blockIdx.x];
     const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
     const int tid = threadIdx.x;
+    const uint32_t base = bits ? tbase[blockIdx.x] : 0u;
     if (bits == 0 || t.row_end == kSplit) {
         for (uint32_t j = tid; j < cnt; j += kBlock) {
-            packed[nb + j] = bits ? (idx[nb + j] << bits) | j : idx[nb + j];
+            packed[nb + j] = bits ? ((idx[nb + j] - base) << bits) | j : idx[nb + j];
             peid[nb + j] = eid[nb + j];
         }
+        if (bits == 0 && tid == 0) tbase[blockIdx.x] = 0u;
         return;
     }
     for (uint32_t j = tid; j < kWaveTileNnz; j += kBlock)
-        key[j] = j < cnt ? (idx[nb + j] << bits) | j : 0xffffffffu;
+        key[j] = j < cnt ? ((idx[nb + j] - base) << bits) | j : 0xffffffffu;
     __syncthreads();
     for (uint32_t k = 2; k <= kWaveTileNnz; k <<= 1) {
         for (uint32_t h = k >> 1; h > 0; h >>= 1) {
@@ -228,11 +271,9 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
     o.nlong = (uint32_t)long_rows.size();
     o.nslots = nslots;
     o.sched_rows = max_rows;
-    // Sources and a kWaveSlotBits-bit slot share one 32-bit word when every
-    // vertex id fits in the remaining bits.
-    o.bits = (uint64_t)n <= (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
     const uint64_t m = off[n];
     GT_TRY(o.tiles.reserve(sizeof(Tile) * std::max<size_t>(1, tiles.size())));
+    GT_TRY(o.tbase.reserve(sizeof(uint32_t) * std::max<size_t>(2, tiles.size() + 1)));
     GT_TRY(o.long_rows.reserve(sizeof(uint32_t) * std::max<size_t>(1, long_rows.size())));
     GT_TRY(o.long_slot.reserve(sizeof(uint32_t) * long_slot.size()));
     GT_TRY(o.packed.reserve(sizeof(uint32_t) * std::max<uint64_t>(1, m)));
@@ -245,17 +286,45 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                          sizeof(uint32_t) * long_rows.size(), hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(o.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
                      hipMemcpyHostToDevice));
-    // Tiles go out in launches of at most 65535 workgroups.
+    // Each tile's smallest source is its packing base.  A source and a
+    // kWaveSlotBits-bit slot share one 32-bit word when every tile's sources
+    // span fewer than 2^(32 - kWaveSlotBits) labels: always below that many
+    // vertices; above, when no tile reaches that wide (a batch of graphs of
+    // that size, a graph with local edges).  Tiles go out in launches of at
+    // most 65535 workgroups; the widest span lands in the word after the
+    // bases.
+    uint32_t *tb = o.tbase.as<uint32_t>();
+    uint32_t *span = tb + std::max<uint32_t>(1, o.ntiles);
+    GT_HIP(hipMemsetAsync(span, 0, sizeof(uint32_t), g->stream));
+    for (uint32_t t0 = 0; t0 < o.ntiles; t0 += 65535) {
+        const uint32_t nt = std::min<uint32_t>(65535, o.ntiles - t0);
+        hipLaunchKernelGGL(tile_span, dim3(nt), dim3(kBlock), 0, g->stream,
+                           o.tiles.as<Tile>() + t0, o.idx.as<uint32_t>(), tb + t0, span);
+        GT_HIP(hipGetLastError());
+    }
+    uint32_t widest = 0;
+    if ((uint64_t)n > (1ull << (32 - kWaveSlotBits)) && o.ntiles) {
+        GT_HIP(hipMemcpyAsync(&widest, span, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+        GT_HIP(hipStreamSynchronize(g->stream));
+    }
+    o.bits = (uint64_t)widest < (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
     for (uint32_t t0 = 0; t0 < o.ntiles; t0 += 65535) {
         const uint32_t nt = std::min<uint32_t>(65535, o.ntiles - t0);
         hipLaunchKernelGGL(pack_tiles, dim3(nt), dim3(kBlock), 0, g->stream,
                            o.tiles.as<Tile>() + t0, o.idx.as<uint32_t>(), o.eid.as<uint32_t>(),
-                           o.bits, o.packed.as<uint32_t>(), o.peid.as<uint32_t>());
+                           o.bits, tb + t0, o.packed.as<uint32_t>(), o.peid.as<uint32_t>());
         GT_HIP(hipGetLastError());
     }
     GT_HIP(hipStreamSynchronize(g->stream));
     o.scheduled = true;
     return GT_OK;
+}
+
+void note_layout(gt_graph *g, const Orient &o) {
+    g->layout = gt_sweep_layout{};
+    g->layout.slot_bits = o.bits;
+    g->layout.tiles = o.ntiles;
+    g->layout.split_rows = o.nlong;
 }
 
 gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz) {
@@ -529,6 +598,13 @@ gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t) {
     clear_error();
     if (!g || !t) return fail(GT_ERR_INVALID_ARG, "null argument");
     g->tune = resolve_tuning(*t);
+    return GT_OK;
+}
+
+gt_status gt_last_layout(const gt_graph *g, gt_sweep_layout *out) {
+    clear_error();
+    if (!g || !out) return fail(GT_ERR_INVALID_ARG, "null argument");
+    *out = g->layout;
     return GT_OK;
 }
 

@@ -79,8 +79,9 @@ constexpr uint32_t kNoEdge = 0xffffffffu;
 
 // Pull-sweep wave tiles: at most kWaveTileNnz entries (16 per lane) and
 // kWaveTileRows rows (4 per lane); longer rows are split into chunks of
-// kWaveTileNnz entries.  An entry is packed as (source << kWaveSlotBits) |
-// slot when vertex ids fit in 32 - kWaveSlotBits bits.
+// kWaveTileNnz entries.  An entry is packed as ((source - base) <<
+// kWaveSlotBits) | slot, base being the tile's smallest source, when every
+// tile's sources span fewer than 2^(32 - kWaveSlotBits) labels.
 constexpr uint32_t kWaveTileNnz = 1024;
 constexpr uint32_t kWaveTileRows = 256;
 constexpr uint32_t kWaveSlotBits = 10;
@@ -128,6 +129,7 @@ struct Orient {
     DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
     DevBuf packed;           // uint32[m]: entries in tile-sorted packed order
     DevBuf peid;             // uint32[m]: edge id of each packed entry
+    DevBuf tbase;            // uint32[ntiles]: source label of packed value 0
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
     uint32_t bits = 0;       // slot bits of `packed` (0: plain sources, row order)
     uint32_t sched_rows = 0; // rows-per-tile bound the schedule was built with
@@ -251,6 +253,7 @@ struct gt_graph {
     // phase of the FP32 form.
     uint64_t sweep_hint[GT_SWEEP_KINDS + 1] = {};
     std::vector<hipEvent_t> ev;  // profiling events
+    gt_sweep_layout layout{};    // the last solve's swept orientation
     SweepState *h_state = nullptr;  // pinned mirror of the device state
 };
 
@@ -260,6 +263,8 @@ namespace gt {
 // host offsets, then the packed entry stream and its edge ids on the device.
 gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                           const std::vector<uint32_t> *bounds = nullptr);
+// Record `o` (scheduled) as the orientation the current solve sweeps.
+void note_layout(gt_graph *g, const Orient &o);
 // Build (once per tile size) the SpMM schedule of an orientation.
 gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz);
 // Allocate a graph shell with its stream.

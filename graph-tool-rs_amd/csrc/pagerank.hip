@@ -36,6 +36,7 @@ struct PrArgs {
     const uint32_t *in_off;
     const uint32_t *packed;   // packed entry stream of the wave tiles
     uint32_t bits;            // slot bits of `packed`
+    const uint32_t *tbase;    // source base of each tile
     uint32_t short_row;        // rows with fewer entries are summed by one lane
     const void *w;            // weights in packed order (value type), weighted only
     const uint32_t *out_off;  // unity out-degree source
@@ -128,7 +129,7 @@ template <typename V, typename S, typename Fin>
 __device__ __forceinline__ void sweep_body(const PrArgs &a) {
     __shared__ WaveLds<V> L;
     Fin fin(a, a.st->p_sink);
-    spmv_waves<V, Fin::kWeighted>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits,
+    spmv_waves<V, Fin::kWeighted>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase,
                                   static_cast<const S *>(a.c_cur), static_cast<const V *>(a.w),
                                   a.long_part, a.short_row, L, fin);
     double dsum = fin.dsum, ssum = fin.ssum;
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32
     const V *c = static_cast<const V *>(a.c_cur);
     fill_hot(c, nhot, hot);
     PrFin<V, WEIGHTED, DPERS, false> fin(a, a.st->p_sink);
-    spmv_waves<V, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
+    spmv_waves<V, WEIGHTED, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase, c,
                                   static_cast<const V *>(a.w), a.long_part, a.short_row, L, fin,
                                   static_cast<const V *>(hot), nhot);
     double dsum = fin.dsum, ssum = fin.ssum;
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(kHotBlock) void pr_refine_hot(const PrArgs a, uint3
     const float *c = static_cast<const float *>(a.c_cur);
     fill_hot(c, nhot, hot);
     RefineFin<false, DPERS, false> fin(a, a.st->p_sink);
-    spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, c,
+    spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase, c,
                                     static_cast<const double *>(a.w), a.long_part, a.short_row,
                                     L, fin, static_cast<const float *>(hot), nhot);
     double dsum = fin.dsum, ssum = fin.ssum;
@@ -590,6 +591,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     }
     if (n == 0) return GT_OK;
     GT_TRY(ensure_schedule(g, *S.pull, n));
+    note_layout(g, *S.pull);
     GT_TRY(prof_prepare(g));
 
     Workspace &ws = g->ws;
@@ -615,6 +617,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     a.in_off = S.pull->off.as<uint32_t>();
     a.packed = S.pull->packed.as<uint32_t>();
     a.bits = S.pull->bits;
+    a.tbase = S.pull->tbase.as<uint32_t>();
     a.short_row = (uint32_t)g->tune.short_row;
     a.damping = damping;
     a.eps_abs = params->epsilon;

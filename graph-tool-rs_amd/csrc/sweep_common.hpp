@@ -5,14 +5,17 @@
 // Work is cut into wave tiles of <= kWaveTileNnz in-entries and <= kWaveTileRows
 // rows; a row longer than kWaveTileNnz is cut into kWaveTileNnz-entry chunks
 // whose partial sums go to slots of long_part.  Inside a tile the entries are
-// stored sorted by source, each packed as (source << bits) | slot, where slot
-// is the entry's position in row order: the 64 lanes of every gather
-// instruction read ascending addresses, so lanes that touch the same cache
-// line share one request, and the gathered value is dropped into LDS at
-// `slot`, which restores row order for the row sums.  The packing costs no
-// bytes over a plain 4-byte index stream.  With bits == 0 (vertex ids too wide
-// to pack) the entries are the plain sources in row order and an entry's slot
-// is its position.
+// stored sorted by source, each packed as ((source - base) << bits) | slot,
+// where base is the tile's smallest source (tbase[tile]) and slot is the
+// entry's position in row order: the 64 lanes of every gather instruction
+// read ascending addresses, so lanes that touch the same cache line share one
+// request, and the gathered value is dropped into LDS at `slot`, which
+// restores row order for the row sums.  The packing costs no bytes over a
+// plain 4-byte index stream.  It needs every tile's sources to span fewer
+// than 2^(32 - bits) labels -- always true below 2^22 vertices, and true for
+// a batch of graphs of that size whatever their total.  Otherwise (bits == 0,
+// every base 0) the entries are the plain sources in row order and an
+// entry's slot is its position.
 //
 // Every wavefront walks its own tiles (wave w of the grid takes tiles w,
 // w + W, ...) with no workgroup barrier:
@@ -129,6 +132,7 @@ struct WavePre {
     static constexpr int K = kWaveTileNnz / kWave;
     static constexpr int R = kWaveTileRows / kWave;
     Tile t;
+    uint32_t base;  // source label of packed value 0 (wave-uniform)
     uint32_t p[K];
     uint32_t lo[R], hi[R];
 };
@@ -153,11 +157,13 @@ __device__ __forceinline__ uint32_t list_entry(const uint32_t *list, uint32_t i)
     return ((const __attribute__((address_space(4))) uint32_t *)list)[i];
 }
 
-// Loads of tile `t` (its descriptor already read).
-__device__ __forceinline__ void wave_fetch(const Tile &t, const uint32_t *__restrict__ off,
+// Loads of tile `t` (its descriptor and source base already read).
+__device__ __forceinline__ void wave_fetch(const Tile &t, uint32_t base,
+                                           const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, int lane,
                                            WavePre &f) {
     f.t = t;
+    f.base = base;
     const uint32_t nb = f.t.nnz_begin, cnt = f.t.nnz_end - f.t.nnz_begin;
     const __amdgpu_buffer_rsrc_t rp = buffer_rsrc(packed + nb, 4 * cnt);
 #pragma unroll
@@ -188,6 +194,7 @@ template <typename V, bool WEIGHTED, bool HOT = false, typename TT, int NW, bool
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
+                                           const uint32_t *__restrict__ tbase,
                                            const S *__restrict__ src, const V *__restrict__ w,
                                            double *__restrict__ long_part, uint32_t short_row,
                                            WaveLds<TT, NW, ROWS> &L, Fin &fin,
@@ -207,17 +214,20 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
     const uint32_t nwaves = gridDim.x * NW;
     WavePre cur, nxt;
     uint32_t ti = blockIdx.x * NW + wid;
-    if (PIPE && ti < ntiles) wave_fetch(wave_tile(tiles, ti), off, packed, lane, cur);
+    if (PIPE && ti < ntiles)
+        wave_fetch(wave_tile(tiles, ti), list_entry(tbase, ti), off, packed, lane, cur);
     for (; ti < ntiles; ti += nwaves) {
         // The tile: the list's ti-th entry when walking a list (finishers
         // that skip tiles only), else ti.
         const uint32_t tid = !PIPE && tlist ? list_entry(tlist, ti) : ti;
         if (!PIPE) {
             if (!fin.tile(tid)) continue;
-            wave_fetch(wave_tile(tiles, tid), off, packed, lane, cur);
+            wave_fetch(wave_tile(tiles, tid), list_entry(tbase, tid), off, packed, lane, cur);
         }
         // The next tile's descriptor (the last tile reads its own again).
-        const Tile tnext = wave_tile(tiles, ti + nwaves < ntiles ? ti + nwaves : ntiles - 1);
+        const uint32_t tin = ti + nwaves < ntiles ? ti + nwaves : ntiles - 1;
+        const Tile tnext = wave_tile(tiles, tin);
+        const uint32_t bnext = list_entry(tbase, tin);
         const Tile t = cur.t;
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
         // 1. The gathers (global ones only above the hot slice).
@@ -225,7 +235,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t j = lane + k * kWave;
-            const uint32_t sx = cur.p[k] >> bits;
+            const uint32_t sx = (cur.p[k] >> bits) + cur.base;
             g[k] = S(0);
             if (j < cnt && (!HOT || sx >= nhot)) g[k] = src[sx];
         }
@@ -239,7 +249,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             for (int k = 0; k < K; ++k)
                 wt[k] = buffer_load(rw, sizeof(V) * (lane + k * kWave), (V *)nullptr);
         }
-        if (PIPE) wave_fetch(tnext, off, packed, lane, nxt);
+        if (PIPE) wave_fetch(tnext, bnext, off, packed, lane, nxt);
         // Nothing that reads a gathered value moves above the loads issued so far.
         __builtin_amdgcn_sched_barrier(0);
         // 3. The hot slice (every lane reads LDS; a lane above it reads slot 0).
@@ -247,7 +257,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t j = lane + k * kWave;
-            const uint32_t sx = cur.p[k] >> bits;
+            const uint32_t sx = (cur.p[k] >> bits) + cur.base;
             S x = g[k];
             if (HOT) {
                 const S h = hot[sx < nhot ? sx : 0u];

@@ -59,6 +59,7 @@ struct UArgs {
     const uint32_t *off;        // pull (forward) / push (adjoint) row offsets
     const uint32_t *packed;
     uint32_t bits;
+    const uint32_t *tbase;      // source base of each tile
     uint32_t short_row;        // rows with fewer entries are summed by one lane
     const float *w;             // packed-order weights (weighted)
     const uint32_t *deg_off;    // unity out-degree: push orientation offsets
@@ -183,9 +184,9 @@ __global__ __launch_bounds__(kBlock) void u_sweep(const UArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.list_n_next = 0u;
     __shared__ WaveLds<float> L;
     Fin fin(a);
-    spmv_waves<float, WEIGHTED>(a.tiles, nt, a.off, a.packed, a.bits, a.src, a.w, a.long_part,
-                                a.short_row, L, fin, static_cast<const float *>(nullptr), 0u,
-                                a.list);
+    spmv_waves<float, WEIGHTED>(a.tiles, nt, a.off, a.packed, a.bits, a.tbase,
+                                a.src, a.w, a.long_part, a.short_row, L, fin,
+                                static_cast<const float *>(nullptr), 0u, a.list);
 }
 
 // Before the first sweep of a call: every tile listed.
@@ -373,7 +374,7 @@ template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void u_mu(const UArgs a, double *mu) {
     __shared__ WaveLds<float> L;
     UMuFin<WEIGHTED> fin(a, mu);
-    spmv_waves<float, WEIGHTED>(a.tiles, a.ntiles, a.off, a.packed, a.bits, a.src, a.w,
+    spmv_waves<float, WEIGHTED>(a.tiles, a.ntiles, a.off, a.packed, a.bits, a.tbase, a.src, a.w,
                                 a.long_part, a.short_row, L, fin);
 }
 
@@ -468,6 +469,7 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     hipStream_t s = g->stream;
     std::vector<uint32_t> bounds(voff, voff + G + 1);
     GT_TRY(ensure_schedule(g, o, n, &bounds));
+    note_layout(g, o);
     const std::vector<Tile> &T = o.h_tiles;
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<uint32_t> tg(std::max<size_t>(1, T.size())), tb(G + 1, 0), lb(G + 1, 0);
@@ -540,6 +542,7 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     a.off = o.off.as<uint32_t>();
     a.packed = o.packed.as<uint32_t>();
     a.bits = o.bits;
+    a.tbase = o.tbase.as<uint32_t>();
     a.short_row = (uint32_t)g->tune.short_row;
     a.long_rows = o.long_rows.as<uint32_t>();
     a.long_slot = o.long_slot.as<uint32_t>();

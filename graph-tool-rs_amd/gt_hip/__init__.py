@@ -320,6 +320,14 @@ class DeviceGraph:
         call("gt_sweep_stats", self._h, kind, C.byref(ms), C.byref(la), C.byref(sw))
         return ms.value, la.value, sw.value
 
+    def layout(self) -> dict:
+        """How the last pull-sweep solve laid out its orientation
+        (gt_last_layout): slot_bits 10 for the packed tile stream, 0 for
+        plain sources; the tile and split-row counts; the sliced rows."""
+        lay = _lib.GtSweepLayout()
+        call("gt_last_layout", self._h, C.byref(lay))
+        return {k: int(getattr(lay, k)) for k, _ in _lib.GtSweepLayout._fields_}
+
     def close(self) -> None:
         if self._h is not None and self._h.value:
             lib().gt_graph_free(self._h)

@@ -82,6 +82,12 @@ class GtBcStats(C.Structure):
                 ("forward_ms", C.c_double), ("backward_ms", C.c_double),
                 ("streams", C.c_uint64)]
 
+class GtSweepLayout(C.Structure):
+    _fields_ = [("slot_bits", C.c_uint32), ("tiles", C.c_uint32), ("split_rows", C.c_uint32),
+                ("slices", C.c_uint32), ("sliced_rows", C.c_uint64),
+                ("sliced_entries", C.c_uint64)]
+
+
 GraphP = C.c_void_p
 
 # name -> (argtypes); every function returns gt_status (c_int).
@@ -115,6 +121,7 @@ _PROTOS = {
     "gt_graph_set_profiling": [GraphP, C.c_int32],
     "gt_sweep_stats": [GraphP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                        C.POINTER(C.c_uint64)],
+    "gt_last_layout": [GraphP, C.POINTER(GtSweepLayout)],
     "gt_gen_er": [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p],
     "gt_pagerank_uneven": [GraphP, C.c_uint32, u32p, C.POINTER(C.c_double), C.POINTER(GtProp),
                            C.POINTER(GtPowerIteration), C.c_void_p, C.c_int32,

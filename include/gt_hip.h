@@ -409,6 +409,23 @@ gt_status gt_graph_set_profiling(gt_graph *g, int32_t enable);
 gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms,
                          uint64_t *launches, uint64_t *last_sweeps);
 
+/* Extension.  How the last pull-sweep solve on g (PageRank, eigenvector,
+ * adjoint or uneven batch) laid out the orientation it swept.  slot_bits is
+ * 10 when every wave tile's entries are packed as (source - base) << 10 |
+ * slot (each tile's sources span fewer than 2^22 labels: always below
+ * 2^22 vertices), 0 when they are plain 32-bit sources in row order.
+ * sliced_rows / sliced_entries / slices describe the rows whose entries are
+ * gathered from on-chip slices of the source vector (0 when none were). */
+typedef struct gt_sweep_layout {
+    uint32_t slot_bits;
+    uint32_t tiles;
+    uint32_t split_rows;
+    uint32_t slices;
+    uint64_t sliced_rows;
+    uint64_t sliced_entries;
+} gt_sweep_layout;
+gt_status gt_last_layout(const gt_graph *g, gt_sweep_layout *out);
+
 /* ------------------------------------------------------------- host side */
 
 /* Extension: seeded workload generators (counter-based SplitMix64, the
