@@ -65,18 +65,30 @@ struct AdjFin : TileAll {
     double dsum = 0.0;
     double psum = 0.0;
     __device__ AdjFin(const AdjArgs &args, double p) : a(args), pl(p) {}
-    __device__ __forceinline__ void operator()(uint32_t s, double acc) {
-        const V *lc = static_cast<const V *>(a.l_cur);
-        V *ln = static_cast<V *>(a.l_next);
-        const double dg =
-            WEIGHTED ? a.deg[s] : static_cast<double>(a.row_off[s + 1] - a.row_off[s]);
-        const double mu = dg != 0.0 ? acc / dg : 0.0;
-        const double v = a.cvec[s] + a.damping * mu + (dg == 0.0 ? a.damping * pl : 0.0);
-        const V vv = static_cast<V>(v);
-        dsum += fabs(static_cast<double>(vv) - static_cast<double>(lc[s]));
-        ln[s] = vv;
-        psum += (DPERS ? a.pers[s] : a.pval) * static_cast<double>(vv);
+    // What row s reads: its degree, dL/dx, lambda and personalisation.
+    static constexpr bool kLoadsFirst = true;
+    struct In {
+        double dg, cv, pv;
+        V lc;
+    };
+    __device__ __forceinline__ In load(uint32_t s) const {
+        In in;
+        in.dg = WEIGHTED ? a.deg[s] : static_cast<double>(a.row_off[s + 1] - a.row_off[s]);
+        in.cv = a.cvec[s];
+        in.pv = DPERS ? a.pers[s] : a.pval;
+        in.lc = static_cast<const V *>(a.l_cur)[s];
+        return in;
     }
+    __device__ __forceinline__ void finish(uint32_t s, double acc, const In &in) {
+        V *ln = static_cast<V *>(a.l_next);
+        const double mu = in.dg != 0.0 ? acc / in.dg : 0.0;
+        const double v = in.cv + a.damping * mu + (in.dg == 0.0 ? a.damping * pl : 0.0);
+        const V vv = static_cast<V>(v);
+        dsum += fabs(static_cast<double>(vv) - static_cast<double>(in.lc));
+        ln[s] = vv;
+        psum += in.pv * static_cast<double>(vv);
+    }
+    __device__ __forceinline__ void operator()(uint32_t s, double acc) { finish(s, acc, load(s)); }
 };
 
 // The SpMV of one transposed sweep (split rows aside), with per-workgroup

@@ -73,16 +73,31 @@ struct EvFin : TileAll {
     double sq = 0.0, dl = 0.0;
     __device__ EvFin(const EvArgs &args, double c, double p, bool f)
         : a(args), sc_cur(c), sc_prev(p), first(f) {}
-    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
-        if (MASK && !a.active[i]) {
-            a.y_next[i] = a.y_cur[i];
+    // What row i reads: its last two iterates (the first launch has one and
+    // reads it twice).
+    static constexpr bool kLoadsFirst = true;
+    struct In {
+        double yc, yp;
+        bool act;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const {
+        In in;
+        in.yc = a.y_cur[i];
+        in.yp = (first ? a.y_cur : a.y_prev)[i];
+        in.act = MASK ? a.active[i] != 0 : true;
+        return in;
+    }
+    __device__ __forceinline__ void finish(uint32_t i, double acc, const In &in) {
+        if (MASK && !in.act) {
+            a.y_next[i] = in.yc;
             return;
         }
         const double s = acc / sc_cur;
         a.y_next[i] = s;
         sq += s * s;
-        if (!first) dl += fabs(a.y_cur[i] / sc_cur - a.y_prev[i] / sc_prev);
+        if (!first) dl += fabs(in.yc / sc_cur - in.yp / sc_prev);
     }
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) { finish(i, acc, load(i)); }
 };
 
 // The SpMV of one sweep (split rows aside), with per-workgroup squared-norm

@@ -83,6 +83,9 @@ gt_tuning resolve_tuning(const gt_tuning &t) {
                                       : (int)kMmTileNnz;
     r.mm_blocks_per_cu = t.mm_blocks_per_cu > 0 ? std::min(t.mm_blocks_per_cu, 8) : 8;
     r.sweep_hot_kb = t.sweep_hot_kb < 0 ? -1 : t.sweep_hot_kb > 0 ? std::min(t.sweep_hot_kb, 128) : 128;
+    r.sliced_min_len =
+        t.sliced_min_len > 0 ? std::max(2, std::min(t.sliced_min_len, 1024)) : -1;
+    r.sliced_share = t.sliced_share > 0 ? std::min(t.sliced_share, 99) : 0;
     return r;
 }
 
@@ -286,36 +289,10 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                          sizeof(uint32_t) * long_rows.size(), hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(o.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
                      hipMemcpyHostToDevice));
-    // Each tile's smallest source is its packing base.  A source and a
-    // kWaveSlotBits-bit slot share one 32-bit word when every tile's sources
-    // span fewer than 2^(32 - kWaveSlotBits) labels: always below that many
-    // vertices; above, when no tile reaches that wide (a batch of graphs of
-    // that size, a graph with local edges).  Tiles go out in launches of at
-    // most 65535 workgroups; the widest span lands in the word after the
-    // bases.
-    uint32_t *tb = o.tbase.as<uint32_t>();
-    uint32_t *span = tb + std::max<uint32_t>(1, o.ntiles);
-    GT_HIP(hipMemsetAsync(span, 0, sizeof(uint32_t), g->stream));
-    for (uint32_t t0 = 0; t0 < o.ntiles; t0 += 65535) {
-        const uint32_t nt = std::min<uint32_t>(65535, o.ntiles - t0);
-        hipLaunchKernelGGL(tile_span, dim3(nt), dim3(kBlock), 0, g->stream,
-                           o.tiles.as<Tile>() + t0, o.idx.as<uint32_t>(), tb + t0, span);
-        GT_HIP(hipGetLastError());
-    }
-    uint32_t widest = 0;
-    if ((uint64_t)n > (1ull << (32 - kWaveSlotBits)) && o.ntiles) {
-        GT_HIP(hipMemcpyAsync(&widest, span, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
-        GT_HIP(hipStreamSynchronize(g->stream));
-    }
-    o.bits = (uint64_t)widest < (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
-    for (uint32_t t0 = 0; t0 < o.ntiles; t0 += 65535) {
-        const uint32_t nt = std::min<uint32_t>(65535, o.ntiles - t0);
-        hipLaunchKernelGGL(pack_tiles, dim3(nt), dim3(kBlock), 0, g->stream,
-                           o.tiles.as<Tile>() + t0, o.idx.as<uint32_t>(), o.eid.as<uint32_t>(),
-                           o.bits, tb + t0, o.packed.as<uint32_t>(), o.peid.as<uint32_t>());
-        GT_HIP(hipGetLastError());
-    }
-    GT_HIP(hipStreamSynchronize(g->stream));
+    GT_TRY(pack_stream(g, o.tiles.as<Tile>(), o.ntiles, o.idx.as<uint32_t>(),
+                       o.eid.as<uint32_t>(), (uint64_t)n > (1ull << (32 - kWaveSlotBits)),
+                       o.tbase.as<uint32_t>(), o.packed.as<uint32_t>(), o.peid.as<uint32_t>(),
+                       &o.bits));
     o.scheduled = true;
     return GT_OK;
 }
@@ -325,6 +302,39 @@ void note_layout(gt_graph *g, const Orient &o) {
     g->layout.slot_bits = o.bits;
     g->layout.tiles = o.ntiles;
     g->layout.split_rows = o.nlong;
+}
+
+gt_status pack_stream(gt_graph *g, const Tile *tiles, uint32_t ntiles, const uint32_t *idx,
+                      const uint32_t *eid, bool wide, uint32_t *tbase, uint32_t *packed,
+                      uint32_t *peid, uint32_t *bits) {
+    // Each tile's smallest source is its packing base.  A source and a
+    // kWaveSlotBits-bit slot share one 32-bit word when every tile's sources
+    // span fewer than 2^(32 - kWaveSlotBits) labels: always below that many
+    // vertices; above, when no tile reaches that wide (a batch of graphs of
+    // that size, a graph with local edges).  Tiles go out in launches of at
+    // most 65535 workgroups; the widest span lands in tbase[ntiles].
+    uint32_t *span = tbase + ntiles;
+    GT_HIP(hipMemsetAsync(span, 0, sizeof(uint32_t), g->stream));
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 65535) {
+        const uint32_t nt = std::min<uint32_t>(65535, ntiles - t0);
+        hipLaunchKernelGGL(tile_span, dim3(nt), dim3(kBlock), 0, g->stream, tiles + t0, idx,
+                           tbase + t0, span);
+        GT_HIP(hipGetLastError());
+    }
+    uint32_t widest = 0;
+    if (wide && ntiles) {
+        GT_HIP(hipMemcpyAsync(&widest, span, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+        GT_HIP(hipStreamSynchronize(g->stream));
+    }
+    *bits = (uint64_t)widest < (1ull << (32 - kWaveSlotBits)) ? kWaveSlotBits : 0;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 65535) {
+        const uint32_t nt = std::min<uint32_t>(65535, ntiles - t0);
+        hipLaunchKernelGGL(pack_tiles, dim3(nt), dim3(kBlock), 0, g->stream, tiles + t0, idx, eid,
+                           *bits, tbase + t0, packed, peid);
+        GT_HIP(hipGetLastError());
+    }
+    GT_HIP(hipStreamSynchronize(g->stream));
+    return GT_OK;
 }
 
 gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz) {

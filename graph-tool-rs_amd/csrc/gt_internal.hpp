@@ -150,6 +150,51 @@ struct Orient {
     DevBuf first_len;
 };
 
+// One stretch of a workgroup's share of the sliced pass: tiles [t0, t1) of
+// one slice, whose sources are the labels [base, base + len).
+struct SliceSeg {
+    uint32_t base, len, t0, t1;
+};
+
+// A relabelled pull orientation split for the sliced sweep.  Sources are
+// cut into slices of `width` labels: slice 0 is the hot slice every light
+// workgroup holds in LDS; the "warm" slices after it are each held by the
+// workgroups that sum the warm pieces.  A row with at least min_len entries
+// ("heavy") is cut into one warm piece per warm slice it has entries in and
+// one main piece with the rest (a piece longer than kWaveTileNnz into
+// chunks), each summed into its own partial.  A heavy row's slots -- main
+// piece first, then warm pieces in slice order -- are the fold's split-row
+// slots, read through long_idx.  Light rows and main pieces are swept by the
+// launch's other workgroups, with the hot slice and global gathers.
+struct Sliced {
+    bool ready = false;
+    bool active = false;     // some row is heavy (else the plain hot sweep runs)
+    uint32_t width = 0, min_len = 0, sched_rows = 0, share = 0, grid = 0;
+    // Light rows: wave tiles over runs of light rows, packed like an
+    // orientation's stream at the orientation's entry positions.
+    DevBuf ltiles, ltbase, lpacked, lpeid;
+    uint32_t nltiles = 0, lbits = 0;
+    uint64_t light_entries = 0;
+    // Heavy rows, their slot ranges, and the partial each slot reads
+    // (long_idx[slot]; nout partials in all).
+    DevBuf long_rows, long_slot, long_idx;
+    uint32_t nlong = 0, nslots = 0, nout = 0;
+    // Warm pieces: wave tiles whose rows are pieces (hoff: piece offsets
+    // into hpacked; a piece's sum goes to the partial of its index) or one
+    // chunk (row_end == kSplit, row_begin = its partial); sources in hpacked
+    // are relative to the slice.  Warm slices are 1 .. warm_end - 1.
+    DevBuf htiles, htbase, hoff, hpacked, hpeid;
+    uint32_t nhtiles = 0, npieces = 0, nslices = 0, warm_end = 0;
+    uint64_t heavy_entries = 0;  // entries of the warm pieces
+    // Main pieces: a heavy row's entries outside the warm slices, swept with
+    // the light rows; a piece's sum goes to partial mpart + its index.
+    DevBuf mtiles, mtbase, moff, mpacked, mpeid;
+    uint32_t nmtiles = 0, mbits = 0, mpart = 0;
+    // Workgroup b < heavy_wgs takes segments seg[wseg[b] .. wseg[b + 1]).
+    DevBuf seg, wseg;
+    uint32_t heavy_wgs = 0;
+};
+
 // A pull orientation in a locality-friendly labelling: vertices renumbered by
 // descending gather frequency (the row length of the opposite orientation:
 // how often a vertex appears as an entry), rows in the new order and each
@@ -161,6 +206,7 @@ struct Relabeled {
     DevBuf iperm;      // uint32[n]: new label -> old
     Orient pull;       // relabelled pull orientation (eid = original edge ids)
     DevBuf push_off;   // uint32[n+1]: scanned push-row lengths in new order
+    Sliced sl[2];      // sliced sweeps: [0] 4-byte values, [1] 8-byte values
 };
 
 struct Workspace {
@@ -280,6 +326,18 @@ int occupancy_cap(const void *kernel, int per_cu = 8);
 // live in (R-MAT scale 22: 3 per CU is the fastest f32 and f64 sweep, 8 per
 // CU is 40% slower).
 constexpr int kSweepBlocksPerCU = 3;
+// The sliced split of R's pull orientation for `width`-label slices (value
+// size `vbytes`), built once per width, row threshold, tile rows and grid.
+gt_status ensure_sliced(gt_graph *g, Relabeled &R, uint32_t width, uint32_t vbytes, int grid,
+                        Sliced **out);
+// Sort and pack the entries of `ntiles` wave tiles (their idx / eid at the
+// tiles' entry positions) into `packed` / `peid`, each tile's sources
+// relative to its smallest (tbase).  Returns the slot bits: kWaveSlotBits,
+// or 0 (plain sources, bases 0) when some tile spans 2^22 labels or more,
+// which is looked for only when `wide` is set.  Waits for the stream.
+gt_status pack_stream(gt_graph *g, const Tile *tiles, uint32_t ntiles, const uint32_t *idx,
+                      const uint32_t *eid, bool wide, uint32_t *tbase, uint32_t *packed,
+                      uint32_t *peid, uint32_t *bits);
 // Default short-row threshold: rows with fewer entries are summed by one lane
 // (R-MAT 22: 64 beats 32 by 6 % on the FP32 sweep; 96..256 are within 1 %).
 constexpr uint32_t kShortRowDefault = 64;

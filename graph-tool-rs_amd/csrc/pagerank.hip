@@ -58,6 +58,7 @@ struct PrArgs {
     double *long_part;        // [nslots]
     const uint32_t *long_rows;
     const uint32_t *long_slot;
+    const uint32_t *long_idx;  // null: a split row's slots are its partials
     SweepState *st;
 };
 
@@ -69,29 +70,43 @@ struct PrFin : TileAll {
     double dsum = 0.0;
     double ssum = 0.0;
     __device__ PrFin(const PrArgs &args, double ps) : a(args), p_sink(ps) {}
-    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
-        const V *xc = static_cast<const V *>(a.x_cur);
+    // What row i reads: its iterate, out-degree and personalisation.
+    static constexpr bool kLoadsFirst = false;
+    struct In {
+        V xc;
+        double dg;
+        double pv;
+        bool act;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const {
+        In in;
+        in.xc = static_cast<const V *>(a.x_cur)[i];
+        in.dg = WEIGHTED ? a.deg[i] : static_cast<double>(a.out_off[i + 1] - a.out_off[i]);
+        in.pv = DPERS ? a.pers[i] : a.pval;
+        in.act = MASK ? a.active[i] != 0 : true;
+        return in;
+    }
+    __device__ __forceinline__ void finish(uint32_t i, double acc, const In &in) {
         V *xn = static_cast<V *>(a.x_next);
         V *cn = static_cast<V *>(a.c_next);
-        if (MASK && !a.active[i]) {
-            xn[i] = xc[i];
+        if (MASK && !in.act) {
+            xn[i] = in.xc;
             cn[i] = V(0);
             return;
         }
-        const double pv = DPERS ? a.pers[i] : a.pval;
+        const double pv = in.pv;
         const double nv = (1.0 - a.damping) * pv + a.damping * (p_sink * pv + acc);
         const V nvv = static_cast<V>(nv);
-        dsum += fabs(static_cast<double>(nvv) - static_cast<double>(xc[i]));
+        dsum += fabs(static_cast<double>(nvv) - static_cast<double>(in.xc));
         xn[i] = nvv;
-        const double dg =
-            WEIGHTED ? a.deg[i] : static_cast<double>(a.out_off[i + 1] - a.out_off[i]);
-        if (dg != 0.0) {
-            cn[i] = static_cast<V>(static_cast<double>(nvv) / dg);
+        if (in.dg != 0.0) {
+            cn[i] = static_cast<V>(static_cast<double>(nvv) / in.dg);
         } else {
             cn[i] = V(0);
             ssum += static_cast<double>(nvv);
         }
     }
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) { finish(i, acc, load(i)); }
 };
 
 // The last sweep of the FP32 form, in FP64: the FP32 phase's final
@@ -107,18 +122,30 @@ struct RefineFin : TileAll {
     double dsum = 0.0;
     double ssum = 0.0;
     __device__ RefineFin(const PrArgs &args, double ps) : a(args), p_sink(ps) {}
-    __device__ __forceinline__ void operator()(uint32_t i, double acc) {
-        const float *xc = static_cast<const float *>(a.x_cur);
+    static constexpr bool kLoadsFirst = false;
+    struct In {
+        float xc;
+        double pv;
+        bool act;
+    };
+    __device__ __forceinline__ In load(uint32_t i) const {
+        In in;
+        in.xc = static_cast<const float *>(a.x_cur)[i];
+        in.pv = DPERS ? a.pers[i] : a.pval;
+        in.act = MASK ? a.active[i] != 0 : true;
+        return in;
+    }
+    __device__ __forceinline__ void finish(uint32_t i, double acc, const In &in) {
         double *xn = static_cast<double *>(a.x_next);
-        if (MASK && !a.active[i]) {
-            xn[i] = static_cast<double>(xc[i]);
+        if (MASK && !in.act) {
+            xn[i] = static_cast<double>(in.xc);
             return;
         }
-        const double pv = DPERS ? a.pers[i] : a.pval;
-        const double nv = (1.0 - a.damping) * pv + a.damping * (p_sink * pv + acc);
-        dsum += fabs(nv - static_cast<double>(xc[i]));
+        const double nv = (1.0 - a.damping) * in.pv + a.damping * (p_sink * in.pv + acc);
+        dsum += fabs(nv - static_cast<double>(in.xc));
         xn[i] = nv;
     }
+    __device__ __forceinline__ void operator()(uint32_t i, double acc) { finish(i, acc, load(i)); }
 };
 
 // The SpMV of one sweep: every row but the split ones, with per-workgroup
@@ -151,7 +178,8 @@ __device__ __forceinline__ void fold_body(const PrArgs &a, uint32_t sgrid) {
     const int tid = threadIdx.x;
     Fin lf(a, a.st->p_sink);
     for (uint32_t k = blockIdx.x * kBlock + tid; k < a.nlong; k += gridDim.x * kBlock)
-        lf(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
+        lf(a.long_rows[k], a.long_idx ? long_row_sum_idx(a.long_part, a.long_slot, a.long_idx, k)
+                                      : long_row_sum(a.long_part, a.long_slot, k));
     double d = lf.dsum, s = lf.ssum;
     block_sum2(d, s, red);
     double *fp = a.partials + 2 * kMaxSweepGrid;
@@ -212,6 +240,38 @@ __global__ __launch_bounds__(kHotBlock) void pr_sweep_hot(const PrArgs a, uint32
     }
 }
 
+// The sliced sweep (relabelled, unweighted): workgroups below
+// sa.heavy_wgs sum the long rows' pieces from LDS slices of the
+// contributions (their partials are zero: those rows are finished by the
+// fold), the others sweep the light rows with the hot slice.
+template <typename V, bool DPERS>
+__global__ __launch_bounds__(kHotBlock) void pr_sweep_sliced(const PrArgs a, const SlicedArgs sa,
+                                                             uint32_t nhot) {
+    if (a.st->done) return;
+    __shared__ WaveLds<V, kHotWaves, false> L;
+    __shared__ V hot[hot_max<V>()];
+    const V *c = static_cast<const V *>(a.c_cur);
+    if (blockIdx.x < sa.heavy_wgs) {
+        sliced_pieces<V, V>(sa, c, a.long_part, a.short_row, L, hot);
+        if (threadIdx.x == 0) {
+            a.partials[blockIdx.x] = 0.0;
+            a.partials[gridDim.x + blockIdx.x] = 0.0;
+        }
+        return;
+    }
+    fill_hot(c, nhot, hot);
+    PrFin<V, false, DPERS, false> fin(a, a.st->p_sink);
+    sliced_light<V, V>(sa, a.in_off, c, a.long_part, a.short_row, L, fin,
+                       static_cast<const V *>(hot), nhot, blockIdx.x - sa.heavy_wgs,
+                       gridDim.x - sa.heavy_wgs);
+    double dsum = fin.dsum, ssum = fin.ssum;
+    block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = dsum;
+        a.partials[gridDim.x + blockIdx.x] = ssum;
+    }
+}
+
 template <typename V, bool WEIGHTED, bool DPERS, bool MASK>
 __global__ __launch_bounds__(kBlock) void pr_fold(const PrArgs a, uint32_t sgrid) {
     if (a.st->done) return;
@@ -237,6 +297,35 @@ __global__ __launch_bounds__(kHotBlock) void pr_refine_hot(const PrArgs a, uint3
     spmv_waves<double, false, true>(a.tiles, a.ntiles, a.in_off, a.packed, a.bits, a.tbase, c,
                                     static_cast<const double *>(a.w), a.long_part, a.short_row,
                                     L, fin, static_cast<const float *>(hot), nhot);
+    double dsum = fin.dsum, ssum = fin.ssum;
+    block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = dsum;
+        a.partials[gridDim.x + blockIdx.x] = ssum;
+    }
+}
+
+// The refinement with the sliced split of the FP32 sweeps (FP32 tiles and
+// slices, FP64 sums).
+template <bool DPERS>
+__global__ __launch_bounds__(kHotBlock) void pr_refine_sliced(const PrArgs a, const SlicedArgs sa,
+                                                              uint32_t nhot) {
+    __shared__ WaveLds<float, kHotWaves, false> L;
+    __shared__ float hot[kHotMax];
+    const float *c = static_cast<const float *>(a.c_cur);
+    if (blockIdx.x < sa.heavy_wgs) {
+        sliced_pieces<double, float>(sa, c, a.long_part, a.short_row, L, hot);
+        if (threadIdx.x == 0) {
+            a.partials[blockIdx.x] = 0.0;
+            a.partials[gridDim.x + blockIdx.x] = 0.0;
+        }
+        return;
+    }
+    fill_hot(c, nhot, hot);
+    RefineFin<false, DPERS, false> fin(a, a.st->p_sink);
+    sliced_light<double, float>(sa, a.in_off, c, a.long_part, a.short_row, L, fin,
+                                static_cast<const float *>(hot), nhot, blockIdx.x - sa.heavy_wgs,
+                                gridDim.x - sa.heavy_wgs);
     double dsum = fin.dsum, ssum = fin.ssum;
     block_sum2_waves<kHotWaves>(dsum, ssum, L.red);
     if (threadIdx.x == 0) {
@@ -361,6 +450,41 @@ int launch_sweep_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
     hipLaunchKernelGGL((pr_sweep_hot<V, W, P>), dim3(sg), dim3(kHotBlock), 0, s, a, nhot);
     return sg;
 }
+SlicedArgs sliced_args(const Sliced &S) {
+    SlicedArgs sa;
+    sa.htiles = S.htiles.as<Tile>();
+    sa.htbase = S.htbase.as<uint32_t>();
+    sa.hoff = S.hoff.as<uint32_t>();
+    sa.hpacked = S.hpacked.as<uint32_t>();
+    sa.seg = S.seg.as<SliceSeg>();
+    sa.wseg = S.wseg.as<uint32_t>();
+    sa.heavy_wgs = S.heavy_wgs;
+    sa.nltiles = S.nltiles;
+    sa.ltiles = S.ltiles.as<Tile>();
+    sa.ltbase = S.ltbase.as<uint32_t>();
+    sa.lpacked = S.lpacked.as<uint32_t>();
+    sa.lbits = S.lbits;
+    sa.nmtiles = S.nmtiles;
+    sa.mtiles = S.mtiles.as<Tile>();
+    sa.mtbase = S.mtbase.as<uint32_t>();
+    sa.moff = S.moff.as<uint32_t>();
+    sa.mpacked = S.mpacked.as<uint32_t>();
+    sa.mbits = S.mbits;
+    sa.mpart = S.mpart;
+    return sa;
+}
+// The sliced sweep on `grid` workgroups (the grid the split was built for).
+template <typename V>
+int dispatch_sweep_sliced(const PrPlan &p, const PrArgs &a, const SlicedArgs &sa, uint32_t nhot,
+                          int grid, hipStream_t s) {
+    if (p.dpers)
+        hipLaunchKernelGGL((pr_sweep_sliced<V, true>), dim3(grid), dim3(kHotBlock), 0, s, a, sa,
+                           nhot);
+    else
+        hipLaunchKernelGGL((pr_sweep_sliced<V, false>), dim3(grid), dim3(kHotBlock), 0, s, a, sa,
+                           nhot);
+    return grid;
+}
 template <typename V>
 int dispatch_sweep_hot(const PrPlan &p, const PrArgs &a, uint32_t nhot, hipStream_t s) {
     switch ((p.weighted ? 2 : 0) | (p.dpers ? 1 : 0)) {
@@ -420,9 +544,31 @@ void launch_refine_hot(const PrArgs &a, uint32_t nhot, hipStream_t s) {
     hipLaunchKernelGGL((pr_refine_fold<false, P, false>), dim3(fold_grid(a.nlong)), dim3(kBlock),
                        0, s, a, (uint32_t)sg);
 }
-// `nhot` > 0: the hot-slice refinement (unweighted, unfiltered, relabelled).
+// `nhot` > 0: the hot-slice refinement (unweighted, unfiltered, relabelled);
+// with `sl` (the FP32 sweeps' sliced split, active), the sliced one.
 void dispatch_refine(const PrPlan &p, const PrArgs &a, int per_cu, uint32_t nhot,
-                     hipStream_t s) {
+                     const Sliced *sl, hipStream_t s) {
+    if (nhot && sl) {
+        PrArgs b = a;
+        b.nlong = sl->nlong;
+        b.long_rows = sl->long_rows.as<uint32_t>();
+        b.long_slot = sl->long_slot.as<uint32_t>();
+        b.long_idx = sl->long_idx.as<uint32_t>();
+        const SlicedArgs sa = sliced_args(*sl);
+        const int grid = (int)sl->grid;
+        if (p.dpers) {
+            hipLaunchKernelGGL((pr_refine_sliced<true>), dim3(grid), dim3(kHotBlock), 0, s, b, sa,
+                               nhot);
+            hipLaunchKernelGGL((pr_refine_fold<false, true, false>), dim3(fold_grid(b.nlong)),
+                               dim3(kBlock), 0, s, b, (uint32_t)grid);
+        } else {
+            hipLaunchKernelGGL((pr_refine_sliced<false>), dim3(grid), dim3(kHotBlock), 0, s, b,
+                               sa, nhot);
+            hipLaunchKernelGGL((pr_refine_fold<false, false, false>), dim3(fold_grid(b.nlong)),
+                               dim3(kBlock), 0, s, b, (uint32_t)grid);
+        }
+        return;
+    }
     if (nhot) {
         if (p.dpers)
             launch_refine_hot<true>(a, nhot, s);
@@ -520,6 +666,7 @@ struct PrSolve {
     // in new labels, rank64 stays in the caller's.
     const uint32_t *perm = nullptr;
     const uint32_t *iperm = nullptr;
+    Relabeled *rel = nullptr;
 };
 
 unsigned vblocks(uint32_t n) {
@@ -562,6 +709,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
         Relabeled *R = nullptr;
         GT_TRY(ensure_relabeled(g, reversed ? 1 : 0, &R));
         S.pull = &R->pull;
+        S.rel = R;
         S.perm = R->perm.as<uint32_t>();
         S.iperm = R->iperm.as<uint32_t>();
         a.out_off = R->push_off.as<uint32_t>();
@@ -631,6 +779,28 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     return GT_OK;
 }
 
+// The sliced split for the sweeps of value type V over `nhot`-label slices,
+// or null: weighted, filtered, the tuning turns it off, the vector fits one
+// slice, or no row is long enough.  Grows the partial slots to its pieces.
+template <typename V>
+gt_status sliced_for(PrSolve &S, uint32_t nhot, const Sliced **out) {
+    *out = nullptr;
+    gt_graph *g = S.g;
+    if (!nhot || !S.rel || S.plan.weighted || S.plan.masked || g->tune.sliced_min_len < 0 ||
+        g->n <= nhot)
+        return GT_OK;
+    const int grid = occupancy_cap((const void *)pr_sweep_sliced<V, false>, 1);
+    Sliced *SL = nullptr;
+    GT_TRY(ensure_sliced(g, *S.rel, nhot, sizeof(V), grid, &SL));
+    if (!SL->active) return GT_OK;
+    GT_TRY(g->ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, SL->nout)));
+    g->layout.slices = SL->nslices;
+    g->layout.sliced_rows = SL->nlong;
+    g->layout.sliced_entries = SL->heavy_entries;
+    *out = SL;
+    return GT_OK;
+}
+
 // Init + sweeps in value type V on buffers x / c; returns the sweep count
 // reached (st->iter) and leaves the latest iterate in x[count & 1].
 template <typename V>
@@ -639,6 +809,22 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     gt_graph *g = S.g;
     hipStream_t s = g->stream;
     g->prof_kind = std::is_same<V, double>::value ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
+    // The hot slice serves sweeps in relabelled order (the most gathered
+    // vertices first); masked views keep the caller's labels.
+    const uint32_t nhot =
+        S.iperm && !S.plan.masked ? hot_len<V>(g->tune.sweep_hot_kb, g->n) : 0u;
+    // With a vector wider than the slice, the sliced split serves the long
+    // rows from LDS slices of that width (unweighted sweeps).
+    const Sliced *SL = nullptr;
+    GT_TRY(sliced_for<V>(S, nhot, &SL));
+    if (SL) {
+        a.nlong = SL->nlong;
+        a.long_rows = SL->long_rows.as<uint32_t>();
+        a.long_slot = SL->long_slot.as<uint32_t>();
+        a.long_idx = SL->long_idx.as<uint32_t>();
+        a.long_part = g->ws.long_part.as<double>();
+    }
+    const SlicedArgs sa = SL ? sliced_args(*SL) : SlicedArgs{};
     a.x_next = x[0];
     a.c_next = c[0];
     dispatch_init<V>(S.plan, a, S.rank64, had, g->n, iter0, s);
@@ -646,18 +832,15 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
     uint64_t launched = 0;
     PrArgs b = a;
     int sg = 0;
-    // The hot slice serves sweeps in relabelled order (the most gathered
-    // vertices first); masked views keep the caller's labels.
-    const uint32_t nhot =
-        S.iperm && !S.plan.masked ? hot_len<V>(g->tune.sweep_hot_kb, g->n) : 0u;
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
         b.x_cur = x[par];
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        sg = nhot ? dispatch_sweep_hot<V>(S.plan, b, nhot, s)
-                  : dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
+        sg = SL     ? dispatch_sweep_sliced<V>(S.plan, b, sa, nhot, (int)SL->grid, s)
+             : nhot ? dispatch_sweep_hot<V>(S.plan, b, nhot, s)
+                    : dispatch_sweep<V>(S.plan, b, g->tune.sweep_blocks_per_cu, s);
     };
     auto fold = [&](uint64_t) { dispatch_fold<V>(S.plan, b, sg, s); };
     if (cap == 1) {
@@ -716,8 +899,10 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
     const size_t vb = sizeof(double) * (size_t)n;
     GT_TRY(ws.x[0].reserve(vb));
     GT_TRY(ws.x[1].reserve(vb));
-    GT_TRY(ws.c[0].reserve(vb));
-    GT_TRY(ws.c[1].reserve(vb));
+    // Contributions carry four spare values: a slice fill of the sliced
+    // sweep reads whole 16-byte groups.
+    GT_TRY(ws.c[0].reserve(vb + 32));
+    GT_TRY(ws.c[1].reserve(vb + 32));
     if (S.plan.weighted) {
         GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
         S.base.w = ws.wcsc64.ptr;
@@ -775,7 +960,7 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     hipStream_t s = g->stream;
     for (int k = 0; k < 2; ++k) {
         GT_TRY(ws.x32[k].reserve(sizeof(float) * n));
-        GT_TRY(ws.c32[k].reserve(sizeof(float) * n));
+        GT_TRY(ws.c32[k].reserve(sizeof(float) * ((size_t)n + 4)));
     }
     float *x32[2] = {ws.x32[0].as<float>(), ws.x32[1].as<float>()};
     float *c32[2] = {ws.c32[0].as<float>(), ws.c32[1].as<float>()};
@@ -816,7 +1001,10 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         const uint32_t nhot = S.iperm && !S.plan.masked && !S.plan.weighted
                                   ? hot_len<float>(g->tune.sweep_hot_kb, n)
                                   : 0u;
-        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, nhot, s);
+        const Sliced *SL = nullptr;
+        if (nhot) GT_TRY(sliced_for<float>(S, nhot, &SL));
+        if (SL) ar.long_part = ws.long_part.as<double>();
+        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, nhot, SL, s);
         GT_TRY(prof_end(g, 0));
         GT_HIP(hipGetLastError());
         GT_TRY(prof_flush(g, 1));

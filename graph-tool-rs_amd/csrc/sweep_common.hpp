@@ -97,6 +97,27 @@ struct TileAll {
     __device__ __forceinline__ void tile_end(uint32_t) const {}
 };
 
+// A finisher may split its per-row work in two: `In load(row)` issues the
+// row's loads (any valid label) and `finish(row, sum, in)` completes it.  The
+// hot-slice walk (register row sums) then issues a tile's row loads with
+// its gathers, so their latency overlaps the gathers' instead of following
+// the row sums, round after round: before the gathers, only the rounds the
+// tile has (kLoadsFirst), or after them, all four without a branch.
+template <typename F, typename = void>
+struct has_row_loads : std::false_type {};
+template <typename F>
+struct has_row_loads<F, std::void_t<typename F::In>> : std::true_type {};
+template <typename F, bool>
+struct RowIn {
+    struct type {};
+    static constexpr bool first = false;
+};
+template <typename F>
+struct RowIn<F, true> {
+    using type = typename F::In;
+    static constexpr bool first = F::kLoadsFirst;
+};
+
 // Orders this wavefront's LDS accesses: writes before the call are visible to
 // its lanes' reads after it.
 __device__ __forceinline__ void wave_sync() {
@@ -189,8 +210,13 @@ __device__ __forceinline__ void wave_fetch(const Tile &t, uint32_t base,
 // override tile()), a wavefront loads its next tile's packed words and row
 // offsets right after the current tile's gathers, so their latency overlaps
 // the current tile's sums.
-template <typename V, bool WEIGHTED, bool HOT = false, typename TT, int NW, bool ROWS,
-          typename S, typename Fin>
+//
+// LDSONLY: every source is below `nhot` (a slice of the sliced sweep, `src`
+// and `hot` both starting at the slice), so no global gather is issued.
+// A wavefront walks tiles first, first + stride, ...; by default first is its
+// index in the grid and stride the grid's wavefront count.
+template <typename V, bool WEIGHTED, bool HOT = false, bool LDSONLY = false, typename TT, int NW,
+          bool ROWS, typename S, typename Fin>
 __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint32_t ntiles,
                                            const uint32_t *__restrict__ off,
                                            const uint32_t *__restrict__ packed, uint32_t bits,
@@ -199,21 +225,27 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                                            double *__restrict__ long_part, uint32_t short_row,
                                            WaveLds<TT, NW, ROWS> &L, Fin &fin,
                                            const S *hot = nullptr, uint32_t nhot = 0,
-                                           const uint32_t *tlist = nullptr) {
+                                           const uint32_t *tlist = nullptr,
+                                           uint32_t first = 0xffffffffu, uint32_t stride = 0) {
     // The LDS tile holds TT: V, or -- unweighted, row sums in registers -- the
     // narrower gathered type S, widened exactly when a row is summed.
     static_assert(std::is_same<TT, V>::value || (std::is_same<TT, S>::value && !WEIGHTED && !ROWS),
                   "a narrower tile holds unweighted gathered values only");
     constexpr int K = WavePre::K;
     constexpr bool PIPE = std::is_base_of<TileAll, Fin>::value;
+    constexpr bool PRE = !ROWS && has_row_loads<Fin>::value;
+    using In = typename RowIn<Fin, PRE>::type;
+    constexpr bool FIRST = RowIn<Fin, PRE>::first;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint32_t slot_mask = (1u << bits) - 1u;
     TT *vals = L.vals[wid];
     TT *rowsum = L.rowsum[wid];  // ROWS only (TT == V)
-    const uint32_t nwaves = gridDim.x * NW;
+    static_assert(!LDSONLY || HOT, "an LDS-only walk gathers from the hot slice");
+    const uint32_t nwaves = stride ? stride : gridDim.x * NW;
     WavePre cur, nxt;
-    uint32_t ti = blockIdx.x * NW + wid;
+    uint32_t ti = first != 0xffffffffu ? __builtin_amdgcn_readfirstlane(first)
+                                       : blockIdx.x * NW + wid;
     if (PIPE && ti < ntiles)
         wave_fetch(wave_tile(tiles, ti), list_entry(tbase, ti), off, packed, lane, cur);
     for (; ti < ntiles; ti += nwaves) {
@@ -230,6 +262,20 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
         const uint32_t bnext = list_entry(tbase, tin);
         const Tile t = cur.t;
         const uint32_t nb = t.nnz_begin, cnt = t.nnz_end - t.nnz_begin;
+        // 0. The rows' own loads, when the finisher takes them first: only
+        //    the rounds of 64 rows the tile has (none for a split chunk).
+        //    Issued before the gathers, they are done when the gathers are,
+        //    whichever rounds were taken.
+        In in[WavePre::R];
+        if constexpr (PRE && FIRST) {
+            const uint32_t nr = t.row_end == kSplit ? 0u : t.row_end - t.row_begin;
+#pragma unroll
+            for (int i = 0; i < WavePre::R; ++i) {
+                if ((uint32_t)i * kWave >= nr) break;
+                const uint32_t q = lane + i * kWave;
+                in[i] = fin.load(t.row_begin + (q < nr ? q : nr - 1u));
+            }
+        }
         // 1. The gathers (global ones only above the hot slice).
         S g[K];
 #pragma unroll
@@ -237,7 +283,7 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             const uint32_t j = lane + k * kWave;
             const uint32_t sx = (cur.p[k] >> bits) + cur.base;
             g[k] = S(0);
-            if (j < cnt && (!HOT || sx >= nhot)) g[k] = src[sx];
+            if (!LDSONLY && j < cnt && (!HOT || sx >= nhot)) g[k] = src[sx];
         }
         // 2. The weights, then the next tile's words and offsets (the last
         //    tile loads itself again): both without branches, so waiting
@@ -249,6 +295,19 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             for (int k = 0; k < K; ++k)
                 wt[k] = buffer_load(rw, sizeof(V) * (lane + k * kWave), (V *)nullptr);
         }
+        // Otherwise after the gathers: every round, without branches (a
+        //    split chunk loads row 0 and uses nothing), so waiting for the
+        //    gathers leaves them in flight.
+        if constexpr (PRE && !FIRST) {
+            const bool split = t.row_end == kSplit;
+            const uint32_t nr = split ? 1u : t.row_end - t.row_begin;
+            const uint32_t r0 = split ? 0u : t.row_begin;
+#pragma unroll
+            for (int i = 0; i < WavePre::R; ++i) {
+                const uint32_t q = lane + i * kWave;
+                in[i] = fin.load(r0 + (q < nr ? q : nr - 1u));
+            }
+        }
         if (PIPE) wave_fetch(tnext, bnext, off, packed, lane, nxt);
         // Nothing that reads a gathered value moves above the loads issued so far.
         __builtin_amdgcn_sched_barrier(0);
@@ -259,7 +318,9 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
             const uint32_t j = lane + k * kWave;
             const uint32_t sx = (cur.p[k] >> bits) + cur.base;
             S x = g[k];
-            if (HOT) {
+            if (LDSONLY) {
+                x = hot[sx < nhot ? sx : 0u];
+            } else if (HOT) {
                 const S h = hot[sx < nhot ? sx : 0u];
                 x = sx < nhot ? h : x;
             }
@@ -325,6 +386,12 @@ __device__ __forceinline__ void spmv_waves(const Tile *__restrict__ tiles, uint3
                 wave_sync();
                 for (uint32_t q = lane; q < rows; q += kWave)
                     fin(rb + q, static_cast<double>(rowsum[q]));
+            } else if constexpr (PRE) {
+#pragma unroll
+                for (int i = 0; i < WavePre::R; ++i) {
+                    const uint32_t q = lane + i * kWave;
+                    if (q < rows) fin.finish(rb + q, static_cast<double>(rs[i]), in[i]);
+                }
             } else {
                 // One copy of the finisher: the sums rotate through rs[0].
 #pragma unroll 1
@@ -390,6 +457,104 @@ inline uint32_t hot_len(int kb, uint32_t n) {
     if (kb <= 0) return 0;
     const uint64_t want = (uint64_t)kb * 1024 / sizeof(V);
     return (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, hot_max<V>()), n) & ~3u;
+}
+
+// The sliced sweep's extra arguments (see Sliced): piece tiles with their
+// offsets, slots and packed stream; the segments of each piece workgroup;
+// the light rows' tiles and stream.
+struct SlicedArgs {
+    const Tile *htiles;
+    const uint32_t *htbase;
+    const uint32_t *hoff;
+    const uint32_t *hpacked;
+    const SliceSeg *seg;
+    const uint32_t *wseg;
+    uint32_t heavy_wgs;
+    uint32_t nltiles;
+    const Tile *ltiles;
+    const uint32_t *ltbase;
+    const uint32_t *lpacked;
+    uint32_t lbits;
+    uint32_t nmtiles;
+    const Tile *mtiles;
+    const uint32_t *mtbase;
+    const uint32_t *moff;
+    const uint32_t *mpacked;
+    uint32_t mbits;
+    uint32_t mpart;
+};
+
+// A piece's sum into the partial of the same index (coalesced: a tile's
+// pieces are consecutive); the fold finds a row's partials by index.
+struct PieceFin : TileAll {
+    double *long_part;  // already offset to the stream's first partial
+    __device__ __forceinline__ void operator()(uint32_t piece, double sum) const {
+        long_part[piece] = sum;
+    }
+};
+
+// The piece workgroup's share of a sliced sweep: for each of its segments,
+// the slice of `src` into LDS (`hot`), then its wavefronts walk the
+// segment's tiles gathering from LDS only.  TT is the tile's element type.
+template <typename V, typename TT, typename S>
+__device__ __forceinline__ void sliced_pieces(const SlicedArgs &sa, const S *src,
+                                              double *long_part, uint32_t short_row,
+                                              WaveLds<TT, kHotWaves, false> &L, S *hot) {
+    const uint32_t wid = threadIdx.x / kWave;
+    PieceFin fin;
+    fin.long_part = long_part;
+    for (uint32_t q = sa.wseg[blockIdx.x]; q < sa.wseg[blockIdx.x + 1]; ++q) {
+        const SliceSeg sg = sa.seg[q];
+        __syncthreads();  // every wavefront is done with the previous slice
+        fill_hot(src + sg.base, sg.len, hot);
+        spmv_waves<V, false, true, true>(sa.htiles + sg.t0, sg.t1 - sg.t0, sa.hoff, sa.hpacked,
+                                         kWaveSlotBits, sa.htbase + sg.t0, src + sg.base,
+                                         static_cast<const V *>(nullptr), long_part, short_row,
+                                         L, fin, static_cast<const S *>(hot), sg.len, nullptr,
+                                         wid, (uint32_t)kHotWaves);
+    }
+}
+
+// The light workgroups' share of a sliced sweep (hot slice already in LDS):
+// the light rows with their finisher, then the heavy rows' main pieces.
+// This workgroup is light workgroup b of nb.
+template <typename V, typename TT, typename S, typename Fin>
+__device__ __forceinline__ void sliced_light(const SlicedArgs &sa, const uint32_t *off,
+                                             const S *src, double *long_part, uint32_t short_row,
+                                             WaveLds<TT, kHotWaves, false> &L, Fin &fin,
+                                             const S *hot, uint32_t nhot, uint32_t b,
+                                             uint32_t nb) {
+    const uint32_t wid = threadIdx.x / kWave;
+    spmv_waves<V, false, true>(sa.ltiles, sa.nltiles, off, sa.lpacked, sa.lbits, sa.ltbase, src,
+                               static_cast<const V *>(nullptr), long_part, short_row, L, fin, hot,
+                               nhot, nullptr, b * kHotWaves + wid, nb * kHotWaves);
+    PieceFin pf;
+    pf.long_part = long_part + sa.mpart;
+    spmv_waves<V, false, true>(sa.mtiles, sa.nmtiles, sa.moff, sa.mpacked, sa.mbits, sa.mtbase,
+                               src, static_cast<const V *>(nullptr), long_part, short_row, L, pf,
+                               hot, nhot, nullptr, b * kHotWaves + wid, nb * kHotWaves);
+}
+
+// Sum of a split row's partials through an index list: long_part[long_idx[q]]
+// for q over the row's slots, in slot order, 16 loads in flight.
+__device__ __forceinline__ double long_row_sum_idx(const double *long_part,
+                                                   const uint32_t *long_slot,
+                                                   const uint32_t *long_idx, uint32_t k) {
+    const uint32_t e = long_slot[k + 1];
+    uint32_t q = long_slot[k];
+    double acc = 0.0;
+    for (; q + 16 <= e; q += 16) {
+        uint32_t ix[16];
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ix[i] = long_idx[q + i];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = long_part[ix[i]];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc += v[i];
+    }
+    for (; q < e; ++q) acc += long_part[long_idx[q]];
+    return acc;
 }
 
 // Sum of a split row's partial slots, in chunk order.

@@ -23,7 +23,7 @@ ARCH = os.environ.get("GT_HIP_ARCH", "gfx950")
 
 HIP_SOURCES = ["graph.hip", "props.hip", "pagerank.hip", "eigenvector.hip", "builder.hip",
                "views.hip", "batched.hip", "adjoint.hip", "relabel.hip", "uneven.hip", "degree.hip",
-               "betweenness.hip"]
+               "betweenness.hip", "sliced.hip"]
 CXX_SOURCES = ["host.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall",

@@ -54,7 +54,7 @@ class GtTuning(C.Structure):
     _fields_ = [("sweep_blocks_per_cu", C.c_int32), ("tile_rows", C.c_int32),
                 ("short_row", C.c_int32), ("mm_tile_nnz", C.c_int32),
                 ("mm_blocks_per_cu", C.c_int32), ("sweep_hot_kb", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("sliced_min_len", C.c_int32), ("sliced_share", C.c_int32)]
 
 
 class GtView(C.Structure):

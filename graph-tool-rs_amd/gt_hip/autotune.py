@@ -36,6 +36,7 @@ SWEEP_SPACE = {
     "tile_rows": (64, 128, 256),
     "short_row": (32, 64, 256),
     "sweep_hot_kb": (128, 64, -1),
+    "sliced_min_len": (-1, 64, 256),
 }
 BATCHED_SPACE = {
     "mm_tile_nnz": (512, 1024, 2048, 4096),

@@ -378,7 +378,13 @@ typedef struct gt_tuning {
     int32_t sweep_hot_kb;        /* FP32 PageRank sweeps on a relabelled graph: KB of the most
                                     gathered contributions held in LDS, one workgroup per CU;
                                     -1 turns it off: -1, 1..128 (128) */
-    int32_t reserved[2];
+    int32_t sliced_min_len;      /* unweighted PageRank sweeps on a relabelled graph: rows with
+                                    at least this many entries gather their entries in the
+                                    "warm" slices (the sweep_hot_kb-sized slices after the hot
+                                    one) from LDS copies held by dedicated workgroups: 2..1024,
+                                    or 0 / -1 for off (off) */
+    int32_t sliced_share;        /* percent of the sliced sweep's workgroups given to the warm
+                                    slices: 1..99 (0: from a cost model) */
 } gt_tuning;
 
 gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);

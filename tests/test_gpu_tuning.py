@@ -23,13 +23,19 @@ def test_tuning_round_trip_and_defaults(gpu):
     dev = gt_hip.DeviceGraph.from_edges(n, s, t)
     d = dev.tuning()
     assert d == {"sweep_blocks_per_cu": 3, "tile_rows": 256, "short_row": 64,
-                 "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 128}
+                 "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 128,
+                 "sliced_min_len": -1, "sliced_share": 0}
     got = dev.set_tuning(tile_rows=1000, short_row=8, sweep_blocks_per_cu=99, mm_tile_nnz=10)
     assert got["tile_rows"] == 256 and got["short_row"] == 8
     assert got["sweep_blocks_per_cu"] == 8 and got["mm_tile_nnz"] == 256
     assert dev.set_tuning(tile_rows=0, short_row=0)["tile_rows"] == 256
     assert dev.set_tuning(sweep_hot_kb=500)["sweep_hot_kb"] == 128
     assert dev.set_tuning(sweep_hot_kb=-7)["sweep_hot_kb"] == -1
+    assert dev.set_tuning(sliced_min_len=1)["sliced_min_len"] == 2
+    assert dev.set_tuning(sliced_min_len=5000, sliced_share=150)["sliced_min_len"] == 1024
+    assert dev.tuning()["sliced_share"] == 99
+    assert dev.set_tuning(sliced_min_len=0, sliced_share=-3)["sliced_min_len"] == -1
+    assert dev.tuning()["sliced_share"] == 0
     with pytest.raises(ValueError):
         dev.set_tuning(bogus=1)
 
@@ -40,6 +46,7 @@ def test_tuning_round_trip_and_defaults(gpu):
     {"tile_rows": 128, "short_row": 16, "sweep_blocks_per_cu": 4},
     {"sweep_hot_kb": -1},
     {"sweep_hot_kb": 1, "tile_rows": 64},
+    {"sweep_hot_kb": 2, "sliced_min_len": 64, "sliced_share": 25},
 ])
 def test_every_setting_matches_the_oracle(gpu, tune):
     n, s, t = _graph()
