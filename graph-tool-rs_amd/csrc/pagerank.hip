@@ -19,6 +19,7 @@
 // dangling-mass partials in index order and decides convergence on the
 // device.  Launches after convergence exit at once.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -667,6 +668,7 @@ struct PrSolve {
     const uint32_t *perm = nullptr;
     const uint32_t *iperm = nullptr;
     Relabeled *rel = nullptr;
+    bool noalloc = false;  // every buffer is given: build nothing
 };
 
 unsigned vblocks(uint32_t n) {
@@ -786,7 +788,8 @@ template <typename V>
 gt_status sliced_for(PrSolve &S, uint32_t nhot, const Sliced **out) {
     *out = nullptr;
     gt_graph *g = S.g;
-    if (!nhot || !S.rel || S.plan.weighted || S.plan.masked || g->tune.sliced_min_len < 0 ||
+    if (!nhot || !S.rel || S.noalloc || S.plan.weighted || S.plan.masked ||
+        g->tune.sliced_min_len < 0 ||
         g->n <= nhot)
         return GT_OK;
     const int grid = occupancy_cap((const void *)pr_sweep_sliced<V, false>, 1);
@@ -1020,11 +1023,190 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     return pr_deliver(g, latest, rank);
 }
 
+// ------------------------------------------------------- no allocation
+
+// The scratch of gt_pagerank_f64_noalloc: its iterates, contributions,
+// partials and loop state are carved from this array instead of allocated.
+// Every device the library runs on holds its own copy until the process
+// ends; calls on one device share it and keep the last call's contents, so
+// calls that overlap on one device race.
+constexpr size_t kNoAllocScratch = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The current device's copy of `scratch`, looked up once per device.
+gt_status scratch_base(void **out) {
+    int dev = 0;
+    GT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(GT_ERR_UNSUPPORTED, "no-alloc pagerank: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return GT_OK;
+}
+
+__global__ __launch_bounds__(64) void zero_state(SweepState *st) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(st);
+    if (threadIdx.x < sizeof(SweepState) / 4) w[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+gt_status pagerank_f64_noalloc_impl(gt_graph *g, const gt_view *view, const gt_prop *pers,
+                                    double damping, const gt_power_iteration *params,
+                                    gt_vertex_out *rank, uint64_t *iters) {
+    clear_error();
+    if (!g) return fail(GT_ERR_INVALID_ARG, "null graph");
+    if (!iters) return fail(GT_ERR_INVALID_ARG, "iters is null");
+    GT_TRY(check_vertex_out(g, rank, "pagerank: rank"));
+    GT_TRY(check_params(g, params));
+    const int kind = view ? view->kind : GT_VIEW_DIRECTED;
+    if (kind == GT_VIEW_DIRECTED_FILTERED || kind == GT_VIEW_REVERSED_FILTERED)
+        return fail(GT_ERR_UNSUPPORTED,
+                    "no-alloc pagerank: a filtered view builds its orientation per call; use "
+                    "gt_pagerank_f64");
+    PrSolve S;
+    S.g = g;
+    S.noalloc = true;
+    bool reversed = false;
+    const uint8_t *active = nullptr;
+    uint32_t nactive = 0;
+    GT_TRY(resolve_view(g, view, &S.pull, &S.push, &reversed, &active, &nactive));
+    if (rank->loc != GT_DEVICE)
+        return fail(GT_ERR_UNSUPPORTED, "no-alloc pagerank: the rank map must be in device memory");
+    if (pers && pers->kind == GT_MAP_DENSE && !(pers->dtype == GT_F64 && pers->loc == GT_DEVICE))
+        return fail(GT_ERR_UNSUPPORTED,
+                    "no-alloc pagerank: a dense personalisation must be f64 in device memory");
+    const uint32_t n = g->n;
+    const bool skip = no_sweep_for(params->epsilon);
+    if (n == 0) {
+        *iters = skip ? 0 : 1;
+        g->last_sweeps = *iters;
+        return GT_OK;
+    }
+    Relabeled *R = &g->rel[reversed ? 1 : 0];
+    if (!R->ready || !R->pull.scheduled || R->pull.sched_rows != (uint32_t)g->tune.tile_rows)
+        return fail(GT_ERR_UNSUPPORTED,
+                    "no-alloc pagerank: the graph's sweep schedule is not built; run "
+                    "gt_pagerank_f64 once on this graph and view first");
+    S.pull = &R->pull;
+    S.rel = R;
+    S.perm = R->perm.as<uint32_t>();
+    S.iperm = R->iperm.as<uint32_t>();
+    PrArgs &a = S.base;
+    a.active = nullptr;
+    a.out_off = R->push_off.as<uint32_t>();
+    a.iperm = S.iperm;
+    DevBuf unused;  // stage_vertex_prop stages nothing for the maps accepted above
+    GT_TRY(stage_vertex_prop(g, pers, unused, &a.pers, &a.pval, &S.plan.dpers,
+                             1.0 / (double)nactive));
+    a.pers_c = a.pers;
+
+    // The scratch, in 256-byte-aligned pieces: partials, loop state, split
+    // row slots, two iterates and two contributions (each with four spare
+    // values), the personalisation in sweep labels.
+    void *base = nullptr;
+    GT_TRY(scratch_base(&base));
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+        unsigned char *p = static_cast<unsigned char *>(base) + used;
+        used += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    double *partials = reinterpret_cast<double *>(carve(sizeof(double) * kPartials));
+    SweepState *st = reinterpret_cast<SweepState *>(carve(sizeof(SweepState)));
+    double *long_part =
+        reinterpret_cast<double *>(carve(sizeof(double) * std::max<uint32_t>(1, S.pull->nslots)));
+    double *x[2], *c[2];
+    for (int k = 0; k < 2; ++k) x[k] = reinterpret_cast<double *>(carve(sizeof(double) * (n + 4ull)));
+    for (int k = 0; k < 2; ++k) c[k] = reinterpret_cast<double *>(carve(sizeof(double) * (n + 4ull)));
+    double *pers2 = S.plan.dpers ? reinterpret_cast<double *>(carve(sizeof(double) * n)) : nullptr;
+    if (used > kNoAllocScratch) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "no-alloc pagerank: this graph needs %zu bytes of scratch; the maximum is "
+                      "%zu (about 1.6 M vertices)",
+                      used, kNoAllocScratch);
+        return fail(GT_ERR_UNSUPPORTED, msg);
+    }
+    hipStream_t s = g->stream;
+    if (S.plan.dpers) {
+        hipLaunchKernelGGL(permute_f64, dim3(vblocks(n)), dim3(kBlock), 0, s, a.pers, S.iperm, n,
+                           pers2);
+        a.pers = pers2;
+    }
+    hipLaunchKernelGGL(zero_state, dim3(1), dim3(64), 0, s, st);
+    GT_HIP(hipGetLastError());
+    note_layout(g, *S.pull);
+    S.rank64 = rank->data;
+    a.tiles = S.pull->tiles.as<Tile>();
+    a.ntiles = S.pull->ntiles;
+    a.nlong = S.pull->nlong;
+    a.in_off = S.pull->off.as<uint32_t>();
+    a.packed = S.pull->packed.as<uint32_t>();
+    a.bits = S.pull->bits;
+    a.tbase = S.pull->tbase.as<uint32_t>();
+    a.short_row = (uint32_t)g->tune.short_row;
+    a.damping = damping;
+    a.eps_abs = params->epsilon;
+    a.eps_rel = 0.0;
+    a.max_iter = params->max_iter;
+    a.partials = partials;
+    a.long_part = long_part;
+    a.long_rows = S.pull->long_rows.as<uint32_t>();
+    a.long_slot = S.pull->long_slot.as<uint32_t>();
+    a.st = st;
+    // Profiling would create events: it is off for the call.
+    const bool profiling = g->profiling;
+    g->profiling = false;
+    uint64_t sweeps = 0;
+    gt_status rc = GT_OK;
+    if (skip) {
+        PrArgs b = a;
+        b.x_next = x[0];
+        b.c_next = c[0];
+        dispatch_init<double>(S.plan, b, S.rank64, rank->had, n, 0, s);
+        if (hipGetLastError() != hipSuccess) rc = fail(GT_ERR_HIP, "no-alloc pagerank: init launch");
+    } else {
+        rc = pr_phase<double>(S, a, x, c, rank->had, 0, params->epsilon, params->max_iter,
+                              &sweeps);
+    }
+    g->profiling = profiling;
+    GT_TRY(rc);
+    g->last_sweeps = sweeps;
+    *iters = sweeps;
+    GT_TRY(pr_unpermute<double>(S, x[sweeps & 1]));
+    return pr_deliver(g, S.rank64, rank);
+}
+
 }  // namespace gt
 
 using namespace gt;
 
 extern "C" {
+
+gt_status gt_pagerank_f64_noalloc(gt_graph *g, const gt_view *view, const gt_prop *weight,
+                                  const gt_prop *pers, double damping,
+                                  const gt_power_iteration *params, gt_vertex_out *rank,
+                                  uint64_t *iters) {
+    if (weight && weight->kind != GT_MAP_UNITY) {
+        clear_error();
+        return fail(GT_ERR_UNSUPPORTED,
+                    "no-alloc pagerank: weights need staging buffers; pass Unity (or null)");
+    }
+    return pagerank_f64_noalloc_impl(g, view, pers, damping, params, rank, iters);
+}
 
 gt_status gt_pagerank_f64(gt_graph *g, const gt_view *view, const gt_prop *weight,
                           const gt_prop *pers, double damping, const gt_power_iteration *params,

@@ -108,6 +108,9 @@ _PROTOS = {
     "gt_pagerank_mixed": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
                           C.c_double, C.POINTER(GtPowerIteration), C.POINTER(GtVertexOut),
                           C.POINTER(C.c_uint64)],
+    "gt_pagerank_f64_noalloc": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), C.POINTER(GtProp),
+                                C.c_double, C.POINTER(GtPowerIteration),
+                                C.POINTER(GtVertexOut), C.POINTER(C.c_uint64)],
     "gt_pagerank_batched": [GraphP, C.POINTER(GtView), C.POINTER(GtProp), u32p, C.c_uint32,
                             C.c_double, C.POINTER(GtPowerIteration), C.c_void_p, C.c_uint64,
                             C.c_int32, C.POINTER(C.c_uint64)],

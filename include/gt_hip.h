@@ -186,6 +186,22 @@ gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view,
                             double damping, const gt_power_iteration *params,
                             gt_vertex_out *rank, uint64_t *iters);
 
+/* No counterpart in the reference: gt_pagerank_f64 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree: its
+ * iterates, contributions, partials and loop state live in a library-owned
+ * 64 MiB static device array, one per device, which limits it to about
+ * 1.6 M vertices (GT_ERR_UNSUPPORTED beyond, with the size in the message).
+ * Serves Directed and Reversed views, Unity (or null) weights, a null,
+ * constant or FP64 device personalisation and a device rank map; the graph's
+ * sweep schedule for that view must already exist (one gt_pagerank_f64 call
+ * on it builds it).  Results equal gt_pagerank_f64's bit for bit.  Calls
+ * that overlap on one device share the array and race. */
+gt_status gt_pagerank_f64_noalloc(gt_graph *g, const gt_view *view,
+                                  const gt_prop *weight, const gt_prop *pers,
+                                  double damping, const gt_power_iteration *params,
+                                  gt_vertex_out *rank, uint64_t *iters);
+
 /* Extension.  k personalised PageRank vectors solved together as one SpMM per
  * sweep: column c is pagerank_with with the one-hot personalisation
  * e_{seeds[c]}, started from it, stopped on its own L1 change (< epsilon) and
