@@ -42,6 +42,62 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_share() -> int:
+    """Host threads the CPU baseline runs on: every core this process may
+    use (its affinity mask), capped by OMP_NUM_THREADS when that is set -- a
+    GPU box gives one GPU's job a share of its cores and says so there."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        usable = min(usable, int(omp))
+    return max(1, usable)
+
+
+def host_info() -> dict:
+    """nproc, the cores this process may use, OMP_NUM_THREADS and the CPU model."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"nproc": os.cpu_count(), "affinity": usable,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def step_times(torch, steps: int, warmup: int, solve):
+    """`warmup` untimed calls, then `steps` calls each bracketed by a device
+    synchronisation; returns (per-step seconds, last result)."""
+    out = None
+    for _ in range(warmup):
+        out = solve()
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = solve()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    return times, out
+
+
+def timing_stats(times) -> dict:
+    a = np.asarray(times)
+    return {"median_ms": round(float(np.median(a)) * 1e3, 4),
+            "mean_ms": round(float(a.mean()) * 1e3, 4),
+            "min_ms": round(float(a.min()) * 1e3, 4), "max_ms": round(float(a.max()) * 1e3, 4),
+            "steps": int(a.shape[0])}
+
+
 def copy_calibration(torch, mib: int = 1024, reps: int = 5) -> dict:
     """Time a plain streaming kernel (torch.neg on `mib` MiB of f32: reads and
     writes every byte once) on the current stream.  Its achieved GB/s is the
@@ -114,17 +170,21 @@ def _tune(args, dev, workload: str, weight) -> dict:
 
 
 def _timed_solves(torch, dev, kind: int, steps: int, warmup: int, solve):
+    """Per-step timed solves with the library's sweep events on; returns
+    (median step x steps, outputs, summed kernel ms, launches, step stats)."""
     for _ in range(warmup):
         solve()
     dev.set_profiling(True)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    outs = [solve() for _ in range(steps)]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t
+    times, outs = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        outs.append(solve())
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
     ms_k, launches, _ = dev.sweep_stats(kind)
     dev.set_profiling(False)
-    return dt, outs, ms_k, launches
+    return float(np.median(times)) * steps, outs, ms_k, launches, timing_stats(times)
 
 
 def bench_batched(args, torch, gt_hip, _lib) -> int:
@@ -162,8 +222,8 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                   iters.ctypes.data_as(C.POINTER(C.c_uint64)))
         return iters.copy()
 
-    dt, outs, ms_k, launches = _timed_solves(torch, dev, _lib.GT_SWEEP_BATCHED, args.steps,
-                                             args.warmup, solve)
+    dt, outs, ms_k, launches, tstats = _timed_solves(torch, dev, _lib.GT_SWEEP_BATCHED,
+                                                     args.steps, args.warmup, solve)
     it = outs[-1].astype(np.int64)
     col_sweeps = int(it.sum())
     sweeps = int(it.max())
@@ -194,11 +254,12 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                      "bytes_per_sweep": B, "bytes_per_solve": B_solve,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
         "column_sweeps": {"min": int(it.min()), "max": int(it.max()), "sum": col_sweeps},
+        "timing": tstats,
         "tuning": tuning,
     }
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_share()
         o = oracle.Adj(n, src, tgt)
         w64 = w_host.astype(np.float64)
         Xh = X.cpu().numpy()
@@ -218,7 +279,7 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
         cpu_sweeps = sum(int(it[c]) for c in range(done))
         result["cpu_baseline"] = {
             "value": round(m * cpu_sweeps / dtc / 1e9, 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_info(),
             "sample": f"{done} of the 64 columns solved one at a time by the oracle's "
                       f"pagerank_with (one-hot pers, Plan grain 4096), {dtc:.1f}s"}
         result["accuracy_vs_fp64_oracle"] = {
@@ -279,10 +340,10 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
                   C.byref(biters))
         return int(biters.value)
 
-    dtf, fo, ms_f, la_f = _timed_solves(torch, dev, _lib.GT_SWEEP_PAGERANK_F32, args.steps,
-                                        args.warmup, forward)
-    dtb, bo, ms_b, la_b = _timed_solves(torch, dev, _lib.GT_SWEEP_ADJOINT, args.steps,
-                                        args.warmup, backward)
+    dtf, fo, ms_f, la_f, tsf = _timed_solves(torch, dev, _lib.GT_SWEEP_PAGERANK_F32,
+                                             args.steps, args.warmup, forward)
+    dtb, bo, ms_b, la_b, tsb = _timed_solves(torch, dev, _lib.GT_SWEEP_ADJOINT, args.steps,
+                                             args.warmup, backward)
     ratio = dtb / dtf
     Bf = 8 * m + 4 * (n + 1) + 20 * n
     ach_b = Bf * bo[-1] * args.steps / (ms_b * 1e-3) / 1e9 if la_b else 0.0
@@ -307,10 +368,87 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
                      "bytes_per_sweep": Bf,
                      "avg_launch_us": round(ms_b / max(la_b, 1) * 1e3, 2)},
         "grad_damping": gd.value,
+        "timing": {"forward": tsf, "backward": tsb},
     }
+    if args.fd_edges > 0:
+        result["finite_differences"] = _adjoint_fd(args, torch, _lib, dev, n, m, w, cvec, view,
+                                                   gw)
     print(json.dumps(result), flush=True)
     dev.close()
     return 0
+
+
+def _adjoint_fd(args, torch, _lib, dev, n, m, w32, cvec, view, gw32) -> dict:
+    """Central differences of L = c . x over `args.fd_edges` uniformly drawn
+    edges (h = 1e-4 on the weight, FP64 forward solves at epsilon 1e-14)
+    against the FP64 adjoint's gradient, and the FP32 adjoint's against the
+    FP64 one.  The noise floor is what the forward's stopping leaves in
+    L(w + h) - L(w - h): |L at 1e-14 - L converged further| / h."""
+    h, eps_fd = 1e-4, 1e-14
+    w64 = w32.double()
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    wprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F64, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(w64.data_ptr()), m, 1)
+    out = _lib.GtVertexOut(C.c_void_p(x.data_ptr()), n, 0, _lib.GT_DEVICE, 0, 1)
+    it = C.c_uint64()
+
+    def loss(eps=eps_fd, cap=0) -> float:
+        params = _lib.GtPowerIteration(eps, cap, _lib.GtPlan(n, -(-n // 4096)))
+        out.had = 0
+        _lib.call("gt_pagerank_f64", dev.handle, C.byref(view), C.byref(wprop), None, 0.85,
+                  C.byref(params), C.byref(out), C.byref(it))
+        return float(torch.dot(cvec, x).item())
+
+    t0 = time.perf_counter()
+    base = loss()
+    sweeps_fd = int(it.value)
+    deeper = loss(1e-300, sweeps_fd + 40)
+    noise = abs(base - deeper) / h
+    # The FP64 adjoint at the FP64 fixed point.
+    loss()
+    gw = torch.empty(m, dtype=torch.float64, device="cuda")
+    gp = torch.empty(n, dtype=torch.float64, device="cuda")
+    gd = C.c_double()
+    bit = C.c_uint64()
+    xprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F64, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(x.data_ptr()), n, 1)
+    cprop = _lib.GtProp(_lib.GT_MAP_DENSE, _lib.GT_F64, _lib.GT_DEVICE, 0, 0.0,
+                        C.c_void_p(cvec.data_ptr()), n, 1)
+    bparams = _lib.GtPowerIteration(1e-14, 0, _lib.GtPlan(n, -(-n // 4096)))
+    _lib.call("gt_pagerank_backward", dev.handle, C.byref(view), C.byref(wprop), None, 0.85,
+              C.byref(bparams), C.byref(xprop), C.byref(cprop), _lib.GT_F64,
+              C.c_void_p(gw.data_ptr()), _lib.GT_F64, _lib.GT_DEVICE,
+              C.c_void_p(gp.data_ptr()), C.byref(gd), C.byref(bit))
+    g64 = gw.cpu().numpy()
+    g32 = gw32.cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(20240601)
+    edges = rng.choice(m, args.fd_edges, replace=False)
+    fd = np.zeros(len(edges))
+    for i, e in enumerate(edges.tolist()):
+        w0 = float(w64[e].item())
+        w64[e] = w0 + h
+        lp = loss()
+        w64[e] = w0 - h
+        lm = loss()
+        w64[e] = w0
+        fd[i] = (lp - lm) / (2 * h)
+    ga = g64[edges]
+    scale = float(np.abs(g64).max())
+    abs_err = np.abs(fd - ga)
+    rel = abs_err / np.maximum(np.abs(ga), 1e-300)
+    dt = time.perf_counter() - t0
+    log(f"[bench] adjoint FD: {len(edges)} edges, median rel err {np.median(rel):.3e}, "
+        f"max abs err {abs_err.max():.3e} (noise floor {noise:.3e}, max |grad| {scale:.3e}), "
+        f"{dt:.1f}s")
+    return {"edges": int(len(edges)), "h": h, "forward_epsilon": eps_fd,
+            "forward_sweeps": sweeps_fd, "adjoint_sweeps": int(bit.value),
+            "fd_vs_fp64_adjoint": {
+                "median_rel": float(np.median(rel)), "p90_rel": float(np.percentile(rel, 90)),
+                "max_abs": float(abs_err.max()), "max_abs_over_max_grad": float(abs_err.max() / scale),
+                "noise_floor_abs": noise,
+                "within_noise_floor": int((abs_err <= 2.0 * noise + 1e-6 * np.abs(ga)).sum())},
+            "fp32_vs_fp64_adjoint_max_rel": float(np.abs(g32 - g64).max() / scale),
+            "seconds": round(dt, 1)}
 
 
 
@@ -412,7 +550,7 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
     }
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_share()
         xh = x.cpu().numpy()
         t0 = time.perf_counter()
         done, edges, worst, sw_ok = 0, 0.0, 0.0, 0
@@ -432,7 +570,7 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
         dtc = time.perf_counter() - t0
         result["cpu_baseline"] = {
             "value": round(edges / dtc / 1e9, 4), "unit": "GTEPS (forward only)",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "host": host_info(),
             "sample": f"{done} random graphs of the batch, each solved alone by the oracle's "
                       f"pagerank_with (forward only, graph build included), {dtc:.1f}s"}
         result["accuracy_vs_fp64_oracle"] = {"graphs_checked": done, "max_l1": worst,
@@ -517,7 +655,7 @@ def bench_betweenness(args, torch, gt_hip, _lib) -> int:
     }
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_share()
         o = oracle.Adj(n, src, tgt)
         deg = np.bincount(src, minlength=n) + np.bincount(tgt, minlength=n)
         cand = np.flatnonzero(deg > 0).astype(np.uint32)
@@ -538,7 +676,7 @@ def bench_betweenness(args, torch, gt_hip, _lib) -> int:
         nz_v, nz_e = wv != 0, we != 0
         result["cpu_baseline"] = {
             "value": round(piv.shape[0] * m / dtc / 1e9, 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_info(),
             "sample": f"{piv.shape[0]} random pivots of the same graph and view through the "
                       f"oracle's Brandes (16 chunks, C restatement of the reference), {dtc:.1f}s"}
         result["accuracy_vs_oracle"] = {
@@ -555,7 +693,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload",
                     choices=("pagerank", "batched", "adjoint", "uneven", "betweenness"),
                     default="pagerank",
@@ -567,6 +705,9 @@ def main() -> int:
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--epsilon", type=float, default=1e-9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle")
+    ap.add_argument("--fd-edges", type=int, default=1000,
+                    help="adjoint workload: edges checked by central finite differences "
+                         "(0: none)")
     ap.add_argument("--no-eigen", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the built-in launch parameters instead of autotuning them")
@@ -631,21 +772,26 @@ def main() -> int:
         return int(iters.value)
 
     def timed(fn: str, kind: int):
+        # Per-step times (each solve between two synchronisations); the
+        # median step is the reported one, the mean is kept beside it.
         for _ in range(args.warmup):
             solve(fn)
         dev.set_profiling(True)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        sweeps = [solve(fn) for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
+        times, sweeps = [], []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            sweeps.append(solve(fn))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
         ms_k, launches, _ = dev.sweep_stats(kind)
         ms_other = {k: dev.sweep_stats(k)[:2] for k in range(4) if k != kind}
         dev.set_profiling(False)
-        return dt, sweeps, ms_k, launches, ms_other
+        dt = float(np.median(times)) * args.steps
+        return dt, sweeps, ms_k, launches, ms_other, timing_stats(times)
 
     # ---- FP32-value / FP64-residual form (the chosen form)
-    dt32, sw32, ms32, la32, other32 = timed("gt_pagerank_mixed", _lib.GT_SWEEP_PAGERANK_F32)
+    dt32, sw32, ms32, la32, other32, st32 = timed("gt_pagerank_mixed", _lib.GT_SWEEP_PAGERANK_F32)
     x32 = rank.cpu().numpy().copy()
     f64_in_mixed = other32[_lib.GT_SWEEP_PAGERANK_F64]
     log(f"[bench] mixed: {dt32/args.steps*1e3:.3f} ms/solve, sweeps {sw32[0]}, "
@@ -653,7 +799,7 @@ def main() -> int:
         f"f64 refinement launches {f64_in_mixed[1]}")
 
     # ---- plain FP64 drop-in
-    dt64, sw64, ms64, la64, _ = timed("gt_pagerank_f64", _lib.GT_SWEEP_PAGERANK_F64)
+    dt64, sw64, ms64, la64, _, st64 = timed("gt_pagerank_f64", _lib.GT_SWEEP_PAGERANK_F64)
     x64 = rank.cpu().numpy().copy()
     log(f"[bench] fp64: {dt64/args.steps*1e3:.3f} ms/solve, sweeps {sw64[0]}, "
         f"launches {la64} ({ms64/max(la64,1)*1e3:.1f} us avg)")
@@ -710,6 +856,7 @@ def main() -> int:
             "vertices": n, "edges": m, "rmat": [0.57, 0.19, 0.19, 0.05], "seed": SEED,
             "damping": 0.85, "epsilon": args.epsilon, "weights": "unity",
             "personalisation": "uniform", "start": "cold (fresh map per step)",
+            "fp32_phase_stop": "delta < max(epsilon, 2^-22 * sum|x|), then one FP64 sweep",
         },
         "roofline": {
             "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -726,7 +873,17 @@ def main() -> int:
             "roofline_frac": round(ach64 / HBM_PEAK_GBS, 4), "bytes_per_sweep": b64,
             "avg_launch_us": round(avg64 * 1e6, 2),
         },
+        "timing": {"mixed": st32, "fp64_dropin": st64,
+                   "stat": f"median of {args.steps} solves after {args.warmup} warm-ups"},
         "mixed_vs_fp64_speedup": round(dt64 / dt32, 3),
+        # The speedup in two factors: fewer sweeps (the FP32 phase stops at
+        # its noise floor, then one FP64 sweep) and a faster sweep kernel.
+        "mixed_vs_fp64_split": {
+            "sweeps_fp64": sw64[0], "sweeps_mixed": sw32[0],
+            "sweep_count_ratio": round(sw64[0] / max(sw32[0], 1), 4),
+            "per_sweep_kernel_ratio": round(avg64 / avg32, 4) if avg32 else None,
+            "rest": round((dt64 / dt32) / (sw64[0] / max(sw32[0], 1)) / (avg64 / avg32), 4)
+            if avg32 else None},
         "host_rank_map_ms_per_solve": round(host_ms, 4),
         "mixed_sweeps": {"total": sw32[0], "fp64_refinement": f64_in_mixed[1] // max(args.steps, 1)},
         "host_adjacency_s": round(t_adj, 2), "upload_s": round(t_up, 3),
@@ -780,30 +937,30 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_share()
+        hinfo = host_info()
         t0 = time.perf_counter()
         o = oracle.Adj(n, src, tgt)
         log(f"[bench] oracle adjacency built in {time.perf_counter()-t0:.1f}s")
-        # Bounded sample: whole cold-start solves, repeated until ~10 s of CPU time.
-        t0 = time.perf_counter()
-        reps = 0
-        while True:
+        # Five whole cold-start solves (graph already built), the median one.
+        solve_s = []
+        for _ in range(5):
+            t0 = time.perf_counter()
             want, it_o = o.pagerank(epsilon=args.epsilon, max_iter=0, grain=4096,
                                     threads=threads)
-            reps += 1
-            dtc = time.perf_counter() - t0
-            if dtc >= 10.0 or reps >= 64:
-                break
+            solve_s.append(time.perf_counter() - t0)
+        dtc = float(np.median(solve_s))
         # The same path on one host thread: a solve capped at 6 sweeps (its
         # setup is spread over fewer sweeps, so this rate is slightly low).
         t0 = time.perf_counter()
         _, it1 = o.pagerank(epsilon=args.epsilon, max_iter=6, grain=4096, threads=1)
         dt1 = time.perf_counter() - t0
         del o
-        cpu_gteps = m * it_o * reps / dtc / 1e9
+        cpu_gteps = m * it_o / dtc / 1e9
         cpu1 = m * it1 / dt1 / 1e9
-        log(f"[bench] oracle pagerank: {reps} solves in {dtc:.1f}s, {it_o} sweeps each, "
-            f"{cpu_gteps:.3f} GTEPS on {threads} threads; {cpu1:.3f} GTEPS on 1 thread")
+        log(f"[bench] oracle pagerank: median of 5 solves {dtc:.2f}s, {it_o} sweeps each, "
+            f"{cpu_gteps:.3f} GTEPS on {threads} threads; {cpu1:.3f} GTEPS on 1 thread; "
+            f"host {hinfo}")
         nz = want != 0
 
         def errs(x):
@@ -812,9 +969,12 @@ def main() -> int:
 
         result["cpu_baseline"] = {
             "value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"{reps} cold-start pagerank_with solves (oracle C restatement of the "
+            "stat": "median of 5",
+            "sample": f"5 cold-start pagerank_with solves (oracle C restatement of the "
                       f"reference's rayon path, Plan grain 4096, {it_o} sweeps each) on the same "
-                      f"R-MAT-{args.scale} graph, {dtc:.1f}s",
+                      f"R-MAT-{args.scale} graph, graph built beforehand; median {dtc:.2f}s "
+                      f"(all: {', '.join(f'{x:.2f}' for x in solve_s)} s)",
+            "host": hinfo,
             "one_thread": {"value": round(cpu1, 4), "unit": "GTEPS", "cores": 1,
                            "sample": f"one solve capped at {it1} sweeps, {dt1:.1f}s"},
         }

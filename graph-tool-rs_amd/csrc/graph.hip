@@ -1,6 +1,7 @@
 // Device graph handle: upload of a host adjacency, download, the SpMV row
 // schedule, streams and the error channel of the C ABI.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -29,16 +30,32 @@ gt_status hip_fail(hipError_t e, const char *what, const char *file, int line) {
                                                                        : GT_ERR_HIP;
 }
 
+// Device bytes held by the library's buffers, and the cap on them (0: none).
+static std::atomic<uint64_t> g_devmem_in_use{0};
+static std::atomic<uint64_t> g_devmem_limit{0};
+
+void devmem_released(size_t b) { g_devmem_in_use.fetch_sub(b); }
+
 gt_status DevBuf::reserve(size_t b) {
     if (b <= bytes && ptr) return GT_OK;
     release();
     if (b == 0) b = 256;
+    const uint64_t lim = g_devmem_limit.load(), used = g_devmem_in_use.load();
+    if (lim && used + b > lim) {
+        char msg[240];
+        std::snprintf(msg, sizeof msg,
+                      "device allocation of %zu bytes exceeds the library's memory limit of "
+                      "%llu bytes (%llu in use)",
+                      b, (unsigned long long)lim, (unsigned long long)used);
+        return fail(GT_ERR_OOM, msg);
+    }
     hipError_t e = hipMalloc(&ptr, b);
     if (e != hipSuccess) {
         ptr = nullptr;
         return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
     }
     bytes = b;
+    g_devmem_in_use.fetch_add(b);
     return GT_OK;
 }
 
@@ -514,6 +531,19 @@ using namespace gt;
 extern "C" {
 
 const char *gt_last_error(void) { return g_last_error.c_str(); }
+
+gt_status gt_set_device_memory_limit(uint64_t bytes) {
+    clear_error();
+    g_devmem_limit.store(bytes);
+    return GT_OK;
+}
+
+gt_status gt_device_memory_in_use(uint64_t *bytes) {
+    clear_error();
+    if (!bytes) return fail(GT_ERR_INVALID_ARG, "bytes is null");
+    *bytes = g_devmem_in_use.load();
+    return GT_OK;
+}
 
 gt_status gt_device_count(int32_t *count) {
     clear_error();

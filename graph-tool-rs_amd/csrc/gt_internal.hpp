@@ -42,6 +42,9 @@ gt_status fail(gt_status code, const std::string &msg);
 
 // ---------------------------------------------------------------- memory
 
+// Device bytes the library holds (every DevBuf), for the memory limit.
+void devmem_released(size_t bytes);
+
 // Owning device buffer (hipMalloc / hipFree).
 struct DevBuf {
     void *ptr = nullptr;
@@ -51,7 +54,10 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            (void)hipFree(ptr);
+            devmem_released(bytes);
+        }
         ptr = nullptr;
         bytes = 0;
     }

@@ -980,7 +980,10 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     (void)s;
     uint64_t total = it32;
     const double *latest = S.rank64;
-    const bool capped = params->max_iter > 0 && it32 >= params->max_iter;
+    // A NaN delta stops the loop as it stops the reference's (`NaN >= eps` is
+    // false) without converging: like a capped run, no refinement sweep.
+    const bool capped = (params->max_iter > 0 && it32 >= params->max_iter) ||
+                        std::isnan(g->h_state->delta);
     if (refine == 0 || capped) {
         // Widen the FP32 iterate into the caller's f64 map (and labels).
         GT_TRY(pr_unpermute<float>(S, x32[it32 & 1]));

@@ -37,6 +37,7 @@ __all__ = [
     "Plan", "PowerIteration", "Unity", "Constant", "DenseProp", "AdjList", "View",
     "DeviceGraph", "pagerank", "pagerank_with", "pagerank_mixed", "eigenvector",
     "gen_er", "gen_rmat", "host_adjacency", "GtError", "GtValueError", "device_count",
+    "set_device_memory_limit", "device_memory_in_use",
     "pagerank_uneven", "pagerank_uneven_grad", "gen_chung_lu_batch",
     "Document", "read_gt", "read_edge_list",
     "weighted_out_degree", "weighted_in_degree", "weighted_degree", "weighted_out_degrees",
@@ -48,6 +49,19 @@ _u32p = C.POINTER(C.c_uint32)
 
 def _ptr(a: np.ndarray, t=_u32p):
     return a.ctypes.data_as(t)
+
+
+def set_device_memory_limit(nbytes: int) -> None:
+    """Cap the device memory the library's buffers may hold (0: no cap); an
+    allocation past it raises GtError with status GT_ERR_OOM."""
+    call("gt_set_device_memory_limit", int(nbytes))
+
+
+def device_memory_in_use() -> int:
+    """Device bytes the library's buffers hold now."""
+    v = C.c_uint64()
+    call("gt_device_memory_in_use", C.byref(v))
+    return int(v.value)
 
 
 def device_count() -> int:

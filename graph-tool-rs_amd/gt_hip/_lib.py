@@ -93,6 +93,8 @@ GraphP = C.c_void_p
 # name -> (argtypes); every function returns gt_status (c_int).
 _PROTOS = {
     "gt_device_count": [C.POINTER(C.c_int32)],
+    "gt_set_device_memory_limit": [C.c_uint64],
+    "gt_device_memory_in_use": [C.POINTER(C.c_uint64)],
     "gt_graph_upload": [C.c_uint64, C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p,
                         C.POINTER(GraphP)],
     "gt_graph_build_edges": [C.c_uint64, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,

@@ -128,6 +128,14 @@ const char *gt_last_error(void);
 /* Number of visible devices; 0 when no GPU is present. */
 gt_status gt_device_count(int32_t *count);
 
+/* Extension: a cap on the device memory the library's own buffers may hold
+ * (graphs, schedules, workspaces), 0 for none (the default).  A call whose
+ * allocation would pass it fails with GT_ERR_OOM, the sizes in
+ * gt_last_error(), exactly as when the device itself runs out.
+ * gt_device_memory_in_use reports the bytes held now. */
+gt_status gt_set_device_memory_limit(uint64_t bytes);
+gt_status gt_device_memory_in_use(uint64_t *bytes);
+
 /* ---------------------------------------------------------------- graphs */
 
 /* Upload an adjacency that already exists on the host, in storage order:

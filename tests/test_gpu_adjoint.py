@@ -48,6 +48,28 @@ def test_backward_matches_oracle_adjoint(gpu, precision, tol):
         assert abs(it - it_o) <= 1
 
 
+def test_fp64_backward_matches_oracle_adjoint_scale16(gpu):
+    """The FP64 GPU adjoint against the oracle's FP64 adjoint on a scale-16
+    R-MAT instance (65,536 vertices, 1,048,576 weighted edges): every edge
+    weight's, the personalisation's and the damping's gradient."""
+    n, s, t, w, c = _problem(16, 23)
+    g = gt_hip.AdjList.with_vertices(n)
+    g.add_edges(s, t)
+    o = oracle.Adj(n, s, t)
+    x, _ = o.pagerank(weight=w, epsilon=1e-15, max_iter=10_000, threads=8)
+    gw_o, gp_o, gd_o, _, it_o = o.pagerank_grad(x, c, weight=w, epsilon=1e-13)
+    wm = gt_hip.DenseProp.from_vec(g.graph_id, w)
+    gw, gp, gd, it = gt_hip.pagerank_grad(g, wm, None, 0.85, _params(n, 1e-13), x, c,
+                                          precision="f64")
+    assert np.abs(gw - gw_o).max() <= 1e-9 * np.abs(gw_o).max()
+    assert np.abs(gp - gp_o).max() <= 1e-9 * np.abs(gp_o).max()
+    assert abs(gd - gd_o) <= 1e-9 * max(1.0, abs(gd_o))
+    # Sweep counts where the L1 change is still well above rounding noise.
+    _, _, _, it = gt_hip.pagerank_grad(g, wm, None, 0.85, _params(n, 1e-8), x, c)
+    _, _, _, _, it_o = o.pagerank_grad(x, c, weight=w, epsilon=1e-8)
+    assert abs(it - it_o) <= 1
+
+
 def test_backward_on_reversed_view_matches_oracle(gpu):
     # Reversed view: the transposed sweep pulls over the in-halves and every
     # edge's gradient row is its target.

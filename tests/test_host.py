@@ -142,3 +142,14 @@ def test_chung_lu_batch_generator():
     few = gt_hip.gen_chung_lu_batch(10, 0x5EED, n_min=64, n_max=4096)
     assert np.array_equal(few[0], voff[:11])
     assert np.array_equal(few[3], s[:eoff[10]])
+
+
+def test_device_memory_limit_api():
+    """The limit round-trips without a device; nothing is held before a graph
+    is built (allocation under the limit is exercised on the GPU)."""
+    before = gt_hip.device_memory_in_use()
+    assert before >= 0
+    gt_hip.set_device_memory_limit(1 << 20)
+    gt_hip.set_device_memory_limit(0)
+    with pytest.raises(gt_hip.GtValueError):
+        _lib.call("gt_device_memory_in_use", None)

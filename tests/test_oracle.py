@@ -309,3 +309,21 @@ def test_eigenvector_matches_dense_eigenpair_on_symmetric_graph():
     v = vecs[:, -1] * np.sign(vecs[:, -1].sum())
     assert abs(eig - vals[-1]) < 1e-9
     assert np.abs(c - v).max() < 1e-7
+
+
+def test_nan_weight_stops_after_one_sweep():
+    """IEEE comparisons fix the reference's behaviour on a NaN weight: the
+    source's degree is NaN (not 0: no sink), its out-neighbours' ranks NaN,
+    delta NaN, and `while delta >= epsilon` ends after one sweep
+    (centrality.rs:284-291, 333-372).  Vertices no NaN reaches keep finite
+    ranks."""
+    n, s, t = thousand()
+    w = np.ones(len(s))
+    w[17] = np.nan
+    x, it = oracle.Adj(n, s, t).pagerank(weight=w, epsilon=1e-10, max_iter=1000)
+    assert it == 1
+    bad = np.isnan(x)
+    assert bad[t[s == s[17]]].all()
+    assert bad.sum() == len(set(t[s == s[17]].tolist()))
+    xe, eig, ite = oracle.Adj(n, s, t).eigenvector(weight=w, epsilon=1e-10, max_iter=1000)
+    assert ite == 1 and np.isnan(eig) and np.isnan(xe).all()
