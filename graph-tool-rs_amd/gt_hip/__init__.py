@@ -354,7 +354,71 @@ class DeviceGraph:
             pass
 
 
-class AdjList:
+class _GraphMethods:
+    """gt-py's shape of the centrality calls on a graph or one of its views
+    (crates/gt-py/src/module.rs:3-18): the read-only maps are sized first (a
+    graph-tool map grows on read); the output map is only checked for its
+    owner -- the solver sizes it, and the length it had decides which slots
+    are seeded and which warm-start (centrality.rs:198-205, 299).  The call
+    runs outside the GIL (ctypes releases it around every library call);
+    errors arrive as
+    ValueError (a map or argument) or RuntimeError (the device).  The four
+    directed view kinds are served; the undirected ones are refused with the
+    NotDirected message (crates/gt-py/src/dispatch.rs:315-336).  Without a
+    map argument the result goes to the graph's own vertex map
+    ``vp["pagerank"]`` / ``vp["eigenvector"]``."""
+
+    def _graph(self) -> "AdjList":
+        raise NotImplementedError
+
+    def new_vertex_property(self, value_type: str = "double") -> "DenseProp":
+        """A new double vertex map of this graph."""
+        return self._graph()._new_map(value_type)
+
+    def new_edge_property(self, value_type: str = "double") -> "DenseProp":
+        """A new double edge map of this graph."""
+        return self._graph()._new_map(value_type)
+
+    def _owned(self, m, what: str) -> None:
+        if isinstance(m, DenseProp) and m.graph_id != self._graph().graph_id:
+            raise GtValueError(_lib.GT_ERR_WRONG_GRAPH, f"{what} map does not belong to this graph")
+
+    def _sized(self, m, bound: int, what: str):
+        if isinstance(m, DenseProp):
+            self._owned(m, what)
+            m.sized_for(m.graph_id, bound)
+        return m
+
+    def pagerank(self, weight=None, pers=None, damping: float = 0.85, epsilon: float = 1e-6,
+                 max_iter: int = 0, rank=None) -> int:
+        """``pagerank`` / ``pagerank_with`` (centrality.rs:234-384) into `rank`
+        (warm start from its existing slots); returns the sweep count."""
+        base = self._graph()
+        if rank is None:
+            rank = base.vp.setdefault("pagerank", DenseProp.new(base.graph_id))
+        self._owned(rank, "pagerank: rank")
+        self._sized(weight, base.edge_bound(), "pagerank: weight")
+        self._sized(pers, base.vertex_bound(), "pagerank: personalisation")
+        params = PowerIteration(epsilon, max_iter, Plan(base.vertex_bound(), 64))
+        w = Unity.NEW if weight is None else weight
+        if pers is None:
+            return pagerank(self, w, damping, params, rank)
+        return pagerank_with(self, w, pers, damping, params, rank)
+
+    def eigenvector(self, weight=None, epsilon: float = 1e-6, max_iter: int = 0,
+                    x=None) -> Tuple[float, int]:
+        """``eigenvector`` (centrality.rs:405-497) into `x`; returns
+        ``(eigenvalue, sweeps)``."""
+        base = self._graph()
+        if x is None:
+            x = base.vp.setdefault("eigenvector", DenseProp.new(base.graph_id))
+        self._owned(x, "eigenvector: centrality")
+        self._sized(weight, base.edge_bound(), "eigenvector: weight")
+        params = PowerIteration(epsilon, max_iter, Plan(base.vertex_bound(), 64))
+        return eigenvector(self, Unity.NEW if weight is None else weight, params, x)
+
+
+class AdjList(_GraphMethods):
     """An adjacency built by ``add_edge`` calls; edge ``i`` is the ``i``-th added.
 
     The device copy is built on first use and rebuilt after any mutation.
@@ -370,6 +434,17 @@ class AdjList:
         self._dev_epoch = -1
         self._epoch = 0
         self._host = None
+        self.vp = {}  # the graph's own vertex maps (name -> DenseProp)
+        self.ep = {}  # and edge maps
+
+    def _graph(self) -> "AdjList":
+        return self
+
+    def _new_map(self, value_type: str) -> "DenseProp":
+        if value_type not in ("double", "float64"):
+            raise GtValueError(_lib.GT_ERR_INVALID_ARG,
+                               f"Invalid property type: {value_type}")
+        return DenseProp.new(self.graph_id)
 
     @classmethod
     def with_vertices(cls, n: int) -> "AdjList":
@@ -475,11 +550,14 @@ class AdjList:
         return View(self, kind, vmask, emask)
 
 
-class View:
+class View(_GraphMethods):
     """One of the six view kinds over an :class:`AdjList`."""
 
     def __init__(self, base: AdjList, kind: int, vmask=None, emask=None):
         self.base, self.kind, self.vmask, self.emask = base, kind, vmask, emask
+
+    def _graph(self) -> AdjList:
+        return self.base
 
     @property
     def graph_id(self) -> int:
