@@ -249,7 +249,11 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                    "vertices": n, "edges": m, "k": k, "weights": "f32 exp(N(0,1))",
                    "seeds": "64 distinct uniform", "damping": 0.85, "epsilon": eps},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(
+                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[0],
+                     "traffic_source": profiled_traffic(
+                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[1],
                      "kernel": "ppr_sweep<true> (batched SpMM sweep)",
                      "bytes_per_sweep": B, "bytes_per_solve": B_solve,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
@@ -544,7 +548,11 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
         "adjoint_sweeps": {"min": int(itb.min()), "max": int(itb.max()),
                            "mean": round(float(itb.mean()), 2)},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("uneven_chung_lu_4096", "u_sweep_forward_call")[0],
+                     "traffic_unit": "bytes per forward call",
+                     "traffic_source": profiled_traffic("uneven_chung_lu_4096",
+                                                        "u_sweep_forward_call")[1],
                      "kernel": "u_sweep<UFwdFin<true>> (forward, wall time of the call)",
                      "bytes_per_solve": int(bytes_f)},
     }
@@ -927,6 +935,7 @@ def main() -> int:
             "launches_per_solve": lae // max(args.steps, 1),
             "eigenvalue": eig.value, "achieved_gbs": round(ache, 1),
             "roofline_frac": round(ache / HBM_PEAK_GBS, 4), "bytes_per_sweep": be,
+            "traffic": profiled_traffic(workload, "ev_sweep_hot")[0],
         }
         x_eig = xs.cpu().numpy().copy()
         log(f"[bench] eigenvector: {dte/args.steps*1e3:.3f} ms/solve, sweeps {swe[0]}, "
