@@ -254,7 +254,7 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
                          f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[0],
                      "traffic_source": profiled_traffic(
                          f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[1],
-                     "kernel": "ppr_sweep<true> (batched SpMM sweep)",
+                     "kernel": "ppr_sweep_w<true, 16> (batched SpMM sweep, wave tiles)",
                      "bytes_per_sweep": B, "bytes_per_solve": B_solve,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
         "column_sweeps": {"min": int(it.min()), "max": int(it.max()), "sum": col_sweeps},

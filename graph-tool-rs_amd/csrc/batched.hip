@@ -5,26 +5,31 @@
 // stopped on its own L1 change and frozen from then on, so its sweep count is
 // the one its single-vector run would report.
 //
-// Layout: the state is [n][64] FP32, one 256-byte row per vertex; a
-// wavefront owns a row and lane c owns column c, so every gathered source row
-// is one coalesced 256-byte load.  Values, weights and the SpMM accumulation
-// are FP32; the teleport update, the per-column dangling mass and the
-// per-column L1 change are FP64.  Tiles of <= 1024 in-entries; longer rows
-// are split into chunks whose [64] partial sums a second kernel per sweep
-// (ppr_fold) combines in chunk order before folding the per-workgroup,
-// per-column partials in index order.  k > 64 runs as consecutive blocks of
-// 64 columns.
+// Layout: the state is [n][64] FP32, one 256-byte row per vertex; lane c of
+// a wavefront owns column c, so every gathered source row is one coalesced
+// 256-byte load.  Values, weights and the SpMM accumulation are FP32; the
+// teleport update, the per-column dangling mass and the per-column L1 change
+// are FP64.  Each wavefront walks its own tiles of at most kMmRows rows and
+// the tuned number of in-entries (ppr_sweep_w); longer rows are split into
+// chunks whose [64] partial sums a second kernel per sweep (ppr_fold)
+// combines in chunk order before folding the per-workgroup, per-column
+// partials in index order.  k > 64 runs as consecutive blocks of 64
+// columns.
 #include <cmath>
 
 #include "device_common.hpp"
 #include "driver.hpp"
 #include "gt_internal.hpp"
 #include "props.hpp"
+#include "sweep_common.hpp"
 
 namespace gt {
 
 constexpr int kCols = 64;
 constexpr uint32_t kNoSeed = 0xffffffffu;
+// Rows of an SpMM wave tile, and gathers in flight per wavefront.
+constexpr uint32_t kMmRows = 16;
+constexpr int kMmGroup = 16;
 
 struct BatchState {
     SweepState h;            // delta = max over active columns, iter = sweeps
@@ -59,32 +64,6 @@ struct BArgs {
     BatchState *st;
 };
 
-// Sum over entries lo, lo+stride, ... < hi of w(j) * c[src(j)][lane], in
-// entry order.  Eight entries are in flight per wavefront.  A converged
-// column's lane loads nothing (its sum is not used).
-template <bool WEIGHTED>
-__device__ __forceinline__ float row_accumulate(const BArgs &a, uint32_t lo, uint32_t hi,
-                                                uint32_t stride, int lane, bool col_on) {
-    float acc = 0.f;
-    for (uint32_t j0 = lo; j0 < hi; j0 += 8 * stride) {
-        const uint32_t mj = j0 + (uint32_t)lane * stride;
-        const bool mine = lane < 8 && mj < hi;
-        const uint32_t my_idx = mine ? a.in_idx[mj] : 0u;
-        const float my_w = (WEIGHTED && mine) ? a.w[mj] : 1.f;
-        float v[8], wv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t s = __shfl(my_idx, u, kWave);
-            wv[u] = WEIGHTED ? __shfl(my_w, u, kWave) : 1.f;
-            v[u] = (col_on && j0 + u * stride < hi) ? a.c_cur[(size_t)s * kCols + lane] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (j0 + u * stride < hi) acc += wv[u] * v[u];
-    }
-    return acc;
-}
-
 // A converged column's values are copied into the other buffers on the sweep
 // after it stopped (col_copy) and left alone afterwards: both buffers then
 // hold its last iterate.
@@ -115,40 +94,167 @@ __device__ __forceinline__ void finish_row(const BArgs &a, uint32_t r, float acc
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double x, uint32_t l) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// finish_row with the row's iterate (xo, this lane's column) and out-degree
+// (dg) already loaded.
 template <bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
+__device__ __forceinline__ void finish_row_pre(const BArgs &a, uint32_t r, float acc, float xo,
+                                               double dg, int lane, bool col_on, bool col_copy,
+                                               uint32_t seed, double psink, double &dsum,
+                                               double &ssum) {
+    const size_t o = (size_t)r * kCols + lane;
+    if (!col_on) {
+        if (col_copy) {
+            a.x_next[o] = xo;
+            a.c_next[o] = a.c_cur[o];
+        }
+        return;
+    }
+    const double pv = (r == seed) ? 1.0 : 0.0;
+    const double nv = (1.0 - a.damping) * pv + a.damping * (psink * pv + (double)acc);
+    const float nf = (float)nv;
+    dsum += fabs((double)nf - (double)xo);
+    a.x_next[o] = nf;
+    if (dg != 0.0) {
+        a.c_next[o] = (float)((double)nf / dg);
+    } else {
+        a.c_next[o] = 0.f;
+        ssum += (double)nf;
+    }
+}
+
+// Adds w(j) * c[src(j)][lane] over the entries u0 + u, u < G, of a chunk
+// whose sources (and weights) lane u holds in `my` (`myw`; lanes at or past
+// `cnt` repeat the chunk's last entry): the sources are read out to scalars,
+// so each gather is one 256-byte row at a scalar address, and all G are in
+// flight before the first is used.  A converged column's lane loads nothing
+// (its sums are never used; a masked lane costs the gather nothing, while a
+// lane that loads, even a byte another lane reads, does).  Entries at or
+// past `cnt` are not added.  `add(u, x)` takes the values in entry order.
+template <bool WEIGHTED, int G, typename Add>
+__device__ __forceinline__ void gather_group(const float *__restrict__ c, uint32_t my, float myw,
+                                             uint32_t u0, uint32_t cnt, int lane, bool col_on,
+                                             Add add) {
+    cnt = __builtin_amdgcn_readfirstlane(cnt);
+    u0 = __builtin_amdgcn_readfirstlane(u0);
+    float v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = 0.f;
+    if (col_on) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const uint32_t s = __builtin_amdgcn_readlane(my, u0 + u);
+            const float *row = c + (size_t)s * kCols;  // scalar: one base per gather
+            v[u] = row[lane];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        if (u0 + u >= cnt) break;
+        const float w = WEIGHTED ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                 __builtin_bit_cast(int, myw),
+                                                                 u0 + u))
+                                 : 1.f;
+        add(u0 + u, WEIGHTED ? w * v[u] : v[u]);
+    }
+}
+
+// The SpMM sweep over wave tiles of at most kMmRows rows (or one chunk of a
+// split row): each wavefront walks its own tiles.  A tile's sources are
+// loaded 64 at a time (one coalesced load), its rows' iterates and
+// out-degrees before the gathers; the gathers go out G at a time; each row's
+// sum is kept in a register in entry order and parked in the wavefront's
+// LDS rows when the row ends; the rows are finished after the entries.
+template <bool WEIGHTED, int G>
+__global__ __launch_bounds__(kBlock) void ppr_sweep_w(const BArgs a) {
     if (a.st->h.done) return;
-    __shared__ float s_part[kBlock / kWave][kCols];
+    __shared__ float s_rows[kBlock / kWave][kMmRows][kCols];
     __shared__ double s_red[2][kBlock / kWave][kCols];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid / kWave;
+    float(*rows)[kCols] = s_rows[wid];
     const uint64_t active = a.st->active;
     const bool col_on = (active >> lane) & 1ull;
     const bool col_copy = !col_on && ((a.st->was_active >> lane) & 1ull);
     const uint32_t seed = a.seeds[lane];
     const double psink = a.st->p_sink[lane];
     double dsum = 0.0, ssum = 0.0;
+    const uint32_t nwaves = gridDim.x * (kBlock / kWave);
 
-    for (uint32_t ti = blockIdx.x; ti < a.ntiles; ti += gridDim.x) {
-        const Tile t = a.tiles[ti];
+    for (uint32_t ti = blockIdx.x * (kBlock / kWave) + wid; ti < a.ntiles; ti += nwaves) {
+        const Tile t = wave_tile(a.tiles, ti);
+        const uint32_t nb = t.nnz_begin, ne = t.nnz_end;
         if (t.row_end == kSplit) {
-            // All four wavefronts share the chunk, entries strided by 4.
-            s_part[wid][lane] =
-                row_accumulate<WEIGHTED>(a, t.nnz_begin + wid, t.nnz_end, kBlock / kWave, lane,
-                                         col_on);
-            __syncthreads();
-            if (wid == 0)
-                a.long_part[(size_t)t.row_begin * kCols + lane] =
-                    ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
-            __syncthreads();
+            // A chunk of a split row: its sum goes to the row's slot.
+            float acc = 0.f;
+            for (uint32_t j0 = nb; j0 < ne; j0 += kWave) {
+                const uint32_t cnt = ne - j0 < (uint32_t)kWave ? ne - j0 : (uint32_t)kWave;
+                const uint32_t j = j0 + ((uint32_t)lane < cnt ? (uint32_t)lane : cnt - 1);
+                const uint32_t my = a.in_idx[j];
+                const float myw = WEIGHTED ? a.w[j] : 1.f;
+                for (uint32_t u0 = 0; u0 < cnt; u0 += G)
+                    gather_group<WEIGHTED, G>(a.c_cur, my, myw, u0, cnt, lane, col_on,
+                                              [&](uint32_t, float x) { acc += x; });
+            }
+            a.long_part[(size_t)t.row_begin * kCols + lane] = acc;
             continue;
         }
-        for (uint32_t r = t.row_begin + wid; r < t.row_end; r += kBlock / kWave) {
-            const float acc =
-                row_accumulate<WEIGHTED>(a, a.in_off[r], a.in_off[r + 1], 1, lane, col_on);
-            finish_row<WEIGHTED>(a, r, acc, lane, col_on, col_copy, seed, psink, dsum, ssum);
+        const uint32_t rb = t.row_begin, nr = t.row_end - rb;  // 1 <= nr <= kMmRows
+        // Lane q <= nr: the offset of row rb + q; lane q < nr: its out-degree.
+        const uint32_t ql = (uint32_t)lane < nr ? (uint32_t)lane : nr;
+        const uint32_t roff = a.in_off[rb + ql];
+        const uint32_t qd = (uint32_t)lane < nr ? (uint32_t)lane : nr - 1;
+        const double dgl = WEIGHTED ? a.deg[rb + qd]
+                                    : (double)(a.out_off[rb + qd + 1] - a.out_off[rb + qd]);
+        float xo[kMmRows];
+#pragma unroll
+        for (int q = 0; q < (int)kMmRows; ++q)
+            xo[q] = a.x_cur[(size_t)(rb + ((uint32_t)q < nr ? (uint32_t)q : nr - 1)) * kCols + lane];
+        uint32_t written = 0, cur = 0;
+        float acc = 0.f;
+        for (uint32_t j0 = nb; j0 < ne; j0 += kWave) {
+            const uint32_t cnt = ne - j0 < (uint32_t)kWave ? ne - j0 : (uint32_t)kWave;
+            const uint32_t j = j0 + ((uint32_t)lane < cnt ? (uint32_t)lane : cnt - 1);
+            const uint32_t my = a.in_idx[j];
+            const float myw = WEIGHTED ? a.w[j] : 1.f;
+            // Row of entry j0 + lane: the rows q >= 1 starting at or before it.
+            uint32_t rq = 0;
+            for (uint32_t q = 1; q < nr; ++q)
+                rq += (uint32_t)(__builtin_amdgcn_readlane(roff, q) <= j0 + (uint32_t)lane);
+            for (uint32_t u0 = 0; u0 < cnt; u0 += G)
+                gather_group<WEIGHTED, G>(a.c_cur, my, myw, u0, cnt, lane, col_on,
+                                          [&](uint32_t u, float x) {
+                                              const uint32_t q = __builtin_amdgcn_readlane(rq, u);
+                                              if (q != cur) {
+                                                  rows[cur][lane] = acc;
+                                                  written |= 1u << cur;
+                                                  acc = 0.f;
+                                                  cur = q;
+                                              }
+                                              acc += x;
+                                          });
         }
+        if (ne > nb) {
+            rows[cur][lane] = acc;
+            written |= 1u << cur;
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < (int)kMmRows; ++q) {
+            if ((uint32_t)q >= nr) break;
+            const float sum = (written >> q) & 1u ? rows[q][lane] : 0.f;
+            const double dg = readlane_f64(dgl, q);
+            finish_row_pre<WEIGHTED>(a, rb + q, sum, xo[q], dg, lane, col_on, col_copy, seed,
+                                     psink, dsum, ssum);
+        }
+        wave_sync();  // the LDS rows are reused by this wavefront's next tile
     }
 
     s_red[0][wid][lane] = dsum;
@@ -164,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep(const BArgs a) {
     }
 }
 
-// Completes the preceding ppr_sweep launch (`sgrid` workgroups): one
+// Completes the preceding ppr_sweep_w launch (`sgrid` workgroups): one
 // wavefront per split row combines its chunk partials in chunk order and
 // finishes the row, and workgroup b also sums its contiguous slice of the
 // sweep's per-column partials; the last workgroup to arrive folds the
@@ -394,11 +500,13 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
         a.deg = ws.deg2.as<double>();
         a.w = ws.wcsc.as<float>();
     }
-    GT_TRY(ensure_schedule_mm(*pull, n, (uint32_t)g->tune.mm_tile_nnz));
+    GT_TRY(ensure_schedule_mm(*pull, n, (uint32_t)g->tune.mm_tile_nnz, kMmRows));
     const Schedule &S = pull->mm;
-    const int cap_w = occupancy_cap((const void *)ppr_sweep<true>, g->tune.mm_blocks_per_cu);
-    const int cap_u = occupancy_cap((const void *)ppr_sweep<false>, g->tune.mm_blocks_per_cu);
-    const int grid_w = fit_grid(cap_w, S.ntiles), grid_u = fit_grid(cap_u, S.ntiles);
+    const int cap_w =
+        occupancy_cap((const void *)ppr_sweep_w<true, kMmGroup>, g->tune.mm_blocks_per_cu);
+    const int cap_u =
+        occupancy_cap((const void *)ppr_sweep_w<false, kMmGroup>, g->tune.mm_blocks_per_cu);
+    const int grid_w = wave_grid(cap_w, S.ntiles), grid_u = wave_grid(cap_u, S.ntiles);
     // At least 64 fold workgroups: each sums at most 32 of the sweep's rows of
     // partials before the last one folds the workgroup sums.
     const unsigned fgrid =
@@ -464,9 +572,11 @@ gt_status pagerank_batched_impl(gt_graph *g, const gt_view *view, const gt_prop 
             b.c_cur = cb[par];
             b.c_next = cb[par ^ 1];
             if (weighted)
-                hipLaunchKernelGGL(ppr_sweep<true>, dim3(grid_w), dim3(kBlock), 0, s, b);
+                hipLaunchKernelGGL((ppr_sweep_w<true, kMmGroup>), dim3(grid_w), dim3(kBlock), 0,
+                                   s, b);
             else
-                hipLaunchKernelGGL(ppr_sweep<false>, dim3(grid_u), dim3(kBlock), 0, s, b);
+                hipLaunchKernelGGL((ppr_sweep_w<false, kMmGroup>), dim3(grid_u), dim3(kBlock), 0,
+                                   s, b);
         };
         auto fold = [&](uint64_t) {
             if (weighted)

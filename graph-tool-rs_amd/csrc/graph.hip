@@ -354,11 +354,11 @@ gt_status pack_stream(gt_graph *g, const Tile *tiles, uint32_t ntiles, const uin
     return GT_OK;
 }
 
-gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz) {
+gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz, uint32_t tile_rows) {
     Schedule &S = o.mm;
-    if (S.ready && S.tile_nnz == tile_nnz) return GT_OK;
+    if (S.ready && S.tile_nnz == tile_nnz && S.tile_rows == tile_rows) return GT_OK;
     const uint32_t tnnz = tile_nnz;
-    const uint32_t trows = std::max<uint32_t>(1, tile_nnz / 4);
+    const uint32_t trows = std::max<uint32_t>(1, tile_rows);
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;
@@ -401,6 +401,7 @@ gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz) {
     GT_HIP(hipMemcpy(S.long_slot.ptr, long_slot.data(), sizeof(uint32_t) * long_slot.size(),
                      hipMemcpyHostToDevice));
     S.tile_nnz = tile_nnz;
+    S.tile_rows = tile_rows;
     S.ready = true;
     return GT_OK;
 }

@@ -101,6 +101,7 @@ struct Schedule {
     DevBuf long_slot;        // uint32[nlong+1]: first partial slot of each
     uint32_t ntiles = 0, nlong = 0, nslots = 0;
     uint32_t tile_nnz = 0;   // entry bound the schedule was built with
+    uint32_t tile_rows = 0;  // row bound (SpMM schedules)
     bool ready = false;
 };
 
@@ -318,7 +319,7 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
 // Record `o` (scheduled) as the orientation the current solve sweeps.
 void note_layout(gt_graph *g, const Orient &o);
 // Build (once per tile size) the SpMM schedule of an orientation.
-gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz);
+gt_status ensure_schedule_mm(Orient &o, uint32_t n, uint32_t tile_nnz, uint32_t tile_rows);
 // Allocate a graph shell with its stream.
 gt_status graph_new(uint64_t graph_id, uint32_t n, uint64_t m, uint64_t edge_bound,
                     gt_graph **out);

@@ -86,3 +86,33 @@ def test_batched_rejects_bad_seed(gpu):
     with pytest.raises(ValueError):
         gt_hip.pagerank_batched(g, gt_hip.Unity.NEW, [5], 0.85,
                                 gt_hip.PowerIteration(1e-6, 10, gt_hip.Plan(4, 64)))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_batched_split_rows_and_empty_rows(gpu, weighted):
+    """256-entry SpMM tiles: a hub row of 3,000 in-entries is summed in
+    twelve chunks, rows with no in-entries (isolated vertices) sit between
+    the tiles' rows, and the columns still equal the single-vector runs."""
+    n, s, t, _ = _graph(10, 21)
+    n_all = n + 300  # the last 300 vertices have no edges
+    rng = np.random.default_rng(5)
+    hub_src = rng.integers(0, n, 3000).astype(np.uint32)
+    s = np.concatenate([s, hub_src]).astype(np.uint32)
+    t = np.concatenate([t, np.full(3000, 17, np.uint32)]).astype(np.uint32)
+    w = np.exp(rng.standard_normal(len(s))).astype(np.float32)
+    g = gt_hip.AdjList.with_vertices(n_all)
+    g.add_edges(s, t)
+    g.device().set_tuning(mm_tile_nnz=256)
+    seeds = [17, 0, 5, n_all - 1] + list(range(100, 100 + 60))
+    params = gt_hip.PowerIteration(1e-7, 1000, gt_hip.Plan(n_all, 64))
+    wp = gt_hip.DenseProp.from_vec(g.graph_id, w) if weighted else gt_hip.Unity.NEW
+    X, iters = gt_hip.pagerank_batched(g, wp, seeds, 0.85, params)
+    o = oracle.Adj(n_all, s, t)
+    for c in (0, 1, 2, 3, 40, 63):
+        pers = np.zeros(n_all)
+        pers[seeds[c]] = 1.0
+        want, it_o = o.pagerank(weight=w.astype(np.float64) if weighted else None, pers=pers,
+                                epsilon=1e-7, max_iter=1000)
+        assert abs(int(iters[c]) - it_o) <= 1, (c, iters[c], it_o)
+        assert np.abs(X[:, c].astype(np.float64) - want).sum() <= 2e-6, c
+    g.device().set_tuning(mm_tile_nnz=0)
