@@ -82,4 +82,35 @@ __device__ __forceinline__ bool last_to_arrive(uint32_t *counter, uint32_t *s_fl
     return *s_flag != 0;
 }
 
+
+// The share of `units` consecutive work items this workgroup's XCD walks when
+// the grid is dealt to the eight XCDs in turn (workgroup b on XCD b % 8):
+// XCD x takes items [begin, begin + count), the x-th eighth, and its
+// workgroups walk them with `first` / `stride` (in wavefronts of NW per
+// workgroup).  A grid that is not a multiple of eight walks all items as one.
+constexpr uint32_t kXcds = 8;
+struct XcdSpan {
+    uint32_t begin, count, index, groups;
+    __device__ uint32_t first(uint32_t nw) const {
+        return index * nw + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    }
+    __device__ uint32_t stride(uint32_t nw) const { return groups * nw; }
+};
+__device__ __forceinline__ XcdSpan xcd_span(uint32_t units) {
+    XcdSpan s;
+    if (gridDim.x % kXcds != 0) {
+        s.begin = 0;
+        s.count = units;
+        s.index = blockIdx.x;
+        s.groups = gridDim.x;
+        return s;
+    }
+    const uint32_t x = blockIdx.x % kXcds;
+    s.begin = (uint32_t)((uint64_t)units * x / kXcds);
+    s.count = (uint32_t)((uint64_t)units * (x + 1) / kXcds) - s.begin;
+    s.index = blockIdx.x / kXcds;
+    s.groups = gridDim.x / kXcds;
+    return s;
+}
+
 }  // namespace gt

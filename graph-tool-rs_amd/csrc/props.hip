@@ -24,27 +24,36 @@ __device__ __forceinline__ double read_w(const void *w, uint32_t e, int kind, do
     return static_cast<double>(static_cast<const S *>(w)[e]);
 }
 
-// One thread per row shorter than kWaveDeg, eight weight loads in flight,
-// added in storage order.
+// One thread per row shorter than kWaveDeg, added in storage order.  The
+// row's entries go in batches of eight, every load of a batch issued before
+// the first add (a short row's tail is one batch with its lanes past the row
+// masked), so a row costs one round trip per eight entries.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void deg_short(const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ eid,
                                                     const void *w, int kind, double value,
                                                     uint32_t n, bool accumulate,
                                                     double *__restrict__ deg) {
-    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+    // Each XCD takes a contiguous eighth of the rows, so the weights its rows
+    // read stay in its L2 (edge ids of nearby rows are near each other when
+    // edges are numbered graph by graph or row by row).
+    const XcdSpan sp = xcd_span(n);
+    for (uint32_t v = sp.begin + sp.index * kBlock + threadIdx.x; v < sp.begin + sp.count;
+         v += sp.groups * kBlock) {
         const uint32_t lo = off[v], hi = off[v + 1];
         if (hi - lo >= kWaveDeg) continue;
         double acc = accumulate ? deg[v] : 0.0;
-        uint32_t j = lo;
-        for (; j + 8 <= hi; j += 8) {
+        for (uint32_t j = lo; j < hi; j += 8) {
+            uint32_t e[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) e[u] = j + u < hi ? eid[j + u] : 0u;
             double x[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = read_w<S>(w, eid[j + u], kind, value);
+            for (int u = 0; u < 8; ++u) x[u] = j + u < hi ? read_w<S>(w, e[u], kind, value) : 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc += x[u];
+            for (int u = 0; u < 8; ++u)
+                if (j + u < hi) acc += x[u];
         }
-        for (; j < hi; ++j) acc += read_w<S>(w, eid[j], kind, value);
         deg[v] = acc;
     }
 }
@@ -116,14 +125,40 @@ __global__ __launch_bounds__(kBlock) void deg_long(const uint32_t *__restrict__ 
     if (threadIdx.x == 0) deg[v] = accumulate ? deg[v] + acc : acc;
 }
 
-// wcsc[j] = w(eid[j]) in the value type of the sweep.
+// wcsc[j] = w(eid[j]) in the value type of the sweep.  Each thread takes
+// eight entries a stride apart per round (coalesced), all eight index loads
+// and then all eight gathers in flight before the stores.
 template <typename S, typename V>
 __global__ __launch_bounds__(kBlock) void permute_w(const uint32_t *__restrict__ eid,
                                                     const void *w, int kind, double value,
                                                     uint64_t m, V *__restrict__ out) {
-    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < m;
-         j += (uint64_t)gridDim.x * kBlock)
-        out[j] = static_cast<V>(read_w<S>(w, eid[j], kind, value));
+    // Each XCD takes a contiguous eighth of the entries (see deg_short).
+    uint64_t begin = 0, end = m, first = (uint64_t)blockIdx.x * kBlock + threadIdx.x,
+             stride = (uint64_t)gridDim.x * kBlock;
+    if (gridDim.x % kXcds == 0) {
+        const uint32_t x = blockIdx.x % kXcds;
+        begin = m * x / kXcds;
+        end = m * (x + 1) / kXcds;
+        first = begin + (uint64_t)(blockIdx.x / kXcds) * kBlock + threadIdx.x;
+        stride = (uint64_t)(gridDim.x / kXcds) * kBlock;
+    }
+    for (uint64_t j0 = first; j0 < end; j0 += 8 * stride) {
+        uint32_t e[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t j = j0 + u * stride;
+            e[u] = j < end ? eid[j] : 0u;
+        }
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            x[u] = j0 + u * stride < end ? read_w<S>(w, e[u], kind, value) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t j = j0 + u * stride;
+            if (j < end) out[j] = static_cast<V>(x[u]);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void f32_to_f64(const float *__restrict__ a, uint64_t n,

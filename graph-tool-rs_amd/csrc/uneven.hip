@@ -184,9 +184,14 @@ __global__ __launch_bounds__(kBlock) void u_sweep(const UArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.list_n_next = 0u;
     __shared__ WaveLds<float> L;
     Fin fin(a);
-    spmv_waves<float, WEIGHTED>(a.tiles, nt, a.off, a.packed, a.bits, a.tbase,
+    // Workgroups are dealt to the XCDs in turn (workgroup b on XCD b % 8), so
+    // XCD x walks the x-th eighth of the list: its L2 then holds the vectors
+    // of the graphs in that eighth, not of every graph at once.
+    const XcdSpan sp = xcd_span(nt);
+    spmv_waves<float, WEIGHTED>(a.tiles, sp.count, a.off, a.packed, a.bits, a.tbase,
                                 a.src, a.w, a.long_part, a.short_row, L, fin,
-                                static_cast<const float *>(nullptr), 0u, a.list);
+                                static_cast<const float *>(nullptr), 0u, a.list + sp.begin,
+                                sp.first(kWavesPerBlock), sp.stride(kWavesPerBlock));
 }
 
 // Before the first sweep of a call: every tile listed.
