@@ -133,6 +133,45 @@ def profiled_traffic(workload: str, kernel: str):
     return int(rec["traffic_bytes"]), f"profiles/{f.name}"
 
 
+def gather_rate() -> dict:
+    """The random-gather rate of this device (libgt_probe.so, one workgroup of
+    eight wavefronts per CU walking 65,536 tiles of 1,024 labels with every
+    lane gathering): nanoseconds per gathered lane per compute unit, with the
+    1 MB window resident in every XCD's L2, and with labels spread over a
+    16 MB window (the FP32 contribution vector of R-MAT 22)."""
+    lib = C.CDLL(str(ROOT / "graph-tool-rs_amd" / "gt_hip" / "libgt_probe.so"))
+    lib.gt_probe_gather.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+    lib.gt_probe_error.restype = C.c_char_p
+    tiles, reps = 65536, 5
+    out = {"tiles": tiles, "entries_per_tile": 1024, "kernel": "gather_walk (libgt_probe.so)"}
+    for name, window in (("l2_resident_1mb", 1 << 18), ("uniform_16mb", 1 << 22)):
+        us, cus = C.c_double(), C.c_uint32()
+        if lib.gt_probe_gather(window, tiles, 1024, reps, C.byref(us), C.byref(cus)) != 0:
+            raise RuntimeError(lib.gt_probe_error().decode())
+        out[name] = {"us_per_launch": round(us.value, 2),
+                     "ns_per_lane_per_cu": round(us.value * 1e3 * cus.value / (tiles * 1024), 4)}
+        out["compute_units"] = cus.value
+    return out
+
+
+def cold_gathers(src, n: int, nhot: int) -> int:
+    """Entries whose source is outside the nhot most gathered vertices (the
+    relabelled order): the gathers a hot-slice sweep sends to memory."""
+    deg = np.bincount(src, minlength=n)
+    top = np.sort(deg)[::-1][:nhot]
+    return int(deg.sum() - top.sum())
+
+
+def hot_len(kb: int, vbytes: int, n: int) -> int:
+    """The hot slice the library uses for a tuning value (0: 128 KB)."""
+    if kb < 0:
+        return 0
+    kb = 128 if kb == 0 else min(kb, 128)
+    cap = 32704 if vbytes == 4 else 12224
+    return min(kb * 1024 // vbytes, cap, n) & ~3
+
+
 def pr_bytes(n: int, m: int, s: int) -> int:
     return 4 * m + 4 * (n + 1) + n * (4 * s + 4)
 
@@ -896,6 +935,25 @@ def main() -> int:
         "mixed_sweeps": {"total": sw32[0], "fp64_refinement": f64_in_mixed[1] // max(args.steps, 1)},
         "host_adjacency_s": round(t_adj, 2), "upload_s": round(t_up, 3),
     }
+
+    # ---- the gather ceiling: the time the sweeps' own memory gathers take
+    # at the device's random-gather rate (every gather an L2 hit, nothing
+    # else costing anything), beside the measured sweep times.
+    rate = gather_rate()
+    ns_l2 = rate["l2_resident_1mb"]["ns_per_lane_per_cu"]
+    cus = rate["compute_units"]
+    kb = dev.tuning()["sweep_hot_kb"]
+    ceiling = {"probe": rate}
+    for key, vbytes, avg in (("fp32_sweep", 4, avg32), ("fp64_sweep", 8, avg64)):
+        nh = hot_len(kb, vbytes, n)
+        cold = cold_gathers(src, n, nh)
+        floor_us = cold * ns_l2 / cus / 1e3
+        ceiling[key] = {"hot_slice": nh, "gathers_from_memory": cold,
+                        "floor_us": round(floor_us, 1), "sweep_us": round(avg * 1e6, 1),
+                        "floor_over_sweep": round(floor_us / (avg * 1e6), 4) if avg else None}
+    result["gather_ceiling"] = ceiling
+    log(f"[bench] gather ceiling: {ns_l2:.3f} ns per L2-hit lane per CU; FP32 sweep floor "
+        f"{ceiling['fp32_sweep']['floor_us']:.0f} us vs {avg32*1e6:.0f} us")
 
     # ---- eigenvector on the symmetrised graph
     if not args.no_eigen:

@@ -25,6 +25,8 @@ HIP_SOURCES = ["graph.hip", "props.hip", "pagerank.hip", "eigenvector.hip", "bui
                "views.hip", "batched.hip", "adjoint.hip", "relabel.hip", "uneven.hip", "degree.hip",
                "betweenness.hip", "sliced.hip"]
 CXX_SOURCES = ["host.cpp", "io.cpp"]
+PROBE_SOURCE = "gather_probe.hip"
+PROBE_LIB = PKG / "libgt_probe.so"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}", "-Wall",
           "-Wno-unused-result"]
@@ -74,6 +76,10 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp),
           *map(str, objs), "-lpthread"])
     os.replace(tmp, LIB)
+    # The measurement library (the random-gather rate behind the roofline).
+    tmp = PROBE_LIB.with_suffix(".so.tmp")
+    _run([hipcc, *COMMON, *HIP_FLAGS, "-shared", str(CSRC / PROBE_SOURCE), "-o", str(tmp)])
+    os.replace(tmp, PROBE_LIB)
     return LIB
 
 

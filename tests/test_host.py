@@ -32,6 +32,17 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTED) == names
 
 
+def test_probe_library_exports_its_header():
+    """libgt_probe.so (the measurement library bench.py times the random-gather
+    rate with) exports what include/gt_probe.h declares."""
+    import ctypes
+    text = (ROOT / "include" / "gt_probe.h").read_text()
+    names = sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(gt_\w+)\s*\(", text, re.M)))
+    assert names == ["gt_probe_error", "gt_probe_gather"]
+    L = ctypes.CDLL(str(ROOT / "graph-tool-rs_amd" / "gt_hip" / "libgt_probe.so"))
+    assert all(hasattr(L, n) for n in names)
+
+
 def test_er_generator_matches_oracle():
     for n, m, seed in [(10_000, 100_000, 0x243F6A8885A308D3), (7, 5, 1), (100, 200_000, 42)]:
         a = gt_hip.gen_er(n, m, seed)
