@@ -290,9 +290,9 @@ def bench_batched(args, torch, gt_hip, _lib) -> int:
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(
-                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[0],
+                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep_w")[0],
                      "traffic_source": profiled_traffic(
-                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep")[1],
+                         f"batched_ppr_rmat{scale}_ef{args.edgefactor}_k{k}", "ppr_sweep_w")[1],
                      "kernel": "ppr_sweep_w<true, 16> (batched SpMM sweep, wave tiles)",
                      "bytes_per_sweep": B, "bytes_per_solve": B_solve,
                      "avg_launch_us": round(avg * 1e6, 2), "copy_calibration": copy},
