@@ -124,3 +124,30 @@ def test_random_graphs_and_filtered_views(gpu):
     hist = np.bincount(deg[vm == 1])
     assert gt_hip.degree_histogram(f) == hist.tolist()
     assert gt_hip.out_degrees(f).data.tolist() == (np.bincount(src[keep], minlength=n) * vm).tolist()
+
+
+def test_weighted_out_degree_follows_storage_order_at_every_row_length(gpu):
+    """Rows of 1 to 20,000 out-entries with weights whose sum depends on the
+    order: up to 4,096 entries the device sum is the reference's sequential
+    sum in storage order (centrality.rs:281-297) bit for bit; longer rows use
+    a fixed tree order, within rounding of it."""
+    import oracle
+    lens = [1, 7, 31, 32, 33, 63, 64, 65, 1000, 4095, 4096, 4097, 9000, 20000]
+    n = len(lens) + 50
+    rng = np.random.default_rng(11)
+    s = np.concatenate([np.full(L, v, np.uint32) for v, L in enumerate(lens)])
+    t = rng.integers(0, n, len(s)).astype(np.uint32)
+    w = rng.uniform(0.0, 1.0, len(s)) * 10.0 ** rng.integers(-8, 8, len(s))
+    g = gt_hip.AdjList.with_vertices(n)
+    g.add_edges(s, t)
+    got = gt_hip.weighted_out_degrees(g, gt_hip.DenseProp.from_vec(g.graph_id, w))
+    _, _, _, out_off, _, out_eid = oracle.Adj(n, s, t).export()
+    for v, L in enumerate(lens):
+        acc = 0.0
+        for e in out_eid[out_off[v]:out_off[v + 1]]:
+            acc += float(w[e])
+        if L <= 4096:
+            assert got[v] == acc, (L, got[v], acc)
+        else:
+            assert abs(got[v] - acc) <= 1e-12 * acc, (L, got[v], acc)
+    assert (got[len(lens):] == 0.0).all()
