@@ -241,22 +241,39 @@ __global__ __launch_bounds__(kBlock) void adj_mu_long(const AdjArgs a, double *m
 // (s, t) = (row, entry) of e in the pull orientation's opposite (es / et:
 // the edge list, swapped for a reversed view).  Ids with no edge, and edges
 // whose deg_s is 0, get 0.  Reads and writes run in edge-id order.
-template <typename V, typename G, bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void grad_edges(
-    const uint32_t *__restrict__ es, const uint32_t *__restrict__ et,
-    const double *__restrict__ deg, const uint32_t *__restrict__ row_off,
-    const double *__restrict__ mu, const V *__restrict__ lam, const double *__restrict__ x,
-    double damping, uint64_t bound, G *__restrict__ grad) {
+//
+// Per source the edge pass needs d x_s / deg_s and mu_s: grad_coef packs
+// them into one 16-byte pair per vertex (in the caller's labels; mu arrives
+// in the relabelled order), so an edge gathers that pair and lambda_t -- two
+// gathers instead of five.  coef is (d x_s) / deg_s, the operations and
+// order of the single formula; a source with deg_s = 0 has coef 0 (and mu
+// 0), so its edges get 0 for every finite lambda.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void grad_coef(const double *__restrict__ deg,
+                                                    const uint32_t *__restrict__ row_off,
+                                                    const double *__restrict__ mu_rel,
+                                                    const uint32_t *__restrict__ perm,
+                                                    const double *__restrict__ x, double damping,
+                                                    uint32_t n, double2 *__restrict__ pair) {
+    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+        const double dg = WEIGHTED ? deg[v] : static_cast<double>(row_off[v + 1] - row_off[v]);
+        pair[v] = make_double2(dg != 0.0 ? damping * x[v] / dg : 0.0, mu_rel[perm[v]]);
+    }
+}
+
+template <typename V, typename G>
+__global__ __launch_bounds__(kBlock) void grad_edges(const uint32_t *__restrict__ es,
+                                                     const uint32_t *__restrict__ et,
+                                                     const double2 *__restrict__ pair,
+                                                     const V *__restrict__ lam, uint64_t bound,
+                                                     G *__restrict__ grad) {
     for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < bound;
          e += (uint64_t)gridDim.x * kBlock) {
         const uint32_t sv = es[e];
         G out = G(0);
         if (sv != kNoEdge) {
-            const double dg =
-                WEIGHTED ? deg[sv] : static_cast<double>(row_off[sv + 1] - row_off[sv]);
-            if (dg != 0.0)
-                out = static_cast<G>(damping * x[sv] / dg *
-                                     (static_cast<double>(lam[et[e]]) - mu[sv]));
+            const double2 p = pair[sv];
+            out = static_cast<G>(p.x * (static_cast<double>(lam[et[e]]) - p.y));
         }
         grad[e] = out;
     }
@@ -538,9 +555,9 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
             gdst = ws.gw.ptr;
         }
         GT_TRY(ensure_edge_list(g));
-        GT_TRY(ws.mu.reserve(sizeof(double) * n));
+        GT_TRY(ws.mu.reserve(sizeof(double2) * n));
         GT_TRY(ws.arl[3].reserve(sizeof(double) * n));
-        double *mu = ws.mu.as<double>();
+        double2 *pair = ws.mu.as<double2>();
         double *mr = ws.arl[3].as<double>();
         const dim3 mg(wave_grid(occupancy_cap((const void *)adj_mu<V, true>, pc), r.ntiles)),
             lg(fold_grid(r.nlong)), bd(kBlock);
@@ -551,29 +568,25 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
             hipLaunchKernelGGL((adj_mu<V, false>), mg, bd, 0, s, r, lr, mr);
             hipLaunchKernelGGL((adj_mu_long<V, false>), lg, bd, 0, s, r, mr);
         }
-        hipLaunchKernelGGL(adj_gather<double>, dim3(vgrid(n)), bd, 0, s, mr, perm, n, mu);
+        const uint32_t *ro = push->off.as<uint32_t>();
+        if (weighted)
+            hipLaunchKernelGGL(grad_coef<true>, dim3(vgrid(n)), bd, 0, s, a.deg, ro, mr, perm, x,
+                               damping, n, pair);
+        else
+            hipLaunchKernelGGL(grad_coef<false>, dim3(vgrid(n)), bd, 0, s, a.deg, ro, mr, perm, x,
+                               damping, n, pair);
         // Rows of the push orientation are sources for a directed view and
         // targets for a reversed one.
         const uint32_t *es = (reversed ? g->etgt : g->esrc).as<uint32_t>();
         const uint32_t *et = (reversed ? g->esrc : g->etgt).as<uint32_t>();
         const uint64_t eb = g->edge_bound;
         const dim3 gd((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(16384, (eb + kBlock - 1) / kBlock)));
-        const uint32_t *ro = push->off.as<uint32_t>();
-        if (gw_dtype == GT_F64) {
-            if (weighted)
-                hipLaunchKernelGGL((grad_edges<V, double, true>), gd, bd, 0, s, es, et, a.deg, ro,
-                                   mu, lf, x, damping, eb, static_cast<double *>(gdst));
-            else
-                hipLaunchKernelGGL((grad_edges<V, double, false>), gd, bd, 0, s, es, et, a.deg, ro,
-                                   mu, lf, x, damping, eb, static_cast<double *>(gdst));
-        } else {
-            if (weighted)
-                hipLaunchKernelGGL((grad_edges<V, float, true>), gd, bd, 0, s, es, et, a.deg, ro,
-                                   mu, lf, x, damping, eb, static_cast<float *>(gdst));
-            else
-                hipLaunchKernelGGL((grad_edges<V, float, false>), gd, bd, 0, s, es, et, a.deg, ro,
-                                   mu, lf, x, damping, eb, static_cast<float *>(gdst));
-        }
+        if (gw_dtype == GT_F64)
+            hipLaunchKernelGGL((grad_edges<V, double>), gd, bd, 0, s, es, et, pair, lf, eb,
+                               static_cast<double *>(gdst));
+        else
+            hipLaunchKernelGGL((grad_edges<V, float>), gd, bd, 0, s, es, et, pair, lf, eb,
+                               static_cast<float *>(gdst));
         GT_HIP(hipGetLastError());
         if (grad_loc == GT_HOST && gb)
             GT_HIP(hipMemcpyAsync(grad_w, gdst, gb, hipMemcpyDeviceToHost, s));

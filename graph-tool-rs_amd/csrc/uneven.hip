@@ -390,23 +390,38 @@ __global__ __launch_bounds__(kBlock) void u_mu_long(const UArgs a, uint32_t nlon
         fin(a.long_rows[k], long_row_sum(a.long_part, a.long_slot, k));
 }
 
-// dL/dw_e = d_g x_s / deg_s (lambda_t - mu_s) in edge-id order.
-template <bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void u_grad_edges(const UArgs a, const uint32_t *es,
-                                                       const uint32_t *et, const uint32_t *vgraph,
-                                                       const double *mu, const float *lam,
-                                                       const double *x, uint64_t bound,
-                                                       float *grad) {
-    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < bound;
-         e += (uint64_t)gridDim.x * kBlock) {
+// dL/dw_e = d_g x_s / deg_s (lambda_t - mu_s) in edge-id order.  The per-
+// source factors come packed (u_grad_coef): pair[s] = ((d_g x_s) / deg_s, mu_s),
+// 0 and 0 for a source with deg_s = 0, so an edge gathers one 16-byte pair
+// and lambda_t.  Each XCD takes a contiguous eighth of the edge ids, so the
+// pairs of the graphs it covers stay in its L2.
+__global__ __launch_bounds__(kBlock) void u_grad_coef(const UArgs a, const uint32_t *vgraph,
+                                                      const double *mu, const double *x,
+                                                      uint32_t n, double2 *pair) {
+    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+        const double dg = a.deg ? a.deg[v] : static_cast<double>(a.deg_off[v + 1] - a.deg_off[v]);
+        pair[v] = make_double2(dg != 0.0 ? a.damp[vgraph[v]] * x[v] / dg : 0.0, mu[v]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void u_grad_edges(const uint32_t *es, const uint32_t *et,
+                                                       const double2 *pair, const float *lam,
+                                                       uint64_t bound, float *grad) {
+    uint64_t begin = 0, end = bound, first = (uint64_t)blockIdx.x * kBlock + threadIdx.x,
+             stride = (uint64_t)gridDim.x * kBlock;
+    if (gridDim.x % kXcds == 0) {
+        const uint32_t x = blockIdx.x % kXcds;
+        begin = bound * x / kXcds;
+        end = bound * (x + 1) / kXcds;
+        first = begin + (uint64_t)(blockIdx.x / kXcds) * kBlock + threadIdx.x;
+        stride = (uint64_t)(gridDim.x / kXcds) * kBlock;
+    }
+    for (uint64_t e = first; e < end; e += stride) {
         const uint32_t sv = es[e];
         float out = 0.f;
         if (sv != kNoEdge) {
-            const double dg =
-                WEIGHTED ? a.deg[sv] : static_cast<double>(a.deg_off[sv + 1] - a.deg_off[sv]);
-            if (dg != 0.0)
-                out = static_cast<float>(a.damp[vgraph[sv]] * x[sv] / dg *
-                                         (static_cast<double>(lam[et[e]]) - mu[sv]));
+            const double2 p = pair[sv];
+            out = static_cast<float>(p.x * (static_cast<double>(lam[et[e]]) - p.y));
         }
         grad[e] = out;
     }
@@ -822,7 +837,9 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     if (grad_w) {
         GT_TRY(ensure_edge_list(g));
         GT_TRY(ws.mu.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
+        GT_TRY(ws.arl[3].reserve(sizeof(double2) * std::max<uint32_t>(1, n)));
         double *mu = ws.mu.as<double>();
+        double2 *pair = ws.arl[3].as<double2>();
         UArgs b = a;
         b.src = lf;
         const dim3 lg(fold_grid(g->out.nlong));
@@ -841,12 +858,11 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
         }
         const dim3 eg((unsigned)std::max<uint64_t>(
             1, std::min<uint64_t>(16384, (eb + kBlock - 1) / kBlock)));
-        if (weighted)
-            hipLaunchKernelGGL(u_grad_edges<true>, eg, bd, 0, cs, a, g->esrc.as<uint32_t>(),
-                               g->etgt.as<uint32_t>(), vg, mu, lf, xd, eb, gdst);
-        else
-            hipLaunchKernelGGL(u_grad_edges<false>, eg, bd, 0, cs, a, g->esrc.as<uint32_t>(),
-                               g->etgt.as<uint32_t>(), vg, mu, lf, xd, eb, gdst);
+        UArgs c = a;
+        c.deg = weighted ? a.deg : nullptr;
+        hipLaunchKernelGGL(u_grad_coef, dim3(ugrid(n)), bd, 0, cs, c, vg, mu, xd, n, pair);
+        hipLaunchKernelGGL(u_grad_edges, eg, bd, 0, cs, g->esrc.as<uint32_t>(),
+                           g->etgt.as<uint32_t>(), pair, lf, eb, gdst);
         if (loc == GT_HOST && eb)
             GT_HIP(hipMemcpyAsync(grad_w, gdst, sizeof(float) * eb, hipMemcpyDeviceToHost, cs));
     }
