@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void u_reset(UArgs a) {
     }
 }
 
-// Forward start: x0 = pers = 1/n_g, c0 = x0 / deg, sinks counted per graph.
+// Forward start: x0 = pers = 1/n_g, c0 = x0 / deg.
 template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void u_fwd_init(UArgs a, const uint32_t *vgraph, uint32_t n) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -301,21 +301,7 @@ __global__ __launch_bounds__(kBlock) void u_fwd_init(UArgs a, const uint32_t *vg
         a.x_next[i] = x0;
         const double dg =
             WEIGHTED ? a.deg[i] : static_cast<double>(a.deg_off[i + 1] - a.deg_off[i]);
-        if (dg != 0.0) {
-            a.c_next[i] = static_cast<float>(static_cast<double>(x0) / dg);
-        } else {
-            a.c_next[i] = 0.f;
-            atomicAdd(&a.st.nsink[g], 1u);
-        }
-    }
-}
-
-// The first dangling mass of each graph: (number of sinks) x x0.
-__global__ __launch_bounds__(kBlock) void u_fwd_psink(UArgs a) {
-    for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < a.ngraphs; g += gridDim.x * kBlock) {
-        const uint32_t ng = a.voff[g + 1] - a.voff[g];
-        if (ng == 0) continue;
-        a.st.p_sink[g] = (double)a.st.nsink[g] * (double)static_cast<float>(1.0 / (double)ng);
+        a.c_next[i] = dg != 0.0 ? static_cast<float>(static_cast<double>(x0) / dg) : 0.f;
     }
 }
 
@@ -336,6 +322,19 @@ __global__ __launch_bounds__(kBlock) void u_graph_sums(const uint32_t *__restric
         }
     }
 }
+
+// The first dangling mass of each graph: (number of sinks) x x0, the sinks
+// counted by the graph's own workgroup (no atomics on a shared counter).
+struct FwdStartSinks {
+    const double *deg;          // weighted out-degree, or null
+    const uint32_t *deg_off;    // unity: push offsets
+    const uint32_t *voff;
+    __device__ void operator()(uint32_t g, uint32_t v, double &ps, double &) const {
+        const double dg = deg ? deg[v] : static_cast<double>(deg_off[v + 1] - deg_off[v]);
+        if (dg == 0.0)
+            ps += static_cast<double>(static_cast<float>(1.0 / (double)(voff[g + 1] - voff[g])));
+    }
+};
 
 // Adjoint start: lambda_0 = c, p . lambda_0 and ||c||_1 per graph.
 struct AdjStartSums {
@@ -723,7 +722,10 @@ gt_status pagerank_uneven_impl(gt_graph *g, uint32_t G, const uint32_t *voff,
             hipLaunchKernelGGL(u_fwd_init<true>, dim3(ugrid(n)), dim3(kBlock), 0, s, b, vg, n);
         else
             hipLaunchKernelGGL(u_fwd_init<false>, dim3(ugrid(n)), dim3(kBlock), 0, s, b, vg, n);
-        hipLaunchKernelGGL(u_fwd_psink, dim3(ugrid(G)), dim3(kBlock), 0, s, b);
+        const unsigned gg = std::min<uint32_t>(G, 65535);
+        hipLaunchKernelGGL(u_graph_sums<FwdStartSinks>, dim3(gg), dim3(kBlock), 0, s, a.voff, G,
+                           FwdStartSinks{weighted ? a.deg : nullptr, a.deg_off, a.voff},
+                           a.st.p_sink, a.st.scale);
         GT_HIP(hipGetLastError());
     }
     const int cap_w = occupancy_cap(
