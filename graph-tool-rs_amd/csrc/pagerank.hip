@@ -992,7 +992,12 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         // HBM): no init kernel, 4-byte gathers, FP64 sums and output.
         PrArgs ar = S.base;
         if (S.plan.weighted) {
-            GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
+            // An f32 map's weights are already staged as f32 in packed order
+            // for the FP32 sweeps: widening them is exact.
+            if (S.wdtype == GT_F32 && weight->kind == GT_MAP_DENSE)
+                GT_TRY(widen_pull_weights(g, *S.pull, ws.wcsc, ws.wcsc64));
+            else
+                GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
             ar.w = ws.wcsc64.ptr;
         }
         GT_TRY(ws.x[0].reserve(sizeof(double) * n));

@@ -302,6 +302,16 @@ gt_status stage_pull_weights(gt_graph *g, const gt_prop *w, Orient &pull, const 
 }
 template gt_status stage_pull_weights<double>(gt_graph *, const gt_prop *, Orient &, const void *,
                                               int, DevBuf &);
+
+gt_status widen_pull_weights(gt_graph *g, const Orient &pull, const DevBuf &f32, DevBuf &out) {
+    const uint64_t m = pull.h_off.empty() ? 0 : pull.h_off.back();
+    GT_TRY(out.reserve(sizeof(double) * std::max<uint64_t>(1, g->m)));
+    if (m == 0) return GT_OK;
+    hipLaunchKernelGGL(f32_to_f64, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, f32.as<float>(), m,
+                       out.as<double>());
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
 template gt_status stage_pull_weights<float>(gt_graph *, const gt_prop *, Orient &, const void *,
                                              int, DevBuf &);
 

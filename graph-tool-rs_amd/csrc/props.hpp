@@ -27,6 +27,10 @@ gt_status stage_csc_weights(gt_graph *g, const gt_prop *w, Orient &pull, const v
 template <typename V>
 gt_status stage_pull_weights(gt_graph *g, const gt_prop *w, Orient &pull, const void *src,
                              int dtype, DevBuf &out);
+// The f32 pull weights (packed order, as stage_pull_weights<float> wrote them)
+// widened to f64: the same values as staging an f32 map in f64 again, without
+// the gather through the edge ids.
+gt_status widen_pull_weights(gt_graph *g, const Orient &pull, const DevBuf &f32, DevBuf &out);
 
 // A vertex map as a scalar (UNITY / CONSTANT / absent -> default_value) or
 // an f64 device array.
