@@ -172,6 +172,15 @@ def hot_len(kb: int, vbytes: int, n: int) -> int:
     return min(kb * 1024 // vbytes, cap, n) & ~3
 
 
+def profiled_misses(workload: str, kernel: str):
+    """L2 misses per launch of `kernel` from the committed counter profile."""
+    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    rec = json.loads(f.read_text()).get("kernels", {}).get(kernel)
+    return rec.get("tcc_miss") if rec else None
+
+
 def pr_bytes(n: int, m: int, s: int) -> int:
     return 4 * m + 4 * (n + 1) + n * (4 * s + 4)
 
@@ -939,21 +948,38 @@ def main() -> int:
     # ---- the gather ceiling: the time the sweeps' own memory gathers take
     # at the device's random-gather rate (every gather an L2 hit, nothing
     # else costing anything), beside the measured sweep times.
+    # A second floor adds the L2 misses the committed counter profile
+    # measured for the kernel, each at the probe's extra cost of a miss (the
+    # 16 MB window's rate, where nearly every gather misses, less the
+    # L2-resident rate).
     rate = gather_rate()
     ns_l2 = rate["l2_resident_1mb"]["ns_per_lane_per_cu"]
+    ns_miss = rate["uniform_16mb"]["ns_per_lane_per_cu"] - ns_l2
     cus = rate["compute_units"]
     kb = dev.tuning()["sweep_hot_kb"]
-    ceiling = {"probe": rate}
-    for key, vbytes, avg in (("fp32_sweep", 4, avg32), ("fp64_sweep", 8, avg64)):
-        nh = hot_len(kb, vbytes, n)
-        cold = cold_gathers(src, n, nh)
+    ceiling = {"probe": rate, "miss_extra_ns_per_cu": round(ns_miss, 4)}
+
+    def floors(key, cold, nh, avg, pmc_kernel):
+        misses = profiled_misses(workload, pmc_kernel)
         floor_us = cold * ns_l2 / cus / 1e3
-        ceiling[key] = {"hot_slice": nh, "gathers_from_memory": cold,
-                        "floor_us": round(floor_us, 1), "sweep_us": round(avg * 1e6, 1),
-                        "floor_over_sweep": round(floor_us / (avg * 1e6), 4) if avg else None}
+        rec = {"hot_slice": nh, "gathers_from_memory": cold, "floor_us": round(floor_us, 1),
+               "sweep_us": round(avg * 1e6, 1),
+               "floor_over_sweep": round(floor_us / (avg * 1e6), 4) if avg else None}
+        if misses is not None:
+            fm = (cold * ns_l2 + misses * ns_miss) / cus / 1e3
+            rec.update({"l2_misses_profiled": int(misses), "floor_with_misses_us": round(fm, 1),
+                        "floor_with_misses_over_sweep": round(fm / (avg * 1e6), 4) if avg else None})
+        ceiling[key] = rec
+
+    for key, vbytes, avg, pk in (("fp32_sweep", 4, avg32, "pr_sweep_hot<float, false, false>"),
+                                 ("fp64_sweep", 8, avg64, "pr_sweep_hot<double, false, false>")):
+        nh = hot_len(kb, vbytes, n)
+        floors(key, cold_gathers(src, n, nh), nh, avg, pk)
     result["gather_ceiling"] = ceiling
     log(f"[bench] gather ceiling: {ns_l2:.3f} ns per L2-hit lane per CU; FP32 sweep floor "
-        f"{ceiling['fp32_sweep']['floor_us']:.0f} us vs {avg32*1e6:.0f} us")
+        f"{ceiling['fp32_sweep']['floor_us']:.0f} us "
+        f"({ceiling['fp32_sweep'].get('floor_with_misses_us', float('nan')):.0f} us with its "
+        f"L2 misses) vs {avg32*1e6:.0f} us")
 
     # ---- eigenvector on the symmetrised graph
     if not args.no_eigen:
@@ -996,6 +1022,10 @@ def main() -> int:
             "traffic": profiled_traffic(workload, "ev_sweep_hot")[0],
         }
         x_eig = xs.cpu().numpy().copy()
+        nh_e = hot_len(kb, 8, n)
+        deg_s = np.bincount(S, minlength=n)
+        cold_e = int(deg_s.sum() - np.sort(deg_s)[::-1][:nh_e].sum())
+        floors("eigenvector_sweep", cold_e, nh_e, avge, "ev_sweep_hot<false>")
         log(f"[bench] eigenvector: {dte/args.steps*1e3:.3f} ms/solve, sweeps {swe[0]}, "
             f"lambda {eig.value:.6f}")
         devs.close()
