@@ -218,11 +218,13 @@ def _tune(args, dev, workload: str, weight) -> dict:
 
 
 def _timed_solves(torch, dev, kind: int, steps: int, warmup: int, solve):
-    """Per-step timed solves with the library's sweep events on; returns
-    (median step x steps, outputs, summed kernel ms, launches, step stats)."""
+    """Per-step timed solves, then the same solves again with the library's
+    sweep events on for the kernel times (an event around every sweep costs
+    the solve a few microseconds per sweep, so the timed solves run without
+    them); returns (median step x steps, outputs, summed kernel ms, launches,
+    step stats)."""
     for _ in range(warmup):
         solve()
-    dev.set_profiling(True)
     times, outs = [], []
     for _ in range(steps):
         torch.cuda.synchronize()
@@ -230,6 +232,10 @@ def _timed_solves(torch, dev, kind: int, steps: int, warmup: int, solve):
         outs.append(solve())
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
+    dev.set_profiling(True)
+    for _ in range(steps):
+        solve()
+    torch.cuda.synchronize()
     ms_k, launches, _ = dev.sweep_stats(kind)
     dev.set_profiling(False)
     return float(np.median(times)) * steps, outs, ms_k, launches, timing_stats(times)
@@ -829,10 +835,12 @@ def main() -> int:
 
     def timed(fn: str, kind: int):
         # Per-step times (each solve between two synchronisations); the
-        # median step is the reported one, the mean is kept beside it.
+        # median step is the reported one, the mean is kept beside it.  The
+        # kernel times come from the same solves run again with the
+        # library's sweep events on (events around every sweep cost a solve
+        # about 140 us, so the timed solves run without them).
         for _ in range(args.warmup):
             solve(fn)
-        dev.set_profiling(True)
         times, sweeps = [], []
         for _ in range(args.steps):
             torch.cuda.synchronize()
@@ -840,6 +848,10 @@ def main() -> int:
             sweeps.append(solve(fn))
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t)
+        dev.set_profiling(True)
+        for _ in range(args.steps):
+            solve(fn)
+        torch.cuda.synchronize()
         ms_k, launches, _ = dev.sweep_stats(kind)
         ms_other = {k: dev.sweep_stats(k)[:2] for k in range(4) if k != kind}
         dev.set_profiling(False)
@@ -930,7 +942,9 @@ def main() -> int:
             "avg_launch_us": round(avg64 * 1e6, 2),
         },
         "timing": {"mixed": st32, "fp64_dropin": st64,
-                   "stat": f"median of {args.steps} solves after {args.warmup} warm-ups"},
+                   "stat": f"median of {args.steps} solves after {args.warmup} warm-ups, sweep "
+                           f"events off; kernel averages from the same solves run again with "
+                           f"events"},
         "mixed_vs_fp64_speedup": round(dt64 / dt32, 3),
         # The speedup in two factors: fewer sweeps (the FP32 phase stops at
         # its noise floor, then one FP64 sweep) and a faster sweep kernel.
@@ -1002,12 +1016,15 @@ def main() -> int:
 
         for _ in range(args.warmup):
             esolve()
-        devs.set_profiling(True)
         torch.cuda.synchronize()
         t = time.perf_counter()
         swe = [esolve() for _ in range(args.steps)]
         torch.cuda.synchronize()
         dte = time.perf_counter() - t
+        devs.set_profiling(True)
+        for _ in range(args.steps):
+            esolve()
+        torch.cuda.synchronize()
         mse, lae, _ = devs.sweep_stats(_lib.GT_SWEEP_EIGENVECTOR)
         devs.set_profiling(False)
         be = ev_bytes(n, ms_)
