@@ -1,0 +1,500 @@
+// Block-tile sweeps of the FP32 PageRank phase: the layout builder, the sweep
+// and fold kernels.  What a sweep computes is pagerank_with's update
+// (crates/gt-algo/src/centrality.rs:345-363) with the FP32 form's types: the
+// FP32 contributions c = x / deg of the last sweep are gathered, each row's
+// sum is formed in 64-bit fixed point (blocks.hpp), and the teleport update,
+// dangling mass and L1 change are FP64, as in the wave engine's FP32 sweep.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <mutex>
+#include <thread>
+
+#include "blocks.hpp"
+#include "device_common.hpp"
+
+namespace gt {
+
+constexpr int kBlkWaves = 8;
+constexpr int kBlkThreads = kBlkWaves * kWave;
+// Groups in flight per wavefront: 16 coalesced word loads, then 16 gathers.
+constexpr int kBlkUnroll = 16;
+constexpr uint32_t kBlkLds = kBlkAcc + kBlkLongMax * kWave;
+static_assert(kBlkLds * 8 <= 160 * 1024, "accumulators fit one CU's LDS");
+static_assert(kBlkNull >= kBlkRows && kBlkNull < kBlkAcc, "padding adds into a dummy");
+static_assert(kBlkLds <= (1u << kBlkCodeBits), "accumulator indices fit their bits");
+// Row codes of the sort key: a short row's r - r0, the j-th long row's
+// kBlkAcc + j (both below 2^kBlkCodeBits).
+constexpr uint32_t kBlkKeyLong0 = kBlkAcc;
+constexpr uint32_t kBlkCodeMask = (1u << kBlkCodeBits) - 1u;
+typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+typedef const __attribute__((address_space(4))) BlkTile ConstBlkTile;
+
+// The fixed-point exponent for an iterate of L1 mass `mass` (<= kBlkMassMax):
+// 2^K * mass < 2^62.  A source's terms sum to at most its |x| (each is
+// x / deg, and it has deg of them), so every partial sum of a sweep's terms,
+// scaled by 2^K, stays below 2^62.
+__device__ __forceinline__ int blk_scale(double mass) {
+    return mass > 0.0 ? 61 - ilogb(mass) : 0;
+}
+
+// round(c * 2^K) (ties to even) as a two's complement 64-bit integer.  In
+// FP64 every step is exact: c * 2^K has at most 24 significant bits and
+// |c| * 2^K < 2^62 (blk_scale), so the high and low 32-bit halves are found
+// by floor and fma and converted without rounding.
+__device__ __forceinline__ unsigned long long blk_fixed(float c, int K) {
+    const double f = rint(ldexp(static_cast<double>(c), K));
+    const double h = floor(ldexp(f, -32));
+    const double l = fma(h, -0x1p32, f);
+    const uint32_t hi = static_cast<uint32_t>(static_cast<int32_t>(h));
+    const uint32_t lo = static_cast<uint32_t>(l);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Tile t's descriptor, as scalar loads (the table is read-only in a sweep).
+__device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
+    ConstBlkTile *c = (ConstBlkTile *)tiles + t;
+    BlkTile r;
+    r.r0 = c->r0;
+    r.r1 = c->r1;
+    r.g0 = c->g0;
+    r.g1 = c->g1;
+    r.lfirst = c->lfirst;
+    r.nlong = c->nlong;
+    return r;
+}
+
+// Row i's update from its sum `acc`: PrFin's FP32 finisher (the teleport
+// update in FP64, the iterate and its contribution rounded to FP32), plus
+// the L1 mass of the new iterate.
+template <bool DPERS>
+__device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc,
+                                           double p_sink, double &dsum, double &ssum,
+                                           double &msum) {
+    const float xc = b.x_cur[i];
+    const uint32_t dg = b.out_off[i + 1] - b.out_off[i];
+    const double pv = DPERS ? b.pers[i] : b.pval;
+    const double nv = (1.0 - b.damping) * pv + b.damping * (p_sink * pv + acc);
+    const float nvv = static_cast<float>(nv);
+    dsum += fabs(static_cast<double>(nvv) - static_cast<double>(xc));
+    msum += fabs(static_cast<double>(nvv));
+    b.x_next[i] = nvv;
+    if (dg != 0) {
+        b.c_next[i] = static_cast<float>(static_cast<double>(nvv) / static_cast<double>(dg));
+    } else {
+        b.c_next[i] = 0.0f;
+        ssum += static_cast<double>(nvv);
+    }
+}
+
+// One sweep.  Workgroups take tiles from bs->next_tile until none is left.
+// Per tile: the wavefronts walk its groups interleaved (wavefront w takes
+// groups w, w + 8, ...), so the workgroup's gathers move up the source
+// labels together; each lane adds its fixed-point term into the row's LDS
+// accumulator, or, for a long row, into its own lane's slot of that row
+// (rotated by the row index, so neither the adds nor the later column sums
+// meet on one bank).  Then the long rows' lane slots are summed into their
+// int64 slots in global memory, and the tile's rows are finished, each
+// accumulator read once and cleared for the next tile.  The tile's delta,
+// dangling and mass partials go to its own entries of b.part, so the fold
+// sums them in tile order whichever workgroup took the tile.
+template <bool DPERS>
+__global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
+    SweepState *st = b.st;
+    if (st->done) return;
+    __shared__ unsigned long long acc[kBlkLds];
+    __shared__ double red[3 * kBlkWaves];
+    __shared__ uint32_t s_tile[2];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & (kWave - 1);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const double mass = b.bs->mass;
+    if (!(mass <= kBlkMassMax)) {
+        if (blockIdx.x == 0 && tid == 0) {
+            st->bail = 1u;
+            st->done = 1u;
+        }
+        return;
+    }
+    const int K = blk_scale(mass);
+    const double p_sink = st->p_sink;
+    for (uint32_t i = tid; i < kBlkLds; i += kBlkThreads) acc[i] = 0ull;
+    if (tid == 0) s_tile[0] = atomicAdd(&b.bs->next_tile, 1u);
+    __syncthreads();
+    for (uint32_t it = 0;; ++it) {
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
+        if (t >= b.ntiles) break;
+        const BlkTile T = blk_tile(b.tiles, t);
+        const uint32_t ng = T.g1 - T.g0;
+        ConstU32 *gbase = (ConstU32 *)b.gbase;
+        for (uint32_t gi = w; gi < ng; gi += kBlkWaves * kBlkUnroll) {
+            uint32_t word[kBlkUnroll], base[kBlkUnroll];
+#pragma unroll
+            for (int u = 0; u < kBlkUnroll; ++u) {
+                const uint32_t gg = gi + u * kBlkWaves;
+                const bool ok = gg < ng;
+                word[u] = ok ? b.words[(size_t)(T.g0 + gg) * kWave + lane] : kBlkNull;
+                base[u] = ok ? gbase[T.g0 + gg] : 0u;
+            }
+            float v[kBlkUnroll];
+#pragma unroll
+            for (int u = 0; u < kBlkUnroll; ++u) v[u] = b.c_cur[base[u] + (word[u] >> kBlkCodeBits)];
+#pragma unroll
+            for (int u = 0; u < kBlkUnroll; ++u)
+                __hip_atomic_fetch_add(&acc[word[u] & kBlkCodeMask], blk_fixed(v[u], K),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        if (tid == 0) s_tile[(it + 1) & 1] = atomicAdd(&b.bs->next_tile, 1u);
+        for (uint32_t j = tid; j < T.nlong; j += kBlkThreads) {
+            unsigned long long s = 0;
+            for (uint32_t l = 0; l < kWave; ++l) {
+                const uint32_t q = kBlkAcc + j * kWave + ((l + j) & (kWave - 1));
+                s += acc[q];
+                acc[q] = 0ull;
+            }
+            atomicAdd(&b.slots[b.ltab[T.lfirst + j]], s);
+        }
+        double dsum = 0.0, ssum = 0.0, msum = 0.0;
+        for (uint32_t i = T.r0 + tid; i < T.r1; i += kBlkThreads) {
+            if ((b.lmask[i >> 5] >> (i & 31)) & 1u) continue;
+            const uint32_t q = i - T.r0;
+            const long long sv = static_cast<long long>(acc[q]);
+            acc[q] = 0ull;
+            blk_finish<DPERS>(b, i, ldexp(static_cast<double>(sv), -K), p_sink, dsum, ssum, msum);
+        }
+        dsum = wave_sum(dsum);
+        ssum = wave_sum(ssum);
+        msum = wave_sum(msum);
+        if (lane == 0) {
+            red[w] = dsum;
+            red[kBlkWaves + w] = ssum;
+            red[2 * kBlkWaves + w] = msum;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double d = 0.0, s = 0.0, ms = 0.0;
+            for (int k = 0; k < kBlkWaves; ++k) {
+                d += red[k];
+                s += red[kBlkWaves + k];
+                ms += red[2 * kBlkWaves + k];
+            }
+            b.part[t] = d;
+            b.part[b.ntiles + t] = s;
+            b.part[2 * b.ntiles + t] = ms;
+        }
+    }
+}
+
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if (lane == 0) {
+        red[wid] = a;
+        red[4 + wid] = b;
+        red[8 + wid] = c;
+    }
+    __syncthreads();
+    a = (red[0] + red[1]) + (red[2] + red[3]);
+    b = (red[4] + red[5]) + (red[6] + red[7]);
+    c = (red[8] + red[9]) + (red[10] + red[11]);
+    __syncthreads();
+}
+
+// Completes the sweep: the long rows (one thread each; each slot is read and
+// cleared for the next sweep), then, in the last workgroup to arrive, the
+// tile partials in tile order and this kernel's in workgroup order, the
+// convergence decision and the next sweep's mass.
+template <bool DPERS>
+__global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
+    SweepState *st = b.st;
+    if (st->done) return;
+    __shared__ double red[12];
+    __shared__ uint32_t flag;
+    const int K = blk_scale(b.bs->mass);
+    const double p_sink = st->p_sink;
+    double d = 0.0, s = 0.0, ms = 0.0;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock) {
+        const uint32_t r = b.long_rows[k];
+        const long long sv = static_cast<long long>(b.slots[k]);
+        b.slots[k] = 0ull;
+        blk_finish<DPERS>(b, r, ldexp(static_cast<double>(sv), -K), p_sink, d, s, ms);
+    }
+    block_sum3(d, s, ms, red);
+    double *fp = b.fold_part;
+    if (threadIdx.x == 0) {
+        fp[blockIdx.x] = d;
+        fp[gridDim.x + blockIdx.x] = s;
+        fp[2 * gridDim.x + blockIdx.x] = ms;
+    }
+    if (!last_to_arrive(&st->counter, &flag)) return;
+    d = 0.0;
+    s = 0.0;
+    ms = 0.0;
+    for (uint32_t k = threadIdx.x; k < b.ntiles; k += kBlock) {
+        d += b.part[k];
+        s += b.part[b.ntiles + k];
+        ms += b.part[2 * b.ntiles + k];
+    }
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        d += fp[k];
+        s += fp[gridDim.x + k];
+        ms += fp[2 * gridDim.x + k];
+    }
+    block_sum3(d, s, ms, red);
+    if (threadIdx.x == 0) {
+        const uint64_t it = st->iter + 1;
+        const double thr = fmax(b.eps_abs, b.eps_rel * st->aux);
+        st->delta = d;
+        st->p_sink = s;
+        st->iter = it;
+        st->launches += 1;
+        st->done = (!(d >= thr) || (b.max_iter > 0 && it >= b.max_iter)) ? 1u : 0u;
+        st->counter = 0;
+        b.bs->mass = ms;
+        b.bs->next_tile = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void pr_start_blk(const BlkArgs b) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock)
+        b.slots[k] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.bs->mass = b.st->aux;
+        b.bs->next_tile = 0;
+        b.st->bail = 0;
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+namespace {
+
+int device_cus() {
+    static std::mutex mu;
+    static int cus = -1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cus < 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 0;
+    }
+    return cus > 0 ? cus : 256;
+}
+
+// A tile as the host builds it: rows [r0, r1) with entries [e0, e1) of the
+// orientation, or a chunk [e0, e1) of one long row (chunk set).
+struct HostTile {
+    uint32_t r0, r1;
+    uint64_t e0, e1;
+    uint32_t lfirst, nlong;
+    bool chunk;
+};
+
+// Runs f(k) for k in [0, count) on the host's cores.
+template <typename F>
+void parallel_for(uint32_t count, F f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (count < 4 || nt == 1) {
+        for (uint32_t k = 0; k < count; ++k) f(k);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t)
+        pool.emplace_back([&]() {
+            for (uint32_t k; (k = next.fetch_add(1)) < count;) f(k);
+        });
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace
+
+gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
+                        Blocks **out) {
+    Blocks &B = R.blk;
+    *out = &B;
+    if (B.ready && B.tile_nnz == tile_nnz && B.long_min == long_min) return GT_OK;
+    B.ready = false;
+    const uint32_t n = g->n;
+    const std::vector<uint32_t> &off = R.pull.h_off;
+    const uint64_t m = off[n];
+    std::vector<uint32_t> idx(m);
+    if (m)
+        GT_HIP(hipMemcpy(idx.data(), R.pull.idx.ptr, sizeof(uint32_t) * m,
+                         hipMemcpyDeviceToHost));
+
+    // Tiles, in row order.
+    std::vector<HostTile> ht;
+    std::vector<uint32_t> ltab, long_rows, code(n);
+    uint32_t r0 = 0, nl = 0, lfirst = 0;
+    uint64_t ent = 0;
+    auto close = [&](uint32_t r1) {
+        if (r1 > r0) ht.push_back(HostTile{r0, r1, off[r0], off[r1], lfirst, nl, false});
+        r0 = r1;
+        ent = 0;
+        nl = 0;
+        lfirst = (uint32_t)ltab.size();
+    };
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint64_t L = (uint64_t)off[r + 1] - off[r];
+        if (L > tile_nnz) {
+            close(r);
+            const uint32_t k = (uint32_t)long_rows.size();
+            long_rows.push_back(r);
+            for (uint64_t e = off[r]; e < off[r + 1]; e += tile_nnz) {
+                ht.push_back(HostTile{r, r, e, std::min<uint64_t>(e + tile_nnz, off[r + 1]),
+                                      (uint32_t)ltab.size(), 1, true});
+                ltab.push_back(k);
+            }
+            r0 = r + 1;
+            lfirst = (uint32_t)ltab.size();
+            continue;
+        }
+        const bool lng = L > long_min;
+        if (ent + L > tile_nnz || r + 1 - r0 > kBlkRows || (lng && nl == kBlkLongMax)) close(r);
+        if (lng) {
+            code[r] = kBlkKeyLong0 + nl++;
+            ltab.push_back((uint32_t)long_rows.size());
+            long_rows.push_back(r);
+        } else {
+            code[r] = r - r0;
+        }
+        ent += L;
+    }
+    close(n);
+    const uint32_t ntiles = (uint32_t)ht.size();
+
+    // Each tile's entries as (source << code bits | code), sorted; then its
+    // groups of 64: a group ends after 64 entries or before a source that
+    // lies kBlkSpan or more past the group's first.
+    std::vector<uint64_t> key(m);
+    std::vector<uint32_t> ngr(ntiles + 1, 0);
+    parallel_for(ntiles, [&](uint32_t t) {
+        const HostTile &T = ht[t];
+        if (T.chunk) {
+            for (uint64_t e = T.e0; e < T.e1; ++e)
+                key[e] = ((uint64_t)idx[e] << kBlkCodeBits) | kBlkKeyLong0;
+        } else {
+            for (uint32_t r = T.r0; r < T.r1; ++r)
+                for (uint64_t e = off[r]; e < off[r + 1]; ++e)
+                    key[e] = ((uint64_t)idx[e] << kBlkCodeBits) | code[r];
+        }
+        std::sort(key.begin() + T.e0, key.begin() + T.e1);
+        uint32_t groups = 0, cnt = 0;
+        uint64_t base = 0;
+        for (uint64_t e = T.e0; e < T.e1; ++e) {
+            const uint64_t src = key[e] >> kBlkCodeBits;
+            if (cnt == 0 || cnt == kWave || src - base >= kBlkSpan) {
+                ++groups;
+                cnt = 0;
+                base = src;
+            }
+            ++cnt;
+        }
+        ngr[t] = groups;
+    });
+    std::vector<uint64_t> g0(ntiles + 1, 0);
+    for (uint32_t t = 0; t < ntiles; ++t) g0[t + 1] = g0[t] + ngr[t];
+    const uint64_t ngroups = g0[ntiles];
+    if (ngroups >= (1ull << 32) / kWave)
+        return fail(GT_ERR_UNSUPPORTED, "block tiles: too many entry groups");
+    std::vector<uint32_t> words(std::max<uint64_t>(1, ngroups * kWave), kBlkNull);
+    std::vector<uint32_t> gbase(std::max<uint64_t>(1, ngroups), 0);
+    parallel_for(ntiles, [&](uint32_t t) {
+        const HostTile &T = ht[t];
+        uint64_t gidx = g0[t];
+        uint32_t cnt = 0;
+        uint64_t base = 0;
+        bool open = false;
+        for (uint64_t e = T.e0; e < T.e1; ++e) {
+            const uint64_t src = key[e] >> kBlkCodeBits;
+            const uint32_t c = (uint32_t)(key[e] & kBlkCodeMask);
+            if (!open || cnt == kWave || src - base >= kBlkSpan) {
+                if (open) ++gidx;
+                open = true;
+                cnt = 0;
+                base = src;
+                gbase[gidx] = (uint32_t)src;
+            }
+            // A long row's entry adds into its lane's slot of the row.
+            const uint32_t j = c - kBlkKeyLong0;
+            const uint32_t at =
+                j < kBlkLongMax ? kBlkAcc + j * kWave + ((cnt + j) & (kWave - 1)) : c;
+            words[gidx * kWave + cnt] = ((uint32_t)(src - base) << kBlkCodeBits) | at;
+            ++cnt;
+        }
+    });
+    std::vector<BlkTile> tiles(std::max<uint32_t>(1, ntiles));
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const HostTile &T = ht[t];
+        tiles[t] = BlkTile{T.r0, T.r1, (uint32_t)g0[t], (uint32_t)g0[t + 1], T.lfirst, T.nlong,
+                           {0, 0}};
+    }
+    std::vector<uint32_t> lmask(((size_t)n + 31) / 32 + 1, 0u);
+    for (uint32_t r : long_rows) lmask[r >> 5] |= 1u << (r & 31);
+    if (ltab.empty()) ltab.push_back(0);
+    const uint32_t nlong = (uint32_t)long_rows.size();
+    if (long_rows.empty()) long_rows.push_back(0);
+
+    GT_TRY(B.tiles.reserve(sizeof(BlkTile) * tiles.size()));
+    GT_TRY(B.words.reserve(sizeof(uint32_t) * words.size()));
+    GT_TRY(B.gbase.reserve(sizeof(uint32_t) * gbase.size()));
+    GT_TRY(B.ltab.reserve(sizeof(uint32_t) * ltab.size()));
+    GT_TRY(B.long_rows.reserve(sizeof(uint32_t) * long_rows.size()));
+    GT_TRY(B.lmask.reserve(sizeof(uint32_t) * lmask.size()));
+    GT_HIP(hipMemcpy(B.tiles.ptr, tiles.data(), sizeof(BlkTile) * tiles.size(),
+                     hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.words.ptr, words.data(), sizeof(uint32_t) * words.size(),
+                     hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.gbase.ptr, gbase.data(), sizeof(uint32_t) * gbase.size(),
+                     hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.ltab.ptr, ltab.data(), sizeof(uint32_t) * ltab.size(),
+                     hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.long_rows.ptr, long_rows.data(), sizeof(uint32_t) * long_rows.size(),
+                     hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.lmask.ptr, lmask.data(), sizeof(uint32_t) * lmask.size(),
+                     hipMemcpyHostToDevice));
+    B.ntiles = ntiles;
+    B.nlong = nlong;
+    B.ngroups = ngroups;
+    B.tile_nnz = tile_nnz;
+    B.long_min = long_min;
+    B.ready = true;
+    return GT_OK;
+}
+
+gt_status blk_reserve(gt_graph *g, const Blocks &B) {
+    Workspace &ws = g->ws;
+    GT_TRY(ws.blk_slots.reserve(sizeof(unsigned long long) * std::max<uint32_t>(1, B.nlong)));
+    GT_TRY(ws.blk_part.reserve(sizeof(double) * 3 * std::max<uint32_t>(1, B.ntiles)));
+    GT_TRY(ws.blk_state.reserve(sizeof(BlkState)));
+    GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
+    return GT_OK;
+}
+
+void blk_launch_start(const BlkArgs &b, hipStream_t s) {
+    hipLaunchKernelGGL(pr_start_blk, dim3(fold_grid(b.nlong)), dim3(kBlock), 0, s, b);
+}
+
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, hipStream_t s) {
+    const int grid = fit_grid(device_cus() * std::max(1, per_cu), b.ntiles);
+    if (dpers)
+        hipLaunchKernelGGL(pr_sweep_blk<true>, dim3(grid), dim3(kBlkThreads), 0, s, b);
+    else
+        hipLaunchKernelGGL(pr_sweep_blk<false>, dim3(grid), dim3(kBlkThreads), 0, s, b);
+}
+
+void blk_launch_fold(const BlkArgs &b, bool dpers, hipStream_t s) {
+    const int grid = fold_grid(b.nlong);
+    if (dpers)
+        hipLaunchKernelGGL(pr_fold_blk<true>, dim3(grid), dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL(pr_fold_blk<false>, dim3(grid), dim3(kBlock), 0, s, b);
+}
+
+}  // namespace gt

@@ -1,0 +1,77 @@
+// Block-tile sweeps for the FP32 PageRank phase (relabelled, unfiltered,
+// unweighted).  The layout is Blocks (gt_internal.hpp); the kernels and
+// their launch helpers are in blocks.hip.
+//
+// A tile's entries are sorted by source across all its rows, so the 64 lanes
+// of a gather instruction read 64 consecutive entries of that order: with
+// 65,536 entries per tile the sources of one instruction are dense enough
+// that many lanes share a 128-byte line (R-MAT scale 22: about 17 M line
+// requests per sweep, against 29 M for 1,024-entry wave tiles with the
+// 128 KB hot slice).  Row order is lost inside a tile, so a row's entries
+// are added into its LDS accumulator as they come.  The accumulators are
+// 64-bit fixed point: a contribution c is rounded to c * 2^K (K from the
+// mass of the gathered iterate, so that no partial sum can reach 2^62) and
+// added as an integer.  Integer addition does not depend on order, so the
+// row sums -- and the sweep -- are the same bits on every run whatever the
+// order the workgroup's wavefronts add in, and they carry 2^-61 of the
+// iterate's L1 mass absolute precision, finer than the FP32 row sums of the
+// wave engine.
+#pragma once
+
+#include "gt_internal.hpp"
+
+namespace gt {
+
+// Device state of a block-sweep loop.
+struct BlkState {
+    double mass;         // L1 mass of the iterate the next sweep gathers
+    uint32_t next_tile;  // tile counter of the running sweep
+    uint32_t pad;
+};
+
+// A mass above this makes the sweep give up (SweepState::bail): the loop is
+// run again on the wave engine, whose FP32 sums follow the values to
+// overflow as the reference's would.
+constexpr double kBlkMassMax = 0x1p900;
+
+struct BlkArgs {
+    const BlkTile *tiles;
+    uint32_t ntiles;
+    uint32_t nlong;
+    const uint32_t *words;
+    const uint32_t *gbase;
+    const uint32_t *ltab;
+    const uint32_t *long_rows;
+    const uint32_t *lmask;
+    unsigned long long *slots;  // [nlong] int64 sums of the long rows
+    double *part;               // [3 * ntiles]: delta, dangling, mass per tile
+    double *fold_part;          // [3 * fold grid]
+    BlkState *bs;
+    SweepState *st;
+    const uint32_t *out_off;    // unity out-degree, sweep labels
+    const double *pers;         // dense personalisation, sweep labels (DPERS)
+    double pval;
+    double damping;
+    double eps_abs, eps_rel;    // stop when delta < max(eps_abs, eps_rel * initial mass)
+    uint64_t max_iter;
+    const float *x_cur;
+    float *x_next;
+    const float *c_cur;
+    float *c_next;
+};
+
+// Build (once per tile size and long-row threshold) the block tiles of R's
+// pull orientation.  Waits for the stream.
+gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
+                        Blocks **out);
+// Sizes the per-call buffers of a block loop over B into the workspace.
+gt_status blk_reserve(gt_graph *g, const Blocks &B);
+// After the init kernel: zero the long-row sums, take the initial mass,
+// clear the bail flag.
+void blk_launch_start(const BlkArgs &b, hipStream_t s);
+// One sweep (workgroups: `per_cu` per compute unit, at most one per tile).
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, hipStream_t s);
+// Completes a sweep: long rows, partials in tile order, convergence.
+void blk_launch_fold(const BlkArgs &b, bool dpers, hipStream_t s);
+
+}  // namespace gt

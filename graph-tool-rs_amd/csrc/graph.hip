@@ -103,6 +103,11 @@ gt_tuning resolve_tuning(const gt_tuning &t) {
     r.sliced_min_len =
         t.sliced_min_len > 0 ? std::max(2, std::min(t.sliced_min_len, 1024)) : -1;
     r.sliced_share = t.sliced_share > 0 ? std::min(t.sliced_share, 99) : 0;
+    r.block_tile_nnz = t.block_tile_nnz < 0    ? -1
+                       : t.block_tile_nnz == 0 ? (int)kBlkTileNnzDefault
+                                               : std::max(1024, std::min(t.block_tile_nnz, 1 << 20));
+    r.block_long_min = t.block_long_min > 0 ? std::max(16, std::min(t.block_long_min, 65536))
+                                            : (int)kBlkLongMinDefault;
     return r;
 }
 

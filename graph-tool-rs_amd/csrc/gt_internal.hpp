@@ -202,6 +202,47 @@ struct Sliced {
     uint32_t heavy_wgs = 0;
 };
 
+// Block tiles of a relabelled pull orientation (the FP32 PageRank sweeps):
+// each tile is a run of rows [r0, r1) with at most tile_nnz entries, its
+// entries sorted by source across all its rows and cut into groups of 64
+// (one gather instruction).  A word is ((source - gbase[group]) << 14) |
+// at, where `at` is the LDS accumulator the entry adds into: the row's,
+// r - r0 < kBlkRows, for a short row; for one of the tile's long rows (more
+// than long_min entries, at most kBlkLongMax per tile, the j-th) the slot of
+// the entry's lane in that row's lane slots, kBlkAcc + 64 j + (lane + j) %
+// 64; kBlkNull (a dummy) for a padding entry.  A row longer than tile_nnz is
+// cut into chunk tiles of its own (r0 == r1).  Long rows are finished by the
+// fold from int64 slots (long_rows[k], slot k; ltab lists each tile's).
+constexpr uint32_t kBlkCodeBits = 14;
+constexpr uint32_t kBlkRows = 8064;      // rows in a tile's range
+constexpr uint32_t kBlkAcc = 8128;       // row accumulators and dummies, then lane slots
+constexpr uint32_t kBlkNull = 8127;      // accumulator of the padding entries (a dummy)
+constexpr uint32_t kBlkLongMax = 64;     // long rows per tile
+constexpr uint32_t kBlkSpan = 1u << (32 - kBlkCodeBits);  // source span of a group
+constexpr uint32_t kBlkTileNnzDefault = 65536;
+constexpr uint32_t kBlkLongMinDefault = 1024;
+
+struct BlkTile {
+    uint32_t r0, r1;      // rows finished by the tile (long rows excepted)
+    uint32_t g0, g1;      // groups of 64 words
+    uint32_t lfirst;      // first entry of ltab
+    uint32_t nlong;       // long rows
+    uint32_t pad[2];
+};
+
+struct Blocks {
+    bool ready = false;
+    uint32_t tile_nnz = 0, long_min = 0;
+    DevBuf tiles;        // BlkTile[ntiles]
+    DevBuf words;        // uint32[ngroups * 64]
+    DevBuf gbase;        // uint32[ngroups]
+    DevBuf ltab;         // uint32[]: long-row slot of each tile's long code
+    DevBuf long_rows;    // uint32[nlong]
+    DevBuf lmask;        // uint32[(n + 31) / 32]: long rows, one bit each
+    uint32_t ntiles = 0, nlong = 0;
+    uint64_t ngroups = 0;
+};
+
 // A pull orientation in a locality-friendly labelling: vertices renumbered by
 // descending gather frequency (the row length of the opposite orientation:
 // how often a vertex appears as an entry), rows in the new order and each
@@ -214,6 +255,7 @@ struct Relabeled {
     Orient pull;       // relabelled pull orientation (eid = original edge ids)
     DevBuf push_off;   // uint32[n+1]: scanned push-row lengths in new order
     Sliced sl[2];      // sliced sweeps: [0] 4-byte values, [1] 8-byte values
+    Blocks blk;        // block tiles (FP32 PageRank sweeps)
 };
 
 struct Workspace {
@@ -245,6 +287,9 @@ struct Workspace {
     DevBuf bpart;       // batched: per-workgroup, per-column partials
     DevBuf blong;       // batched: split-row [64] partials
     DevBuf bstate;      // batched: loop state + seeds
+    DevBuf blk_slots;   // block sweeps: int64 sums of the long rows
+    DevBuf blk_part;    // block sweeps: per-tile delta, dangling and mass partials
+    DevBuf blk_state;   // block sweeps: BlkState
     DevBuf masks;       // staged vertex + edge masks
     DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
     DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces
@@ -269,7 +314,8 @@ struct SweepState {
     uint32_t done;      // set once converged or capped
     uint32_t counter;   // arrival ticket for the last-workgroup fold
     uint32_t launches;  // launches that found work
-    uint32_t pad;
+    uint32_t bail;      // set by a block sweep that cannot represent its sums: the loop
+                        // ends and the solve runs again on the wave engine
 };
 
 struct gt_graph {

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <type_traits>
 
+#include "blocks.hpp"
 #include "device_common.hpp"
 #include "gt_internal.hpp"
 #include "props.hpp"
@@ -804,6 +805,73 @@ gt_status sliced_for(PrSolve &S, uint32_t nhot, const Sliced **out) {
     return GT_OK;
 }
 
+// Whether the FP32 sweeps of S run on block tiles (blocks.hpp): relabelled,
+// unfiltered and unweighted, unless the tuning turns them off.
+bool blocks_apply(const PrSolve &S) {
+    return S.rel && S.iperm && !S.plan.masked && !S.plan.weighted && !S.noalloc &&
+           S.g->tune.block_tile_nnz > 0;
+}
+
+// Init + FP32 sweeps on block tiles, on buffers x / c; returns the sweep
+// count reached and leaves the latest iterate in x[count & 1], as pr_phase.
+// Sets *bailed when a sweep met an iterate whose mass its fixed-point sums
+// cannot hold (not finite, or above kBlkMassMax): nothing is then delivered
+// and the caller runs the phase again on the wave engine.
+gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64_t had,
+                          double eps_pred, uint64_t cap, uint64_t *iter_out, bool *bailed) {
+    gt_graph *g = S.g;
+    hipStream_t s = g->stream;
+    *bailed = false;
+    Blocks *B = nullptr;
+    GT_TRY(ensure_blocks(g, *S.rel, (uint32_t)g->tune.block_tile_nnz,
+                         (uint32_t)g->tune.block_long_min, &B));
+    GT_TRY(blk_reserve(g, *B));
+    g->layout.block_tiles = B->ntiles;
+    g->layout.block_long_rows = B->nlong;
+    g->layout.block_groups = B->ngroups;
+    g->prof_kind = GT_SWEEP_PAGERANK_F32;
+    a.x_next = x[0];
+    a.c_next = c[0];
+    dispatch_init<float>(S.plan, a, S.rank64, had, g->n, 0, s);
+    BlkArgs b{};
+    b.tiles = B->tiles.as<BlkTile>();
+    b.ntiles = B->ntiles;
+    b.nlong = B->nlong;
+    b.words = B->words.as<uint32_t>();
+    b.gbase = B->gbase.as<uint32_t>();
+    b.ltab = B->ltab.as<uint32_t>();
+    b.long_rows = B->long_rows.as<uint32_t>();
+    b.lmask = B->lmask.as<uint32_t>();
+    b.slots = g->ws.blk_slots.as<unsigned long long>();
+    b.part = g->ws.blk_part.as<double>();
+    b.fold_part = g->ws.partials.as<double>();
+    b.bs = g->ws.blk_state.as<BlkState>();
+    b.st = a.st;
+    b.out_off = a.out_off;
+    b.pers = a.pers;
+    b.pval = a.pval;
+    b.damping = a.damping;
+    b.eps_abs = a.eps_abs;
+    b.eps_rel = a.eps_rel;
+    b.max_iter = a.max_iter;
+    blk_launch_start(b, s);
+    GT_HIP(hipGetLastError());
+    auto sweep = [&](uint64_t k) {
+        const int par = (int)(k & 1);
+        b.x_cur = x[par];
+        b.x_next = x[par ^ 1];
+        b.c_cur = c[par];
+        b.c_next = c[par ^ 1];
+        blk_launch_sweep(b, S.plan.dpers, 1, s);
+    };
+    auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, s); };
+    uint64_t launched = 0;
+    GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched, g->prof_kind));
+    *bailed = g->h_state->bail != 0;
+    *iter_out = g->h_state->iter;
+    return GT_OK;
+}
+
 // Init + sweeps in value type V on buffers x / c; returns the sweep count
 // reached (st->iter) and leaves the latest iterate in x[count & 1].
 template <typename V>
@@ -811,6 +879,16 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
                    double eps_pred, uint64_t cap, uint64_t *iter_out) {
     gt_graph *g = S.g;
     hipStream_t s = g->stream;
+    if constexpr (std::is_same<V, float>::value) {
+        if (iter0 == 0 && cap != 1 && blocks_apply(S)) {
+            bool bailed = false;
+            GT_TRY(pr_phase_blocks(S, a, x, c, had, eps_pred, cap, iter_out, &bailed));
+            if (!bailed) return GT_OK;
+            g->layout.block_tiles = 0;
+            g->layout.block_long_rows = 0;
+            g->layout.block_groups = 0;
+        }
+    }
     g->prof_kind = std::is_same<V, double>::value ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
     // The hot slice serves sweeps in relabelled order (the most gathered
     // vertices first); masked views keep the caller's labels.

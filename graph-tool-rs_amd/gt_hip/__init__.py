@@ -304,7 +304,8 @@ class DeviceGraph:
         call("gt_graph_set_stream", self._h, C.c_void_p(stream_ptr or 0))
 
     TUNING_FIELDS = ("sweep_blocks_per_cu", "tile_rows", "short_row", "mm_tile_nnz",
-                     "mm_blocks_per_cu", "sweep_hot_kb", "sliced_min_len", "sliced_share")
+                     "mm_blocks_per_cu", "sweep_hot_kb", "sliced_min_len", "sliced_share",
+                     "block_tile_nnz", "block_long_min")
 
     def tuning(self) -> dict:
         """Launch / schedule parameters in effect (gt_graph_get_tuning)."""

@@ -23,7 +23,7 @@ ARCH = os.environ.get("GT_HIP_ARCH", "gfx950")
 
 HIP_SOURCES = ["graph.hip", "props.hip", "pagerank.hip", "eigenvector.hip", "builder.hip",
                "views.hip", "batched.hip", "adjoint.hip", "relabel.hip", "uneven.hip", "degree.hip",
-               "betweenness.hip", "sliced.hip"]
+               "betweenness.hip", "sliced.hip", "blocks.hip"]
 CXX_SOURCES = ["host.cpp", "io.cpp"]
 PROBE_SOURCE = "gather_probe.hip"
 PROBE_LIB = PKG / "libgt_probe.so"

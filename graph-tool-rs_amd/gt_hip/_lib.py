@@ -54,7 +54,8 @@ class GtTuning(C.Structure):
     _fields_ = [("sweep_blocks_per_cu", C.c_int32), ("tile_rows", C.c_int32),
                 ("short_row", C.c_int32), ("mm_tile_nnz", C.c_int32),
                 ("mm_blocks_per_cu", C.c_int32), ("sweep_hot_kb", C.c_int32),
-                ("sliced_min_len", C.c_int32), ("sliced_share", C.c_int32)]
+                ("sliced_min_len", C.c_int32), ("sliced_share", C.c_int32),
+                ("block_tile_nnz", C.c_int32), ("block_long_min", C.c_int32)]
 
 
 class GtView(C.Structure):
@@ -85,7 +86,8 @@ class GtBcStats(C.Structure):
 class GtSweepLayout(C.Structure):
     _fields_ = [("slot_bits", C.c_uint32), ("tiles", C.c_uint32), ("split_rows", C.c_uint32),
                 ("slices", C.c_uint32), ("sliced_rows", C.c_uint64),
-                ("sliced_entries", C.c_uint64)]
+                ("sliced_entries", C.c_uint64), ("block_tiles", C.c_uint32),
+                ("block_long_rows", C.c_uint32), ("block_groups", C.c_uint64)]
 
 
 GraphP = C.c_void_p

@@ -415,6 +415,12 @@ typedef struct gt_tuning {
                                     or 0 / -1 for off (off) */
     int32_t sliced_share;        /* percent of the sliced sweep's workgroups given to the warm
                                     slices: 1..99 (0: from a cost model) */
+    int32_t block_tile_nnz;      /* FP32 PageRank sweeps on a relabelled, unfiltered, unweighted
+                                    graph: entries per block tile (rows sorted together by source,
+                                    summed in 64-bit fixed point); -1 turns the block sweep off and
+                                    the wave engine runs: -1, 1024..1048576 (65536) */
+    int32_t block_long_min;      /* block tiles: rows with more entries than this are summed in
+                                    lane-private slots (at most 64 per tile): 16..65536 (1024) */
 } gt_tuning;
 
 gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);
@@ -451,7 +457,9 @@ gt_status gt_sweep_stats(const gt_graph *g, int32_t kind, double *total_ms,
  * slot (each tile's sources span fewer than 2^22 labels: always below
  * 2^22 vertices), 0 when they are plain 32-bit sources in row order.
  * sliced_rows / sliced_entries / slices describe the rows whose entries are
- * gathered from on-chip slices of the source vector (0 when none were). */
+ * gathered from on-chip slices of the source vector (0 when none were).
+ * block_tiles / block_long_rows / block_groups describe the block tiles the
+ * FP32 sweeps ran on (0 when they ran on the wave engine). */
 typedef struct gt_sweep_layout {
     uint32_t slot_bits;
     uint32_t tiles;
@@ -459,6 +467,9 @@ typedef struct gt_sweep_layout {
     uint32_t slices;
     uint64_t sliced_rows;
     uint64_t sliced_entries;
+    uint32_t block_tiles;
+    uint32_t block_long_rows;
+    uint64_t block_groups;
 } gt_sweep_layout;
 gt_status gt_last_layout(const gt_graph *g, gt_sweep_layout *out);
 

@@ -1,0 +1,187 @@
+"""The block-tile FP32 sweep against the oracle.
+
+This is the default FP32 engine of pagerank_mixed on relabelled, unfiltered,
+unweighted graphs. Each result is checked against the oracle at the FP32
+form's bar (L1 <= 1e-6 of the FP64 fixed point), against the wave engine (the
+same solve with block_tile_nnz = -1), and bit for bit run to run.
+
+The cases cover the layout's edge cases:
+- long rows summed in lane slots;
+- rows longer than a tile, cut into chunk tiles;
+- tiles capped by their row range (empty rows);
+- groups cut by their source span;
+- warm starts, a sweep cap, and a personalisation with negative entries;
+- an iterate whose mass the fixed point cannot hold, where the solve must
+  come from the wave engine.
+"""
+import numpy as np
+import pytest
+
+import gt_hip
+import oracle
+from graphs import Rng
+
+pytestmark = pytest.mark.gpu
+
+
+def adj(n, s, t):
+    g = gt_hip.AdjList.with_vertices(n)
+    g.add_edges(np.asarray(s, np.uint32), np.asarray(t, np.uint32))
+    return g
+
+
+def params(n, eps, max_iter):
+    return gt_hip.PowerIteration(eps, max_iter, gt_hip.Plan(n, 64))
+
+
+def mixed(g, eps=1e-9, max_iter=1000, pers=None, rank=None, reversed_=False):
+    r = gt_hip.DenseProp.new(g.graph_id) if rank is None else rank
+    target = g.reversed() if reversed_ else g
+    it = gt_hip.pagerank_mixed(target, gt_hip.Unity.NEW, pers, 0.85,
+                               params(g.vertex_bound(), eps, max_iter), r)
+    return np.asarray(r.data).copy(), it
+
+
+def run_both(g, **kw):
+    """(block result, its layout), (wave-engine result) for one solve."""
+    dev = g.device()
+    dev.set_tuning(block_tile_nnz=kw.pop("tile", 0), block_long_min=kw.pop("long_min", 0))
+    blk = mixed(g, **kw)
+    lay = dev.layout()
+    dev.set_tuning(block_tile_nnz=-1)
+    wave = mixed(g, **kw)
+    assert dev.layout()["block_tiles"] == 0
+    dev.set_tuning(block_tile_nnz=0, block_long_min=0)
+    return blk, lay, wave
+
+
+def check(n, s, t, pers=None, tile=0, long_min=0, reversed_=False):
+    g = adj(n, s, t)
+    pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
+    (x, it), lay, (xw, itw) = run_both(g, tile=tile, long_min=long_min, pers=pp,
+                                       reversed_=reversed_)
+    assert lay["block_tiles"] > 0, lay
+    want, _ = oracle.Adj(n, s, t).pagerank(pers=pers, epsilon=1e-12, max_iter=1000,
+                                           reversed=reversed_, threads=8)
+    scale = max(1.0, float(np.abs(want).sum()))
+    assert np.abs(x - want).sum() <= 1e-6 * scale, (np.abs(x - want).sum(), it)
+    # The two engines differ only in the FP32 row sums' rounding.
+    assert abs(it - itw) <= 1
+    assert np.abs(x - xw).sum() <= 2e-7 * scale
+    return g, lay, x, it
+
+
+def test_rmat_default_tiles(gpu):
+    """R-MAT 16 (1 M entries, 16 default tiles, hub rows longer than the
+    long-row threshold) and R-MAT 12 (one tile)."""
+    for scale, seed in ((16, 41), (12, 42)):
+        n = 1 << scale
+        s, t = oracle.gen_rmat(scale, 16 * n, 0.57, 0.19, 0.19, seed)
+        g, lay, _, _ = check(n, s, t)
+        if scale == 16:
+            assert lay["block_tiles"] >= 16 and lay["block_long_rows"] > 0, lay
+
+
+def test_small_tiles_long_rows_and_chunks(gpu):
+    """1,024-entry tiles and a long-row threshold of 16: many long rows per
+    tile, and every hub longer than a tile cut into chunk tiles."""
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 43)
+    indeg = np.bincount(t, minlength=n)
+    assert indeg.max() > 1024
+    _, lay, _, _ = check(n, s, t, tile=1024, long_min=16)
+    assert lay["block_long_rows"] >= int((indeg > 16).sum())
+
+
+def test_reversed_view_and_personalisation_with_negative_entries(gpu):
+    """Out-halves as the pull orientation; personalisation with negative
+    entries (normalised to sum 1), so the iterate's L1 mass exceeds its sum
+    and the fixed-point scale follows the mass."""
+    n = 1 << 14
+    s, t = oracle.gen_rmat(14, 16 * n, 0.57, 0.19, 0.19, 44)
+    p = np.random.default_rng(7).uniform(-0.5, 1.5, n)
+    p /= p.sum()
+    check(n, s, t, pers=p, reversed_=True)
+    check(n, s, t, pers=p, tile=2048, long_min=64)
+
+
+def test_empty_rows_cap_the_tile_ranges(gpu):
+    """100,003 vertices and 20,000 edges: most rows are empty, so tiles are
+    cut by their row range (8,064 rows) rather than their entries."""
+    r = Rng(45)
+    n, m = 100_003, 20_000
+    s = np.array([r.below(n) for _ in range(m)], np.uint32)
+    t = np.array([r.below(n) for _ in range(m)], np.uint32)
+    _, lay, _, _ = check(n, s, t)
+    assert lay["block_tiles"] >= n // 8064
+
+
+def test_groups_cut_by_source_span(gpu):
+    """2^22 + 5 vertices, vertex v's one out-edge to v mod 300,000: every
+    vertex is gathered once (so relabelling keeps the ids), and row r's
+    sources are r + 300,000 k.  A 1,024-entry tile of 73 rows sorts into runs
+    of 73 consecutive sources 300,000 labels apart, so each run ends a group
+    early (a group spans fewer than 2^18 labels) and groups are padded."""
+    n = (1 << 22) + 5
+    s = np.arange(n, dtype=np.uint32)
+    t = (s % 300_000).astype(np.uint32)
+    _, lay, _, _ = check(n, s, t, tile=1024)
+    assert lay["block_groups"] * 64 > 3 * n // 2, lay
+
+
+def test_warm_start_and_cap(gpu):
+    """A full and a partial warm start, and a cap of three sweeps (no FP64
+    refinement) against the oracle's three sweeps.
+
+    The FP32 phase stops once delta < 2^-22 * mass.  From a random start the
+    slowest mode decays at the damping d, so that iterate lies within
+    delta * d / (1 - d) of the fixed point, and the FP64 sweep multiplies
+    that by d: about 1.2e-6 in L1 for mass 1 at d = 0.85.  That is the bar
+    here (the cold starts above stay far inside 1e-6); the wave engine must
+    agree with the block sweep to rounding."""
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 47)
+    g = adj(n, s, t)
+    warm = np.random.default_rng(8).uniform(0.0, 2.0 / n, n)
+    bar = 2.0**-22 * 0.85 / 0.15 * 0.85 * 1.05
+    for had in (n, n // 3):
+        r = gt_hip.DenseProp.from_vec(g.graph_id, warm[:had].copy())
+        x, it = mixed(g, rank=r)
+        assert g.device().layout()["block_tiles"] > 0
+        g.device().set_tuning(block_tile_nnz=-1)
+        r = gt_hip.DenseProp.from_vec(g.graph_id, warm[:had].copy())
+        xw, itw = mixed(g, rank=r)
+        g.device().set_tuning(block_tile_nnz=0)
+        want, _ = oracle.Adj(n, s, t).pagerank(epsilon=1e-12, max_iter=1000,
+                                               rank=warm[:had].copy(), threads=8)
+        assert np.abs(x - want).sum() <= bar, (np.abs(x - want).sum(), had)
+        assert np.abs(xw - want).sum() <= bar, (np.abs(xw - want).sum(), had)
+        assert abs(it - itw) <= 1 and np.abs(x - xw).sum() <= 2e-7
+    x, it = mixed(g, max_iter=3)
+    want, it_o = oracle.Adj(n, s, t).pagerank(epsilon=1e-9, max_iter=3, threads=8)
+    assert it == it_o == 3
+    assert np.abs(x - want).sum() <= 1e-6
+
+
+def test_bit_for_bit_run_to_run(gpu):
+    """Integer row sums: the same bits whatever order the workgroups add in."""
+    n = 1 << 15
+    s, t = oracle.gen_rmat(15, 16 * n, 0.57, 0.19, 0.19, 48)
+    g = adj(n, s, t)
+    a, ia = mixed(g)
+    b, ib = mixed(g)
+    assert ia == ib and np.array_equal(a, b)
+
+
+def test_unrepresentable_mass_runs_on_the_wave_engine(gpu):
+    """A personalisation of 1e300 overflows FP32, so the first iterate's mass is
+    infinite: the block sweep gives up and the solve is the wave engine's, bit
+    for bit (NaN where it has NaN), with its sweep count."""
+    n = 1 << 10
+    s, t = oracle.gen_rmat(10, 16 * n, 0.57, 0.19, 0.19, 49)
+    g = adj(n, s, t)
+    p = np.full(n, 1e300)
+    pp = gt_hip.DenseProp.from_vec(g.graph_id, p)
+    (x, it), lay, (xw, itw) = run_both(g, pers=pp)
+    assert lay["block_tiles"] == 0
+    assert it == itw and np.array_equal(x, xw, equal_nan=True)

@@ -224,10 +224,11 @@ def test_mixed_precision_within_1e6(gpu):
 @pytest.mark.parametrize("weighted", [False, True])
 @pytest.mark.parametrize("personalised", [False, True])
 def test_mixed_hot_slice(gpu, weighted, personalised):
-    """The FP32 sweep that keeps the most gathered contributions in LDS (the
-    default on unfiltered graphs) against the oracle, and against the same
-    solve with the slice off: graphs wholly inside the slice (scale 12), larger
-    than it (scale 16), with n not a multiple of 4 (1003), and a 1 KB slice."""
+    """The wave engine's FP32 sweep that keeps the most gathered contributions
+    in LDS (block tiles off) against the oracle, against the same solve with
+    the slice off, and against the block-tile sweep (the default for
+    unweighted graphs): graphs wholly inside the slice (scale 12), larger than
+    it (scale 16), with n not a multiple of 4 (1003), and a 1 KB slice."""
     cases = [oracle.gen_rmat(12, 16 << 12, 0.57, 0.19, 0.19, 5) + (1 << 12, 0),
              oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 0),
              oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 1)]
@@ -249,18 +250,22 @@ def test_mixed_hot_slice(gpu, weighted, personalised):
         want, _ = oracle.Adj(n, s, t).pagerank(weight=w, pers=p, epsilon=1e-12, max_iter=1000,
                                                threads=8)
         got = {}
-        for hot in (kb, -1):
-            dev.set_tuning(sweep_hot_kb=hot)
+        for hot, blk in ((kb, -1), (-1, -1), ("blk", 0)):
+            dev.set_tuning(sweep_hot_kb=kb if hot == "blk" else hot, block_tile_nnz=blk)
             wp = None if w is None else gt_hip.DenseProp.from_vec(g.graph_id, w)
             pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
             got[hot] = pr(g, 1e-9, 1000, weight=wp, pers=pp, fn=gt_hip.pagerank_mixed)
-        dev.set_tuning(sweep_hot_kb=0)
+            assert (dev.layout()["block_tiles"] > 0) == (blk == 0 and not weighted)
+        dev.set_tuning(sweep_hot_kb=0, block_tile_nnz=0)
         scale = max(1.0, np.abs(want).sum())
         for hot, (x, it) in got.items():
             assert np.abs(x - want).sum() <= 1e-6 * scale, (n, kb, hot)
         # Only the order of the FP64 partial folds differs between the two.
         assert abs(got[kb][1] - got[-1][1]) <= 1
         assert np.abs(got[kb][0] - got[-1][0]).sum() <= 1e-7 * scale
+        # The block sweep's row sums are exact to 2^-61 of the mass.
+        assert abs(got["blk"][1] - got[-1][1]) <= 1
+        assert np.abs(got["blk"][0] - got[-1][0]).sum() <= 2e-7 * scale
 
 
 @pytest.mark.parametrize("personalised", [False, True])

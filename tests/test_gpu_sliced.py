@@ -90,7 +90,8 @@ def test_sliced_matches_oracle_and_unsliced(gpu, case, personalised):
         p /= p.sum()
     g = _graph(n, s, t)
     dev = g.device()
-    dev.set_tuning(sweep_hot_kb=kb, sliced_min_len=min_len)
+    # The FP32 solves on the wave engine (the sliced split is one of its sweeps).
+    dev.set_tuning(sweep_hot_kb=kb, sliced_min_len=min_len, block_tile_nnz=-1)
     o = oracle.Adj(n, s, t)
     want, it_o = o.pagerank(pers=p, epsilon=1e-12, max_iter=1000, threads=8)
     pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
@@ -108,13 +109,13 @@ def test_sliced_matches_oracle_and_unsliced(gpu, case, personalised):
     _parity(off64, it_off, want, it_o, 1e-12)
     off32, _ = _solve(g, "mixed", 1e-9, pp)
     assert np.abs(off32 - x32).sum() <= 1e-7
-    dev.set_tuning(sweep_hot_kb=0, sliced_min_len=0)
+    dev.set_tuning(sweep_hot_kb=0, sliced_min_len=0, block_tile_nnz=0)
 
 
 def test_sliced_repeats_bit_for_bit(gpu):
     n, s, t, kb, min_len = CASES["hub_1kb"]()
     g = _graph(n, s, t)
-    g.device().set_tuning(sweep_hot_kb=kb, sliced_min_len=min_len)
+    g.device().set_tuning(sweep_hot_kb=kb, sliced_min_len=min_len, block_tile_nnz=-1)
     a, ia = _solve(g, "f64", 1e-12)
     b, ib = _solve(g, "f64", 1e-12)
     assert ia == ib and (a.view(np.uint64) == b.view(np.uint64)).all()
@@ -131,7 +132,7 @@ def test_sliced_workgroup_split(gpu, share):
     s, t = oracle.gen_rmat(15, 16 * n, 0.57, 0.19, 0.19, 13)
     g = _graph(n, s, t)
     dev = g.device()
-    dev.set_tuning(sweep_hot_kb=4, sliced_min_len=64, sliced_share=share)
+    dev.set_tuning(sweep_hot_kb=4, sliced_min_len=64, sliced_share=share, block_tile_nnz=-1)
     want, it_o = oracle.Adj(n, s, t).pagerank(epsilon=1e-12, max_iter=1000, threads=8)
     got, it = _solve(g, "f64", 1e-12)
     assert dev.layout()["slices"] >= 1

@@ -116,12 +116,19 @@ def test_plain_stream_weighted_personalised(gpu, plain_graph):
 
 
 def test_plain_stream_mixed(gpu, plain_graph):
+    """The FP32 form on the block tiles (the default) and on the wave engine's
+    plain stream."""
     n, s, t, g, o = plain_graph
-    got, it = _pr(g, n, gt_hip.pagerank_mixed, eps=1e-9)
-    assert g.device().layout()["slot_bits"] == 0
     want, _ = o.pagerank(epsilon=1e-12, grain=4096, threads=THREADS)
-    assert np.abs(got - want).sum() <= 1e-6
-    assert it >= 1
+    dev = g.device()
+    for blk in (0, -1):
+        dev.set_tuning(block_tile_nnz=blk)
+        got, it = _pr(g, n, gt_hip.pagerank_mixed, eps=1e-9)
+        lay = dev.layout()
+        assert lay["slot_bits"] == 0 and (lay["block_tiles"] > 0) == (blk == 0), lay
+        assert np.abs(got - want).sum() <= 1e-6
+        assert it >= 1
+    dev.set_tuning(block_tile_nnz=0)
 
 
 def test_plain_stream_eigenvector(gpu, plain_graph):
@@ -176,9 +183,14 @@ def test_packed_stream_above_2_22(gpu, local_graph):
     assert lay["slot_bits"] == 10, lay
     want, it_o = o.pagerank(epsilon=1e-10, grain=4096, threads=THREADS)
     _parity(got, it, want, it_o, 1e-10)
-    got32, _ = _pr(g, n, gt_hip.pagerank_mixed, eps=1e-9)
-    assert g.device().layout()["slot_bits"] == 10
-    assert np.abs(got32 - want).sum() <= 1e-6
+    dev = g.device()
+    for blk in (0, -1):
+        dev.set_tuning(block_tile_nnz=blk)
+        got32, _ = _pr(g, n, gt_hip.pagerank_mixed, eps=1e-9)
+        lay = dev.layout()
+        assert lay["slot_bits"] == 10 and (lay["block_tiles"] > 0) == (blk == 0), lay
+        assert np.abs(got32 - want).sum() <= 1e-6
+    dev.set_tuning(block_tile_nnz=0)
     x = gt_hip.DenseProp.new(g.graph_id)
     eig, it = gt_hip.eigenvector(g, gt_hip.Unity.NEW, _params(n, 1e-300, 20), x)
     xo, eo, ito = o.eigenvector(epsilon=1e-300, max_iter=20, grain=4096, threads=THREADS)
