@@ -38,6 +38,8 @@ struct BlkArgs {
     const BlkTile *tiles;
     uint32_t ntiles;
     uint32_t nlong;
+    uint32_t glast;             // last group (loads ahead of a tile's end clamp to it)
+    uint32_t n;                 // rows
     const uint32_t *words;
     const uint32_t *gbase;
     const uint32_t *ltab;

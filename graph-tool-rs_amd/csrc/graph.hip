@@ -108,6 +108,7 @@ gt_tuning resolve_tuning(const gt_tuning &t) {
                                                : std::max(1024, std::min(t.block_tile_nnz, 1 << 20));
     r.block_long_min = t.block_long_min > 0 ? std::max(16, std::min(t.block_long_min, 65536))
                                             : (int)kBlkLongMinDefault;
+    r.block_per_cu = t.block_per_cu > 0 ? std::min(t.block_per_cu, 3) : 2;
     return r;
 }
 

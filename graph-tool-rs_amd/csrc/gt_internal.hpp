@@ -205,7 +205,7 @@ struct Sliced {
 // Block tiles of a relabelled pull orientation (the FP32 PageRank sweeps):
 // each tile is a run of rows [r0, r1) with at most tile_nnz entries, its
 // entries sorted by source across all its rows and cut into groups of 64
-// (one gather instruction).  A word is ((source - gbase[group]) << 14) |
+// (one gather instruction).  A word is ((source - gbase[group]) << 13) |
 // at, where `at` is the LDS accumulator the entry adds into: the row's,
 // r - r0 < kBlkRows, for a short row; for one of the tile's long rows (more
 // than long_min entries, at most kBlkLongMax per tile, the j-th) the slot of
@@ -213,14 +213,14 @@ struct Sliced {
 // 64; kBlkNull (a dummy) for a padding entry.  A row longer than tile_nnz is
 // cut into chunk tiles of its own (r0 == r1).  Long rows are finished by the
 // fold from int64 slots (long_rows[k], slot k; ltab lists each tile's).
-constexpr uint32_t kBlkCodeBits = 14;
-constexpr uint32_t kBlkRows = 8064;      // rows in a tile's range
-constexpr uint32_t kBlkAcc = 8128;       // row accumulators and dummies, then lane slots
-constexpr uint32_t kBlkNull = 8127;      // accumulator of the padding entries (a dummy)
-constexpr uint32_t kBlkLongMax = 64;     // long rows per tile
+constexpr uint32_t kBlkCodeBits = 13;
+constexpr uint32_t kBlkRows = 4032;      // rows in a tile's range
+constexpr uint32_t kBlkAcc = 4096;       // row accumulators and dummies, then lane slots
+constexpr uint32_t kBlkNull = 4095;      // accumulator of the padding entries (a dummy)
+constexpr uint32_t kBlkLongMax = 32;     // long rows per tile
 constexpr uint32_t kBlkSpan = 1u << (32 - kBlkCodeBits);  // source span of a group
-constexpr uint32_t kBlkTileNnzDefault = 65536;
-constexpr uint32_t kBlkLongMinDefault = 1024;
+constexpr uint32_t kBlkTileNnzDefault = 131072;
+constexpr uint32_t kBlkLongMinDefault = 2048;
 
 struct BlkTile {
     uint32_t r0, r1;      // rows finished by the tile (long rows excepted)

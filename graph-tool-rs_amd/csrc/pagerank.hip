@@ -837,6 +837,8 @@ gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64
     b.tiles = B->tiles.as<BlkTile>();
     b.ntiles = B->ntiles;
     b.nlong = B->nlong;
+    b.glast = B->ngroups ? (uint32_t)(B->ngroups - 1) : 0u;
+    b.n = g->n;
     b.words = B->words.as<uint32_t>();
     b.gbase = B->gbase.as<uint32_t>();
     b.ltab = B->ltab.as<uint32_t>();
@@ -862,7 +864,7 @@ gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        blk_launch_sweep(b, S.plan.dpers, 1, s);
+        blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, s);
     };
     auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, s); };
     uint64_t launched = 0;

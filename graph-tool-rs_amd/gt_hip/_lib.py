@@ -55,7 +55,8 @@ class GtTuning(C.Structure):
                 ("short_row", C.c_int32), ("mm_tile_nnz", C.c_int32),
                 ("mm_blocks_per_cu", C.c_int32), ("sweep_hot_kb", C.c_int32),
                 ("sliced_min_len", C.c_int32), ("sliced_share", C.c_int32),
-                ("block_tile_nnz", C.c_int32), ("block_long_min", C.c_int32)]
+                ("block_tile_nnz", C.c_int32), ("block_long_min", C.c_int32),
+                ("block_per_cu", C.c_int32)]
 
 
 class GtView(C.Structure):

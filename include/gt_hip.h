@@ -418,9 +418,10 @@ typedef struct gt_tuning {
     int32_t block_tile_nnz;      /* FP32 PageRank sweeps on a relabelled, unfiltered, unweighted
                                     graph: entries per block tile (rows sorted together by source,
                                     summed in 64-bit fixed point); -1 turns the block sweep off and
-                                    the wave engine runs: -1, 1024..1048576 (65536) */
+                                    the wave engine runs: -1, 1024..1048576 (131072) */
     int32_t block_long_min;      /* block tiles: rows with more entries than this are summed in
-                                    lane-private slots (at most 64 per tile): 16..65536 (1024) */
+                                    lane-private slots (at most 32 per tile): 16..65536 (2048) */
+    int32_t block_per_cu;        /* block sweeps: resident workgroups per CU: 1..3 (2) */
 } gt_tuning;
 
 gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);

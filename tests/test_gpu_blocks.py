@@ -117,16 +117,17 @@ def test_empty_rows_cap_the_tile_ranges(gpu):
 
 
 def test_groups_cut_by_source_span(gpu):
-    """2^22 + 5 vertices, vertex v's one out-edge to v mod 300,000: every
+    """2^22 + 5 vertices, vertex v's one out-edge to v mod 600,000: every
     vertex is gathered once (so relabelling keeps the ids), and row r's
-    sources are r + 300,000 k.  A 1,024-entry tile of 73 rows sorts into runs
-    of 73 consecutive sources 300,000 labels apart, so each run ends a group
-    early (a group spans fewer than 2^18 labels) and groups are padded."""
+    sources are r + 600,000 k.  A 1,024-entry tile sorts into runs of about
+    146 consecutive sources 600,000 labels apart; a group spans fewer than
+    2^19 labels, so every run ends a group early.  Without such cuts a tile
+    needs at most entries / 64 + 1 groups."""
     n = (1 << 22) + 5
     s = np.arange(n, dtype=np.uint32)
-    t = (s % 300_000).astype(np.uint32)
+    t = (s % 600_000).astype(np.uint32)
     _, lay, _, _ = check(n, s, t, tile=1024)
-    assert lay["block_groups"] * 64 > 3 * n // 2, lay
+    assert lay["block_groups"] > n // 64 + lay["block_tiles"] + 10_000, lay
 
 
 def test_warm_start_and_cap(gpu):
