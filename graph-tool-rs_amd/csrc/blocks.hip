@@ -67,13 +67,21 @@ __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
 }
 
 // Row i's update from its sum `acc` and its inputs (iterate xc, out-degree
-// dg, personalisation pv): PrFin's FP32 finisher (the teleport update in
-// FP64, the iterate and its contribution rounded to FP32), plus the L1 mass
-// of the new iterate.
+// dg, personalisation pv).  A sweep of the FP32 phase follows PrFin's FP32
+// finisher (the teleport update in FP64, the iterate and its contribution
+// rounded to FP32) and adds the L1 mass of the new iterate; the refinement
+// sweep follows RefineFin (the FP64 update is the output; delta against the
+// FP32 iterate; no contribution, no dangling mass).
+template <bool REFINE>
 __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc, float xc,
                                            uint32_t dg, double pv, double p_sink, double &dsum,
                                            double &ssum, double &msum) {
     const double nv = (1.0 - b.damping) * pv + b.damping * (p_sink * pv + acc);
+    if (REFINE) {
+        dsum += fabs(nv - static_cast<double>(xc));
+        b.x_next64[i] = nv;
+        return;
+    }
     const float nvv = static_cast<float>(nv);
     dsum += fabs(static_cast<double>(nvv) - static_cast<double>(xc));
     msum += fabs(static_cast<double>(nvv));
@@ -110,10 +118,12 @@ __device__ __forceinline__ unsigned long long sum16(unsigned long long v) {
 // accumulator read once and cleared for the next tile.  The tile's delta,
 // dangling and mass partials go to its own entries of b.part, so the fold
 // sums them in tile order whichever workgroup took the tile.
-template <bool DPERS>
+template <bool DPERS, bool REFINE>
 __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
     SweepState *st = b.st;
-    if (st->done) return;
+    // The refinement runs once, after the FP32 loop finished (its state says
+    // done), so it does not look at the flag.
+    if (!REFINE && st->done) return;
     __shared__ unsigned long long acc[kBlkLds];
     __shared__ double red[3 * kBlkWaves];
     __shared__ uint32_t s_tile[2];
@@ -213,8 +223,8 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
                 const uint32_t q = i - T.r0;
                 const long long sv = static_cast<long long>(acc[q]);
                 acc[q] = 0ull;
-                blk_finish(b, i, ldexp(static_cast<double>(sv), -K), xr[k], dr[k], pr[k], p_sink,
-                           dsum, ssum, msum);
+                blk_finish<REFINE>(b, i, ldexp(static_cast<double>(sv), -K), xr[k], dr[k], pr[k],
+                                   p_sink, dsum, ssum, msum);
             }
         }
         dsum = wave_sum(dsum);
@@ -262,10 +272,10 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 // cleared for the next sweep), then, in the last workgroup to arrive, the
 // tile partials in tile order and this kernel's in workgroup order, the
 // convergence decision and the next sweep's mass.
-template <bool DPERS>
+template <bool DPERS, bool REFINE>
 __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     SweepState *st = b.st;
-    if (st->done) return;
+    if (!REFINE && st->done) return;
     __shared__ double red[12];
     __shared__ uint32_t flag;
     const int K = blk_scale(b.bs->mass);
@@ -275,8 +285,9 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         const uint32_t r = b.long_rows[k];
         const long long sv = static_cast<long long>(b.slots[k]);
         b.slots[k] = 0ull;
-        blk_finish(b, r, ldexp(static_cast<double>(sv), -K), b.x_cur[r],
-                   b.out_off[r + 1] - b.out_off[r], DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
+        blk_finish<REFINE>(b, r, ldexp(static_cast<double>(sv), -K), b.x_cur[r],
+                           b.out_off[r + 1] - b.out_off[r], DPERS ? b.pers[r] : b.pval, p_sink, d,
+                           s, ms);
     }
     block_sum3(d, s, ms, red);
     double *fp = b.fold_part;
@@ -536,20 +547,24 @@ void blk_launch_start(const BlkArgs &b, hipStream_t s) {
     hipLaunchKernelGGL(pr_start_blk, dim3(fold_grid(b.nlong)), dim3(kBlock), 0, s, b);
 }
 
-void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, hipStream_t s) {
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, bool refine, hipStream_t s) {
     const int grid = fit_grid(device_cus() * std::max(1, per_cu), b.ntiles);
-    if (dpers)
-        hipLaunchKernelGGL(pr_sweep_blk<true>, dim3(grid), dim3(kBlkThreads), 0, s, b);
-    else
-        hipLaunchKernelGGL(pr_sweep_blk<false>, dim3(grid), dim3(kBlkThreads), 0, s, b);
+    switch ((dpers ? 2 : 0) | (refine ? 1 : 0)) {
+        case 0: hipLaunchKernelGGL((pr_sweep_blk<false, false>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
+        case 1: hipLaunchKernelGGL((pr_sweep_blk<false, true>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
+        case 2: hipLaunchKernelGGL((pr_sweep_blk<true, false>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
+        default: hipLaunchKernelGGL((pr_sweep_blk<true, true>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
+    }
 }
 
-void blk_launch_fold(const BlkArgs &b, bool dpers, hipStream_t s) {
+void blk_launch_fold(const BlkArgs &b, bool dpers, bool refine, hipStream_t s) {
     const int grid = fold_grid(b.nlong);
-    if (dpers)
-        hipLaunchKernelGGL(pr_fold_blk<true>, dim3(grid), dim3(kBlock), 0, s, b);
-    else
-        hipLaunchKernelGGL(pr_fold_blk<false>, dim3(grid), dim3(kBlock), 0, s, b);
+    switch ((dpers ? 2 : 0) | (refine ? 1 : 0)) {
+        case 0: hipLaunchKernelGGL((pr_fold_blk<false, false>), dim3(grid), dim3(kBlock), 0, s, b); break;
+        case 1: hipLaunchKernelGGL((pr_fold_blk<false, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+        case 2: hipLaunchKernelGGL((pr_fold_blk<true, false>), dim3(grid), dim3(kBlock), 0, s, b); break;
+        default: hipLaunchKernelGGL((pr_fold_blk<true, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+    }
 }
 
 }  // namespace gt

@@ -60,6 +60,7 @@ struct BlkArgs {
     float *x_next;
     const float *c_cur;
     float *c_next;
+    double *x_next64;           // the refinement sweep's output
 };
 
 // Build (once per tile size and long-row threshold) the block tiles of R's
@@ -72,8 +73,10 @@ gt_status blk_reserve(gt_graph *g, const Blocks &B);
 // clear the bail flag.
 void blk_launch_start(const BlkArgs &b, hipStream_t s);
 // One sweep (workgroups: `per_cu` per compute unit, at most one per tile).
-void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, hipStream_t s);
+// With `refine`, the FP32 form's last sweep: FP64 output to x_next64, run
+// whatever the loop state says.
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, bool refine, hipStream_t s);
 // Completes a sweep: long rows, partials in tile order, convergence.
-void blk_launch_fold(const BlkArgs &b, bool dpers, hipStream_t s);
+void blk_launch_fold(const BlkArgs &b, bool dpers, bool refine, hipStream_t s);
 
 }  // namespace gt

@@ -812,6 +812,35 @@ bool blocks_apply(const PrSolve &S) {
            S.g->tune.block_tile_nnz > 0;
 }
 
+// The block-sweep arguments of a solve over B.
+BlkArgs blk_args(const PrSolve &S, const PrArgs &a, const Blocks &B) {
+    gt_graph *g = S.g;
+    BlkArgs b{};
+    b.tiles = B.tiles.as<BlkTile>();
+    b.ntiles = B.ntiles;
+    b.nlong = B.nlong;
+    b.glast = B.ngroups ? (uint32_t)(B.ngroups - 1) : 0u;
+    b.n = g->n;
+    b.words = B.words.as<uint32_t>();
+    b.gbase = B.gbase.as<uint32_t>();
+    b.ltab = B.ltab.as<uint32_t>();
+    b.long_rows = B.long_rows.as<uint32_t>();
+    b.lmask = B.lmask.as<uint32_t>();
+    b.slots = g->ws.blk_slots.as<unsigned long long>();
+    b.part = g->ws.blk_part.as<double>();
+    b.fold_part = g->ws.partials.as<double>();
+    b.bs = g->ws.blk_state.as<BlkState>();
+    b.st = a.st;
+    b.out_off = a.out_off;
+    b.pers = a.pers;
+    b.pval = a.pval;
+    b.damping = a.damping;
+    b.eps_abs = a.eps_abs;
+    b.eps_rel = a.eps_rel;
+    b.max_iter = a.max_iter;
+    return b;
+}
+
 // Init + FP32 sweeps on block tiles, on buffers x / c; returns the sweep
 // count reached and leaves the latest iterate in x[count & 1], as pr_phase.
 // Sets *bailed when a sweep met an iterate whose mass its fixed-point sums
@@ -833,29 +862,7 @@ gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64
     a.x_next = x[0];
     a.c_next = c[0];
     dispatch_init<float>(S.plan, a, S.rank64, had, g->n, 0, s);
-    BlkArgs b{};
-    b.tiles = B->tiles.as<BlkTile>();
-    b.ntiles = B->ntiles;
-    b.nlong = B->nlong;
-    b.glast = B->ngroups ? (uint32_t)(B->ngroups - 1) : 0u;
-    b.n = g->n;
-    b.words = B->words.as<uint32_t>();
-    b.gbase = B->gbase.as<uint32_t>();
-    b.ltab = B->ltab.as<uint32_t>();
-    b.long_rows = B->long_rows.as<uint32_t>();
-    b.lmask = B->lmask.as<uint32_t>();
-    b.slots = g->ws.blk_slots.as<unsigned long long>();
-    b.part = g->ws.blk_part.as<double>();
-    b.fold_part = g->ws.partials.as<double>();
-    b.bs = g->ws.blk_state.as<BlkState>();
-    b.st = a.st;
-    b.out_off = a.out_off;
-    b.pers = a.pers;
-    b.pval = a.pval;
-    b.damping = a.damping;
-    b.eps_abs = a.eps_abs;
-    b.eps_rel = a.eps_rel;
-    b.max_iter = a.max_iter;
+    BlkArgs b = blk_args(S, a, *B);
     blk_launch_start(b, s);
     GT_HIP(hipGetLastError());
     auto sweep = [&](uint64_t k) {
@@ -864,9 +871,9 @@ gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64
         b.x_next = x[par ^ 1];
         b.c_cur = c[par];
         b.c_next = c[par ^ 1];
-        blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, s);
+        blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, false, s);
     };
-    auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, s); };
+    auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, false, s); };
     uint64_t launched = 0;
     GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched, g->prof_kind));
     *bailed = g->h_state->bail != 0;
@@ -1089,14 +1096,26 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         g->prof_kind = GT_SWEEP_PAGERANK_F64;
         GT_TRY(prof_prepare(g));
         GT_TRY(prof_begin(g, 0));
-        const uint32_t nhot = S.iperm && !S.plan.masked && !S.plan.weighted
-                                  ? hot_len<float>(g->tune.sweep_hot_kb, n)
-                                  : 0u;
-        const Sliced *SL = nullptr;
-        if (nhot) GT_TRY(sliced_for<float>(S, nhot, &SL));
-        if (SL) ar.long_part = ws.long_part.as<double>();
-        dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, nhot, SL, s);
-        GT_TRY(prof_end(g, 0));
+        if (g->layout.block_tiles) {
+            // The FP32 sweeps ran on block tiles: so does the refinement
+            // (fixed-point sums, FP64 update and output).
+            BlkArgs b = blk_args(S, ar, S.rel->blk);
+            b.x_cur = x32[it32 & 1];
+            b.c_cur = c32[it32 & 1];
+            b.x_next64 = x64;
+            blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, true, s);
+            GT_TRY(prof_end(g, 0));
+            blk_launch_fold(b, S.plan.dpers, true, s);
+        } else {
+            const uint32_t nhot = S.iperm && !S.plan.masked && !S.plan.weighted
+                                      ? hot_len<float>(g->tune.sweep_hot_kb, n)
+                                      : 0u;
+            const Sliced *SL = nullptr;
+            if (nhot) GT_TRY(sliced_for<float>(S, nhot, &SL));
+            if (SL) ar.long_part = ws.long_part.as<double>();
+            dispatch_refine(S.plan, ar, g->tune.sweep_blocks_per_cu, nhot, SL, s);
+            GT_TRY(prof_end(g, 0));
+        }
         GT_HIP(hipGetLastError());
         GT_TRY(prof_flush(g, 1));
         total = it32 + 1;
