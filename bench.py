@@ -861,6 +861,7 @@ def main() -> int:
     # ---- FP32-value / FP64-residual form (the chosen form)
     dt32, sw32, ms32, la32, other32, st32 = timed("gt_pagerank_mixed", _lib.GT_SWEEP_PAGERANK_F32)
     x32 = rank.cpu().numpy().copy()
+    lay32 = dev.layout()
     f64_in_mixed = other32[_lib.GT_SWEEP_PAGERANK_F64]
     log(f"[bench] mixed: {dt32/args.steps*1e3:.3f} ms/solve, sweeps {sw32[0]}, "
         f"f32 launches {la32} ({ms32/max(la32,1)*1e3:.1f} us avg), "
@@ -904,8 +905,13 @@ def main() -> int:
 
     workload = f"pagerank_rmat{args.scale}_ef{args.edgefactor}"
     hot = dev.tuning()["sweep_hot_kb"] > 0
-    traffic32, traffic_src = profiled_traffic(workload,
-                                              "pr_sweep_hot" if hot else "pr_sweep<float>")
+    # The FP32 sweeps ran on block tiles unless the tuning turned them off
+    # (the timed solves' layout, read before the FP64 solves).
+    blk = {k: lay32[k] for k in ("block_tiles", "block_groups", "block_long_rows")}
+    on_blocks = blk["block_tiles"] > 0
+    k32 = ("pr_sweep_blk<false, false>" if on_blocks
+           else "pr_sweep_hot<float, false, false>" if hot else "pr_sweep<float>")
+    traffic32, traffic_src = profiled_traffic(workload, k32)
     result = {
         "metric": METRIC,
         "value": round(gteps32, 3),
@@ -929,10 +935,13 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
-            "kernel": ("pr_sweep_hot (FP32 PageRank sweep, hot contributions in LDS)" if hot
+            "kernel": ("pr_sweep_blk (FP32 PageRank sweep on block tiles: source-sorted tiles, "
+                       "64-bit fixed-point LDS row sums)" if on_blocks
+                       else "pr_sweep_hot (FP32 PageRank sweep, hot contributions in LDS)" if hot
                        else "pr_sweep<float> (FP32 PageRank sweep)"),
             "bytes_per_sweep": b32, "avg_launch_us": round(avg32 * 1e6, 2),
             "traffic_source": traffic_src, "copy_calibration": copy,
+            "layout": blk,
         },
         "tuning": tuning,
         "fp64_dropin": {
@@ -985,15 +994,20 @@ def main() -> int:
                         "floor_with_misses_over_sweep": round(fm / (avg * 1e6), 4) if avg else None})
         ceiling[key] = rec
 
-    for key, vbytes, avg, pk in (("fp32_sweep", 4, avg32, "pr_sweep_hot<float, false, false>"),
-                                 ("fp64_sweep", 8, avg64, "pr_sweep_hot<double, false, false>")):
+    # The floors model the wave engine's sweeps (hot slice in LDS, one
+    # request per gathered lane outside it): the FP64 and eigenvector sweeps,
+    # and the FP32 sweep when it ran there.
+    sweeps = [("fp64_sweep", 8, avg64, "pr_sweep_hot<double, false, false>")]
+    if not on_blocks:
+        sweeps.insert(0, ("fp32_sweep", 4, avg32, "pr_sweep_hot<float, false, false>"))
+    for key, vbytes, avg, pk in sweeps:
         nh = hot_len(kb, vbytes, n)
         floors(key, cold_gathers(src, n, nh), nh, avg, pk)
     result["gather_ceiling"] = ceiling
-    log(f"[bench] gather ceiling: {ns_l2:.3f} ns per L2-hit lane per CU; FP32 sweep floor "
-        f"{ceiling['fp32_sweep']['floor_us']:.0f} us "
-        f"({ceiling['fp32_sweep'].get('floor_with_misses_us', float('nan')):.0f} us with its "
-        f"L2 misses) vs {avg32*1e6:.0f} us")
+    log(f"[bench] gather ceiling: {ns_l2:.3f} ns per L2-hit lane per CU; FP64 sweep floor "
+        f"{ceiling['fp64_sweep']['floor_us']:.0f} us "
+        f"({ceiling['fp64_sweep'].get('floor_with_misses_us', float('nan')):.0f} us with its "
+        f"L2 misses) vs {avg64*1e6:.0f} us")
 
     # ---- eigenvector on the symmetrised graph
     if not args.no_eigen:

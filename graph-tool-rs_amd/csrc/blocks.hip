@@ -17,10 +17,15 @@ namespace gt {
 
 constexpr int kBlkWaves = 8;
 constexpr int kBlkThreads = kBlkWaves * kWave;
-// Groups in flight per wavefront: 16 coalesced word loads, then 16 gathers.
-constexpr int kBlkUnroll = 16;
+// Blocks of kBlkQuad groups per wavefront batch: 4 word loads of 16 bytes a
+// lane, then 16 gathers in flight.
+constexpr int kBlkBatch = 4;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kBlkLds = kBlkAcc + kBlkLongMax * kWave;
 constexpr int kBlkRowsPerThread = (kBlkRows + kBlkThreads - 1) / kBlkThreads;
+// The row accumulator of a long row in the tile's range: no sum reaches
+// 2^62 in magnitude, so no sum equals it.
+constexpr unsigned long long kBlkMarked = 1ull << 63;
 static_assert(kBlkLongMax * 16 <= kBlkThreads, "16 threads sum each long row");
 static_assert(kBlkLds * 8 <= 160 * 1024, "accumulators fit one CU's LDS");
 static_assert(kBlkNull >= kBlkRows && kBlkNull < kBlkAcc, "padding adds into a dummy");
@@ -147,56 +152,74 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
         if (t >= b.ntiles) break;
         const BlkTile T = blk_tile(b.tiles, t);
-        const uint32_t ng = T.g1 - T.g0;
+        // Groups come in blocks of kBlkQuad: lane l's 16-byte load of block q
+        // holds its word of each of the block's groups.  Wavefront w takes
+        // blocks w, w + 8, ..., kBlkBatch blocks per batch; the next batch's
+        // words and bases are loaded while this batch's gathers are in
+        // flight (a load past the tile's blocks reads a clamped block and is
+        // not added).  The words are read once per sweep: non-temporal loads
+        // keep them from evicting the gathered contributions.
+        const uint32_t q0 = T.g0 / kBlkQuad;
+        const uint32_t nq = (T.g1 - T.g0) / kBlkQuad;
         ConstU32 *gbase = (ConstU32 *)b.gbase;
-        // Batches of kBlkUnroll groups per wavefront.  The next batch's words
-        // and bases are loaded while this batch's gathers are in flight; a
-        // load past the tile's groups reads a clamped group and is not added.
-        // The words are read once per sweep: non-temporal loads keep them
-        // from evicting the gathered contributions (R-MAT 22: 158.5 -> 153.0
-        // us per sweep).
-        uint32_t word[kBlkUnroll], base[kBlkUnroll];
+        const u32x4 *wq = reinterpret_cast<const u32x4 *>(b.words);
+        u32x4 W[kBlkBatch];
+        uint32_t base[kBlkBatch * kBlkQuad];
 #pragma unroll
-        for (int u = 0; u < kBlkUnroll; ++u) {
-            const uint32_t g = min(T.g0 + w + u * kBlkWaves, b.glast);
-            word[u] = __builtin_nontemporal_load(b.words + (size_t)g * kWave + lane);
-            base[u] = gbase[g];
+        for (int j = 0; j < kBlkBatch; ++j) {
+            const uint32_t q = min(q0 + w + j * kBlkWaves, b.qlast);
+            W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
+#pragma unroll
+            for (int k = 0; k < kBlkQuad; ++k) base[j * kBlkQuad + k] = gbase[q * kBlkQuad + k];
         }
         // This thread's rows of the tile (r0 + tid + 512 k) and what the
         // finisher reads of them, loaded now: they arrive under the gathers
         // and the finisher waits on no load.  Loads past the tile read a
         // clamped row and are not used.
         float xr[kBlkRowsPerThread];
-        uint32_t dr[kBlkRowsPerThread], lm[kBlkRowsPerThread];
+        uint32_t dr[kBlkRowsPerThread];
         double pr[kBlkRowsPerThread];
 #pragma unroll
         for (int k = 0; k < kBlkRowsPerThread; ++k) {
             const uint32_t i = min(T.r0 + tid + k * kBlkThreads, b.n - 1);
             xr[k] = b.x_cur[i];
-            dr[k] = b.out_off[i + 1] - b.out_off[i];
-            lm[k] = b.lmask[i >> 5];
+            dr[k] = b.odeg[i];
             pr[k] = DPERS ? b.pers[i] : b.pval;
         }
-        for (uint32_t gi = w; gi < ng; gi += kBlkWaves * kBlkUnroll) {
-            float v[kBlkUnroll];
-            uint32_t at[kBlkUnroll];
+        // The tile's long rows inside its row range are finished by the
+        // fold: their row accumulators (no entry adds into them) are marked.
+        if (tid < T.nlong) {
+            const uint32_t r = b.long_rows[b.ltab[T.lfirst + tid]];
+            if (r >= T.r0 && r < T.r1) acc[r - T.r0] = kBlkMarked;
+        }
+        for (uint32_t qi = w; qi < nq; qi += kBlkWaves * kBlkBatch) {
+            float v[kBlkBatch * kBlkQuad];
+            uint32_t at[kBlkBatch * kBlkQuad];
 #pragma unroll
-            for (int u = 0; u < kBlkUnroll; ++u) {
-                v[u] = b.c_cur[base[u] + (word[u] >> kBlkCodeBits)];
-                at[u] = word[u] & kBlkCodeMask;
+            for (int j = 0; j < kBlkBatch; ++j) {
+#pragma unroll
+                for (int k = 0; k < kBlkQuad; ++k) {
+                    const uint32_t word = W[j][k];
+                    v[j * kBlkQuad + k] = b.c_cur[base[j * kBlkQuad + k] + (word >> kBlkCodeBits)];
+                    at[j * kBlkQuad + k] = word & kBlkCodeMask;
+                }
             }
 #pragma unroll
-            for (int u = 0; u < kBlkUnroll; ++u) {
-                const uint32_t g =
-                    min(T.g0 + gi + (kBlkUnroll + u) * kBlkWaves, b.glast);
-                word[u] = __builtin_nontemporal_load(b.words + (size_t)g * kWave + lane);
-                base[u] = gbase[g];
+            for (int j = 0; j < kBlkBatch; ++j) {
+                const uint32_t q = min(q0 + qi + (kBlkBatch + j) * kBlkWaves, b.qlast);
+                W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
+#pragma unroll
+                for (int k = 0; k < kBlkQuad; ++k) base[j * kBlkQuad + k] = gbase[q * kBlkQuad + k];
             }
 #pragma unroll
-            for (int u = 0; u < kBlkUnroll; ++u)
-                if (gi + u * kBlkWaves < ng)
-                    __hip_atomic_fetch_add(&acc[at[u]], blk_fixed(v[u], K), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int j = 0; j < kBlkBatch; ++j)
+                if (qi + j * kBlkWaves < nq) {
+#pragma unroll
+                    for (int k = 0; k < kBlkQuad; ++k)
+                        __hip_atomic_fetch_add(&acc[at[j * kBlkQuad + k]],
+                                               blk_fixed(v[j * kBlkQuad + k], K), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
         }
         __syncthreads();
         if (tid == 0) s_tile[(it + 1) & 1] = atomicAdd(&b.bs->next_tile, 1u);
@@ -219,12 +242,13 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
 #pragma unroll
         for (int k = 0; k < kBlkRowsPerThread; ++k) {
             const uint32_t i = T.r0 + tid + k * kBlkThreads;
-            if (i < T.r1 && !((lm[k] >> (i & 31)) & 1u)) {
+            if (i < T.r1) {
                 const uint32_t q = i - T.r0;
-                const long long sv = static_cast<long long>(acc[q]);
+                const unsigned long long a = acc[q];
                 acc[q] = 0ull;
-                blk_finish<REFINE>(b, i, ldexp(static_cast<double>(sv), -K), xr[k], dr[k], pr[k],
-                                   p_sink, dsum, ssum, msum);
+                if (a != kBlkMarked)
+                    blk_finish<REFINE>(b, i, ldexp(static_cast<double>(static_cast<long long>(a)), -K),
+                                       xr[k], dr[k], pr[k], p_sink, dsum, ssum, msum);
             }
         }
         dsum = wave_sum(dsum);
@@ -285,9 +309,8 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         const uint32_t r = b.long_rows[k];
         const long long sv = static_cast<long long>(b.slots[k]);
         b.slots[k] = 0ull;
-        blk_finish<REFINE>(b, r, ldexp(static_cast<double>(sv), -K), b.x_cur[r],
-                           b.out_off[r + 1] - b.out_off[r], DPERS ? b.pers[r] : b.pval, p_sink, d,
-                           s, ms);
+        blk_finish<REFINE>(b, r, ldexp(static_cast<double>(sv), -K), b.x_cur[r], b.odeg[r],
+                           DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
     }
     block_sum3(d, s, ms, red);
     double *fp = b.fold_part;
@@ -333,6 +356,14 @@ __global__ __launch_bounds__(kBlock) void pr_start_blk(const BlkArgs b) {
         b.bs->next_tile = 0;
         b.st->bail = 0;
     }
+}
+
+// odeg[i] = off[i + 1] - off[i]: the out-degree in sweep labels, read by the
+// block sweeps' finishers in one load.
+__global__ __launch_bounds__(kBlock) void blk_out_degree(const uint32_t *__restrict__ off,
+                                                         uint32_t n, uint32_t *__restrict__ odeg) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        odeg[i] = off[i + 1] - off[i];
 }
 
 // ------------------------------------------------------------------ host
@@ -462,13 +493,14 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
             }
             ++cnt;
         }
-        ngr[t] = groups;
+        ngr[t] = (groups + kBlkQuad - 1) / kBlkQuad * kBlkQuad;
     });
     std::vector<uint64_t> g0(ntiles + 1, 0);
     for (uint32_t t = 0; t < ntiles; ++t) g0[t + 1] = g0[t] + ngr[t];
     const uint64_t ngroups = g0[ntiles];
     if (ngroups >= (1ull << 32) / kWave)
         return fail(GT_ERR_UNSUPPORTED, "block tiles: too many entry groups");
+    // Padding groups (to whole blocks) hold null words over source 0.
     std::vector<uint32_t> words(std::max<uint64_t>(1, ngroups * kWave), kBlkNull);
     std::vector<uint32_t> gbase(std::max<uint64_t>(1, ngroups), 0);
     parallel_for(ntiles, [&](uint32_t t) {
@@ -491,7 +523,10 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
             const uint32_t j = c - kBlkKeyLong0;
             const uint32_t at =
                 j < kBlkLongMax ? kBlkAcc + j * kWave + ((cnt + j) & (kWave - 1)) : c;
-            words[gidx * kWave + cnt] = ((uint32_t)(src - base) << kBlkCodeBits) | at;
+            // Block gidx / 4 holds, for each lane, its word of the block's
+            // four groups side by side.
+            words[(gidx / kBlkQuad) * (kWave * kBlkQuad) + cnt * kBlkQuad + gidx % kBlkQuad] =
+                ((uint32_t)(src - base) << kBlkCodeBits) | at;
             ++cnt;
         }
     });
@@ -501,8 +536,6 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
         tiles[t] = BlkTile{T.r0, T.r1, (uint32_t)g0[t], (uint32_t)g0[t + 1], T.lfirst, T.nlong,
                            {0, 0}};
     }
-    std::vector<uint32_t> lmask(((size_t)n + 31) / 32 + 1, 0u);
-    for (uint32_t r : long_rows) lmask[r >> 5] |= 1u << (r & 31);
     if (ltab.empty()) ltab.push_back(0);
     const uint32_t nlong = (uint32_t)long_rows.size();
     if (long_rows.empty()) long_rows.push_back(0);
@@ -512,7 +545,14 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     GT_TRY(B.gbase.reserve(sizeof(uint32_t) * gbase.size()));
     GT_TRY(B.ltab.reserve(sizeof(uint32_t) * ltab.size()));
     GT_TRY(B.long_rows.reserve(sizeof(uint32_t) * long_rows.size()));
-    GT_TRY(B.lmask.reserve(sizeof(uint32_t) * lmask.size()));
+    GT_TRY(B.odeg.reserve(sizeof(uint32_t) * std::max<uint32_t>(1, n)));
+    if (n) {
+        hipLaunchKernelGGL(blk_out_degree, dim3(std::min<uint32_t>(2048, (n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, g->stream, R.push_off.as<uint32_t>(), n,
+                           B.odeg.as<uint32_t>());
+        GT_HIP(hipGetLastError());
+        GT_HIP(hipStreamSynchronize(g->stream));
+    }
     GT_HIP(hipMemcpy(B.tiles.ptr, tiles.data(), sizeof(BlkTile) * tiles.size(),
                      hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(B.words.ptr, words.data(), sizeof(uint32_t) * words.size(),
@@ -522,8 +562,6 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     GT_HIP(hipMemcpy(B.ltab.ptr, ltab.data(), sizeof(uint32_t) * ltab.size(),
                      hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(B.long_rows.ptr, long_rows.data(), sizeof(uint32_t) * long_rows.size(),
-                     hipMemcpyHostToDevice));
-    GT_HIP(hipMemcpy(B.lmask.ptr, lmask.data(), sizeof(uint32_t) * lmask.size(),
                      hipMemcpyHostToDevice));
     B.ntiles = ntiles;
     B.nlong = nlong;

@@ -38,19 +38,18 @@ struct BlkArgs {
     const BlkTile *tiles;
     uint32_t ntiles;
     uint32_t nlong;
-    uint32_t glast;             // last group (loads ahead of a tile's end clamp to it)
+    uint32_t qlast;             // last block of groups (loads ahead of a tile's end clamp to it)
     uint32_t n;                 // rows
     const uint32_t *words;
     const uint32_t *gbase;
     const uint32_t *ltab;
     const uint32_t *long_rows;
-    const uint32_t *lmask;
+    const uint32_t *odeg;       // out-degree, sweep labels
     unsigned long long *slots;  // [nlong] int64 sums of the long rows
     double *part;               // [3 * ntiles]: delta, dangling, mass per tile
     double *fold_part;          // [3 * fold grid]
     BlkState *bs;
     SweepState *st;
-    const uint32_t *out_off;    // unity out-degree, sweep labels
     const double *pers;         // dense personalisation, sweep labels (DPERS)
     double pval;
     double damping;

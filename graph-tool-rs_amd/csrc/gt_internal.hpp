@@ -205,7 +205,9 @@ struct Sliced {
 // Block tiles of a relabelled pull orientation (the FP32 PageRank sweeps):
 // each tile is a run of rows [r0, r1) with at most tile_nnz entries, its
 // entries sorted by source across all its rows and cut into groups of 64
-// (one gather instruction).  A word is ((source - gbase[group]) << 13) |
+// (one gather instruction), padded to whole blocks of kBlkQuad groups; a
+// block's words are stored lane-major (lane l's words of its groups side by
+// side).  A word is ((source - gbase[group]) << 13) |
 // at, where `at` is the LDS accumulator the entry adds into: the row's,
 // r - r0 < kBlkRows, for a short row; for one of the tile's long rows (more
 // than long_min entries, at most kBlkLongMax per tile, the j-th) the slot of
@@ -219,6 +221,7 @@ constexpr uint32_t kBlkAcc = 4096;       // row accumulators and dummies, then l
 constexpr uint32_t kBlkNull = 4095;      // accumulator of the padding entries (a dummy)
 constexpr uint32_t kBlkLongMax = 32;     // long rows per tile
 constexpr uint32_t kBlkSpan = 1u << (32 - kBlkCodeBits);  // source span of a group
+constexpr uint32_t kBlkQuad = 4;         // groups per block (one 16-byte load a lane)
 constexpr uint32_t kBlkTileNnzDefault = 131072;
 constexpr uint32_t kBlkLongMinDefault = 2048;
 
@@ -238,7 +241,7 @@ struct Blocks {
     DevBuf gbase;        // uint32[ngroups]
     DevBuf ltab;         // uint32[]: long-row slot of each tile's long code
     DevBuf long_rows;    // uint32[nlong]
-    DevBuf lmask;        // uint32[(n + 31) / 32]: long rows, one bit each
+    DevBuf odeg;         // uint32[n]: out-degree in sweep labels
     uint32_t ntiles = 0, nlong = 0;
     uint64_t ngroups = 0;
 };

@@ -819,19 +819,18 @@ BlkArgs blk_args(const PrSolve &S, const PrArgs &a, const Blocks &B) {
     b.tiles = B.tiles.as<BlkTile>();
     b.ntiles = B.ntiles;
     b.nlong = B.nlong;
-    b.glast = B.ngroups ? (uint32_t)(B.ngroups - 1) : 0u;
+    b.qlast = B.ngroups ? (uint32_t)(B.ngroups / kBlkQuad - 1) : 0u;
     b.n = g->n;
     b.words = B.words.as<uint32_t>();
     b.gbase = B.gbase.as<uint32_t>();
     b.ltab = B.ltab.as<uint32_t>();
     b.long_rows = B.long_rows.as<uint32_t>();
-    b.lmask = B.lmask.as<uint32_t>();
+    b.odeg = B.odeg.as<uint32_t>();
     b.slots = g->ws.blk_slots.as<unsigned long long>();
     b.part = g->ws.blk_part.as<double>();
     b.fold_part = g->ws.partials.as<double>();
     b.bs = g->ws.blk_state.as<BlkState>();
     b.st = a.st;
-    b.out_off = a.out_off;
     b.pers = a.pers;
     b.pval = a.pval;
     b.damping = a.damping;

@@ -3,10 +3,12 @@
 The pull sweeps (PageRank, eigenvector, adjoint, uneven batch) depend on
 three parameters: the resident workgroups per compute unit, the rows per wave
 tile, and the row length below which one lane sums a row (above it the whole
-wavefront does).  The FP32 PageRank sweep on a relabelled graph runs one
+wavefront does).  The FP64 PageRank sweep on a relabelled graph runs one
 workgroup per compute unit and also depends on how many of the most gathered
-contributions it keeps in LDS; the PageRank probe therefore times an FP32
-sweep and an FP64 sweep, and minimises their sum.  The batched SpMM depends on its tile size and its resident
+contributions it keeps in LDS.  The FP32 PageRank sweep on a relabelled,
+unweighted graph runs on block tiles, whose size and resident workgroups per
+compute unit are tuned too; the PageRank probe therefore times an FP32 sweep
+and an FP64 sweep, and minimises their sum.  The batched SpMM depends on its tile size and its resident
 workgroups per compute unit.  ``autotune`` times candidates on the caller's
 own graph -- the sweep kernels alone, measured with the HIP events the library
 records on its stream (gt_graph_set_profiling / gt_sweep_stats) -- walks each
@@ -37,6 +39,8 @@ SWEEP_SPACE = {
     "short_row": (32, 64, 256),
     "sweep_hot_kb": (128, 64, -1),
     "sliced_min_len": (-1, 64, 256),
+    "block_tile_nnz": (131072, 65536),
+    "block_per_cu": (2, 1),
 }
 BATCHED_SPACE = {
     "mm_tile_nnz": (512, 1024, 2048, 4096),

@@ -122,12 +122,12 @@ def test_groups_cut_by_source_span(gpu):
     sources are r + 600,000 k.  A 1,024-entry tile sorts into runs of about
     146 consecutive sources 600,000 labels apart; a group spans fewer than
     2^19 labels, so every run ends a group early.  Without such cuts a tile
-    needs at most entries / 64 + 1 groups."""
+    needs at most entries / 64 + 4 groups (whole blocks of four)."""
     n = (1 << 22) + 5
     s = np.arange(n, dtype=np.uint32)
     t = (s % 600_000).astype(np.uint32)
     _, lay, _, _ = check(n, s, t, tile=1024)
-    assert lay["block_groups"] > n // 64 + lay["block_tiles"] + 10_000, lay
+    assert lay["block_groups"] > n // 64 + 4 * lay["block_tiles"] + 6_000, lay
 
 
 def test_warm_start_and_cap(gpu):
