@@ -27,7 +27,7 @@ constexpr int kBlkRowsPerThread = (kBlkRows + kBlkThreads - 1) / kBlkThreads;
 // 2^62 in magnitude, so no sum equals it.
 constexpr unsigned long long kBlkMarked = 1ull << 63;
 static_assert(kBlkLongMax * 16 <= kBlkThreads, "16 threads sum each long row");
-static_assert(kBlkLds * 8 <= 160 * 1024, "accumulators fit one CU's LDS");
+static_assert(2 * kBlkLds * 8 <= 160 * 1024, "two accumulator words fit one CU's LDS");
 static_assert(kBlkNull >= kBlkRows && kBlkNull < kBlkAcc, "padding adds into a dummy");
 static_assert(kBlkLds <= (1u << kBlkCodeBits), "accumulator indices fit their bits");
 // Row codes of the sort key: a short row's r - r0, the j-th long row's
@@ -45,18 +45,57 @@ __device__ __forceinline__ int blk_scale(double mass) {
     return mass > 0.0 ? 61 - ilogb(mass) : 0;
 }
 
-// round(c * 2^K) (ties to even) as a two's complement 64-bit integer.  In
-// FP64 every step is exact: c * 2^K has at most 24 significant bits and
-// |c| * 2^K < 2^62 (blk_scale), so the high and low 32-bit halves are found
-// by floor and fma and converted without rounding.
-__device__ __forceinline__ unsigned long long blk_fixed(float c, int K) {
-    const double f = rint(ldexp(static_cast<double>(c), K));
-    const double h = floor(ldexp(f, -32));
-    const double l = fma(h, -0x1p32, f);
-    const uint32_t hi = static_cast<uint32_t>(static_cast<int32_t>(h));
-    const uint32_t lo = static_cast<uint32_t>(l);
+// An integer-valued double of magnitude below 2^62 as a two's complement
+// 64-bit integer: its high and low 32-bit halves by floor and fma, each
+// converted exactly.
+__device__ __forceinline__ unsigned long long blk_i64(double h) {
+    const double hh = floor(ldexp(h, -32));
+    const double ll = fma(hh, -0x1p32, h);
+    const uint32_t hi = static_cast<uint32_t>(static_cast<int32_t>(hh));
+    const uint32_t lo = static_cast<uint32_t>(ll);
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
+
+// round(c * 2^K) (ties to even) as a 64-bit integer.  Exact in FP64: c * 2^K
+// has at most 24 significant bits and |c| * 2^K < 2^62 (blk_scale).
+__device__ __forceinline__ unsigned long long blk_fixed(float c, int K) {
+    return blk_i64(rint(ldexp(static_cast<double>(c), K)));
+}
+
+// An FP64 term in two words: hi = round(c * 2^K) and lo = round(r *
+// 2^kBlkLoBits), r = c * 2^K - hi (|r| <= 1/2, found exactly), so c = (hi +
+// lo * 2^-kBlkLoBits) * 2^-K to 2^-(K + kBlkLoBits + 1).  The low words of
+// a row (at most 2^23 terms) sum below 2^62.
+constexpr int kBlkLoBits = 38;
+__device__ __forceinline__ void blk_fixed2(double c, int K, unsigned long long &hi,
+                                           unsigned long long &lo) {
+    const double f = ldexp(c, K);
+    const double h = rint(f);
+    hi = blk_i64(h);
+    lo = blk_i64(rint(ldexp(f - h, kBlkLoBits)));
+}
+
+// A row's value from its one or two summed words.
+__device__ __forceinline__ double blk_value(unsigned long long hi, int K) {
+    return ldexp(static_cast<double>(static_cast<long long>(hi)), -K);
+}
+__device__ __forceinline__ double blk_value2(unsigned long long hi, unsigned long long lo, int K) {
+    return ldexp(static_cast<double>(static_cast<long long>(hi)) +
+                     ldexp(static_cast<double>(static_cast<long long>(lo)), -kBlkLoBits),
+                 -K);
+}
+
+template <int MODE> struct BlkTypes {
+    typedef float X;  // the iterate a row update reads
+    typedef float C;  // the gathered contribution
+    static constexpr bool kTwo = false;
+};
+template <> struct BlkTypes<kBlkF64> {
+    typedef double X;
+    typedef double C;
+    static constexpr bool kTwo = true;
+};
+template <> struct BlkTypes<kBlkEv> : BlkTypes<kBlkF64> {};
 
 // Tile t's descriptor, as scalar loads (the table is read-only in a sweep).
 __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
@@ -72,23 +111,35 @@ __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
 }
 
 // Row i's update from its sum `acc` and its inputs (iterate xc, out-degree
-// dg, personalisation pv).  A sweep of the FP32 phase follows PrFin's FP32
-// finisher (the teleport update in FP64, the iterate and its contribution
-// rounded to FP32) and adds the L1 mass of the new iterate; the refinement
-// sweep follows RefineFin (the FP64 update is the output; delta against the
-// FP32 iterate; no contribution, no dangling mass).
-template <bool REFINE>
-__device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc, float xc,
+// dg, personalisation pv).  The FP32 sweep follows PrFin's FP32 finisher
+// (the teleport update in FP64, the iterate and its contribution rounded to
+// FP32), the FP64 sweep PrFin's FP64 one; both add the L1 mass of the new
+// iterate.  The refinement follows RefineFin (the FP64 update is the
+// output; delta against the FP32 iterate; no contribution, no dangling mass).
+template <int MODE>
+__device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc, double xc,
                                            uint32_t dg, double pv, double p_sink, double &dsum,
                                            double &ssum, double &msum) {
     const double nv = (1.0 - b.damping) * pv + b.damping * (p_sink * pv + acc);
-    if (REFINE) {
-        dsum += fabs(nv - static_cast<double>(xc));
+    if (MODE == kBlkRefine) {
+        dsum += fabs(nv - xc);
         b.x_next64[i] = nv;
         return;
     }
+    if (MODE == kBlkF64) {
+        dsum += fabs(nv - xc);
+        msum += fabs(nv);
+        b.x_next64[i] = nv;
+        if (dg != 0) {
+            b.c_next64[i] = nv / static_cast<double>(dg);
+        } else {
+            b.c_next64[i] = 0.0;
+            ssum += nv;
+        }
+        return;
+    }
     const float nvv = static_cast<float>(nv);
-    dsum += fabs(static_cast<double>(nvv) - static_cast<double>(xc));
+    dsum += fabs(static_cast<double>(nvv) - xc);
     msum += fabs(static_cast<double>(nvv));
     b.x_next[i] = nvv;
     if (dg != 0) {
@@ -97,6 +148,22 @@ __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double 
         b.c_next[i] = 0.0f;
         ssum += static_cast<double>(nvv);
     }
+}
+
+// The eigenvector's row update, as EvFin (eigenvector.hip): row i's new
+// unscaled iterate sv = acc / sc_cur (acc sums y_{L-1} over the row), its
+// square into the norm, the delta of sweep L-1 from the row's two previous
+// iterates yc = y_{L-1}(i), yp = y_{L-2}(i), and |sv| * out-degree into the
+// mass the next launch's terms are bounded by.
+__device__ __forceinline__ void blk_finish_ev(const BlkArgs &b, uint32_t i, double acc, double yc,
+                                              double yp, uint32_t dg, double sc_cur,
+                                              double sc_prev, double &dsum, double &ssum,
+                                              double &msum) {
+    const double sv = acc / sc_cur;
+    b.x_next64[i] = sv;
+    ssum += sv * sv;
+    dsum += fabs(yc / sc_cur - yp / sc_prev);
+    msum += fabs(sv) * static_cast<double>(dg);
 }
 
 // Sum of a 64-bit value over each aligned group of 16 lanes.
@@ -123,13 +190,17 @@ __device__ __forceinline__ unsigned long long sum16(unsigned long long v) {
 // accumulator read once and cleared for the next tile.  The tile's delta,
 // dangling and mass partials go to its own entries of b.part, so the fold
 // sums them in tile order whichever workgroup took the tile.
-template <bool DPERS, bool REFINE>
+template <bool DPERS, int MODE>
 __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
+    typedef BlkTypes<MODE> M;
+    constexpr bool EV = MODE == kBlkEv;
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
     // done), so it does not look at the flag.
-    if (!REFINE && st->done) return;
+    if (MODE != kBlkRefine && st->done) return;
     __shared__ unsigned long long acc[kBlkLds];
+    // The FP64 sweep's low words (a dummy otherwise).
+    __shared__ unsigned long long acc_lo[M::kTwo ? kBlkLds : 1];
     __shared__ double red[3 * kBlkWaves];
     __shared__ uint32_t s_tile[2];
     const uint32_t tid = threadIdx.x;
@@ -144,8 +215,13 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         return;
     }
     const int K = blk_scale(mass);
-    const double p_sink = st->p_sink;
-    for (uint32_t i = tid; i < kBlkLds; i += kBlkThreads) acc[i] = 0ull;
+    const double p_sink = EV ? 0.0 : st->p_sink;
+    const double sc_cur = EV ? b.ev->sc_cur : 1.0;
+    const double sc_prev = EV ? b.ev->sc_prev : 1.0;
+    for (uint32_t i = tid; i < kBlkLds; i += kBlkThreads) {
+        acc[i] = 0ull;
+        if (M::kTwo) acc_lo[i] = 0ull;
+    }
     if (tid == 0) s_tile[0] = atomicAdd(&b.bs->next_tile, 1u);
     __syncthreads();
     for (uint32_t it = 0;; ++it) {
@@ -175,16 +251,23 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         // This thread's rows of the tile (r0 + tid + 512 k) and what the
         // finisher reads of them, loaded now: they arrive under the gathers
         // and the finisher waits on no load.  Loads past the tile read a
-        // clamped row and are not used.
-        float xr[kBlkRowsPerThread];
+        // clamped row and are not used.  pr is the personalisation, or for
+        // the eigenvector y_{L-2}.
+        typename M::X xr[kBlkRowsPerThread];
         uint32_t dr[kBlkRowsPerThread];
         double pr[kBlkRowsPerThread];
 #pragma unroll
         for (int k = 0; k < kBlkRowsPerThread; ++k) {
             const uint32_t i = min(T.r0 + tid + k * kBlkThreads, b.n - 1);
-            xr[k] = b.x_cur[i];
+            if constexpr (M::kTwo)
+                xr[k] = b.x_cur64[i];
+            else
+                xr[k] = b.x_cur[i];
             dr[k] = b.odeg[i];
-            pr[k] = DPERS ? b.pers[i] : b.pval;
+            if constexpr (EV)
+                pr[k] = b.x_prev64[i];
+            else
+                pr[k] = DPERS ? b.pers[i] : b.pval;
         }
         // The tile's long rows inside its row range are finished by the
         // fold: their row accumulators (no entry adds into them) are marked.
@@ -193,14 +276,18 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
             if (r >= T.r0 && r < T.r1) acc[r - T.r0] = kBlkMarked;
         }
         for (uint32_t qi = w; qi < nq; qi += kBlkWaves * kBlkBatch) {
-            float v[kBlkBatch * kBlkQuad];
+            typename M::C v[kBlkBatch * kBlkQuad];
             uint32_t at[kBlkBatch * kBlkQuad];
 #pragma unroll
             for (int j = 0; j < kBlkBatch; ++j) {
 #pragma unroll
                 for (int k = 0; k < kBlkQuad; ++k) {
                     const uint32_t word = W[j][k];
-                    v[j * kBlkQuad + k] = b.c_cur[base[j * kBlkQuad + k] + (word >> kBlkCodeBits)];
+                    const uint32_t src = base[j * kBlkQuad + k] + (word >> kBlkCodeBits);
+                    if constexpr (M::kTwo)
+                        v[j * kBlkQuad + k] = b.c_cur64[src];
+                    else
+                        v[j * kBlkQuad + k] = b.c_cur[src];
                     at[j * kBlkQuad + k] = word & kBlkCodeMask;
                 }
             }
@@ -215,10 +302,20 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
             for (int j = 0; j < kBlkBatch; ++j)
                 if (qi + j * kBlkWaves < nq) {
 #pragma unroll
-                    for (int k = 0; k < kBlkQuad; ++k)
-                        __hip_atomic_fetch_add(&acc[at[j * kBlkQuad + k]],
-                                               blk_fixed(v[j * kBlkQuad + k], K), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    for (int k = 0; k < kBlkQuad; ++k) {
+                        const uint32_t a = at[j * kBlkQuad + k];
+                        if constexpr (M::kTwo) {
+                            unsigned long long hi, lo;
+                            blk_fixed2(v[j * kBlkQuad + k], K, hi, lo);
+                            __hip_atomic_fetch_add(&acc[a], hi, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&acc_lo[a], lo, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        } else {
+                            __hip_atomic_fetch_add(&acc[a], blk_fixed(v[j * kBlkQuad + k], K),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
                 }
         }
         __syncthreads();
@@ -226,17 +323,26 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         if (T.nlong) {
             // Long row j's 64 lane slots, 16 threads a row, 4 slots each.
             const uint32_t j = tid >> 4, part = tid & 15;
-            unsigned long long sl = 0;
+            unsigned long long sl = 0, sll = 0;
             if (j < T.nlong) {
 #pragma unroll
                 for (uint32_t l = 0; l < 4; ++l) {
                     const uint32_t q = kBlkAcc + j * kWave + part * 4 + l;
                     sl += acc[q];
                     acc[q] = 0ull;
+                    if (M::kTwo) {
+                        sll += acc_lo[q];
+                        acc_lo[q] = 0ull;
+                    }
                 }
             }
             sl = sum16(sl);
-            if (j < T.nlong && part == 0) atomicAdd(&b.slots[b.ltab[T.lfirst + j]], sl);
+            if (M::kTwo) sll = sum16(sll);
+            if (j < T.nlong && part == 0) {
+                const uint32_t slot = b.ltab[T.lfirst + j];
+                atomicAdd(&b.slots[slot], sl);
+                if (M::kTwo) atomicAdd(&b.slots_lo[slot], sll);
+            }
         }
         double dsum = 0.0, ssum = 0.0, msum = 0.0;
 #pragma unroll
@@ -246,9 +352,20 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
                 const uint32_t q = i - T.r0;
                 const unsigned long long a = acc[q];
                 acc[q] = 0ull;
-                if (a != kBlkMarked)
-                    blk_finish<REFINE>(b, i, ldexp(static_cast<double>(static_cast<long long>(a)), -K),
-                                       xr[k], dr[k], pr[k], p_sink, dsum, ssum, msum);
+                unsigned long long al = 0;
+                if (M::kTwo) {
+                    al = acc_lo[q];
+                    acc_lo[q] = 0ull;
+                }
+                if (a != kBlkMarked) {
+                    const double v = M::kTwo ? blk_value2(a, al, K) : blk_value(a, K);
+                    if constexpr (EV)
+                        blk_finish_ev(b, i, v, xr[k], pr[k], dr[k], sc_cur, sc_prev, dsum, ssum,
+                                      msum);
+                    else
+                        blk_finish<MODE>(b, i, v, static_cast<double>(xr[k]), dr[k], pr[k],
+                                         p_sink, dsum, ssum, msum);
+                }
             }
         }
         dsum = wave_sum(dsum);
@@ -296,21 +413,36 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 // cleared for the next sweep), then, in the last workgroup to arrive, the
 // tile partials in tile order and this kernel's in workgroup order, the
 // convergence decision and the next sweep's mass.
-template <bool DPERS, bool REFINE>
+template <bool DPERS, int MODE>
 __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
+    constexpr bool EV = MODE == kBlkEv;
     SweepState *st = b.st;
-    if (!REFINE && st->done) return;
+    if (MODE != kBlkRefine && st->done) return;
     __shared__ double red[12];
     __shared__ uint32_t flag;
     const int K = blk_scale(b.bs->mass);
-    const double p_sink = st->p_sink;
+    const double p_sink = EV ? 0.0 : st->p_sink;
+    const double sc_cur = EV ? b.ev->sc_cur : 1.0;
+    const double sc_prev = EV ? b.ev->sc_prev : 1.0;
     double d = 0.0, s = 0.0, ms = 0.0;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock) {
         const uint32_t r = b.long_rows[k];
-        const long long sv = static_cast<long long>(b.slots[k]);
+        const unsigned long long hi = b.slots[k];
         b.slots[k] = 0ull;
-        blk_finish<REFINE>(b, r, ldexp(static_cast<double>(sv), -K), b.x_cur[r], b.odeg[r],
-                           DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
+        if constexpr (MODE == kBlkF64 || EV) {
+            const unsigned long long lo = b.slots_lo[k];
+            b.slots_lo[k] = 0ull;
+            const double v = blk_value2(hi, lo, K);
+            if constexpr (EV)
+                blk_finish_ev(b, r, v, b.x_cur64[r], b.x_prev64[r], b.odeg[r], sc_cur, sc_prev, d,
+                              s, ms);
+            else
+                blk_finish<MODE>(b, r, v, b.x_cur64[r], b.odeg[r], DPERS ? b.pers[r] : b.pval,
+                                 p_sink, d, s, ms);
+        } else {
+            blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), b.odeg[r],
+                             DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
+        }
     }
     block_sum3(d, s, ms, red);
     double *fp = b.fold_part;
@@ -334,7 +466,13 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         ms += fp[2 * gridDim.x + k];
     }
     block_sum3(d, s, ms, red);
-    if (threadIdx.x == 0) {
+    if (EV && threadIdx.x == 0) {
+        // d: delta of the previous sweep; s: squared norm of this one.
+        ev_close(b.ev, s, d, b.eps_abs, b.max_iter);
+        b.bs->mass = ms;
+        b.bs->next_tile = 0;
+    }
+    if (!EV && threadIdx.x == 0) {
         const uint64_t it = st->iter + 1;
         const double thr = fmax(b.eps_abs, b.eps_rel * st->aux);
         st->delta = d;
@@ -348,9 +486,38 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void pr_start_blk(const BlkArgs b) {
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock)
+// The eigenvector's start: zero the long-row sums, then the mass of y_0 (sum
+// of |y_0| * out-degree), per workgroup and then in workgroup order by the
+// last to arrive.
+__global__ __launch_bounds__(kBlock) void ev_start_blk(const BlkArgs b) {
+    __shared__ double red[4];
+    __shared__ uint32_t flag;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock) {
         b.slots[k] = 0ull;
+        b.slots_lo[k] = 0ull;
+    }
+    double m = 0.0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < b.n; i += gridDim.x * kBlock)
+        m += fabs(b.x_cur64[i]) * static_cast<double>(b.odeg[i]);
+    m = block_sum(m, red);
+    if (threadIdx.x == 0) b.fold_part[blockIdx.x] = m;
+    if (!last_to_arrive(&b.st->counter, &flag)) return;
+    m = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) m += b.fold_part[k];
+    m = block_sum(m, red);
+    if (threadIdx.x == 0) {
+        b.bs->mass = m;
+        b.bs->next_tile = 0;
+        b.st->bail = 0;
+        b.st->counter = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void pr_start_blk(const BlkArgs b) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock) {
+        b.slots[k] = 0ull;
+        if (b.slots_lo) b.slots_lo[k] = 0ull;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.bs->mass = b.st->aux;
         b.bs->next_tile = 0;
@@ -500,9 +667,12 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     const uint64_t ngroups = g0[ntiles];
     if (ngroups >= (1ull << 32) / kWave)
         return fail(GT_ERR_UNSUPPORTED, "block tiles: too many entry groups");
-    // Padding groups (to whole blocks) hold null words over source 0.
-    std::vector<uint32_t> words(std::max<uint64_t>(1, ngroups * kWave), kBlkNull);
-    std::vector<uint32_t> gbase(std::max<uint64_t>(1, ngroups), 0);
+    // Padding groups (to whole blocks) hold null words over source 0.  A
+    // graph without entries still gets one block: a sweep's first loads
+    // read block 0 whether or not its tile has groups.
+    const uint64_t nstore = std::max<uint64_t>(kBlkQuad, ngroups);
+    std::vector<uint32_t> words(nstore * kWave, kBlkNull);
+    std::vector<uint32_t> gbase(nstore, 0);
     parallel_for(ntiles, [&](uint32_t t) {
         const HostTile &T = ht[t];
         uint64_t gidx = g0[t];
@@ -565,6 +735,8 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
                      hipMemcpyHostToDevice));
     B.ntiles = ntiles;
     B.nlong = nlong;
+    B.max_row = 0;
+    for (uint32_t r = 0; r < n; ++r) B.max_row = std::max(B.max_row, off[r + 1] - off[r]);
     B.ngroups = ngroups;
     B.tile_nnz = tile_nnz;
     B.long_min = long_min;
@@ -575,33 +747,85 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
 gt_status blk_reserve(gt_graph *g, const Blocks &B) {
     Workspace &ws = g->ws;
     GT_TRY(ws.blk_slots.reserve(sizeof(unsigned long long) * std::max<uint32_t>(1, B.nlong)));
+    GT_TRY(ws.blk_slots_lo.reserve(sizeof(unsigned long long) * std::max<uint32_t>(1, B.nlong)));
     GT_TRY(ws.blk_part.reserve(sizeof(double) * 3 * std::max<uint32_t>(1, B.ntiles)));
     GT_TRY(ws.blk_state.reserve(sizeof(BlkState)));
     GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     return GT_OK;
 }
 
+BlkArgs blk_layout_args(gt_graph *g, const Blocks &B) {
+    BlkArgs b{};
+    b.tiles = B.tiles.as<BlkTile>();
+    b.ntiles = B.ntiles;
+    b.nlong = B.nlong;
+    b.qlast = B.ngroups ? (uint32_t)(B.ngroups / kBlkQuad - 1) : 0u;
+    b.n = g->n;
+    b.words = B.words.as<uint32_t>();
+    b.gbase = B.gbase.as<uint32_t>();
+    b.ltab = B.ltab.as<uint32_t>();
+    b.long_rows = B.long_rows.as<uint32_t>();
+    b.odeg = B.odeg.as<uint32_t>();
+    b.slots = g->ws.blk_slots.as<unsigned long long>();
+    b.slots_lo = g->ws.blk_slots_lo.as<unsigned long long>();
+    b.part = g->ws.blk_part.as<double>();
+    b.fold_part = g->ws.partials.as<double>();
+    b.bs = g->ws.blk_state.as<BlkState>();
+    g->layout.block_tiles = B.ntiles;
+    g->layout.block_long_rows = B.nlong;
+    g->layout.block_groups = B.ngroups;
+    return b;
+}
+
 void blk_launch_start(const BlkArgs &b, hipStream_t s) {
     hipLaunchKernelGGL(pr_start_blk, dim3(fold_grid(b.nlong)), dim3(kBlock), 0, s, b);
 }
 
-void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, bool refine, hipStream_t s) {
-    const int grid = fit_grid(device_cus() * std::max(1, per_cu), b.ntiles);
-    switch ((dpers ? 2 : 0) | (refine ? 1 : 0)) {
-        case 0: hipLaunchKernelGGL((pr_sweep_blk<false, false>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
-        case 1: hipLaunchKernelGGL((pr_sweep_blk<false, true>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
-        case 2: hipLaunchKernelGGL((pr_sweep_blk<true, false>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
-        default: hipLaunchKernelGGL((pr_sweep_blk<true, true>), dim3(grid), dim3(kBlkThreads), 0, s, b); break;
+void blk_launch_start_ev(const BlkArgs &b, hipStream_t s) {
+    const int grid = fit_grid(1024, std::max((b.n + kBlock - 1) / kBlock, b.nlong / kBlock + 1));
+    hipLaunchKernelGGL(ev_start_blk, dim3(grid), dim3(kBlock), 0, s, b);
+}
+
+template <int MODE>
+void launch_mode(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
+    if (dpers)
+        hipLaunchKernelGGL((pr_sweep_blk<true, MODE>), dim3(grid), dim3(kBlkThreads), 0, s, b);
+    else
+        hipLaunchKernelGGL((pr_sweep_blk<false, MODE>), dim3(grid), dim3(kBlkThreads), 0, s, b);
+}
+
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStream_t s) {
+    // The two-word sweeps' accumulators take 96 KB: one workgroup per CU.
+    const int per = mode >= kBlkF64 ? 1 : std::max(1, per_cu);
+    const int grid = fit_grid(device_cus() * per, b.ntiles);
+    switch (mode) {
+        case kBlkF32: launch_mode<kBlkF32>(b, dpers, grid, s); break;
+        case kBlkRefine: launch_mode<kBlkRefine>(b, dpers, grid, s); break;
+        case kBlkF64: launch_mode<kBlkF64>(b, dpers, grid, s); break;
+        default:
+            hipLaunchKernelGGL((pr_sweep_blk<false, kBlkEv>), dim3(grid), dim3(kBlkThreads), 0, s,
+                               b);
+            break;
     }
 }
 
-void blk_launch_fold(const BlkArgs &b, bool dpers, bool refine, hipStream_t s) {
+template <int MODE>
+void fold_mode(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
+    if (dpers)
+        hipLaunchKernelGGL((pr_fold_blk<true, MODE>), dim3(grid), dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL((pr_fold_blk<false, MODE>), dim3(grid), dim3(kBlock), 0, s, b);
+}
+
+void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
     const int grid = fold_grid(b.nlong);
-    switch ((dpers ? 2 : 0) | (refine ? 1 : 0)) {
-        case 0: hipLaunchKernelGGL((pr_fold_blk<false, false>), dim3(grid), dim3(kBlock), 0, s, b); break;
-        case 1: hipLaunchKernelGGL((pr_fold_blk<false, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
-        case 2: hipLaunchKernelGGL((pr_fold_blk<true, false>), dim3(grid), dim3(kBlock), 0, s, b); break;
-        default: hipLaunchKernelGGL((pr_fold_blk<true, true>), dim3(grid), dim3(kBlock), 0, s, b); break;
+    switch (mode) {
+        case kBlkF32: fold_mode<kBlkF32>(b, dpers, grid, s); break;
+        case kBlkRefine: fold_mode<kBlkRefine>(b, dpers, grid, s); break;
+        case kBlkF64: fold_mode<kBlkF64>(b, dpers, grid, s); break;
+        default:
+            hipLaunchKernelGGL((pr_fold_blk<false, kBlkEv>), dim3(grid), dim3(kBlock), 0, s, b);
+            break;
     }
 }
 

@@ -18,9 +18,21 @@
 // wave engine.
 #pragma once
 
+#include "ev_state.hpp"
 #include "gt_internal.hpp"
 
 namespace gt {
+
+// The sweeps on block tiles: the FP32 form's FP32 sweep, its FP64
+// refinement (FP32 contributions gathered, FP64 output), the FP64 PageRank
+// sweep (FP64 contributions, each term held in two 64-bit words: round(c
+// 2^K) and the rest at 2^(K + 38), so the sums keep about 100 bits of the
+// iterate's mass), and the eigenvector sweep (the FP64 iterate gathered, in
+// two words).
+enum : int { kBlkF32 = 0, kBlkRefine = 1, kBlkF64 = 2, kBlkEv = 3 };
+// The two-word sweeps' low words stay below 2^62 for rows of at most this
+// many entries; a graph with a longer row runs them on the wave engine.
+constexpr uint32_t kBlkF64MaxRow = 1u << 23;
 
 // Device state of a block-sweep loop.
 struct BlkState {
@@ -46,10 +58,13 @@ struct BlkArgs {
     const uint32_t *long_rows;
     const uint32_t *odeg;       // out-degree, sweep labels
     unsigned long long *slots;  // [nlong] int64 sums of the long rows
+    unsigned long long *slots_lo;  // [nlong] their low words (FP64 sweep), else null
     double *part;               // [3 * ntiles]: delta, dangling, mass per tile
     double *fold_part;          // [3 * fold grid]
     BlkState *bs;
-    SweepState *st;
+    SweepState *st;             // PageRank state; the eigenvector's EvState (same offsets
+                                // of done, bail and counter) for the eigenvector sweep
+    EvState *ev;                // eigenvector sweep: its state, else null
     const double *pers;         // dense personalisation, sweep labels (DPERS)
     double pval;
     double damping;
@@ -59,7 +74,11 @@ struct BlkArgs {
     float *x_next;
     const float *c_cur;
     float *c_next;
-    double *x_next64;           // the refinement sweep's output
+    double *x_next64;           // the refinement's and the FP64 sweeps' output
+    const double *x_cur64;      // FP64 sweeps: iterate and contributions (for the
+    const double *c_cur64;      // eigenvector both y_{L-1})
+    double *c_next64;
+    const double *x_prev64;     // eigenvector: y_{L-2} (y_{L-1} on the first launch)
 };
 
 // Build (once per tile size and long-row threshold) the block tiles of R's
@@ -68,14 +87,20 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
                         Blocks **out);
 // Sizes the per-call buffers of a block loop over B into the workspace.
 gt_status blk_reserve(gt_graph *g, const Blocks &B);
+// The layout and workspace fields of the sweep arguments over B (after
+// blk_reserve); the caller adds the loop's state, vectors and parameters.
+// Also reports B in g->layout.
+BlkArgs blk_layout_args(gt_graph *g, const Blocks &B);
 // After the init kernel: zero the long-row sums, take the initial mass,
-// clear the bail flag.
+// clear the bail flag.  The eigenvector's mass (sum of |y_0| * out-degree)
+// is a reduction over the iterate, in a fixed order.
 void blk_launch_start(const BlkArgs &b, hipStream_t s);
-// One sweep (workgroups: `per_cu` per compute unit, at most one per tile).
-// With `refine`, the FP32 form's last sweep: FP64 output to x_next64, run
-// whatever the loop state says.
-void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, bool refine, hipStream_t s);
+void blk_launch_start_ev(const BlkArgs &b, hipStream_t s);
+// One sweep of `mode` (workgroups: `per_cu` per compute unit for the FP32
+// and refinement sweeps, one for the two-word sweeps; at most one per tile).
+// The refinement runs whatever the loop state says.
+void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStream_t s);
 // Completes a sweep: long rows, partials in tile order, convergence.
-void blk_launch_fold(const BlkArgs &b, bool dpers, bool refine, hipStream_t s);
+void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s);
 
 }  // namespace gt

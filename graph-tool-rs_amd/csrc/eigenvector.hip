@@ -19,28 +19,12 @@
 #include "device_common.hpp"
 #include "gt_internal.hpp"
 #include "props.hpp"
+#include "blocks.hpp"
 #include "driver.hpp"
+#include "ev_state.hpp"
 #include "sweep_common.hpp"
 
 namespace gt {
-
-// Eigenvector loop state, laid over SweepState's storage.
-// delta, iter and done sit at SweepState's offsets so the shared driver can
-// read them.
-struct EvState {
-    double sc_cur;      // scale of y_{L-1}
-    double delta;       // delta of the last completed check
-    double sc_prev;     // scale of y_{L-2}
-    double norm_last;   // norm_{L-1}
-    double eig;         // norm of the final iterate
-    uint64_t iter;      // reference sweep count reached
-    uint32_t done;
-    uint32_t counter;
-    uint32_t launches;  // launches that did work (= L)
-    uint32_t pad;
-    double final_scale; // scale of the final iterate
-    uint64_t final_iter;
-};
 
 struct EvArgs {
     const Tile *tiles;
@@ -170,38 +154,7 @@ __global__ __launch_bounds__(kBlock) void ev_fold(const EvArgs a, uint32_t sgrid
         d += fp[gridDim.x + k];
     }
     block_sum2(s2, d, red);
-    if (tid == 0) {
-        EvState *st = a.st;
-        const uint32_t L = st->launches + 1;
-        const double norm = sqrt(s2);
-        bool done = false;
-        if (L >= 2) {
-            st->delta = d;
-            if (!(d >= a.epsilon)) {
-                // Sweep L-1 converged: its iterate is the answer.
-                done = true;
-                st->final_iter = L - 1;
-                st->final_scale = st->sc_cur;
-                st->eig = st->norm_last;
-                st->iter = L - 1;
-            }
-        }
-        if (!done) {
-            st->iter = L;
-            if (a.max_iter > 0 && L >= a.max_iter) {
-                done = true;
-                st->final_iter = L;
-                st->final_scale = norm == 0.0 ? 1.0 : norm;
-                st->eig = norm;
-            }
-        }
-        st->sc_prev = st->sc_cur;
-        st->sc_cur = norm == 0.0 ? 1.0 : norm;
-        st->norm_last = norm;
-        st->launches = L;
-        st->done = done ? 1u : 0u;
-        st->counter = 0;
-    }
+    if (tid == 0) ev_close(a.st, s2, d, a.epsilon, a.max_iter);
 }
 
 // y0[k] = the caller's slot iperm[k] (identity when null), with slots
@@ -225,6 +178,7 @@ __global__ __launch_bounds__(kBlock) void ev_init(const double *x, const uint32_
         st->launches = 0;
         st->final_scale = 1.0;
         st->final_iter = 0;
+        st->bail = 0;
     }
 }
 
@@ -288,8 +242,8 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     const double eps = params->epsilon;
     const bool skip = std::isnan(eps) || !(eps + 1.0 >= eps);
     const uint32_t *perm = nullptr, *iperm = nullptr;
+    Relabeled *R = nullptr;
     if (!masked && n > 0) {
-        Relabeled *R = nullptr;
         GT_TRY(ensure_relabeled(g, reversed ? 1 : 0, &R));
         pull = &R->pull;
         perm = R->perm.as<uint32_t>();
@@ -345,7 +299,58 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
 
     uint64_t T = 0;
     double *out = y[0];
-    if (!skip) {
+    bool on_blocks = false;
+    if (!skip && R && !weighted && g->tune.block_tile_nnz > 0) {
+        // Block tiles (blocks.hpp): the unweighted sweep of a relabelled
+        // graph, its row sums in two-word fixed point.  The loop runs on the
+        // wave engine instead when a row is too long for the low words, or
+        // when a launch meets an iterate whose terms the fixed point cannot
+        // hold (its mass not finite or above kBlkMassMax).
+        Blocks *B = nullptr;
+        GT_TRY(ensure_blocks(g, *R, (uint32_t)g->tune.block_tile_nnz,
+                             (uint32_t)g->tune.block_long_min, &B));
+        if (B->max_row <= kBlkF64MaxRow) {
+            GT_TRY(blk_reserve(g, *B));
+            BlkArgs b = blk_layout_args(g, *B);
+            b.st = reinterpret_cast<SweepState *>(st);
+            b.ev = st;
+            b.eps_abs = eps;
+            b.max_iter = params->max_iter;
+            b.x_cur64 = y[0];
+            blk_launch_start_ev(b, s);
+            GT_HIP(hipGetLastError());
+            auto sweep = [&](uint64_t k) {
+                // Launch L = k + 1 reads y_{L-1}, y_{L-2}, writes y_L.
+                const uint64_t L = k + 1;
+                b.x_cur64 = y[(L - 1) % 3];
+                b.c_cur64 = b.x_cur64;
+                b.x_prev64 = k == 0 ? b.x_cur64 : y[(L + 1) % 3];
+                b.x_next64 = y[L % 3];
+                blk_launch_sweep(b, false, 1, kBlkEv, s);
+            };
+            auto fold = [&](uint64_t) { blk_launch_fold(b, false, kBlkEv, s); };
+            uint64_t launched = 0;
+            g->prof_kind = GT_SWEEP_EIGENVECTOR;
+            GT_TRY(drive(g, reinterpret_cast<SweepState *>(st), eps, params->max_iter, sweep, fold,
+                         &launched, GT_SWEEP_EIGENVECTOR));
+            EvState h;
+            GT_HIP(hipMemcpy(&h, st, sizeof(EvState), hipMemcpyDeviceToHost));
+            if (!h.bail) {
+                on_blocks = true;
+                T = h.final_iter;
+                *eig = h.eig;
+                out = y[T % 3];
+            } else {
+                g->layout.block_tiles = 0;
+                g->layout.block_long_rows = 0;
+                g->layout.block_groups = 0;
+                hipLaunchKernelGGL(ev_init, dim3(vgrid(n)), dim3(kBlock), 0, s, xio, iperm,
+                                   x->had, n, uniform, y[0], st);
+                GT_HIP(hipGetLastError());
+            }
+        }
+    }
+    if (!skip && !on_blocks) {
         EvArgs a{};
         a.tiles = pull->tiles.as<Tile>();
         a.ntiles = pull->ntiles;
@@ -390,11 +395,7 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
             else ev_launch_fold<false, false>(b, sg, s);
         };
         // The driver reads done / delta / iter through the SweepState layout;
-        // EvState keeps delta, iter and done at the same offsets.
-        static_assert(offsetof(EvState, delta) == offsetof(SweepState, delta), "layout");
-        static_assert(offsetof(EvState, iter) == offsetof(SweepState, iter), "layout");
-        static_assert(offsetof(EvState, done) == offsetof(SweepState, done), "layout");
-        static_assert(sizeof(EvState) >= sizeof(SweepState), "layout");
+        // EvState keeps them at the same offsets.
         uint64_t launched = 0;
         g->prof_kind = GT_SWEEP_EIGENVECTOR;
         // One launch beyond the answer is needed to learn its delta.

@@ -243,6 +243,7 @@ struct Blocks {
     DevBuf long_rows;    // uint32[nlong]
     DevBuf odeg;         // uint32[n]: out-degree in sweep labels
     uint32_t ntiles = 0, nlong = 0;
+    uint32_t max_row = 0;  // entries of the longest row
     uint64_t ngroups = 0;
 };
 
@@ -291,6 +292,7 @@ struct Workspace {
     DevBuf blong;       // batched: split-row [64] partials
     DevBuf bstate;      // batched: loop state + seeds
     DevBuf blk_slots;   // block sweeps: int64 sums of the long rows
+    DevBuf blk_slots_lo;  // block sweeps: their low words (FP64)
     DevBuf blk_part;    // block sweeps: per-tile delta, dangling and mass partials
     DevBuf blk_state;   // block sweeps: BlkState
     DevBuf masks;       // staged vertex + edge masks

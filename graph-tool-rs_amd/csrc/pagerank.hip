@@ -805,7 +805,7 @@ gt_status sliced_for(PrSolve &S, uint32_t nhot, const Sliced **out) {
     return GT_OK;
 }
 
-// Whether the FP32 sweeps of S run on block tiles (blocks.hpp): relabelled,
+// Whether the sweeps of S can run on block tiles (blocks.hpp): relabelled,
 // unfiltered and unweighted, unless the tuning turns them off.
 bool blocks_apply(const PrSolve &S) {
     return S.rel && S.iperm && !S.plan.masked && !S.plan.weighted && !S.noalloc &&
@@ -814,22 +814,7 @@ bool blocks_apply(const PrSolve &S) {
 
 // The block-sweep arguments of a solve over B.
 BlkArgs blk_args(const PrSolve &S, const PrArgs &a, const Blocks &B) {
-    gt_graph *g = S.g;
-    BlkArgs b{};
-    b.tiles = B.tiles.as<BlkTile>();
-    b.ntiles = B.ntiles;
-    b.nlong = B.nlong;
-    b.qlast = B.ngroups ? (uint32_t)(B.ngroups / kBlkQuad - 1) : 0u;
-    b.n = g->n;
-    b.words = B.words.as<uint32_t>();
-    b.gbase = B.gbase.as<uint32_t>();
-    b.ltab = B.ltab.as<uint32_t>();
-    b.long_rows = B.long_rows.as<uint32_t>();
-    b.odeg = B.odeg.as<uint32_t>();
-    b.slots = g->ws.blk_slots.as<unsigned long long>();
-    b.part = g->ws.blk_part.as<double>();
-    b.fold_part = g->ws.partials.as<double>();
-    b.bs = g->ws.blk_state.as<BlkState>();
+    BlkArgs b = blk_layout_args(S.g, B);
     b.st = a.st;
     b.pers = a.pers;
     b.pval = a.pval;
@@ -840,39 +825,51 @@ BlkArgs blk_args(const PrSolve &S, const PrArgs &a, const Blocks &B) {
     return b;
 }
 
-// Init + FP32 sweeps on block tiles, on buffers x / c; returns the sweep
-// count reached and leaves the latest iterate in x[count & 1], as pr_phase.
-// Sets *bailed when a sweep met an iterate whose mass its fixed-point sums
-// cannot hold (not finite, or above kBlkMassMax): nothing is then delivered
-// and the caller runs the phase again on the wave engine.
-gt_status pr_phase_blocks(PrSolve &S, PrArgs a, float *x[2], float *c[2], uint64_t had,
+// Init + sweeps on block tiles in value type V (FP32: the FP32 form's
+// phase; FP64: the drop-in), on buffers x / c; returns the sweep count
+// reached and leaves the latest iterate in x[count & 1], as pr_phase.  Sets
+// *bailed when the phase must run on the wave engine instead: a sweep met an
+// iterate whose mass its fixed-point sums cannot hold (not finite, or above
+// kBlkMassMax), or (FP64) a row is longer than kBlkF64MaxRow.
+template <typename V>
+gt_status pr_phase_blocks(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had,
                           double eps_pred, uint64_t cap, uint64_t *iter_out, bool *bailed) {
+    constexpr bool F64 = std::is_same<V, double>::value;
+    constexpr int mode = F64 ? kBlkF64 : kBlkF32;
     gt_graph *g = S.g;
     hipStream_t s = g->stream;
     *bailed = false;
     Blocks *B = nullptr;
     GT_TRY(ensure_blocks(g, *S.rel, (uint32_t)g->tune.block_tile_nnz,
                          (uint32_t)g->tune.block_long_min, &B));
+    if (F64 && B->max_row > kBlkF64MaxRow) {
+        *bailed = true;
+        return GT_OK;
+    }
     GT_TRY(blk_reserve(g, *B));
-    g->layout.block_tiles = B->ntiles;
-    g->layout.block_long_rows = B->nlong;
-    g->layout.block_groups = B->ngroups;
-    g->prof_kind = GT_SWEEP_PAGERANK_F32;
+    g->prof_kind = F64 ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
     a.x_next = x[0];
     a.c_next = c[0];
-    dispatch_init<float>(S.plan, a, S.rank64, had, g->n, 0, s);
+    dispatch_init<V>(S.plan, a, S.rank64, had, g->n, 0, s);
     BlkArgs b = blk_args(S, a, *B);
     blk_launch_start(b, s);
     GT_HIP(hipGetLastError());
     auto sweep = [&](uint64_t k) {
         const int par = (int)(k & 1);
-        b.x_cur = x[par];
-        b.x_next = x[par ^ 1];
-        b.c_cur = c[par];
-        b.c_next = c[par ^ 1];
-        blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, false, s);
+        if constexpr (F64) {
+            b.x_cur64 = x[par];
+            b.x_next64 = x[par ^ 1];
+            b.c_cur64 = c[par];
+            b.c_next64 = c[par ^ 1];
+        } else {
+            b.x_cur = x[par];
+            b.x_next = x[par ^ 1];
+            b.c_cur = c[par];
+            b.c_next = c[par ^ 1];
+        }
+        blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, mode, s);
     };
-    auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, false, s); };
+    auto fold = [&](uint64_t) { blk_launch_fold(b, S.plan.dpers, mode, s); };
     uint64_t launched = 0;
     GT_TRY(drive(g, a.st, eps_pred, cap, sweep, fold, &launched, g->prof_kind));
     *bailed = g->h_state->bail != 0;
@@ -887,15 +884,18 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
                    double eps_pred, uint64_t cap, uint64_t *iter_out) {
     gt_graph *g = S.g;
     hipStream_t s = g->stream;
-    if constexpr (std::is_same<V, float>::value) {
-        if (iter0 == 0 && cap != 1 && blocks_apply(S)) {
-            bool bailed = false;
-            GT_TRY(pr_phase_blocks(S, a, x, c, had, eps_pred, cap, iter_out, &bailed));
-            if (!bailed) return GT_OK;
-            g->layout.block_tiles = 0;
-            g->layout.block_long_rows = 0;
-            g->layout.block_groups = 0;
-        }
+    // The FP64 drop-in's sweeps run on block tiles only when tuned so: the
+    // wave engine sums a row in the order the weighted sweep does, so unit
+    // weights give the unweighted result bit for bit, as centrality.rs's
+    // unity fast path does.
+    const bool blk = std::is_same<V, float>::value || g->tune.block_f64 > 0;
+    if (iter0 == 0 && cap != 1 && blk && blocks_apply(S)) {
+        bool bailed = false;
+        GT_TRY(pr_phase_blocks<V>(S, a, x, c, had, eps_pred, cap, iter_out, &bailed));
+        if (!bailed) return GT_OK;
+        g->layout.block_tiles = 0;
+        g->layout.block_long_rows = 0;
+        g->layout.block_groups = 0;
     }
     g->prof_kind = std::is_same<V, double>::value ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
     // The hot slice serves sweeps in relabelled order (the most gathered
@@ -1102,9 +1102,9 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
             b.x_cur = x32[it32 & 1];
             b.c_cur = c32[it32 & 1];
             b.x_next64 = x64;
-            blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, true, s);
+            blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, kBlkRefine, s);
             GT_TRY(prof_end(g, 0));
-            blk_launch_fold(b, S.plan.dpers, true, s);
+            blk_launch_fold(b, S.plan.dpers, kBlkRefine, s);
         } else {
             const uint32_t nhot = S.iperm && !S.plan.masked && !S.plan.weighted
                                       ? hot_len<float>(g->tune.sweep_hot_kb, n)

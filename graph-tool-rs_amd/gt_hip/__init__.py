@@ -305,7 +305,8 @@ class DeviceGraph:
 
     TUNING_FIELDS = ("sweep_blocks_per_cu", "tile_rows", "short_row", "mm_tile_nnz",
                      "mm_blocks_per_cu", "sweep_hot_kb", "sliced_min_len", "sliced_share",
-                     "block_tile_nnz", "block_long_min", "block_per_cu")
+                     "block_tile_nnz", "block_long_min", "block_per_cu",
+                     "block_f64")
 
     def tuning(self) -> dict:
         """Launch / schedule parameters in effect (gt_graph_get_tuning)."""

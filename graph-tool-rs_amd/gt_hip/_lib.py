@@ -56,7 +56,7 @@ class GtTuning(C.Structure):
                 ("mm_blocks_per_cu", C.c_int32), ("sweep_hot_kb", C.c_int32),
                 ("sliced_min_len", C.c_int32), ("sliced_share", C.c_int32),
                 ("block_tile_nnz", C.c_int32), ("block_long_min", C.c_int32),
-                ("block_per_cu", C.c_int32)]
+                ("block_per_cu", C.c_int32), ("block_f64", C.c_int32)]
 
 
 class GtView(C.Structure):

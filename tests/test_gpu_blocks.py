@@ -1,9 +1,12 @@
-"""The block-tile FP32 sweep against the oracle.
+"""The block-tile sweeps against the oracle.
 
-This is the default FP32 engine of pagerank_mixed on relabelled, unfiltered,
-unweighted graphs. Each result is checked against the oracle at the FP32
-form's bar (L1 <= 1e-6 of the FP64 fixed point), against the wave engine (the
-same solve with block_tile_nnz = -1), and bit for bit run to run.
+The FP32 sweep is the default FP32 engine of pagerank_mixed on relabelled,
+unfiltered, unweighted graphs. Each result is checked against the oracle at
+the FP32 form's bar (L1 <= 1e-6 of the FP64 fixed point), against the wave
+engine (the same solve with block_tile_nnz = -1), and bit for bit run to run.
+The two-word FP64 sweeps -- the FP64 PageRank drop-in with block_f64 on, and
+the eigenvector (on by default) -- are held to the drop-in's bar (L1 <= 1e-12
+and the same sweep count +-1) and to the wave engine's result to rounding.
 
 The cases cover the layout's edge cases:
 - long rows summed in lane slots;
@@ -186,3 +189,205 @@ def test_unrepresentable_mass_runs_on_the_wave_engine(gpu):
     (x, it), lay, (xw, itw) = run_both(g, pers=pp)
     assert lay["block_tiles"] == 0
     assert it == itw and np.array_equal(x, xw, equal_nan=True)
+
+
+# ------------------------------------------------ FP64 PageRank (block_f64)
+
+def parity(got, it, want, it_o, eps, tol=1e-12):
+    """The drop-in's bar: same sweep count and L1 <= tol, or one sweep apart
+    when the oracle's last delta sat within rounding of epsilon (then the
+    answers are within epsilon)."""
+    l1 = float(np.abs(np.asarray(got) - np.asarray(want)).sum())
+    if it == it_o:
+        assert l1 <= tol, f"L1 {l1:.3e} at {it} sweeps"
+    else:
+        assert abs(it - it_o) == 1, f"sweeps {it} vs {it_o}"
+        assert l1 <= eps, f"L1 {l1:.3e}, sweeps {it} vs {it_o}"
+
+
+def pr64(g, eps=1e-12, max_iter=1000, pers=None, rank=None, reversed_=False):
+    r = gt_hip.DenseProp.new(g.graph_id) if rank is None else rank
+    target = g.reversed() if reversed_ else g
+    p = params(g.vertex_bound(), eps, max_iter)
+    if pers is None:
+        it = gt_hip.pagerank(target, gt_hip.Unity.NEW, 0.85, p, r)
+    else:
+        it = gt_hip.pagerank_with(target, gt_hip.Unity.NEW, pers, 0.85, p, r)
+    return np.asarray(r.data).copy(), it
+
+
+def check64(n, s, t, pers=None, tile=0, long_min=0, reversed_=False):
+    """The FP64 drop-in with block_f64 on, against the oracle at the drop-in's
+    bar and against the wave engine (the default) to rounding."""
+    g = adj(n, s, t)
+    dev = g.device()
+    pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
+    dev.set_tuning(block_f64=1, block_tile_nnz=tile, block_long_min=long_min)
+    x, it = pr64(g, pers=pp, reversed_=reversed_)
+    lay = dev.layout()
+    dev.set_tuning(block_f64=0, block_tile_nnz=0, block_long_min=0)
+    pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
+    xw, itw = pr64(g, pers=pp, reversed_=reversed_)
+    assert lay["block_tiles"] > 0, lay
+    want, it_o = oracle.Adj(n, s, t).pagerank(pers=pers, epsilon=1e-12, max_iter=1000,
+                                              reversed=reversed_, threads=8)
+    parity(x, it, want, it_o, 1e-12)
+    assert abs(it - itw) <= 1
+    if it == itw:
+        assert np.abs(x - xw).sum() <= 1e-12
+    return g, lay, x, it
+
+
+def test_f64_rmat_and_small_tiles(gpu):
+    """R-MAT 16 on default tiles; R-MAT 13 on 1,024-entry tiles with a long-row
+    threshold of 16 (long rows in lane slots, hubs cut into chunk tiles)."""
+    n = 1 << 16
+    s, t = oracle.gen_rmat(16, 16 * n, 0.57, 0.19, 0.19, 51)
+    _, lay, _, _ = check64(n, s, t)
+    assert lay["block_long_rows"] > 0
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 52)
+    check64(n, s, t, tile=1024, long_min=16)
+
+
+def test_f64_reversed_and_negative_personalisation(gpu):
+    n = 1 << 14
+    s, t = oracle.gen_rmat(14, 16 * n, 0.57, 0.19, 0.19, 53)
+    p = np.random.default_rng(9).uniform(-0.5, 1.5, n)
+    p /= p.sum()
+    check64(n, s, t, pers=p, reversed_=True)
+    check64(n, s, t, pers=p, tile=2048, long_min=64)
+
+
+def test_f64_empty_rows_warm_start_and_truncation(gpu):
+    """Mostly empty rows; a partial warm start; caps of 1, 2 and 3 sweeps
+    against the oracle's."""
+    r = Rng(54)
+    n, m = 100_003, 20_000
+    s = np.array([r.below(n) for _ in range(m)], np.uint32)
+    t = np.array([r.below(n) for _ in range(m)], np.uint32)
+    check64(n, s, t)
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 55)
+    g = adj(n, s, t)
+    g.device().set_tuning(block_f64=1)
+    warm = np.random.default_rng(10).uniform(0.0, 2.0 / n, n // 3)
+    x, it = pr64(g, rank=gt_hip.DenseProp.from_vec(g.graph_id, warm.copy()))
+    assert g.device().layout()["block_tiles"] > 0
+    want, it_o = oracle.Adj(n, s, t).pagerank(epsilon=1e-12, max_iter=1000, rank=warm.copy(),
+                                              threads=8)
+    parity(x, it, want, it_o, 1e-12)
+    for k in (1, 2, 3):
+        x, it = pr64(g, eps=1e-30, max_iter=k)
+        want, it_o = oracle.Adj(n, s, t).pagerank(epsilon=1e-30, max_iter=k, threads=8)
+        assert it == it_o == k
+        assert np.abs(x - want).sum() <= 1e-14
+    g.device().set_tuning(block_f64=0)
+
+
+def test_f64_bit_for_bit_and_unrepresentable_mass(gpu):
+    """Integer row sums: the same bits run to run.  A personalisation of
+    1e300 puts the iterate's mass above what the fixed point holds: the solve
+    is the wave engine's, bit for bit."""
+    n = 1 << 15
+    s, t = oracle.gen_rmat(15, 16 * n, 0.57, 0.19, 0.19, 56)
+    g = adj(n, s, t)
+    g.device().set_tuning(block_f64=1)
+    a, ia = pr64(g)
+    b, ib = pr64(g)
+    assert g.device().layout()["block_tiles"] > 0
+    assert ia == ib and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    p = np.full(n, 1e300)
+    x, it = pr64(g, eps=1e-6, pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
+    assert g.device().layout()["block_tiles"] == 0
+    g.device().set_tuning(block_f64=0)
+    xw, itw = pr64(g, eps=1e-6, pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
+    assert it == itw and np.array_equal(x, xw, equal_nan=True)
+
+
+# ---------------------------------------------------------- eigenvector
+
+def ev(g, eps, max_iter, x=None, reversed_=False):
+    xm = gt_hip.DenseProp.new(g.graph_id) if x is None else x
+    target = g.reversed() if reversed_ else g
+    eig, it = gt_hip.eigenvector(target, gt_hip.Unity.NEW, params(g.vertex_bound(), eps, max_iter),
+                                 xm)
+    return np.asarray(xm.data).copy(), eig, it
+
+
+def sym_rmat(scale, seed):
+    s, t = oracle.gen_rmat(scale, 16 << scale, 0.57, 0.19, 0.19, seed)
+    return 1 << scale, np.concatenate([s, t]), np.concatenate([t, s])
+
+
+def check_ev(n, s, t, eps, max_iter, tile=0, long_min=0, x0=None, reversed_=False):
+    """The block eigenvector (the default) against the oracle at the drop-in's
+    bar, and against the wave engine to rounding."""
+    g = adj(n, s, t)
+    dev = g.device()
+    dev.set_tuning(block_tile_nnz=tile, block_long_min=long_min)
+    xm = None if x0 is None else gt_hip.DenseProp.from_vec(g.graph_id, x0.copy())
+    c, eig, it = ev(g, eps, max_iter, x=xm, reversed_=reversed_)
+    lay = dev.layout()
+    dev.set_tuning(block_tile_nnz=-1)
+    xm = None if x0 is None else gt_hip.DenseProp.from_vec(g.graph_id, x0.copy())
+    cw, eigw, itw = ev(g, eps, max_iter, x=xm, reversed_=reversed_)
+    assert dev.layout()["block_tiles"] == 0
+    dev.set_tuning(block_tile_nnz=0, block_long_min=0)
+    assert lay["block_tiles"] > 0, lay
+    co, eo, ito = oracle.Adj(n, s, t).eigenvector(epsilon=eps, max_iter=max_iter,
+                                                  x=None if x0 is None else x0.copy(),
+                                                  reversed=reversed_, threads=8)
+    parity(c, it, co, ito, eps)
+    assert abs(eig - eo) <= max(1e-12, eps) * eo
+    assert abs(it - itw) <= 1
+    if it == itw:
+        assert np.abs(c - cw).sum() <= 1e-12 and abs(eig - eigw) <= 1e-13 * eigw
+    return g, lay, c, eig, it
+
+
+def test_ev_symmetrised_rmat_and_truncation(gpu):
+    """Symmetrised R-MAT 13 at caps of 1, 2 and 3 sweeps and to epsilon 1e-9;
+    R-MAT 16 (directed, hub rows in lane slots) to 1e-10."""
+    n, S, T = sym_rmat(13, 57)
+    for k in (1, 2, 3):
+        check_ev(n, S, T, 1e-30, k)
+    check_ev(n, S, T, 1e-9, 1000)
+    n = 1 << 16
+    s, t = oracle.gen_rmat(16, 16 * n, 0.57, 0.19, 0.19, 58)
+    S, T = np.concatenate([s, t]), np.concatenate([t, s])
+    _, lay, _, _, _ = check_ev(n, S, T, 1e-10, 1000)
+    assert lay["block_long_rows"] > 0
+
+
+def test_ev_small_tiles_reversed_and_warm_start(gpu):
+    n, S, T = sym_rmat(13, 59)
+    check_ev(n, S, T, 1e-10, 1000, tile=1024, long_min=16)
+    s, t = oracle.gen_rmat(12, 16 << 12, 0.45, 0.15, 0.15, 60)
+    n = 1 << 12
+    # A strongly connected core keeps the directed iteration convergent.
+    S = np.concatenate([s, np.arange(n, dtype=np.uint32)])
+    T = np.concatenate([t, (np.arange(n, dtype=np.uint32) + 1) % n]).astype(np.uint32)
+    check_ev(n, S, T, 1e-10, 2000, reversed_=True)
+    warm = np.random.default_rng(11).uniform(0.5, 1.5, n // 2)
+    check_ev(n, S, T, 1e-10, 2000, x0=warm)
+
+
+def test_ev_bit_for_bit_and_unrepresentable_start(gpu):
+    """Integer row sums: the same bits run to run.  A start of 1e300 per
+    vertex puts the mass (sum of |x| * out-degree) above what the fixed point
+    holds: the solve is the wave engine's, bit for bit."""
+    n, S, T = sym_rmat(14, 61)
+    g = adj(n, S, T)
+    a, ea, ia = ev(g, 1e-10, 1000)
+    assert g.device().layout()["block_tiles"] > 0
+    b, eb, ib = ev(g, 1e-10, 1000)
+    assert ia == ib and ea == eb and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    big = np.full(n, 1e300)
+    x, e, it = ev(g, 1e-10, 1000, x=gt_hip.DenseProp.from_vec(g.graph_id, big.copy()))
+    assert g.device().layout()["block_tiles"] == 0
+    g.device().set_tuning(block_tile_nnz=-1)
+    xw, ew, itw = ev(g, 1e-10, 1000, x=gt_hip.DenseProp.from_vec(g.graph_id, big.copy()))
+    g.device().set_tuning(block_tile_nnz=0)
+    assert it == itw and np.array_equal(x, xw, equal_nan=True)
+    assert e == ew or (np.isnan(e) and np.isnan(ew))
