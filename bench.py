@@ -873,6 +873,18 @@ def main() -> int:
     log(f"[bench] fp64: {dt64/args.steps*1e3:.3f} ms/solve, sweeps {sw64[0]}, "
         f"launches {la64} ({ms64/max(la64,1)*1e3:.1f} us avg)")
 
+    # The FP64 drop-in with its sweeps on block tiles (tuning block_f64:
+    # off by default, since unit weights then no longer give the unweighted
+    # result bit for bit).
+    dev.set_tuning(block_f64=1)
+    dt64b, sw64b, ms64b, la64b, _, st64b = timed("gt_pagerank_f64", _lib.GT_SWEEP_PAGERANK_F64)
+    x64b = rank.cpu().numpy().copy()
+    lay64b = dev.layout()
+    dev.set_tuning(block_f64=0)
+    log(f"[bench] fp64 on block tiles: {dt64b/args.steps*1e3:.3f} ms/solve, sweeps {sw64b[0]}, "
+        f"launches {la64b} ({ms64b/max(la64b,1)*1e3:.1f} us avg), "
+        f"block tiles {lay64b['block_tiles']}")
+
     # The same solve handing over a host rank map (copy in / out over PCIe
     # inside the call): reported beside `value`, never as it.
     rank_h = np.zeros(n, np.float64)
@@ -909,7 +921,7 @@ def main() -> int:
     # (the timed solves' layout, read before the FP64 solves).
     blk = {k: lay32[k] for k in ("block_tiles", "block_groups", "block_long_rows")}
     on_blocks = blk["block_tiles"] > 0
-    k32 = ("pr_sweep_blk<false, false>" if on_blocks
+    k32 = ("pr_sweep_blk<false, 0>" if on_blocks
            else "pr_sweep_hot<float, false, false>" if hot else "pr_sweep<float>")
     traffic32, traffic_src = profiled_traffic(workload, k32)
     result = {
@@ -950,7 +962,16 @@ def main() -> int:
             "roofline_frac": round(ach64 / HBM_PEAK_GBS, 4), "bytes_per_sweep": b64,
             "avg_launch_us": round(avg64 * 1e6, 2),
         },
-        "timing": {"mixed": st32, "fp64_dropin": st64,
+        "fp64_dropin_block_tiles": {
+            "tuning": {"block_f64": 1}, "on_block_tiles": lay64b["block_tiles"] > 0,
+            "gteps": round(m * sw64b[0] * args.steps / dt64b / 1e9, 3),
+            "ms_per_solve": round(dt64b / args.steps * 1e3, 4), "sweeps": sw64b[0],
+            "avg_launch_us": round(ms64b / max(la64b, 1) * 1e3, 2),
+            "roofline_frac": round(b64 * sw64b[0] * args.steps / max(la64b, 1)
+                                   / (ms64b / max(la64b, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            if la64b else None,
+        },
+        "timing": {"mixed": st32, "fp64_dropin": st64, "fp64_dropin_block_tiles": st64b,
                    "stat": f"median of {args.steps} solves after {args.warmup} warm-ups, sweep "
                            f"events off; kernel averages from the same solves run again with "
                            f"events"},
@@ -1041,6 +1062,9 @@ def main() -> int:
         torch.cuda.synchronize()
         mse, lae, _ = devs.sweep_stats(_lib.GT_SWEEP_EIGENVECTOR)
         devs.set_profiling(False)
+        lay_e = devs.layout()
+        ev_blocks = lay_e["block_tiles"] > 0
+        ke = "pr_sweep_blk<false, 3>" if ev_blocks else "ev_sweep_hot"
         be = ev_bytes(n, ms_)
         avge = mse / max(lae, 1) * 1e-3
         ache = be * lae / max(lae, 1) / avge / 1e9 if lae else 0.0
@@ -1050,13 +1074,18 @@ def main() -> int:
             "launches_per_solve": lae // max(args.steps, 1),
             "eigenvalue": eig.value, "achieved_gbs": round(ache, 1),
             "roofline_frac": round(ache / HBM_PEAK_GBS, 4), "bytes_per_sweep": be,
-            "traffic": profiled_traffic(workload, "ev_sweep_hot")[0],
+            "kernel": ("pr_sweep_blk<false, 3> (eigenvector sweep on block tiles, two-word "
+                       "fixed-point row sums)" if ev_blocks
+                       else "ev_sweep_hot (eigenvector sweep, hot iterate in LDS)"),
+            "layout": {k: lay_e[k] for k in ("block_tiles", "block_groups", "block_long_rows")},
+            "traffic": profiled_traffic(workload, ke)[0],
         }
         x_eig = xs.cpu().numpy().copy()
-        nh_e = hot_len(kb, 8, n)
-        deg_s = np.bincount(S, minlength=n)
-        cold_e = int(deg_s.sum() - np.sort(deg_s)[::-1][:nh_e].sum())
-        floors("eigenvector_sweep", cold_e, nh_e, avge, "ev_sweep_hot<false>")
+        if not ev_blocks:
+            nh_e = hot_len(kb, 8, n)
+            deg_s = np.bincount(S, minlength=n)
+            cold_e = int(deg_s.sum() - np.sort(deg_s)[::-1][:nh_e].sum())
+            floors("eigenvector_sweep", cold_e, nh_e, avge, "ev_sweep_hot<false>")
         log(f"[bench] eigenvector: {dte/args.steps*1e3:.3f} ms/solve, sweeps {swe[0]}, "
             f"lambda {eig.value:.6f}")
         devs.close()
@@ -1107,7 +1136,7 @@ def main() -> int:
                            "sample": f"one solve capped at {it1} sweeps, {dt1:.1f}s"},
         }
         result["accuracy_vs_fp64_oracle"] = {
-            "mixed": errs(x32), "fp64_dropin": errs(x64),
+            "mixed": errs(x32), "fp64_dropin": errs(x64), "fp64_dropin_block_tiles": errs(x64b),
             "oracle_sweeps": it_o, "fp64_sweeps": sw64[0],
         }
         if not args.no_eigen:
