@@ -873,17 +873,14 @@ def main() -> int:
     log(f"[bench] fp64: {dt64/args.steps*1e3:.3f} ms/solve, sweeps {sw64[0]}, "
         f"launches {la64} ({ms64/max(la64,1)*1e3:.1f} us avg)")
 
-    # The FP64 drop-in with its sweeps on block tiles (tuning block_f64:
-    # off by default, since unit weights then no longer give the unweighted
-    # result bit for bit).
-    dev.set_tuning(block_f64=1)
+    lay64 = dev.layout()
+    # The same drop-in on the wave engine (tuning block_f64 = -1), beside it.
+    dev.set_tuning(block_f64=-1)
     dt64b, sw64b, ms64b, la64b, _, st64b = timed("gt_pagerank_f64", _lib.GT_SWEEP_PAGERANK_F64)
     x64b = rank.cpu().numpy().copy()
-    lay64b = dev.layout()
     dev.set_tuning(block_f64=0)
-    log(f"[bench] fp64 on block tiles: {dt64b/args.steps*1e3:.3f} ms/solve, sweeps {sw64b[0]}, "
-        f"launches {la64b} ({ms64b/max(la64b,1)*1e3:.1f} us avg), "
-        f"block tiles {lay64b['block_tiles']}")
+    log(f"[bench] fp64 on the wave engine: {dt64b/args.steps*1e3:.3f} ms/solve, "
+        f"sweeps {sw64b[0]}, launches {la64b} ({ms64b/max(la64b,1)*1e3:.1f} us avg)")
 
     # The same solve handing over a host rank map (copy in / out over PCIe
     # inside the call): reported beside `value`, never as it.
@@ -961,9 +958,14 @@ def main() -> int:
             "sweeps": sw64[0], "achieved_gbs": round(ach64, 1),
             "roofline_frac": round(ach64 / HBM_PEAK_GBS, 4), "bytes_per_sweep": b64,
             "avg_launch_us": round(avg64 * 1e6, 2),
+            "kernel": ("pr_sweep_blk<false, 2> (block tiles, two-word fixed-point sums)"
+                       if lay64["block_tiles"] else "pr_sweep_hot<double> (wave engine)"),
+            "traffic": profiled_traffic(workload, "pr_sweep_blk<false, 2>"
+                                        if lay64["block_tiles"]
+                                        else "pr_sweep_hot<double, false, false>")[0],
         },
-        "fp64_dropin_block_tiles": {
-            "tuning": {"block_f64": 1}, "on_block_tiles": lay64b["block_tiles"] > 0,
+        "fp64_dropin_wave_engine": {
+            "tuning": {"block_f64": -1},
             "gteps": round(m * sw64b[0] * args.steps / dt64b / 1e9, 3),
             "ms_per_solve": round(dt64b / args.steps * 1e3, 4), "sweeps": sw64b[0],
             "avg_launch_us": round(ms64b / max(la64b, 1) * 1e3, 2),
@@ -971,7 +973,7 @@ def main() -> int:
                                    / (ms64b / max(la64b, 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
             if la64b else None,
         },
-        "timing": {"mixed": st32, "fp64_dropin": st64, "fp64_dropin_block_tiles": st64b,
+        "timing": {"mixed": st32, "fp64_dropin": st64, "fp64_dropin_wave_engine": st64b,
                    "stat": f"median of {args.steps} solves after {args.warmup} warm-ups, sweep "
                            f"events off; kernel averages from the same solves run again with "
                            f"events"},
@@ -1018,7 +1020,8 @@ def main() -> int:
     # The floors model the wave engine's sweeps (hot slice in LDS, one
     # request per gathered lane outside it): the FP64 and eigenvector sweeps,
     # and the FP32 sweep when it ran there.
-    sweeps = [("fp64_sweep", 8, avg64, "pr_sweep_hot<double, false, false>")]
+    avg64w = ms64b / max(la64b, 1) * 1e-3
+    sweeps = [("fp64_sweep", 8, avg64w, "pr_sweep_hot<double, false, false>")]
     if not on_blocks:
         sweeps.insert(0, ("fp32_sweep", 4, avg32, "pr_sweep_hot<float, false, false>"))
     for key, vbytes, avg, pk in sweeps:
@@ -1028,7 +1031,7 @@ def main() -> int:
     log(f"[bench] gather ceiling: {ns_l2:.3f} ns per L2-hit lane per CU; FP64 sweep floor "
         f"{ceiling['fp64_sweep']['floor_us']:.0f} us "
         f"({ceiling['fp64_sweep'].get('floor_with_misses_us', float('nan')):.0f} us with its "
-        f"L2 misses) vs {avg64*1e6:.0f} us")
+        f"L2 misses) vs {avg64w*1e6:.0f} us (wave engine)")
 
     # ---- eigenvector on the symmetrised graph
     if not args.no_eigen:
@@ -1136,7 +1139,7 @@ def main() -> int:
                            "sample": f"one solve capped at {it1} sweeps, {dt1:.1f}s"},
         }
         result["accuracy_vs_fp64_oracle"] = {
-            "mixed": errs(x32), "fp64_dropin": errs(x64), "fp64_dropin_block_tiles": errs(x64b),
+            "mixed": errs(x32), "fp64_dropin": errs(x64), "fp64_dropin_wave_engine": errs(x64b),
             "oracle_sweeps": it_o, "fp64_sweeps": sw64[0],
         }
         if not args.no_eigen:

@@ -15,14 +15,30 @@
 
 namespace gt {
 
-constexpr int kBlkWaves = 8;
-constexpr int kBlkThreads = kBlkWaves * kWave;
 // Blocks of kBlkQuad groups per wavefront batch: 4 word loads of 16 bytes a
 // lane, then 16 gathers in flight.
 constexpr int kBlkBatch = 4;
+// Wavefronts per sweep workgroup: 8 for the one-word sweeps (two or three
+// workgroups share a CU), 16 for the two-word sweeps, whose 96 KB of
+// accumulators leave room for one workgroup per CU.
+constexpr int kBlkWaves = 8;
+constexpr int kBlkThreads = kBlkWaves * kWave;
+template <int MODE, bool WGT = false> struct BlkShape {
+    static constexpr int kWaves = MODE >= kBlkF64 ? 16 : kBlkWaves;
+    static constexpr int kThreads = kWaves * kWave;
+    // Blocks per wavefront batch.  A weighted two-word sweep holds each
+    // lane's FP64 weights beside its terms: half the batch keeps it within
+    // the 128 VGPRs of 16 wavefronts.  (Its shape must be the unweighted
+    // sweep's: the rows each thread finishes, and so the order of the
+    // partial sums, follow the thread count, and unit weights must give the
+    // unweighted result bit for bit.)
+    static constexpr int kBatch = MODE >= kBlkF64 && WGT ? 2 : kBlkBatch;
+    static constexpr int kRows = (kBlkRows + kThreads - 1) / kThreads;  // finisher rows a thread
+};
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t kBlkLds = kBlkAcc + kBlkLongMax * kWave;
-constexpr int kBlkRowsPerThread = (kBlkRows + kBlkThreads - 1) / kBlkThreads;
 // The row accumulator of a long row in the tile's range: no sum reaches
 // 2^62 in magnitude, so no sum equals it.
 constexpr unsigned long long kBlkMarked = 1ull << 63;
@@ -56,10 +72,15 @@ __device__ __forceinline__ unsigned long long blk_i64(double h) {
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-// round(c * 2^K) (ties to even) as a 64-bit integer.  Exact in FP64: c * 2^K
-// has at most 24 significant bits and |c| * 2^K < 2^62 (blk_scale).
+// round(t * 2^K) (ties to even) as a 64-bit integer, |t| * 2^K < 2^62
+// (blk_scale).  An FP32 term -- or a product of two FP32 values, 48
+// significant bits -- is exact in FP64, so only this rounding to the grid
+// touches it.
+__device__ __forceinline__ unsigned long long blk_fixed_d(double t, int K) {
+    return blk_i64(rint(ldexp(t, K)));
+}
 __device__ __forceinline__ unsigned long long blk_fixed(float c, int K) {
-    return blk_i64(rint(ldexp(static_cast<double>(c), K)));
+    return blk_fixed_d(static_cast<double>(c), K);
 }
 
 // An FP64 term in two words: hi = round(c * 2^K) and lo = round(r *
@@ -86,13 +107,20 @@ __device__ __forceinline__ double blk_value2(unsigned long long hi, unsigned lon
 }
 
 template <int MODE> struct BlkTypes {
-    typedef float X;  // the iterate a row update reads
-    typedef float C;  // the gathered contribution
+    typedef float X;   // the iterate a row update reads
+    typedef float C;   // the gathered contribution
+    typedef float Wt;  // an edge weight (weighted sweeps)
     static constexpr bool kTwo = false;
+};
+// The refinement gathers the FP32 contributions and weighs them with the
+// map's FP64 weights, as the wave engine's refinement does.
+template <> struct BlkTypes<kBlkRefine> : BlkTypes<kBlkF32> {
+    typedef double Wt;
 };
 template <> struct BlkTypes<kBlkF64> {
     typedef double X;
     typedef double C;
+    typedef double Wt;
     static constexpr bool kTwo = true;
 };
 template <> struct BlkTypes<kBlkEv> : BlkTypes<kBlkF64> {};
@@ -118,7 +146,7 @@ __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
 // output; delta against the FP32 iterate; no contribution, no dangling mass).
 template <int MODE>
 __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc, double xc,
-                                           uint32_t dg, double pv, double p_sink, double &dsum,
+                                           double dg, double pv, double p_sink, double &dsum,
                                            double &ssum, double &msum) {
     const double nv = (1.0 - b.damping) * pv + b.damping * (p_sink * pv + acc);
     if (MODE == kBlkRefine) {
@@ -130,8 +158,8 @@ __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double 
         dsum += fabs(nv - xc);
         msum += fabs(nv);
         b.x_next64[i] = nv;
-        if (dg != 0) {
-            b.c_next64[i] = nv / static_cast<double>(dg);
+        if (dg != 0.0) {
+            b.c_next64[i] = nv / dg;
         } else {
             b.c_next64[i] = 0.0;
             ssum += nv;
@@ -142,8 +170,8 @@ __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double 
     dsum += fabs(static_cast<double>(nvv) - xc);
     msum += fabs(static_cast<double>(nvv));
     b.x_next[i] = nvv;
-    if (dg != 0) {
-        b.c_next[i] = static_cast<float>(static_cast<double>(nvv) / static_cast<double>(dg));
+    if (dg != 0.0) {
+        b.c_next[i] = static_cast<float>(static_cast<double>(nvv) / dg);
     } else {
         b.c_next[i] = 0.0f;
         ssum += static_cast<double>(nvv);
@@ -190,10 +218,16 @@ __device__ __forceinline__ unsigned long long sum16(unsigned long long v) {
 // accumulator read once and cleared for the next tile.  The tile's delta,
 // dangling and mass partials go to its own entries of b.part, so the fold
 // sums them in tile order whichever workgroup took the tile.
-template <bool DPERS, int MODE>
-__global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
+template <bool DPERS, int MODE, bool WGT>
+__global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(const BlkArgs b) {
     typedef BlkTypes<MODE> M;
+    typedef typename M::Wt Wt;
+    constexpr int NW = BlkShape<MODE, WGT>::kWaves;
+    constexpr int NT = BlkShape<MODE, WGT>::kThreads;
+    constexpr int NR = BlkShape<MODE, WGT>::kRows;
+    constexpr int NB = BlkShape<MODE, WGT>::kBatch;
     constexpr bool EV = MODE == kBlkEv;
+    static_assert(!(EV && WGT), "the eigenvector's block sweep is unweighted");
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
     // done), so it does not look at the flag.
@@ -201,7 +235,7 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
     __shared__ unsigned long long acc[kBlkLds];
     // The FP64 sweep's low words (a dummy otherwise).
     __shared__ unsigned long long acc_lo[M::kTwo ? kBlkLds : 1];
-    __shared__ double red[3 * kBlkWaves];
+    __shared__ double red[3 * NW];
     __shared__ uint32_t s_tile[2];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (kWave - 1);
@@ -218,7 +252,7 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
     const double p_sink = EV ? 0.0 : st->p_sink;
     const double sc_cur = EV ? b.ev->sc_cur : 1.0;
     const double sc_prev = EV ? b.ev->sc_prev : 1.0;
-    for (uint32_t i = tid; i < kBlkLds; i += kBlkThreads) {
+    for (uint32_t i = tid; i < kBlkLds; i += NT) {
         acc[i] = 0ull;
         if (M::kTwo) acc_lo[i] = 0ull;
     }
@@ -230,7 +264,7 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         const BlkTile T = blk_tile(b.tiles, t);
         // Groups come in blocks of kBlkQuad: lane l's 16-byte load of block q
         // holds its word of each of the block's groups.  Wavefront w takes
-        // blocks w, w + 8, ..., kBlkBatch blocks per batch; the next batch's
+        // blocks w, w + 8, ..., NB blocks per batch; the next batch's
         // words and bases are loaded while this batch's gathers are in
         // flight (a load past the tile's blocks reads a clamped block and is
         // not added).  The words are read once per sweep: non-temporal loads
@@ -239,11 +273,11 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         const uint32_t nq = (T.g1 - T.g0) / kBlkQuad;
         ConstU32 *gbase = (ConstU32 *)b.gbase;
         const u32x4 *wq = reinterpret_cast<const u32x4 *>(b.words);
-        u32x4 W[kBlkBatch];
-        uint32_t base[kBlkBatch * kBlkQuad];
+        u32x4 W[NB];
+        uint32_t base[NB * kBlkQuad];
 #pragma unroll
-        for (int j = 0; j < kBlkBatch; ++j) {
-            const uint32_t q = min(q0 + w + j * kBlkWaves, b.qlast);
+        for (int j = 0; j < NB; ++j) {
+            const uint32_t q = min(q0 + w + j * NW, b.qlast);
             W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
 #pragma unroll
             for (int k = 0; k < kBlkQuad; ++k) base[j * kBlkQuad + k] = gbase[q * kBlkQuad + k];
@@ -253,17 +287,20 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         // and the finisher waits on no load.  Loads past the tile read a
         // clamped row and are not used.  pr is the personalisation, or for
         // the eigenvector y_{L-2}.
-        typename M::X xr[kBlkRowsPerThread];
-        uint32_t dr[kBlkRowsPerThread];
-        double pr[kBlkRowsPerThread];
+        typename M::X xr[NR];
+        typename std::conditional<WGT, double, uint32_t>::type dr[NR];
+        double pr[NR];
 #pragma unroll
-        for (int k = 0; k < kBlkRowsPerThread; ++k) {
-            const uint32_t i = min(T.r0 + tid + k * kBlkThreads, b.n - 1);
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t i = min(T.r0 + tid + k * NT, b.n - 1);
             if constexpr (M::kTwo)
                 xr[k] = b.x_cur64[i];
             else
                 xr[k] = b.x_cur[i];
-            dr[k] = b.odeg[i];
+            if constexpr (WGT)
+                dr[k] = b.wdeg[i];
+            else
+                dr[k] = b.odeg[i];
             if constexpr (EV)
                 pr[k] = b.x_prev64[i];
             else
@@ -275,11 +312,38 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
             const uint32_t r = b.long_rows[b.ltab[T.lfirst + tid]];
             if (r >= T.r0 && r < T.r1) acc[r - T.r0] = kBlkMarked;
         }
-        for (uint32_t qi = w; qi < nq; qi += kBlkWaves * kBlkBatch) {
-            typename M::C v[kBlkBatch * kBlkQuad];
-            uint32_t at[kBlkBatch * kBlkQuad];
+        for (uint32_t qi = w; qi < nq; qi += NW * NB) {
+            typename M::C v[NB * kBlkQuad];
+            uint32_t at[NB * kBlkQuad];
+            // A weighted sweep's weights of this batch, stored lane-major as
+            // the words are: lane l's kBlkQuad weights of block q side by
+            // side.  Read once per sweep, like the words.
+            Wt wt[WGT ? NB * kBlkQuad : 1];
+            if constexpr (WGT) {
+                const Wt *wb = static_cast<const Wt *>(b.wblk);
 #pragma unroll
-            for (int j = 0; j < kBlkBatch; ++j) {
+                for (int j = 0; j < NB; ++j) {
+                    const uint32_t q = min(q0 + qi + j * NW, b.qlast);
+                    const Wt *at4 = wb + ((size_t)q * kWave + lane) * kBlkQuad;
+                    if constexpr (sizeof(Wt) == 4) {
+                        const f32x4 w4 = __builtin_nontemporal_load(
+                            reinterpret_cast<const f32x4 *>(at4));
+#pragma unroll
+                        for (int k = 0; k < kBlkQuad; ++k) wt[j * kBlkQuad + k] = w4[k];
+                    } else {
+                        const f64x2 wa = __builtin_nontemporal_load(
+                            reinterpret_cast<const f64x2 *>(at4));
+                        const f64x2 wc = __builtin_nontemporal_load(
+                            reinterpret_cast<const f64x2 *>(at4) + 1);
+                        wt[j * kBlkQuad + 0] = wa[0];
+                        wt[j * kBlkQuad + 1] = wa[1];
+                        wt[j * kBlkQuad + 2] = wc[0];
+                        wt[j * kBlkQuad + 3] = wc[1];
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
 #pragma unroll
                 for (int k = 0; k < kBlkQuad; ++k) {
                     const uint32_t word = W[j][k];
@@ -292,27 +356,31 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kBlkBatch; ++j) {
-                const uint32_t q = min(q0 + qi + (kBlkBatch + j) * kBlkWaves, b.qlast);
+            for (int j = 0; j < NB; ++j) {
+                const uint32_t q = min(q0 + qi + (NB + j) * NW, b.qlast);
                 W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
 #pragma unroll
                 for (int k = 0; k < kBlkQuad; ++k) base[j * kBlkQuad + k] = gbase[q * kBlkQuad + k];
             }
 #pragma unroll
-            for (int j = 0; j < kBlkBatch; ++j)
-                if (qi + j * kBlkWaves < nq) {
+            for (int j = 0; j < NB; ++j)
+                if (qi + j * NW < nq) {
 #pragma unroll
                     for (int k = 0; k < kBlkQuad; ++k) {
                         const uint32_t a = at[j * kBlkQuad + k];
                         if constexpr (M::kTwo) {
                             unsigned long long hi, lo;
-                            blk_fixed2(v[j * kBlkQuad + k], K, hi, lo);
+                            double term = v[j * kBlkQuad + k];
+                            if constexpr (WGT) term *= wt[j * kBlkQuad + k];
+                            blk_fixed2(term, K, hi, lo);
                             __hip_atomic_fetch_add(&acc[a], hi, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                             __hip_atomic_fetch_add(&acc_lo[a], lo, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         } else {
-                            __hip_atomic_fetch_add(&acc[a], blk_fixed(v[j * kBlkQuad + k], K),
+                            double term = static_cast<double>(v[j * kBlkQuad + k]);
+                            if constexpr (WGT) term *= static_cast<double>(wt[j * kBlkQuad + k]);
+                            __hip_atomic_fetch_add(&acc[a], blk_fixed_d(term, K),
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
@@ -346,8 +414,8 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         }
         double dsum = 0.0, ssum = 0.0, msum = 0.0;
 #pragma unroll
-        for (int k = 0; k < kBlkRowsPerThread; ++k) {
-            const uint32_t i = T.r0 + tid + k * kBlkThreads;
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t i = T.r0 + tid + k * NT;
             if (i < T.r1) {
                 const uint32_t q = i - T.r0;
                 const unsigned long long a = acc[q];
@@ -363,8 +431,9 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
                         blk_finish_ev(b, i, v, xr[k], pr[k], dr[k], sc_cur, sc_prev, dsum, ssum,
                                       msum);
                     else
-                        blk_finish<MODE>(b, i, v, static_cast<double>(xr[k]), dr[k], pr[k],
-                                         p_sink, dsum, ssum, msum);
+                        blk_finish<MODE>(b, i, v, static_cast<double>(xr[k]),
+                                         static_cast<double>(dr[k]), pr[k], p_sink, dsum, ssum,
+                                         msum);
                 }
             }
         }
@@ -373,16 +442,16 @@ __global__ __launch_bounds__(kBlkThreads) void pr_sweep_blk(const BlkArgs b) {
         msum = wave_sum(msum);
         if (lane == 0) {
             red[w] = dsum;
-            red[kBlkWaves + w] = ssum;
-            red[2 * kBlkWaves + w] = msum;
+            red[NW + w] = ssum;
+            red[2 * NW + w] = msum;
         }
         __syncthreads();
         if (tid == 0) {
             double d = 0.0, s = 0.0, ms = 0.0;
-            for (int k = 0; k < kBlkWaves; ++k) {
+            for (int k = 0; k < NW; ++k) {
                 d += red[k];
-                s += red[kBlkWaves + k];
-                ms += red[2 * kBlkWaves + k];
+                s += red[NW + k];
+                ms += red[2 * NW + k];
             }
             b.part[t] = d;
             b.part[b.ntiles + t] = s;
@@ -413,7 +482,7 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 // cleared for the next sweep), then, in the last workgroup to arrive, the
 // tile partials in tile order and this kernel's in workgroup order, the
 // convergence decision and the next sweep's mass.
-template <bool DPERS, int MODE>
+template <bool DPERS, int MODE, bool WGT>
 __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     constexpr bool EV = MODE == kBlkEv;
     SweepState *st = b.st;
@@ -429,6 +498,7 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         const uint32_t r = b.long_rows[k];
         const unsigned long long hi = b.slots[k];
         b.slots[k] = 0ull;
+        const double dg = WGT ? b.wdeg[r] : static_cast<double>(b.odeg[r]);
         if constexpr (MODE == kBlkF64 || EV) {
             const unsigned long long lo = b.slots_lo[k];
             b.slots_lo[k] = 0ull;
@@ -437,10 +507,10 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
                 blk_finish_ev(b, r, v, b.x_cur64[r], b.x_prev64[r], b.odeg[r], sc_cur, sc_prev, d,
                               s, ms);
             else
-                blk_finish<MODE>(b, r, v, b.x_cur64[r], b.odeg[r], DPERS ? b.pers[r] : b.pval,
-                                 p_sink, d, s, ms);
+                blk_finish<MODE>(b, r, v, b.x_cur64[r], dg, DPERS ? b.pers[r] : b.pval, p_sink, d,
+                                 s, ms);
         } else {
-            blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), b.odeg[r],
+            blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg,
                              DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
         }
     }
@@ -587,10 +657,13 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     const uint32_t n = g->n;
     const std::vector<uint32_t> &off = R.pull.h_off;
     const uint64_t m = off[n];
-    std::vector<uint32_t> idx(m);
-    if (m)
+    std::vector<uint32_t> idx(m), eid(m);
+    if (m) {
         GT_HIP(hipMemcpy(idx.data(), R.pull.idx.ptr, sizeof(uint32_t) * m,
                          hipMemcpyDeviceToHost));
+        GT_HIP(hipMemcpy(eid.data(), R.pull.eid.ptr, sizeof(uint32_t) * m,
+                         hipMemcpyDeviceToHost));
+    }
 
     // Tiles, in row order.
     std::vector<HostTile> ht;
@@ -673,15 +746,44 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     const uint64_t nstore = std::max<uint64_t>(kBlkQuad, ngroups);
     std::vector<uint32_t> words(nstore * kWave, kBlkNull);
     std::vector<uint32_t> gbase(nstore, 0);
+    // The edge id of each word, for weights staged in block order: an
+    // entry's position in its row (rows are sorted by source) is the first
+    // of its source there, plus the entries of the same source and row
+    // before it in the tile's order.  `seen` checks that every entry of the
+    // orientation is taken exactly once.
+    std::vector<uint32_t> weid(nstore * kWave, 0);
+    std::vector<uint8_t> seen(m, 0);
+    std::atomic<bool> bad{false};
     parallel_for(ntiles, [&](uint32_t t) {
         const HostTile &T = ht[t];
         uint64_t gidx = g0[t];
         uint32_t cnt = 0;
         uint64_t base = 0;
         bool open = false;
+        uint32_t dup = 0;
         for (uint64_t e = T.e0; e < T.e1; ++e) {
             const uint64_t src = key[e] >> kBlkCodeBits;
             const uint32_t c = (uint32_t)(key[e] & kBlkCodeMask);
+            dup = (e > T.e0 && key[e] == key[e - 1]) ? dup + 1 : 0;
+            uint64_t lo, hi;
+            if (T.chunk) {
+                lo = T.e0;
+                hi = T.e1;
+            } else {
+                const uint32_t row = c < kBlkKeyLong0
+                                         ? T.r0 + c
+                                         : long_rows[ltab[T.lfirst + (c - kBlkKeyLong0)]];
+                lo = off[row];
+                hi = off[row + 1];
+            }
+            const uint64_t pos =
+                (uint64_t)(std::lower_bound(idx.begin() + lo, idx.begin() + hi, (uint32_t)src) -
+                           idx.begin()) + dup;
+            if (pos >= hi || idx[pos] != src || seen[pos]) {
+                bad = true;
+                break;
+            }
+            seen[pos] = 1;
             if (!open || cnt == kWave || src - base >= kBlkSpan) {
                 if (open) ++gidx;
                 open = true;
@@ -695,11 +797,14 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
                 j < kBlkLongMax ? kBlkAcc + j * kWave + ((cnt + j) & (kWave - 1)) : c;
             // Block gidx / 4 holds, for each lane, its word of the block's
             // four groups side by side.
-            words[(gidx / kBlkQuad) * (kWave * kBlkQuad) + cnt * kBlkQuad + gidx % kBlkQuad] =
-                ((uint32_t)(src - base) << kBlkCodeBits) | at;
+            const uint64_t slot =
+                (gidx / kBlkQuad) * (kWave * kBlkQuad) + cnt * kBlkQuad + gidx % kBlkQuad;
+            words[slot] = ((uint32_t)(src - base) << kBlkCodeBits) | at;
+            weid[slot] = eid[pos];
             ++cnt;
         }
     });
+    if (bad) return fail(GT_ERR_UNSUPPORTED, "block tiles: an entry's edge id was not found");
     std::vector<BlkTile> tiles(std::max<uint32_t>(1, ntiles));
     for (uint32_t t = 0; t < ntiles; ++t) {
         const HostTile &T = ht[t];
@@ -712,6 +817,7 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
 
     GT_TRY(B.tiles.reserve(sizeof(BlkTile) * tiles.size()));
     GT_TRY(B.words.reserve(sizeof(uint32_t) * words.size()));
+    GT_TRY(B.eids.reserve(sizeof(uint32_t) * weid.size()));
     GT_TRY(B.gbase.reserve(sizeof(uint32_t) * gbase.size()));
     GT_TRY(B.ltab.reserve(sizeof(uint32_t) * ltab.size()));
     GT_TRY(B.long_rows.reserve(sizeof(uint32_t) * long_rows.size()));
@@ -727,6 +833,8 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
                      hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(B.words.ptr, words.data(), sizeof(uint32_t) * words.size(),
                      hipMemcpyHostToDevice));
+    GT_HIP(hipMemcpy(B.eids.ptr, weid.data(), sizeof(uint32_t) * weid.size(),
+                     hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(B.gbase.ptr, gbase.data(), sizeof(uint32_t) * gbase.size(),
                      hipMemcpyHostToDevice));
     GT_HIP(hipMemcpy(B.ltab.ptr, ltab.data(), sizeof(uint32_t) * ltab.size(),
@@ -741,6 +849,64 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     B.tile_nnz = tile_nnz;
     B.long_min = long_min;
     B.ready = true;
+    return GT_OK;
+}
+
+// A weighted sweep's weights in block order: out[k] = w(eids[k]) converted
+// to V, for every word slot.  A real entry (not padding) whose weight is
+// negative or not finite sets *bad: the block sweeps bound their sums by the
+// iterate's mass, which holds for non-negative weights only.
+template <typename S, typename V>
+__global__ __launch_bounds__(kBlock) void blk_stage_w(const uint32_t *__restrict__ words,
+                                                      const uint32_t *__restrict__ eids,
+                                                      const void *w, int kind, double value,
+                                                      uint64_t count, V *__restrict__ out,
+                                                      uint32_t *bad) {
+    bool neg = false;
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
+         k += (uint64_t)gridDim.x * kBlock) {
+        const double x = kind == GT_MAP_CONSTANT
+                             ? value
+                             : static_cast<double>(static_cast<const S *>(w)[eids[k]]);
+        out[k] = static_cast<V>(x);
+        if ((words[k] & kBlkCodeMask) != kBlkNull && !(x >= 0.0 && x <= 1.7976931348623157e308))
+            neg = true;
+    }
+    if (__any(neg) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(bad, 1u);
+}
+
+gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, const void *src,
+                            int dtype, bool f64, DevBuf &out, bool *usable) {
+    const uint64_t count = std::max<uint64_t>(kBlkQuad, B.ngroups) * kWave;
+    GT_TRY(out.reserve((f64 ? sizeof(double) : sizeof(float)) * count));
+    GT_TRY(g->ws.blk_state.reserve(sizeof(BlkState)));
+    BlkState *bs = g->ws.blk_state.as<BlkState>();
+    hipStream_t s = g->stream;
+    GT_HIP(hipMemsetAsync(&bs->pad, 0, sizeof(uint32_t), s));
+    const unsigned grid = (unsigned)std::min<uint64_t>(8192, (count + kBlock - 1) / kBlock);
+    const uint32_t *words = B.words.as<uint32_t>(), *eids = B.eids.as<uint32_t>();
+    const int kind = w->kind;
+    const double value = w->value;
+    if (dtype == GT_F32) {
+        if (f64)
+            hipLaunchKernelGGL((blk_stage_w<float, double>), dim3(grid), dim3(kBlock), 0, s, words,
+                               eids, src, kind, value, count, out.as<double>(), &bs->pad);
+        else
+            hipLaunchKernelGGL((blk_stage_w<float, float>), dim3(grid), dim3(kBlock), 0, s, words,
+                               eids, src, kind, value, count, out.as<float>(), &bs->pad);
+    } else {
+        if (f64)
+            hipLaunchKernelGGL((blk_stage_w<double, double>), dim3(grid), dim3(kBlock), 0, s,
+                               words, eids, src, kind, value, count, out.as<double>(), &bs->pad);
+        else
+            hipLaunchKernelGGL((blk_stage_w<double, float>), dim3(grid), dim3(kBlock), 0, s,
+                               words, eids, src, kind, value, count, out.as<float>(), &bs->pad);
+    }
+    GT_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    GT_HIP(hipMemcpyAsync(&bad, &bs->pad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    *usable = bad == 0;
     return GT_OK;
 }
 
@@ -786,12 +952,20 @@ void blk_launch_start_ev(const BlkArgs &b, hipStream_t s) {
     hipLaunchKernelGGL(ev_start_blk, dim3(grid), dim3(kBlock), 0, s, b);
 }
 
+template <int MODE, bool WGT>
+void launch_mode_w(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
+    constexpr int nt = BlkShape<MODE, WGT>::kThreads;
+    if (dpers)
+        hipLaunchKernelGGL((pr_sweep_blk<true, MODE, WGT>), dim3(grid), dim3(nt), 0, s, b);
+    else
+        hipLaunchKernelGGL((pr_sweep_blk<false, MODE, WGT>), dim3(grid), dim3(nt), 0, s, b);
+}
 template <int MODE>
 void launch_mode(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
-    if (dpers)
-        hipLaunchKernelGGL((pr_sweep_blk<true, MODE>), dim3(grid), dim3(kBlkThreads), 0, s, b);
+    if (b.wblk)
+        launch_mode_w<MODE, true>(b, dpers, grid, s);
     else
-        hipLaunchKernelGGL((pr_sweep_blk<false, MODE>), dim3(grid), dim3(kBlkThreads), 0, s, b);
+        launch_mode_w<MODE, false>(b, dpers, grid, s);
 }
 
 void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStream_t s) {
@@ -803,18 +977,25 @@ void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStr
         case kBlkRefine: launch_mode<kBlkRefine>(b, dpers, grid, s); break;
         case kBlkF64: launch_mode<kBlkF64>(b, dpers, grid, s); break;
         default:
-            hipLaunchKernelGGL((pr_sweep_blk<false, kBlkEv>), dim3(grid), dim3(kBlkThreads), 0, s,
-                               b);
+            hipLaunchKernelGGL((pr_sweep_blk<false, kBlkEv, false>), dim3(grid),
+                               dim3(BlkShape<kBlkEv>::kThreads), 0, s, b);
             break;
     }
 }
 
+template <int MODE, bool WGT>
+void fold_mode_w(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
+    if (dpers)
+        hipLaunchKernelGGL((pr_fold_blk<true, MODE, WGT>), dim3(grid), dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL((pr_fold_blk<false, MODE, WGT>), dim3(grid), dim3(kBlock), 0, s, b);
+}
 template <int MODE>
 void fold_mode(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
-    if (dpers)
-        hipLaunchKernelGGL((pr_fold_blk<true, MODE>), dim3(grid), dim3(kBlock), 0, s, b);
+    if (b.wblk)
+        fold_mode_w<MODE, true>(b, dpers, grid, s);
     else
-        hipLaunchKernelGGL((pr_fold_blk<false, MODE>), dim3(grid), dim3(kBlock), 0, s, b);
+        fold_mode_w<MODE, false>(b, dpers, grid, s);
 }
 
 void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
@@ -824,7 +1005,8 @@ void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
         case kBlkRefine: fold_mode<kBlkRefine>(b, dpers, grid, s); break;
         case kBlkF64: fold_mode<kBlkF64>(b, dpers, grid, s); break;
         default:
-            hipLaunchKernelGGL((pr_fold_blk<false, kBlkEv>), dim3(grid), dim3(kBlock), 0, s, b);
+            hipLaunchKernelGGL((pr_fold_blk<false, kBlkEv, false>), dim3(grid), dim3(kBlock), 0, s,
+                               b);
             break;
     }
 }

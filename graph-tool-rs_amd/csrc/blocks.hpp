@@ -38,7 +38,7 @@ constexpr uint32_t kBlkF64MaxRow = 1u << 23;
 struct BlkState {
     double mass;         // L1 mass of the iterate the next sweep gathers
     uint32_t next_tile;  // tile counter of the running sweep
-    uint32_t pad;
+    uint32_t pad;        // set by blk_stage_weights on a weight it cannot use
 };
 
 // A mass above this makes the sweep give up (SweepState::bail): the loop is
@@ -79,12 +79,25 @@ struct BlkArgs {
     const double *c_cur64;      // eigenvector both y_{L-1})
     double *c_next64;
     const double *x_prev64;     // eigenvector: y_{L-2} (y_{L-1} on the first launch)
+    // Weighted PageRank sweeps: the weights in block order (FP32 for the FP32
+    // sweep, FP64 for the refinement and the FP64 sweep), null when
+    // unweighted; the weighted out-degree in sweep labels.
+    const void *wblk;
+    const double *wdeg;
 };
 
 // Build (once per tile size and long-row threshold) the block tiles of R's
 // pull orientation.  Waits for the stream.
 gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
                         Blocks **out);
+// A weighted solve's weights in block order into `out` (FP64 when `f64`,
+// else FP32), from the map's device source (stage_edge_source).  Sets
+// *usable to whether every weight is finite and non-negative: the block
+// sweeps bound their fixed-point sums by the iterate's mass, which other
+// weights break, so such a solve runs on the wave engine.  Waits for the
+// stream.
+gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, const void *src,
+                            int dtype, bool f64, DevBuf &out, bool *usable);
 // Sizes the per-call buffers of a block loop over B into the workspace.
 gt_status blk_reserve(gt_graph *g, const Blocks &B);
 // The layout and workspace fields of the sweep arguments over B (after

@@ -109,7 +109,7 @@ gt_tuning resolve_tuning(const gt_tuning &t) {
     r.block_long_min = t.block_long_min > 0 ? std::max(16, std::min(t.block_long_min, 65536))
                                             : (int)kBlkLongMinDefault;
     r.block_per_cu = t.block_per_cu > 0 ? std::min(t.block_per_cu, 3) : 2;
-    r.block_f64 = t.block_f64 > 0 ? 1 : -1;
+    r.block_f64 = t.block_f64 < 0 ? -1 : 1;
     return r;
 }
 

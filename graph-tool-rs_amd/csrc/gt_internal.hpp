@@ -242,6 +242,7 @@ struct Blocks {
     DevBuf ltab;         // uint32[]: long-row slot of each tile's long code
     DevBuf long_rows;    // uint32[nlong]
     DevBuf odeg;         // uint32[n]: out-degree in sweep labels
+    DevBuf eids;         // uint32[ngroups * 64]: edge id of each word (0 for padding)
     uint32_t ntiles = 0, nlong = 0;
     uint32_t max_row = 0;  // entries of the longest row
     uint64_t ngroups = 0;
@@ -295,6 +296,7 @@ struct Workspace {
     DevBuf blk_slots_lo;  // block sweeps: their low words (FP64)
     DevBuf blk_part;    // block sweeps: per-tile delta, dangling and mass partials
     DevBuf blk_state;   // block sweeps: BlkState
+    DevBuf blk_w, blk_w64;  // weighted block sweeps: weights in block order (f32, f64)
     DevBuf masks;       // staged vertex + edge masks
     DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
     DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces

@@ -662,6 +662,7 @@ struct PrSolve {
     PrPlan plan;
     PrArgs base{};
     double *rank64 = nullptr;  // the f64 map on the device (caller's or staged)
+    const gt_prop *weight = nullptr;  // the weight map of a weighted solve
     const void *wsrc = nullptr;
     int wdtype = GT_F64;
     // Set when the sweeps run in the relabelled orientation: sweep vectors are
@@ -729,6 +730,7 @@ gt_status pr_prepare(gt_graph *g, const gt_view *view, const gt_prop *weight,
     }
     if (weight && weight->kind != GT_MAP_UNITY) {
         S.plan.weighted = true;
+        S.weight = weight;
         GT_TRY(stage_edge_source(g, weight, &S.wsrc, &S.wdtype));
         GT_TRY(stage_degree(g, weight, *push_caller, S.wsrc, S.wdtype, g->ws.deg));
         a.deg = g->ws.deg.as<double>();
@@ -805,11 +807,11 @@ gt_status sliced_for(PrSolve &S, uint32_t nhot, const Sliced **out) {
     return GT_OK;
 }
 
-// Whether the sweeps of S can run on block tiles (blocks.hpp): relabelled,
-// unfiltered and unweighted, unless the tuning turns them off.
+// Whether the sweeps of S can run on block tiles (blocks.hpp): relabelled
+// and unfiltered, unless the tuning turns them off.  A weighted solve also
+// needs every weight finite and non-negative (checked when they are staged).
 bool blocks_apply(const PrSolve &S) {
-    return S.rel && S.iperm && !S.plan.masked && !S.plan.weighted && !S.noalloc &&
-           S.g->tune.block_tile_nnz > 0;
+    return S.rel && S.iperm && !S.plan.masked && !S.noalloc && S.g->tune.block_tile_nnz > 0;
 }
 
 // The block-sweep arguments of a solve over B.
@@ -846,12 +848,25 @@ gt_status pr_phase_blocks(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had,
         *bailed = true;
         return GT_OK;
     }
+    DevBuf &wbuf = F64 ? g->ws.blk_w64 : g->ws.blk_w;
+    if (S.plan.weighted) {
+        bool usable = false;
+        GT_TRY(blk_stage_weights(g, *B, S.weight, S.wsrc, S.wdtype, F64, wbuf, &usable));
+        if (!usable) {
+            *bailed = true;
+            return GT_OK;
+        }
+    }
     GT_TRY(blk_reserve(g, *B));
     g->prof_kind = F64 ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
     a.x_next = x[0];
     a.c_next = c[0];
     dispatch_init<V>(S.plan, a, S.rank64, had, g->n, 0, s);
     BlkArgs b = blk_args(S, a, *B);
+    if (S.plan.weighted) {
+        b.wblk = wbuf.ptr;
+        b.wdeg = a.deg;
+    }
     blk_launch_start(b, s);
     GT_HIP(hipGetLastError());
     auto sweep = [&](uint64_t k) {
@@ -884,10 +899,10 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
                    double eps_pred, uint64_t cap, uint64_t *iter_out) {
     gt_graph *g = S.g;
     hipStream_t s = g->stream;
-    // The FP64 drop-in's sweeps run on block tiles only when tuned so: the
-    // wave engine sums a row in the order the weighted sweep does, so unit
-    // weights give the unweighted result bit for bit, as centrality.rs's
-    // unity fast path does.
+    // The FP64 drop-in's sweeps run on block tiles unless tuned off.  Its
+    // weighted and unweighted block sweeps share their shape and fixed-point
+    // sums, so unit weights give the unweighted result bit for bit, as
+    // centrality.rs's unity fast path does.
     const bool blk = std::is_same<V, float>::value || g->tune.block_f64 > 0;
     if (iter0 == 0 && cap != 1 && blk && blocks_apply(S)) {
         bool bailed = false;
@@ -896,6 +911,12 @@ gt_status pr_phase(PrSolve &S, PrArgs a, V *x[2], V *c[2], uint64_t had, uint64_
         g->layout.block_tiles = 0;
         g->layout.block_long_rows = 0;
         g->layout.block_groups = 0;
+    }
+    if (S.plan.weighted && !a.w) {
+        // The wave engine's weights, in its packed order.
+        DevBuf &wp = std::is_same<V, double>::value ? g->ws.wcsc64 : g->ws.wcsc;
+        GT_TRY(stage_pull_weights<V>(g, S.weight, *S.pull, S.wsrc, S.wdtype, wp));
+        a.w = wp.ptr;
     }
     g->prof_kind = std::is_same<V, double>::value ? GT_SWEEP_PAGERANK_F64 : GT_SWEEP_PAGERANK_F32;
     // The hot slice serves sweeps in relabelled order (the most gathered
@@ -992,10 +1013,6 @@ gt_status pagerank_f64_impl(gt_graph *g, const gt_view *view, const gt_prop *wei
     // sweep reads whole 16-byte groups.
     GT_TRY(ws.c[0].reserve(vb + 32));
     GT_TRY(ws.c[1].reserve(vb + 32));
-    if (S.plan.weighted) {
-        GT_TRY(stage_pull_weights<double>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc64));
-        S.base.w = ws.wcsc64.ptr;
-    }
     // Without relabelling the caller's map is the first iterate itself.
     double *x[2] = {S.iperm ? ws.x[0].as<double>() : S.rank64, ws.x[1].as<double>()};
     double *c[2] = {ws.c[0].as<double>(), ws.c[1].as<double>()};
@@ -1054,10 +1071,6 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
     float *x32[2] = {ws.x32[0].as<float>(), ws.x32[1].as<float>()};
     float *c32[2] = {ws.c32[0].as<float>(), ws.c32[1].as<float>()};
     PrArgs a32 = S.base;
-    if (S.plan.weighted) {
-        GT_TRY(stage_pull_weights<float>(g, weight, *S.pull, S.wsrc, S.wdtype, ws.wcsc));
-        a32.w = ws.wcsc.ptr;
-    }
     a32.eps_rel = std::ldexp(1.0, -22);
     uint64_t cap32 = params->max_iter;
     uint64_t it32 = 0;
@@ -1077,7 +1090,15 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         // One FP64 sweep from the FP32 phase's last contributions (already in
         // HBM): no init kernel, 4-byte gathers, FP64 sums and output.
         PrArgs ar = S.base;
-        if (S.plan.weighted) {
+        const bool on_blocks = g->layout.block_tiles != 0;
+        if (S.plan.weighted && on_blocks) {
+            // The map's weights in FP64 and block order, as the wave
+            // engine's refinement uses them.
+            bool usable = false;
+            GT_TRY(blk_stage_weights(g, S.rel->blk, weight, S.wsrc, S.wdtype, true, ws.blk_w64,
+                                     &usable));
+            if (!usable) return fail(GT_ERR_UNSUPPORTED, "block tiles: weights changed");
+        } else if (S.plan.weighted) {
             // An f32 map's weights are already staged as f32 in packed order
             // for the FP32 sweeps: widening them is exact.
             if (S.wdtype == GT_F32 && weight->kind == GT_MAP_DENSE)
@@ -1095,13 +1116,17 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         g->prof_kind = GT_SWEEP_PAGERANK_F64;
         GT_TRY(prof_prepare(g));
         GT_TRY(prof_begin(g, 0));
-        if (g->layout.block_tiles) {
+        if (on_blocks) {
             // The FP32 sweeps ran on block tiles: so does the refinement
             // (fixed-point sums, FP64 update and output).
             BlkArgs b = blk_args(S, ar, S.rel->blk);
             b.x_cur = x32[it32 & 1];
             b.c_cur = c32[it32 & 1];
             b.x_next64 = x64;
+            if (S.plan.weighted) {
+                b.wblk = ws.blk_w64.ptr;
+                b.wdeg = ar.deg;
+            }
             blk_launch_sweep(b, S.plan.dpers, g->tune.block_per_cu, kBlkRefine, s);
             GT_TRY(prof_end(g, 0));
             blk_launch_fold(b, S.plan.dpers, kBlkRefine, s);

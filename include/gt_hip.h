@@ -415,17 +415,17 @@ typedef struct gt_tuning {
                                     or 0 / -1 for off (off) */
     int32_t sliced_share;        /* percent of the sliced sweep's workgroups given to the warm
                                     slices: 1..99 (0: from a cost model) */
-    int32_t block_tile_nnz;      /* FP32 PageRank and eigenvector sweeps on a relabelled,
-                                    unfiltered, unweighted graph: entries per block tile (rows
-                                    sorted together by source, summed in 64-bit fixed point);
-                                    -1 turns the block sweeps off and the wave engine runs:
-                                    -1, 1024..1048576 (131072) */
+    int32_t block_tile_nnz;      /* PageRank and eigenvector sweeps on a relabelled, unfiltered
+                                    graph (PageRank: unweighted, or with finite non-negative
+                                    weights; eigenvector: unweighted): entries per block tile
+                                    (rows sorted together by source, summed in 64-bit fixed
+                                    point); -1 turns the block sweeps off and the wave engine
+                                    runs: -1, 1024..1048576 (131072) */
     int32_t block_long_min;      /* block tiles: rows with more entries than this are summed in
                                     lane-private slots (at most 32 per tile): 16..65536 (2048) */
     int32_t block_per_cu;        /* FP32 block sweeps: resident workgroups per CU: 1..3 (2) */
-    int32_t block_f64;           /* 1: the FP64 PageRank drop-in's unweighted sweeps run on the
-                                    block tiles too (faster; unit weights then no longer give
-                                    the unweighted result bit for bit); 0 / -1: off (off) */
+    int32_t block_f64;           /* the FP64 PageRank drop-in's sweeps on block tiles (two-word
+                                    fixed point): 1 on, -1 off (1) */
 } gt_tuning;
 
 gt_status gt_graph_set_tuning(gt_graph *g, const gt_tuning *t);

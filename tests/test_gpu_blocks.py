@@ -4,9 +4,11 @@ The FP32 sweep is the default FP32 engine of pagerank_mixed on relabelled,
 unfiltered, unweighted graphs. Each result is checked against the oracle at
 the FP32 form's bar (L1 <= 1e-6 of the FP64 fixed point), against the wave
 engine (the same solve with block_tile_nnz = -1), and bit for bit run to run.
-The two-word FP64 sweeps -- the FP64 PageRank drop-in with block_f64 on, and
-the eigenvector (on by default) -- are held to the drop-in's bar (L1 <= 1e-12
-and the same sweep count +-1) and to the wave engine's result to rounding.
+The two-word FP64 sweeps -- the FP64 PageRank drop-in's and the eigenvector's,
+both on by default -- are held to the drop-in's bar (L1 <= 1e-12 and the same
+sweep count +-1) and to the wave engine's result (block_f64 = -1, or
+block_tile_nnz = -1) to rounding.  Weighted PageRank runs on the block tiles
+too when every weight is finite and non-negative.
 
 The cases cover the layout's edge cases:
 - long rows summed in lane slots;
@@ -37,11 +39,11 @@ def params(n, eps, max_iter):
     return gt_hip.PowerIteration(eps, max_iter, gt_hip.Plan(n, 64))
 
 
-def mixed(g, eps=1e-9, max_iter=1000, pers=None, rank=None, reversed_=False):
+def mixed(g, eps=1e-9, max_iter=1000, pers=None, rank=None, reversed_=False, weight=None):
     r = gt_hip.DenseProp.new(g.graph_id) if rank is None else rank
     target = g.reversed() if reversed_ else g
-    it = gt_hip.pagerank_mixed(target, gt_hip.Unity.NEW, pers, 0.85,
-                               params(g.vertex_bound(), eps, max_iter), r)
+    w = gt_hip.Unity.NEW if weight is None else weight
+    it = gt_hip.pagerank_mixed(target, w, pers, 0.85, params(g.vertex_bound(), eps, max_iter), r)
     return np.asarray(r.data).copy(), it
 
 
@@ -191,7 +193,7 @@ def test_unrepresentable_mass_runs_on_the_wave_engine(gpu):
     assert it == itw and np.array_equal(x, xw, equal_nan=True)
 
 
-# ------------------------------------------------ FP64 PageRank (block_f64)
+# ------------------------------------------------ FP64 PageRank
 
 def parity(got, it, want, it_o, eps, tol=1e-12):
     """The drop-in's bar: same sweep count and L1 <= tol, or one sweep apart
@@ -205,29 +207,33 @@ def parity(got, it, want, it_o, eps, tol=1e-12):
         assert l1 <= eps, f"L1 {l1:.3e}, sweeps {it} vs {it_o}"
 
 
-def pr64(g, eps=1e-12, max_iter=1000, pers=None, rank=None, reversed_=False):
+def pr64(g, eps=1e-12, max_iter=1000, pers=None, rank=None, reversed_=False, weight=None):
     r = gt_hip.DenseProp.new(g.graph_id) if rank is None else rank
     target = g.reversed() if reversed_ else g
     p = params(g.vertex_bound(), eps, max_iter)
+    w = gt_hip.Unity.NEW if weight is None else weight
     if pers is None:
-        it = gt_hip.pagerank(target, gt_hip.Unity.NEW, 0.85, p, r)
+        it = gt_hip.pagerank(target, w, 0.85, p, r)
     else:
-        it = gt_hip.pagerank_with(target, gt_hip.Unity.NEW, pers, 0.85, p, r)
+        it = gt_hip.pagerank_with(target, w, pers, 0.85, p, r)
     return np.asarray(r.data).copy(), it
 
 
 def check64(n, s, t, pers=None, tile=0, long_min=0, reversed_=False):
-    """The FP64 drop-in with block_f64 on, against the oracle at the drop-in's
-    bar and against the wave engine (the default) to rounding."""
+    """The FP64 drop-in on block tiles (the default), against the oracle at
+    the drop-in's bar and against the wave engine (block_f64 = -1) to
+    rounding."""
     g = adj(n, s, t)
     dev = g.device()
     pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
     dev.set_tuning(block_f64=1, block_tile_nnz=tile, block_long_min=long_min)
     x, it = pr64(g, pers=pp, reversed_=reversed_)
     lay = dev.layout()
-    dev.set_tuning(block_f64=0, block_tile_nnz=0, block_long_min=0)
+    dev.set_tuning(block_f64=-1, block_tile_nnz=0, block_long_min=0)
     pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
     xw, itw = pr64(g, pers=pp, reversed_=reversed_)
+    assert dev.layout()["block_tiles"] == 0
+    dev.set_tuning(block_f64=0)
     assert lay["block_tiles"] > 0, lay
     want, it_o = oracle.Adj(n, s, t).pagerank(pers=pers, epsilon=1e-12, max_iter=1000,
                                               reversed=reversed_, threads=8)
@@ -300,8 +306,9 @@ def test_f64_bit_for_bit_and_unrepresentable_mass(gpu):
     p = np.full(n, 1e300)
     x, it = pr64(g, eps=1e-6, pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
     assert g.device().layout()["block_tiles"] == 0
-    g.device().set_tuning(block_f64=0)
+    g.device().set_tuning(block_f64=-1)
     xw, itw = pr64(g, eps=1e-6, pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
+    g.device().set_tuning(block_f64=0)
     assert it == itw and np.array_equal(x, xw, equal_nan=True)
 
 
@@ -391,3 +398,114 @@ def test_ev_bit_for_bit_and_unrepresentable_start(gpu):
     g.device().set_tuning(block_tile_nnz=0)
     assert it == itw and np.array_equal(x, xw, equal_nan=True)
     assert e == ew or (np.isnan(e) and np.isnan(ew))
+
+
+# ------------------------------------------------------- weighted solves
+
+def _weight_maps(g, w):
+    """(map, the values the oracle is given) for an f64, an f32 and a
+    Constant weight map."""
+    w32 = w.astype(np.float32)
+    return [(gt_hip.DenseProp.from_vec(g.graph_id, w.copy()), w),
+            (gt_hip.DenseProp.from_vec(g.graph_id, w32.copy()), w32.astype(np.float64)),
+            (gt_hip.Constant(2.5), np.full(len(w), 2.5))]
+
+
+def test_weighted_fp32_form_on_block_tiles(gpu):
+    """The weighted FP32 form on block tiles (weights in block order), with
+    and without a personalisation, against the oracle at the FP32 form's bar
+    and against the wave engine."""
+    n = 1 << 14
+    s, t = oracle.gen_rmat(14, 16 * n, 0.57, 0.19, 0.19, 62)
+    rng = np.random.default_rng(12)
+    w = 1.0 - 0.9 * rng.random(len(s))
+    p = rng.random(n) + 0.05
+    p /= p.sum()  # a larger sum makes the iteration diverge (dangling mass times sum p)
+    g = adj(n, s, t)
+    dev = g.device()
+    for wm, wv in _weight_maps(g, w):
+        for pers in (None, p):
+            pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers.copy())
+            x, it = mixed(g, weight=wm, pers=pp)
+            assert dev.layout()["block_tiles"] > 0
+            dev.set_tuning(block_tile_nnz=-1)
+            pp = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers.copy())
+            xw, itw = mixed(g, weight=wm, pers=pp)
+            dev.set_tuning(block_tile_nnz=0)
+            want, _ = oracle.Adj(n, s, t).pagerank(weight=wv, pers=pers, epsilon=1e-12,
+                                                   max_iter=1000, threads=8)
+            scale = max(1.0, float(np.abs(want).sum()))
+            assert np.abs(x - want).sum() <= 1e-6 * scale
+            assert abs(it - itw) <= 1 and np.abs(x - xw).sum() <= 2e-7 * scale
+
+
+def test_weighted_fp64_on_block_tiles(gpu):
+    """The weighted FP64 drop-in on block tiles against the oracle at the
+    drop-in's bar, on default and small tiles, and against the wave engine
+    (block_f64 = -1)."""
+    n = 1 << 14
+    s, t = oracle.gen_rmat(14, 16 * n, 0.57, 0.19, 0.19, 63)
+    rng = np.random.default_rng(13)
+    w = 0.1 + 0.9 * rng.random(len(s))
+    g = adj(n, s, t)
+    dev = g.device()
+    for tile in (0, 1024):
+        for wm, wv in _weight_maps(g, w):
+            dev.set_tuning(block_f64=1, block_tile_nnz=tile, block_long_min=16 if tile else 0)
+            x, it = pr64(g, weight=wm)
+            assert dev.layout()["block_tiles"] > 0
+            dev.set_tuning(block_f64=-1, block_tile_nnz=0, block_long_min=0)
+            xw, itw = pr64(g, weight=wm)
+            dev.set_tuning(block_f64=0)
+            want, it_o = oracle.Adj(n, s, t).pagerank(weight=wv, epsilon=1e-12, max_iter=1000,
+                                                      threads=8)
+            parity(x, it, want, it_o, 1e-12)
+            assert abs(it - itw) <= 1
+            if it == itw:
+                assert np.abs(x - xw).sum() <= 1e-12
+
+
+def test_unit_weights_give_the_unweighted_result_bitwise(gpu):
+    """centrality.rs's unity fast path must agree with the general sum: on
+    block tiles, unit weights (an f64, an f32 and a Constant map) give the
+    Unity solve bit for bit, for the FP64 drop-in and the FP32 form."""
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 64)
+    g = adj(n, s, t)
+    dev = g.device()
+    ones = [gt_hip.DenseProp.from_vec(g.graph_id, np.ones(len(s))),
+            gt_hip.DenseProp.from_vec(g.graph_id, np.ones(len(s), np.float32)),
+            gt_hip.Constant(1.0)]
+    dev.set_tuning(block_f64=1)
+    for solve in (pr64, mixed):
+        a, ia = solve(g)
+        assert dev.layout()["block_tiles"] > 0
+        for wm in ones:
+            b, ib = solve(g, weight=wm)
+            assert dev.layout()["block_tiles"] > 0
+            assert ia == ib and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    dev.set_tuning(block_f64=0)
+
+
+def test_negative_weight_runs_on_the_wave_engine(gpu):
+    """The block sweeps bound their sums by the iterate's mass, which a
+    negative weight breaks: such a solve runs on the wave engine, bit for bit
+    as with the block sweeps turned off."""
+    n = 1 << 12
+    s, t = oracle.gen_rmat(12, 16 * n, 0.57, 0.19, 0.19, 65)
+    w = np.random.default_rng(14).uniform(0.5, 1.5, len(s))
+    w[7] = -0.25
+    g = adj(n, s, t)
+    dev = g.device()
+    dev.set_tuning(block_f64=1)
+    for solve in (pr64, mixed):
+        x, it = solve(g, weight=gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
+        assert dev.layout()["block_tiles"] == 0
+        dev.set_tuning(block_tile_nnz=-1)
+        xw, itw = solve(g, weight=gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
+        dev.set_tuning(block_tile_nnz=0)
+        assert it == itw and np.array_equal(x.view(np.uint64), xw.view(np.uint64))
+    want, it_o = oracle.Adj(n, s, t).pagerank(weight=w, epsilon=1e-12, max_iter=1000, threads=8)
+    x, it = pr64(g, weight=gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
+    parity(x, it, want, it_o, 1e-12)
+    dev.set_tuning(block_f64=0)

@@ -226,9 +226,9 @@ def test_mixed_precision_within_1e6(gpu):
 def test_mixed_hot_slice(gpu, weighted, personalised):
     """The wave engine's FP32 sweep that keeps the most gathered contributions
     in LDS (block tiles off) against the oracle, against the same solve with
-    the slice off, and against the block-tile sweep (the default for
-    unweighted graphs): graphs wholly inside the slice (scale 12), larger than
-    it (scale 16), with n not a multiple of 4 (1003), and a 1 KB slice."""
+    the slice off, and against the block-tile sweep (the default, weighted or
+    not): graphs wholly inside the slice (scale 12), larger than it (scale
+    16), with n not a multiple of 4 (1003), and a 1 KB slice."""
     cases = [oracle.gen_rmat(12, 16 << 12, 0.57, 0.19, 0.19, 5) + (1 << 12, 0),
              oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 0),
              oracle.gen_rmat(16, 16 << 16, 0.57, 0.19, 0.19, 6) + (1 << 16, 1)]
@@ -255,7 +255,7 @@ def test_mixed_hot_slice(gpu, weighted, personalised):
             wp = None if w is None else gt_hip.DenseProp.from_vec(g.graph_id, w)
             pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
             got[hot] = pr(g, 1e-9, 1000, weight=wp, pers=pp, fn=gt_hip.pagerank_mixed)
-            assert (dev.layout()["block_tiles"] > 0) == (blk == 0 and not weighted)
+            assert (dev.layout()["block_tiles"] > 0) == (blk == 0)
         dev.set_tuning(sweep_hot_kb=0, block_tile_nnz=0)
         scale = max(1.0, np.abs(want).sum())
         for hot, (x, it) in got.items():
@@ -270,9 +270,10 @@ def test_mixed_hot_slice(gpu, weighted, personalised):
 
 @pytest.mark.parametrize("personalised", [False, True])
 def test_fp64_hot_slice(gpu, personalised):
-    """The FP64 drop-in with its hot slice (unweighted, unfiltered graphs: on by
-    default) and without, against the oracle at the drop-in's bar: graphs
-    inside the slice, larger than it, a 1 KB slice, n not a multiple of 4."""
+    """The FP64 drop-in on the wave engine (block_f64 = -1) with its hot slice
+    (unweighted, unfiltered graphs) and without, and on block tiles (the
+    default), against the oracle at the drop-in's bar: graphs inside the
+    slice, larger than it, a 1 KB slice, n not a multiple of 4."""
     r = Rng(37)
     n = 1003
     cases = [oracle.gen_rmat(12, 16 << 12, 0.57, 0.19, 0.19, 8) + (1 << 12, 0),
@@ -288,12 +289,13 @@ def test_fp64_hot_slice(gpu, personalised):
         dev = g.device()
         want, it_o = oracle.Adj(n, s, t).pagerank(pers=p, epsilon=1e-12, max_iter=1000,
                                                   threads=8)
-        for hot in (kb, -1):
-            dev.set_tuning(sweep_hot_kb=hot)
+        for hot, blk in ((kb, -1), (-1, -1), (kb, 1)):
+            dev.set_tuning(sweep_hot_kb=hot, block_f64=blk)
             pp = None if p is None else gt_hip.DenseProp.from_vec(g.graph_id, p)
             got, it = pr(g, 1e-12, 1000, pers=pp)
+            assert (dev.layout()["block_tiles"] > 0) == (blk == 1)
             parity(got, it, want, it_o, 1e-12)
-        dev.set_tuning(sweep_hot_kb=0)
+        dev.set_tuning(sweep_hot_kb=0, block_f64=0)
 
 
 # ---------------------------------------------------------------- errors

@@ -25,7 +25,7 @@ def test_tuning_round_trip_and_defaults(gpu):
     assert d == {"sweep_blocks_per_cu": 3, "tile_rows": 256, "short_row": 64,
                  "mm_tile_nnz": 1024, "mm_blocks_per_cu": 8, "sweep_hot_kb": 128,
                  "sliced_min_len": -1, "sliced_share": 0, "block_tile_nnz": 131072,
-                 "block_long_min": 2048, "block_per_cu": 2, "block_f64": -1}
+                 "block_long_min": 2048, "block_per_cu": 2, "block_f64": 1}
     got = dev.set_tuning(tile_rows=1000, short_row=8, sweep_blocks_per_cu=99, mm_tile_nnz=10)
     assert got["tile_rows"] == 256 and got["short_row"] == 8
     assert got["sweep_blocks_per_cu"] == 8 and got["mm_tile_nnz"] == 256
@@ -40,8 +40,8 @@ def test_tuning_round_trip_and_defaults(gpu):
     got = dev.set_tuning(block_tile_nnz=-5, block_long_min=3)
     assert got["block_tile_nnz"] == -1 and got["block_long_min"] == 16
     assert dev.set_tuning(block_per_cu=9)["block_per_cu"] == 3
-    assert dev.set_tuning(block_f64=7)["block_f64"] == 1
-    assert dev.set_tuning(block_f64=0)["block_f64"] == -1
+    assert dev.set_tuning(block_f64=-7)["block_f64"] == -1
+    assert dev.set_tuning(block_f64=0)["block_f64"] == 1
     assert dev.set_tuning(sliced_min_len=0, sliced_share=-3)["sliced_min_len"] == -1
     assert dev.tuning()["sliced_share"] == 0
     with pytest.raises(ValueError):

@@ -95,12 +95,24 @@ def _f64(g, w, pers, d, params, r):
     return gt_hip.pagerank_with(g, w, pers, d, params, r)
 
 
+def _engines(g):
+    """Runs the body on block tiles (the default) and on the wave engine,
+    checking which one ran."""
+    dev = g.device()
+    for blk in (0, -1):
+        dev.set_tuning(block_tile_nnz=blk)
+        yield blk
+        assert (dev.layout()["block_tiles"] > 0) == (blk == 0), dev.layout()
+    dev.set_tuning(block_tile_nnz=0)
+
+
 def test_plain_stream_pagerank_f64(gpu, plain_graph):
     n, s, t, g, o = plain_graph
-    got, it = _pr(g, n, _f64)
-    assert g.device().layout()["slot_bits"] == 0
     want, it_o = o.pagerank(epsilon=1e-10, grain=4096, threads=THREADS)
-    _parity(got, it, want, it_o, 1e-10)
+    for _ in _engines(g):
+        got, it = _pr(g, n, _f64)
+        assert g.device().layout()["slot_bits"] == 0
+        _parity(got, it, want, it_o, 1e-10)
 
 
 def test_plain_stream_weighted_personalised(gpu, plain_graph):
@@ -108,11 +120,12 @@ def test_plain_stream_weighted_personalised(gpu, plain_graph):
     rng = np.random.default_rng(3)
     w = 0.1 + 0.9 * rng.random(len(s))
     p = rng.random(n) + 0.05
-    got, it = _pr(g, n, _f64, weight=gt_hip.DenseProp.from_vec(g.graph_id, w),
-                  pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
-    assert g.device().layout()["slot_bits"] == 0
     want, it_o = o.pagerank(weight=w, pers=p, epsilon=1e-10, grain=4096, threads=THREADS)
-    _parity(got, it, want, it_o, 1e-10, tol=1e-12 * max(1.0, float(p.sum())))
+    for _ in _engines(g):
+        got, it = _pr(g, n, _f64, weight=gt_hip.DenseProp.from_vec(g.graph_id, w),
+                      pers=gt_hip.DenseProp.from_vec(g.graph_id, p))
+        assert g.device().layout()["slot_bits"] == 0
+        _parity(got, it, want, it_o, 1e-10, tol=1e-12 * max(1.0, float(p.sum())))
 
 
 def test_plain_stream_mixed(gpu, plain_graph):
@@ -134,13 +147,14 @@ def test_plain_stream_mixed(gpu, plain_graph):
 def test_plain_stream_eigenvector(gpu, plain_graph):
     # A random functional graph need not converge: both sides run 25 sweeps.
     n, s, t, g, o = plain_graph
-    x = gt_hip.DenseProp.new(g.graph_id)
-    eig, it = gt_hip.eigenvector(g, gt_hip.Unity.NEW, _params(n, 1e-300, 25), x)
-    assert g.device().layout()["slot_bits"] == 0
     xo, eo, ito = o.eigenvector(epsilon=1e-300, max_iter=25, grain=4096, threads=THREADS)
-    assert it == ito == 25
-    assert np.abs(np.asarray(x.data) - xo).sum() <= 1e-12 * max(1.0, np.abs(xo).sum())
-    assert abs(eig - eo) <= 1e-12 * abs(eo)
+    for _ in _engines(g):
+        x = gt_hip.DenseProp.new(g.graph_id)
+        eig, it = gt_hip.eigenvector(g, gt_hip.Unity.NEW, _params(n, 1e-300, 25), x)
+        assert g.device().layout()["slot_bits"] == 0
+        assert it == ito == 25
+        assert np.abs(np.asarray(x.data) - xo).sum() <= 1e-12 * max(1.0, np.abs(xo).sum())
+        assert abs(eig - eo) <= 1e-12 * abs(eo)
 
 
 def test_plain_stream_filtered_view(gpu, plain_graph):
@@ -178,11 +192,12 @@ def test_plain_stream_adjoint(gpu, plain_graph, precision, tol):
 
 def test_packed_stream_above_2_22(gpu, local_graph):
     n, s, t, g, o = local_graph
-    got, it = _pr(g, n, _f64, eps=1e-10)
-    lay = g.device().layout()
-    assert lay["slot_bits"] == 10, lay
     want, it_o = o.pagerank(epsilon=1e-10, grain=4096, threads=THREADS)
-    _parity(got, it, want, it_o, 1e-10)
+    for _ in _engines(g):
+        got, it = _pr(g, n, _f64, eps=1e-10)
+        lay = g.device().layout()
+        assert lay["slot_bits"] == 10, lay
+        _parity(got, it, want, it_o, 1e-10)
     dev = g.device()
     for blk in (0, -1):
         dev.set_tuning(block_tile_nnz=blk)
@@ -191,13 +206,14 @@ def test_packed_stream_above_2_22(gpu, local_graph):
         assert lay["slot_bits"] == 10 and (lay["block_tiles"] > 0) == (blk == 0), lay
         assert np.abs(got32 - want).sum() <= 1e-6
     dev.set_tuning(block_tile_nnz=0)
-    x = gt_hip.DenseProp.new(g.graph_id)
-    eig, it = gt_hip.eigenvector(g, gt_hip.Unity.NEW, _params(n, 1e-300, 20), x)
     xo, eo, ito = o.eigenvector(epsilon=1e-300, max_iter=20, grain=4096, threads=THREADS)
-    assert it == ito == 20
-    # Unit L2 norm over millions of entries: the bar is relative to ||x||_1.
-    assert np.abs(np.asarray(x.data) - xo).sum() <= 1e-12 * max(1.0, np.abs(xo).sum())
-    assert abs(eig - eo) <= 1e-12 * abs(eo)
+    for _ in _engines(g):
+        x = gt_hip.DenseProp.new(g.graph_id)
+        eig, it = gt_hip.eigenvector(g, gt_hip.Unity.NEW, _params(n, 1e-300, 20), x)
+        assert it == ito == 20
+        # Unit L2 norm over millions of entries: the bar is relative to ||x||_1.
+        assert np.abs(np.asarray(x.data) - xo).sum() <= 1e-12 * max(1.0, np.abs(xo).sum())
+        assert abs(eig - eo) <= 1e-12 * abs(eo)
 
 
 def test_uneven_batch_union_above_2_22(gpu):
