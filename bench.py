@@ -918,7 +918,7 @@ def main() -> int:
     # (the timed solves' layout, read before the FP64 solves).
     blk = {k: lay32[k] for k in ("block_tiles", "block_groups", "block_long_rows")}
     on_blocks = blk["block_tiles"] > 0
-    k32 = ("pr_sweep_blk<false, 0>" if on_blocks
+    k32 = ("pr_sweep_blk<false, 0, false>" if on_blocks
            else "pr_sweep_hot<float, false, false>" if hot else "pr_sweep<float>")
     traffic32, traffic_src = profiled_traffic(workload, k32)
     result = {
@@ -958,9 +958,9 @@ def main() -> int:
             "sweeps": sw64[0], "achieved_gbs": round(ach64, 1),
             "roofline_frac": round(ach64 / HBM_PEAK_GBS, 4), "bytes_per_sweep": b64,
             "avg_launch_us": round(avg64 * 1e6, 2),
-            "kernel": ("pr_sweep_blk<false, 2> (block tiles, two-word fixed-point sums)"
+            "kernel": ("pr_sweep_blk<false, 2, false> (block tiles, two-word fixed-point sums)"
                        if lay64["block_tiles"] else "pr_sweep_hot<double> (wave engine)"),
-            "traffic": profiled_traffic(workload, "pr_sweep_blk<false, 2>"
+            "traffic": profiled_traffic(workload, "pr_sweep_blk<false, 2, false>"
                                         if lay64["block_tiles"]
                                         else "pr_sweep_hot<double, false, false>")[0],
         },
@@ -1067,7 +1067,7 @@ def main() -> int:
         devs.set_profiling(False)
         lay_e = devs.layout()
         ev_blocks = lay_e["block_tiles"] > 0
-        ke = "pr_sweep_blk<false, 3>" if ev_blocks else "ev_sweep_hot"
+        ke = "pr_sweep_blk<false, 3, false>" if ev_blocks else "ev_sweep_hot"
         be = ev_bytes(n, ms_)
         avge = mse / max(lae, 1) * 1e-3
         ache = be * lae / max(lae, 1) / avge / 1e9 if lae else 0.0
@@ -1077,7 +1077,7 @@ def main() -> int:
             "launches_per_solve": lae // max(args.steps, 1),
             "eigenvalue": eig.value, "achieved_gbs": round(ache, 1),
             "roofline_frac": round(ache / HBM_PEAK_GBS, 4), "bytes_per_sweep": be,
-            "kernel": ("pr_sweep_blk<false, 3> (eigenvector sweep on block tiles, two-word "
+            "kernel": ("pr_sweep_blk<false, 3, false> (eigenvector sweep on block tiles, two-word "
                        "fixed-point row sums)" if ev_blocks
                        else "ev_sweep_hot (eigenvector sweep, hot iterate in LDS)"),
             "layout": {k: lay_e[k] for k in ("block_tiles", "block_groups", "block_long_rows")},

@@ -910,6 +910,23 @@ gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, cons
     return GT_OK;
 }
 
+__global__ __launch_bounds__(kBlock) void blk_widen_w(const float *__restrict__ in, uint64_t count,
+                                                      double *__restrict__ out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
+         k += (uint64_t)gridDim.x * kBlock)
+        out[k] = static_cast<double>(in[k]);
+}
+
+gt_status blk_widen_weights(gt_graph *g, const Blocks &B, const DevBuf &f32, DevBuf &out) {
+    const uint64_t count = std::max<uint64_t>(kBlkQuad, B.ngroups) * kWave;
+    GT_TRY(out.reserve(sizeof(double) * count));
+    const unsigned grid = (unsigned)std::min<uint64_t>(8192, (count + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(blk_widen_w, dim3(grid), dim3(kBlock), 0, g->stream, f32.as<float>(), count,
+                       out.as<double>());
+    GT_HIP(hipGetLastError());
+    return GT_OK;
+}
+
 gt_status blk_reserve(gt_graph *g, const Blocks &B) {
     Workspace &ws = g->ws;
     GT_TRY(ws.blk_slots.reserve(sizeof(unsigned long long) * std::max<uint32_t>(1, B.nlong)));

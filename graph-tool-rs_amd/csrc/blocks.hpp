@@ -98,6 +98,10 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
 // stream.
 gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, const void *src,
                             int dtype, bool f64, DevBuf &out, bool *usable);
+// The FP32 block-order weights of an f32 map (as blk_stage_weights wrote
+// them) widened to FP64: the same values as staging the map in FP64 again,
+// without the gather through the edge ids.
+gt_status blk_widen_weights(gt_graph *g, const Blocks &B, const DevBuf &f32, DevBuf &out);
 // Sizes the per-call buffers of a block loop over B into the workspace.
 gt_status blk_reserve(gt_graph *g, const Blocks &B);
 // The layout and workspace fields of the sweep arguments over B (after

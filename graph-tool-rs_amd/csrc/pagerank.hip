@@ -1093,11 +1093,16 @@ gt_status pagerank_mixed_impl(gt_graph *g, const gt_view *view, const gt_prop *w
         const bool on_blocks = g->layout.block_tiles != 0;
         if (S.plan.weighted && on_blocks) {
             // The map's weights in FP64 and block order, as the wave
-            // engine's refinement uses them.
-            bool usable = false;
-            GT_TRY(blk_stage_weights(g, S.rel->blk, weight, S.wsrc, S.wdtype, true, ws.blk_w64,
-                                     &usable));
-            if (!usable) return fail(GT_ERR_UNSUPPORTED, "block tiles: weights changed");
+            // engine's refinement uses them: an f32 map's, widened from the
+            // FP32 sweeps' copy.
+            if (S.wdtype == GT_F32 && weight->kind == GT_MAP_DENSE) {
+                GT_TRY(blk_widen_weights(g, S.rel->blk, ws.blk_w, ws.blk_w64));
+            } else {
+                bool usable = false;
+                GT_TRY(blk_stage_weights(g, S.rel->blk, weight, S.wsrc, S.wdtype, true,
+                                         ws.blk_w64, &usable));
+                if (!usable) return fail(GT_ERR_UNSUPPORTED, "block tiles: weights changed");
+            }
         } else if (S.plan.weighted) {
             // An f32 map's weights are already staged as f32 in packed order
             // for the FP32 sweeps: widening them is exact.
