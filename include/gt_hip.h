@@ -176,9 +176,13 @@ gt_status gt_graph_set_stream(gt_graph *g, void *hip_stream);
 
 /* ------------------------------------------------------------- centrality */
 
-/* PageRank, FP64 throughout: the drop-in for pagerank / pagerank_with.
+/* PageRank in FP64: the drop-in for pagerank / pagerank_with.
  * pers == NULL means the uniform 1/num_vertices of `pagerank`.
- * On return *iters is the sweep count the reference returns. */
+ * On return *iters is the sweep count the reference returns.  Unfiltered
+ * solves, unweighted or with finite non-negative weights, sum their rows on
+ * block tiles in two-word fixed point (to about 2^-100 of the iterate's L1
+ * mass, the same bits on every run); tuning block_f64 = -1 selects the wave
+ * engine's FP64 row sums. */
 gt_status gt_pagerank_f64(gt_graph *g, const gt_view *view,
                           const gt_prop *weight, const gt_prop *pers,
                           double damping, const gt_power_iteration *params,
@@ -209,8 +213,10 @@ gt_status gt_pagerank_mixed(gt_graph *g, const gt_view *view,
  * Serves Directed and Reversed views, Unity (or null) weights, a null,
  * constant or FP64 device personalisation and a device rank map; the graph's
  * sweep schedule for that view must already exist (one gt_pagerank_f64 call
- * on it builds it).  Results equal gt_pagerank_f64's bit for bit.  Calls
- * that overlap on one device share the array and race. */
+ * on it builds it).  It runs the wave engine: results equal those of
+ * gt_pagerank_f64 with tuning block_f64 = -1 bit for bit, and its default
+ * block-tile solve's within the drop-in's bar.  Calls that overlap on one
+ * device share the array and race. */
 gt_status gt_pagerank_f64_noalloc(gt_graph *g, const gt_view *view,
                                   const gt_prop *weight, const gt_prop *pers,
                                   double damping, const gt_power_iteration *params,
