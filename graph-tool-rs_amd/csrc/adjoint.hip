@@ -23,6 +23,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "blocks.hpp"
 #include "device_common.hpp"
 #include "driver.hpp"
 #include "gt_internal.hpp"
@@ -486,8 +487,6 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
             hipLaunchKernelGGL(adj_gather<double>, vg, bd, 0, s, a.deg, iperm, n,
                                ws.arl[1].as<double>());
             r.deg = ws.arl[1].as<double>();
-            GT_TRY(stage_pull_weights<V>(g, weight, rp, wsrc, wdt, ws.wcsr));
-            r.w = ws.wcsr.ptr;
         }
         if (dpers) {
             GT_TRY(ws.arl[2].reserve(sizeof(double) * n));
@@ -497,8 +496,8 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         }
         GT_HIP(hipGetLastError());
     }
-    GT_HIP(hipMemsetAsync(a.st, 0, sizeof(SweepState), s));
-    {
+    auto init = [&]() -> gt_status {
+        GT_HIP(hipMemsetAsync(a.st, 0, sizeof(SweepState), s));
         AdjArgs b = r;
         b.l_next = lam[0];
         if (dpers)
@@ -506,11 +505,81 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         else
             hipLaunchKernelGGL((adj_init<V, false>), dim3(vgrid(n)), dim3(kBlock), 0, s, b, n);
         GT_HIP(hipGetLastError());
-    }
+        return GT_OK;
+    };
+    GT_TRY(init());
     g->prof_kind = GT_SWEEP_ADJOINT;
+    const int pc = g->tune.sweep_blocks_per_cu;
+    constexpr bool kF64 = std::is_same<V, double>::value;
+    constexpr int bmode = kF64 ? kBlkAdj64 : kBlkAdj32;
+    // Block tiles (blocks.hpp) of the transposed orientation, unless the
+    // tuning turns them off: the same sweep kernel with the adjoint's row
+    // update.  The loop runs on the wave engine instead when a row is too
+    // long for the two-word sums (FP64), a weight is negative or not finite,
+    // or a launch meets an iterate whose terms the fixed point cannot hold.
+    bool on_blocks = false;
+    BlkArgs bk{};
+    if (g->tune.block_tile_nnz > 0) {
+        Blocks *B = nullptr;
+        GT_TRY(ensure_blocks(g, *R, (uint32_t)g->tune.block_tile_nnz,
+                             (uint32_t)g->tune.block_long_min, &B));
+        bool usable = !kF64 || B->max_row <= kBlkF64MaxRow;
+        DevBuf &wbuf = kF64 ? ws.blk_w64 : ws.blk_w;
+        if (usable && weighted)
+            GT_TRY(blk_stage_weights(g, *B, weight, wsrc, wdt, kF64, wbuf, &usable));
+        if (usable) {
+            GT_TRY(blk_reserve(g, *B));
+            bk = blk_layout_args(g, *B);
+            bk.st = a.st;
+            bk.pers = r.pers;
+            bk.pval = r.pval;
+            bk.damping = damping;
+            bk.eps_abs = a.eps;
+            bk.eps_rel = a.eps_rel;
+            bk.max_iter = a.max_iter;
+            bk.cvec = r.cvec;
+            bk.row_off = r.row_off;
+            if (weighted) {
+                bk.wblk = wbuf.ptr;
+                bk.wdeg = r.deg;
+            }
+            bk.x_cur64 = r.cvec;  // the start mass: lambda_0 = dL/dx
+            blk_launch_start_ev(bk, s);
+            GT_HIP(hipGetLastError());
+            auto bsweep = [&](uint64_t q) {
+                const int par = (int)(q & 1);
+                if constexpr (kF64) {
+                    bk.x_cur64 = reinterpret_cast<const double *>(lam[par]);
+                    bk.c_cur64 = bk.x_cur64;
+                    bk.x_next64 = reinterpret_cast<double *>(lam[par ^ 1]);
+                } else {
+                    bk.x_cur = reinterpret_cast<const float *>(lam[par]);
+                    bk.c_cur = bk.x_cur;
+                    bk.x_next = reinterpret_cast<float *>(lam[par ^ 1]);
+                }
+                blk_launch_sweep(bk, dpers, g->tune.block_per_cu, bmode, s);
+            };
+            auto bfold = [&](uint64_t) { blk_launch_fold(bk, dpers, bmode, s); };
+            uint64_t launched = 0;
+            GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, bsweep, bfold, &launched,
+                         GT_SWEEP_ADJOINT));
+            if (!g->h_state->bail) {
+                on_blocks = true;
+            } else {
+                g->layout.block_tiles = 0;
+                g->layout.block_long_rows = 0;
+                g->layout.block_groups = 0;
+                GT_TRY(init());
+            }
+        }
+    }
+    if (weighted && !on_blocks) {
+        // The wave engine's weights, in its packed order.
+        GT_TRY(stage_pull_weights<V>(g, weight, rp, wsrc, wdt, ws.wcsr));
+        r.w = ws.wcsr.ptr;
+    }
     AdjArgs b = r;
     int sg = 0;
-    const int pc = g->tune.sweep_blocks_per_cu;
     // FP32 sweeps keep the most gathered entries of lambda in LDS.
     const uint32_t nhot =
         std::is_same<V, float>::value ? hot_len<float>(g->tune.sweep_hot_kb, n) : 0u;
@@ -537,8 +606,9 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         else adj_launch_fold<V, false, false>(b, sg, s);
     };
     uint64_t launched = 0;
-    GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched,
-                 GT_SWEEP_ADJOINT));
+    if (!on_blocks)
+        GT_TRY(drive(g, a.st, params->epsilon, params->max_iter, sweep, fold, &launched,
+                     GT_SWEEP_ADJOINT));
     const uint64_t sweeps = g->h_state->iter;
     const V *lr = lam[sweeps & 1];  // relabelled order
     V *lf = lam[(sweeps & 1) ^ 1];  // the caller's labels
@@ -561,7 +631,20 @@ static gt_status backward_impl(gt_graph *g, const gt_view *view, const gt_prop *
         double *mr = ws.arl[3].as<double>();
         const dim3 mg(wave_grid(occupancy_cap((const void *)adj_mu<V, true>, pc), r.ntiles)),
             lg(fold_grid(r.nlong)), bd(kBlock);
-        if (weighted) {
+        if (on_blocks) {
+            // mu from one more pass over the block tiles, gathering the last
+            // iterate (whose mass the last fold left for this pass).
+            if constexpr (kF64) {
+                bk.x_cur64 = reinterpret_cast<const double *>(lr);
+                bk.c_cur64 = bk.x_cur64;
+            } else {
+                bk.x_cur = reinterpret_cast<const float *>(lr);
+                bk.c_cur = bk.x_cur;
+            }
+            bk.mu_out = mr;
+            blk_launch_sweep(bk, dpers, g->tune.block_per_cu, bmode, s);
+            blk_launch_fold(bk, dpers, bmode, s);
+        } else if (weighted) {
             hipLaunchKernelGGL((adj_mu<V, true>), mg, bd, 0, s, r, lr, mr);
             hipLaunchKernelGGL((adj_mu_long<V, true>), lg, bd, 0, s, r, mr);
         } else {

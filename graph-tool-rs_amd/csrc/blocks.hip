@@ -23,18 +23,6 @@ constexpr int kBlkBatch = 4;
 // accumulators leave room for one workgroup per CU.
 constexpr int kBlkWaves = 8;
 constexpr int kBlkThreads = kBlkWaves * kWave;
-template <int MODE, bool WGT = false> struct BlkShape {
-    static constexpr int kWaves = MODE >= kBlkF64 ? 16 : kBlkWaves;
-    static constexpr int kThreads = kWaves * kWave;
-    // Blocks per wavefront batch.  A weighted two-word sweep holds each
-    // lane's FP64 weights beside its terms: half the batch keeps it within
-    // the 128 VGPRs of 16 wavefronts.  (Its shape must be the unweighted
-    // sweep's: the rows each thread finishes, and so the order of the
-    // partial sums, follow the thread count, and unit weights must give the
-    // unweighted result bit for bit.)
-    static constexpr int kBatch = MODE >= kBlkF64 && WGT ? 2 : kBlkBatch;
-    static constexpr int kRows = (kBlkRows + kThreads - 1) / kThreads;  // finisher rows a thread
-};
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
@@ -124,6 +112,22 @@ template <> struct BlkTypes<kBlkF64> {
     static constexpr bool kTwo = true;
 };
 template <> struct BlkTypes<kBlkEv> : BlkTypes<kBlkF64> {};
+template <> struct BlkTypes<kBlkAdj32> : BlkTypes<kBlkF32> {};
+template <> struct BlkTypes<kBlkAdj64> : BlkTypes<kBlkF64> {};
+
+template <int MODE, bool WGT = false> struct BlkShape {
+    static constexpr int kWaves = BlkTypes<MODE>::kTwo ? 16 : kBlkWaves;
+    static constexpr int kThreads = kWaves * kWave;
+    // Blocks per wavefront batch.  A weighted two-word sweep holds each
+    // lane's FP64 weights beside its terms: half the batch keeps it within
+    // the 128 VGPRs of 16 wavefronts.  (Its shape must be the unweighted
+    // sweep's: the rows each thread finishes, and so the order of the
+    // partial sums, follow the thread count, and unit weights must give the
+    // unweighted result bit for bit.)
+    static constexpr int kBatch = BlkTypes<MODE>::kTwo && WGT ? 2 : kBlkBatch;
+    static constexpr int kRows = (kBlkRows + kThreads - 1) / kThreads;  // finisher rows a thread
+};
+
 
 // Tile t's descriptor, as scalar loads (the table is read-only in a sweep).
 __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
@@ -194,6 +198,32 @@ __device__ __forceinline__ void blk_finish_ev(const BlkArgs &b, uint32_t i, doub
     msum += fabs(sv) * static_cast<double>(dg);
 }
 
+// The adjoint's row update, as AdjFin (adjoint.hip): mu = acc / dg (0 for
+// a sink) and lambda_next = dL/dx + d mu (+ d (p . lambda) for a sink),
+// rounded to the sweep's type; the delta against lambda_cur = lc, p .
+// lambda_next, and |lambda_next| times the times row i is gathered (the
+// next launch's mass).  The pass that computes mu writes mu alone.
+template <int MODE>
+__device__ __forceinline__ void blk_finish_adj(const BlkArgs &b, uint32_t i, double acc, double lc,
+                                               double dg, double pv, double pl, double &dsum,
+                                               double &psum, double &msum) {
+    typedef typename BlkTypes<MODE>::X V;
+    const double mu = dg != 0.0 ? acc / dg : 0.0;
+    if (b.mu_out) {
+        b.mu_out[i] = mu;
+        return;
+    }
+    const double v = b.cvec[i] + b.damping * mu + (dg == 0.0 ? b.damping * pl : 0.0);
+    const V vv = static_cast<V>(v);
+    dsum += fabs(static_cast<double>(vv) - lc);
+    if constexpr (BlkTypes<MODE>::kTwo)
+        b.x_next64[i] = vv;
+    else
+        b.x_next[i] = vv;
+    psum += pv * static_cast<double>(vv);
+    msum += fabs(static_cast<double>(vv)) * static_cast<double>(b.odeg[i]);
+}
+
 // Sum of a 64-bit value over each aligned group of 16 lanes.
 __device__ __forceinline__ unsigned long long sum16(unsigned long long v) {
 #pragma unroll
@@ -227,11 +257,12 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
     constexpr int NR = BlkShape<MODE, WGT>::kRows;
     constexpr int NB = BlkShape<MODE, WGT>::kBatch;
     constexpr bool EV = MODE == kBlkEv;
+    constexpr bool ADJ = MODE == kBlkAdj32 || MODE == kBlkAdj64;
     static_assert(!(EV && WGT), "the eigenvector's block sweep is unweighted");
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
-    // done), so it does not look at the flag.
-    if (MODE != kBlkRefine && st->done) return;
+    // done), and so does the adjoint's mu pass: they do not look at the flag.
+    if (MODE != kBlkRefine && !(ADJ && b.mu_out) && st->done) return;
     __shared__ unsigned long long acc[kBlkLds];
     // The FP64 sweep's low words (a dummy otherwise).
     __shared__ unsigned long long acc_lo[M::kTwo ? kBlkLds : 1];
@@ -288,7 +319,7 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
         // clamped row and are not used.  pr is the personalisation, or for
         // the eigenvector y_{L-2}.
         typename M::X xr[NR];
-        typename std::conditional<WGT, double, uint32_t>::type dr[NR];
+        typename std::conditional<WGT || ADJ, double, uint32_t>::type dr[NR];
         double pr[NR];
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
@@ -299,6 +330,8 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
                 xr[k] = b.x_cur[i];
             if constexpr (WGT)
                 dr[k] = b.wdeg[i];
+            else if constexpr (ADJ)
+                dr[k] = static_cast<double>(b.row_off[i + 1] - b.row_off[i]);
             else
                 dr[k] = b.odeg[i];
             if constexpr (EV)
@@ -430,6 +463,10 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
                     if constexpr (EV)
                         blk_finish_ev(b, i, v, xr[k], pr[k], dr[k], sc_cur, sc_prev, dsum, ssum,
                                       msum);
+                    else if constexpr (ADJ)
+                        blk_finish_adj<MODE>(b, i, v, static_cast<double>(xr[k]),
+                                             static_cast<double>(dr[k]), pr[k], p_sink, dsum,
+                                             ssum, msum);
                     else
                         blk_finish<MODE>(b, i, v, static_cast<double>(xr[k]),
                                          static_cast<double>(dr[k]), pr[k], p_sink, dsum, ssum,
@@ -485,8 +522,9 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 template <bool DPERS, int MODE, bool WGT>
 __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     constexpr bool EV = MODE == kBlkEv;
+    constexpr bool ADJ = MODE == kBlkAdj32 || MODE == kBlkAdj64;
     SweepState *st = b.st;
-    if (MODE != kBlkRefine && st->done) return;
+    if (MODE != kBlkRefine && !(ADJ && b.mu_out) && st->done) return;
     __shared__ double red[12];
     __shared__ uint32_t flag;
     const int K = blk_scale(b.bs->mass);
@@ -498,22 +536,31 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         const uint32_t r = b.long_rows[k];
         const unsigned long long hi = b.slots[k];
         b.slots[k] = 0ull;
-        const double dg = WGT ? b.wdeg[r] : static_cast<double>(b.odeg[r]);
-        if constexpr (MODE == kBlkF64 || EV) {
+        const double dg = WGT   ? b.wdeg[r]
+                          : ADJ ? static_cast<double>(b.row_off[r + 1] - b.row_off[r])
+                                : static_cast<double>(b.odeg[r]);
+        const double pv = DPERS ? b.pers[r] : b.pval;
+        if constexpr (BlkTypes<MODE>::kTwo) {
             const unsigned long long lo = b.slots_lo[k];
             b.slots_lo[k] = 0ull;
             const double v = blk_value2(hi, lo, K);
             if constexpr (EV)
                 blk_finish_ev(b, r, v, b.x_cur64[r], b.x_prev64[r], b.odeg[r], sc_cur, sc_prev, d,
                               s, ms);
+            else if constexpr (ADJ)
+                blk_finish_adj<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, d, s, ms);
             else
-                blk_finish<MODE>(b, r, v, b.x_cur64[r], dg, DPERS ? b.pers[r] : b.pval, p_sink, d,
-                                 s, ms);
+                blk_finish<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, d, s, ms);
+        } else if constexpr (ADJ) {
+            blk_finish_adj<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg, pv,
+                                 p_sink, d, s, ms);
         } else {
-            blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg,
-                             DPERS ? b.pers[r] : b.pval, p_sink, d, s, ms);
+            blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg, pv,
+                             p_sink, d, s, ms);
         }
     }
+    // The mu pass has no partials and decides nothing.
+    if (ADJ && b.mu_out) return;
     block_sum3(d, s, ms, red);
     double *fp = b.fold_part;
     if (threadIdx.x == 0) {
@@ -542,7 +589,21 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         b.bs->mass = ms;
         b.bs->next_tile = 0;
     }
-    if (!EV && threadIdx.x == 0) {
+    if (ADJ && threadIdx.x == 0) {
+        // As adj_fold: s is p . lambda of the new iterate; the stop rule is
+        // relative to ||dL/dx||_1 (st->aux).
+        const uint64_t it = st->iter + 1;
+        st->delta = d;
+        st->p_sink = s;
+        st->iter = it;
+        st->launches += 1;
+        const double thr = fmax(b.eps_abs, b.eps_rel) * (st->aux > 0.0 ? st->aux : 1.0);
+        st->done = (!(d >= thr) || (b.max_iter > 0 && it >= b.max_iter)) ? 1u : 0u;
+        st->counter = 0;
+        b.bs->mass = b.wblk ? ms * __longlong_as_double(static_cast<long long>(b.bs->wmax)) : ms;
+        b.bs->next_tile = 0;
+    }
+    if (!EV && !ADJ && threadIdx.x == 0) {
         const uint64_t it = st->iter + 1;
         const double thr = fmax(b.eps_abs, b.eps_rel * st->aux);
         st->delta = d;
@@ -556,8 +617,9 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     }
 }
 
-// The eigenvector's start: zero the long-row sums, then the mass of y_0 (sum
-// of |y_0| * out-degree), per workgroup and then in workgroup order by the
+// The eigenvector's and the adjoint's start: zero the long-row sums, then
+// the mass of the start vector v (sum of |v| * out-degree, times the largest
+// weight when weighted), per workgroup and then in workgroup order by the
 // last to arrive.
 __global__ __launch_bounds__(kBlock) void ev_start_blk(const BlkArgs b) {
     __shared__ double red[4];
@@ -576,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void ev_start_blk(const BlkArgs b) {
     for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) m += b.fold_part[k];
     m = block_sum(m, red);
     if (threadIdx.x == 0) {
-        b.bs->mass = m;
+        b.bs->mass = b.wblk ? m * __longlong_as_double(static_cast<long long>(b.bs->wmax)) : m;
         b.bs->next_tile = 0;
         b.st->bail = 0;
         b.st->counter = 0;
@@ -861,18 +923,26 @@ __global__ __launch_bounds__(kBlock) void blk_stage_w(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ eids,
                                                       const void *w, int kind, double value,
                                                       uint64_t count, V *__restrict__ out,
-                                                      uint32_t *bad) {
+                                                      uint32_t *bad, unsigned long long *wmax) {
     bool neg = false;
+    double top = 0.0;
     for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
          k += (uint64_t)gridDim.x * kBlock) {
         const double x = kind == GT_MAP_CONSTANT
                              ? value
                              : static_cast<double>(static_cast<const S *>(w)[eids[k]]);
         out[k] = static_cast<V>(x);
-        if ((words[k] & kBlkCodeMask) != kBlkNull && !(x >= 0.0 && x <= 1.7976931348623157e308))
-            neg = true;
+        if ((words[k] & kBlkCodeMask) != kBlkNull) {
+            if (!(x >= 0.0 && x <= 1.7976931348623157e308)) neg = true;
+            top = fmax(top, static_cast<double>(static_cast<V>(x)));
+        }
     }
     if (__any(neg) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(bad, 1u);
+    // Non-negative doubles order as their bit patterns.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0 && !neg)
+        atomicMax(wmax, static_cast<unsigned long long>(__double_as_longlong(top)));
 }
 
 gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, const void *src,
@@ -882,7 +952,8 @@ gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, cons
     GT_TRY(g->ws.blk_state.reserve(sizeof(BlkState)));
     BlkState *bs = g->ws.blk_state.as<BlkState>();
     hipStream_t s = g->stream;
-    GT_HIP(hipMemsetAsync(&bs->pad, 0, sizeof(uint32_t), s));
+    static_assert(offsetof(BlkState, wmax) == offsetof(BlkState, pad) + 4, "pad, wmax adjacent");
+    GT_HIP(hipMemsetAsync(&bs->pad, 0, sizeof(uint32_t) + sizeof(unsigned long long), s));
     const unsigned grid = (unsigned)std::min<uint64_t>(8192, (count + kBlock - 1) / kBlock);
     const uint32_t *words = B.words.as<uint32_t>(), *eids = B.eids.as<uint32_t>();
     const int kind = w->kind;
@@ -890,17 +961,17 @@ gt_status blk_stage_weights(gt_graph *g, const Blocks &B, const gt_prop *w, cons
     if (dtype == GT_F32) {
         if (f64)
             hipLaunchKernelGGL((blk_stage_w<float, double>), dim3(grid), dim3(kBlock), 0, s, words,
-                               eids, src, kind, value, count, out.as<double>(), &bs->pad);
+                               eids, src, kind, value, count, out.as<double>(), &bs->pad, &bs->wmax);
         else
             hipLaunchKernelGGL((blk_stage_w<float, float>), dim3(grid), dim3(kBlock), 0, s, words,
-                               eids, src, kind, value, count, out.as<float>(), &bs->pad);
+                               eids, src, kind, value, count, out.as<float>(), &bs->pad, &bs->wmax);
     } else {
         if (f64)
             hipLaunchKernelGGL((blk_stage_w<double, double>), dim3(grid), dim3(kBlock), 0, s,
-                               words, eids, src, kind, value, count, out.as<double>(), &bs->pad);
+                               words, eids, src, kind, value, count, out.as<double>(), &bs->pad, &bs->wmax);
         else
             hipLaunchKernelGGL((blk_stage_w<double, float>), dim3(grid), dim3(kBlock), 0, s,
-                               words, eids, src, kind, value, count, out.as<float>(), &bs->pad);
+                               words, eids, src, kind, value, count, out.as<float>(), &bs->pad, &bs->wmax);
     }
     GT_HIP(hipGetLastError());
     uint32_t bad = 0;
@@ -987,12 +1058,15 @@ void launch_mode(const BlkArgs &b, bool dpers, int grid, hipStream_t s) {
 
 void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStream_t s) {
     // The two-word sweeps' accumulators take 96 KB: one workgroup per CU.
-    const int per = mode >= kBlkF64 ? 1 : std::max(1, per_cu);
+    const bool two = mode == kBlkF64 || mode == kBlkEv || mode == kBlkAdj64;
+    const int per = two ? 1 : std::max(1, per_cu);
     const int grid = fit_grid(device_cus() * per, b.ntiles);
     switch (mode) {
         case kBlkF32: launch_mode<kBlkF32>(b, dpers, grid, s); break;
         case kBlkRefine: launch_mode<kBlkRefine>(b, dpers, grid, s); break;
         case kBlkF64: launch_mode<kBlkF64>(b, dpers, grid, s); break;
+        case kBlkAdj32: launch_mode<kBlkAdj32>(b, dpers, grid, s); break;
+        case kBlkAdj64: launch_mode<kBlkAdj64>(b, dpers, grid, s); break;
         default:
             hipLaunchKernelGGL((pr_sweep_blk<false, kBlkEv, false>), dim3(grid),
                                dim3(BlkShape<kBlkEv>::kThreads), 0, s, b);
@@ -1021,6 +1095,8 @@ void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
         case kBlkF32: fold_mode<kBlkF32>(b, dpers, grid, s); break;
         case kBlkRefine: fold_mode<kBlkRefine>(b, dpers, grid, s); break;
         case kBlkF64: fold_mode<kBlkF64>(b, dpers, grid, s); break;
+        case kBlkAdj32: fold_mode<kBlkAdj32>(b, dpers, grid, s); break;
+        case kBlkAdj64: fold_mode<kBlkAdj64>(b, dpers, grid, s); break;
         default:
             hipLaunchKernelGGL((pr_fold_blk<false, kBlkEv, false>), dim3(grid), dim3(kBlock), 0, s,
                                b);

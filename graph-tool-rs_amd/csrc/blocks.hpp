@@ -27,18 +27,20 @@ namespace gt {
 // refinement (FP32 contributions gathered, FP64 output), the FP64 PageRank
 // sweep (FP64 contributions, each term held in two 64-bit words: round(c
 // 2^K) and the rest at 2^(K + 38), so the sums keep about 100 bits of the
-// iterate's mass), and the eigenvector sweep (the FP64 iterate gathered, in
-// two words).
-enum : int { kBlkF32 = 0, kBlkRefine = 1, kBlkF64 = 2, kBlkEv = 3 };
+// iterate's mass), the eigenvector sweep (the FP64 iterate gathered, in two
+// words), and the adjoint's transposed sweep in FP32 (one word) and FP64
+// (two words).
+enum : int { kBlkF32 = 0, kBlkRefine = 1, kBlkF64 = 2, kBlkEv = 3, kBlkAdj32 = 4, kBlkAdj64 = 5 };
 // The two-word sweeps' low words stay below 2^62 for rows of at most this
 // many entries; a graph with a longer row runs them on the wave engine.
 constexpr uint32_t kBlkF64MaxRow = 1u << 23;
 
 // Device state of a block-sweep loop.
 struct BlkState {
-    double mass;         // L1 mass of the iterate the next sweep gathers
+    double mass;         // bound on the sum of |terms| a gathered value feeds (see blk_scale)
     uint32_t next_tile;  // tile counter of the running sweep
     uint32_t pad;        // set by blk_stage_weights on a weight it cannot use
+    unsigned long long wmax;  // largest staged weight, as its bits (adjoint's mass bound)
 };
 
 // A mass above this makes the sweep give up (SweepState::bail): the loop is
@@ -84,6 +86,12 @@ struct BlkArgs {
     // unweighted; the weighted out-degree in sweep labels.
     const void *wblk;
     const double *wdeg;
+    // The adjoint's sweep: dL/dx (sweep labels), the rows' offsets (their
+    // unweighted degree), and, for the pass that computes mu = M^T lambda
+    // instead of iterating, its output (null otherwise).
+    const double *cvec;
+    const uint32_t *row_off;
+    double *mu_out;
 };
 
 // Build (once per tile size and long-row threshold) the block tiles of R's
@@ -109,8 +117,9 @@ gt_status blk_reserve(gt_graph *g, const Blocks &B);
 // Also reports B in g->layout.
 BlkArgs blk_layout_args(gt_graph *g, const Blocks &B);
 // After the init kernel: zero the long-row sums, take the initial mass,
-// clear the bail flag.  The eigenvector's mass (sum of |y_0| * out-degree)
-// is a reduction over the iterate, in a fixed order.
+// clear the bail flag.  The eigenvector's and the adjoint's mass (sum of
+// |v| * the times v is gathered, over the start vector v = b.x_cur64, times
+// the largest weight when weighted) is a reduction in a fixed order.
 void blk_launch_start(const BlkArgs &b, hipStream_t s);
 void blk_launch_start_ev(const BlkArgs &b, hipStream_t s);
 // One sweep of `mode` (workgroups: `per_cu` per compute unit for the FP32

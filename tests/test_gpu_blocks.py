@@ -509,3 +509,68 @@ def test_negative_weight_runs_on_the_wave_engine(gpu):
     x, it = pr64(g, weight=gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
     parity(x, it, want, it_o, 1e-12)
     dev.set_tuning(block_f64=0)
+
+
+# --------------------------------------------------------------- adjoint
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-9), ("f32", 2e-4)])
+def test_adjoint_on_block_tiles(gpu, precision, tol):
+    """The adjoint's transposed sweeps and its mu pass on block tiles (the
+    default), weighted and unweighted, directed and reversed, on default and
+    1,024-entry tiles: against the oracle's FP64 adjoint and against the wave
+    engine (block_tile_nnz = -1)."""
+    n = 1 << 13
+    s, t = oracle.gen_rmat(13, 16 * n, 0.57, 0.19, 0.19, 66)
+    rng = np.random.default_rng(15)
+    w = (1.0 - 0.9 * rng.random(len(s))).astype(np.float32).astype(np.float64)
+    c = rng.standard_normal(n)
+    p = rng.random(n) + 0.1
+    p /= p.sum()
+    g = adj(n, s, t)
+    dev = g.device()
+    o = oracle.Adj(n, s, t)
+    for weighted, reversed_, tile, pers in ((True, False, 0, None), (False, False, 0, p),
+                                            (True, True, 1024, p), (False, True, 1024, None)):
+        wv = w if weighted else None
+        x, _ = o.pagerank(weight=wv, pers=pers, epsilon=1e-15, max_iter=10_000,
+                          reversed=reversed_, threads=8)
+        gw_o, gp_o, gd_o, _, _ = o.pagerank_grad(x, c, weight=wv, pers=pers, epsilon=1e-13,
+                                                 reversed=reversed_)
+        view = g.reversed() if reversed_ else g
+        wm = gt_hip.DenseProp.from_vec(g.graph_id, w) if weighted else gt_hip.Unity.NEW
+        pm = None if pers is None else gt_hip.DenseProp.from_vec(g.graph_id, pers)
+        got = {}
+        for blk in (tile, -1):
+            dev.set_tuning(block_tile_nnz=blk, block_long_min=16 if blk > 0 else 0)
+            got[blk] = gt_hip.pagerank_grad(view, wm, pm, 0.85, params(n, 1e-13, 0), x, c,
+                                            precision=precision)
+            assert (dev.layout()["block_tiles"] > 0) == (blk != -1), (blk, dev.layout())
+        dev.set_tuning(block_tile_nnz=0, block_long_min=0)
+        for gw, gp, gd, _ in got.values():
+            assert np.abs(gw - gw_o).max() <= tol * np.abs(gw_o).max()
+            assert np.abs(gp - gp_o).max() <= tol * np.abs(gp_o).max()
+            assert abs(gd - gd_o) <= tol * max(1.0, abs(gd_o))
+        if precision == "f64":
+            assert abs(got[tile][3] - got[-1][3]) <= 1
+
+
+def test_adjoint_negative_weight_runs_on_the_wave_engine(gpu):
+    """A negative weight breaks the fixed point's bound: the adjoint runs on
+    the wave engine, bit for bit as with the block tiles turned off."""
+    n = 1 << 11
+    s, t = oracle.gen_rmat(11, 16 * n, 0.57, 0.19, 0.19, 67)
+    rng = np.random.default_rng(16)
+    w = rng.uniform(0.5, 1.5, len(s))
+    w[3] = -0.125
+    c = rng.standard_normal(n)
+    x, _ = oracle.Adj(n, s, t).pagerank(weight=w, epsilon=1e-15, max_iter=10_000)
+    g = adj(n, s, t)
+    dev = g.device()
+    wm = gt_hip.DenseProp.from_vec(g.graph_id, w)
+    a = gt_hip.pagerank_grad(g, wm, None, 0.85, params(n, 1e-12, 0), x, c, precision="f64")
+    assert dev.layout()["block_tiles"] == 0
+    dev.set_tuning(block_tile_nnz=-1)
+    b = gt_hip.pagerank_grad(g, wm, None, 0.85, params(n, 1e-12, 0), x, c, precision="f64")
+    dev.set_tuning(block_tile_nnz=0)
+    assert a[3] == b[3] and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
