@@ -403,6 +403,10 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
     dtb, bo, ms_b, la_b, tsb = _timed_solves(torch, dev, _lib.GT_SWEEP_ADJOINT, args.steps,
                                              args.warmup, backward)
     ratio = dtb / dtf
+    blk_b = dev.layout()["block_tiles"] > 0
+    kb_ = ("pr_sweep_blk<false, 4, true>" if blk_b
+           else "adj_sweep_hot<true, false>" if dev.tuning()["sweep_hot_kb"] > 0
+           else "adj_sweep<float, true, false>")
     Bf = 8 * m + 4 * (n + 1) + 20 * n
     ach_b = Bf * bo[-1] * args.steps / (ms_b * 1e-3) / 1e9 if la_b else 0.0
     log(f"[bench] adjoint: forward {dtf/args.steps*1e3:.2f} ms ({fo[-1]} sweeps), backward "
@@ -419,10 +423,11 @@ def bench_adjoint(args, torch, gt_hip, _lib) -> int:
         "forward_ms": round(dtf / args.steps * 1e3, 4), "forward_sweeps": fo[-1],
         "backward_ms": round(dtb / args.steps * 1e3, 4), "backward_sweeps": bo[-1],
         "roofline": {"bound": "hbm", "achieved": round(ach_b, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach_b / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("adj_sweep_hot<true, false> (adjoint sweep, hot lambda in LDS)"
-                                if dev.tuning()["sweep_hot_kb"] > 0
-                                else "adj_sweep<float, true, false> (adjoint sweep)"),
+                     "unit": "GB/s", "frac": round(ach_b / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(f"adjoint_rmat{scale}_ef{args.edgefactor}",
+                                                 kb_)[0],
+                     "kernel": kb_ + (" (adjoint sweep on block tiles, weights in block order)"
+                                      if blk_b else " (adjoint sweep, wave engine)"),
                      "bytes_per_sweep": Bf,
                      "avg_launch_us": round(ms_b / max(la_b, 1) * 1e3, 2)},
         "grad_damping": gd.value,
