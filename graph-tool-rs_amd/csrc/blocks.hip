@@ -183,10 +183,11 @@ __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double 
 }
 
 // The eigenvector's row update, as EvFin (eigenvector.hip): row i's new
-// unscaled iterate sv = acc / sc_cur (acc sums y_{L-1} over the row), its
-// square into the norm, the delta of sweep L-1 from the row's two previous
-// iterates yc = y_{L-1}(i), yp = y_{L-2}(i), and |sv| * out-degree into the
-// mass the next launch's terms are bounded by.
+// unscaled iterate sv = acc / sc_cur (acc sums y_{L-1}, times the weight
+// when weighted, over the row), its square into the norm, the delta of
+// sweep L-1 from the row's two previous iterates yc = y_{L-1}(i), yp =
+// y_{L-2}(i), and |sv| * the times i is gathered (dg) into the mass the next
+// launch's terms are bounded by (times the largest weight, in the fold).
 __device__ __forceinline__ void blk_finish_ev(const BlkArgs &b, uint32_t i, double acc, double yc,
                                               double yp, uint32_t dg, double sc_cur,
                                               double sc_prev, double &dsum, double &ssum,
@@ -258,7 +259,6 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
     constexpr int NB = BlkShape<MODE, WGT>::kBatch;
     constexpr bool EV = MODE == kBlkEv;
     constexpr bool ADJ = MODE == kBlkAdj32 || MODE == kBlkAdj64;
-    static_assert(!(EV && WGT), "the eigenvector's block sweep is unweighted");
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
     // done), and so does the adjoint's mu pass: they do not look at the flag.
@@ -328,7 +328,7 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
                 xr[k] = b.x_cur64[i];
             else
                 xr[k] = b.x_cur[i];
-            if constexpr (WGT)
+            if constexpr (WGT && !EV)
                 dr[k] = b.wdeg[i];
             else if constexpr (ADJ)
                 dr[k] = static_cast<double>(b.row_off[i + 1] - b.row_off[i]);
@@ -536,9 +536,9 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
         const uint32_t r = b.long_rows[k];
         const unsigned long long hi = b.slots[k];
         b.slots[k] = 0ull;
-        const double dg = WGT   ? b.wdeg[r]
-                          : ADJ ? static_cast<double>(b.row_off[r + 1] - b.row_off[r])
-                                : static_cast<double>(b.odeg[r]);
+        const double dg = WGT && !EV ? b.wdeg[r]
+                          : ADJ       ? static_cast<double>(b.row_off[r + 1] - b.row_off[r])
+                                      : static_cast<double>(b.odeg[r]);
         const double pv = DPERS ? b.pers[r] : b.pval;
         if constexpr (BlkTypes<MODE>::kTwo) {
             const unsigned long long lo = b.slots_lo[k];
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
     if (EV && threadIdx.x == 0) {
         // d: delta of the previous sweep; s: squared norm of this one.
         ev_close(b.ev, s, d, b.eps_abs, b.max_iter);
-        b.bs->mass = ms;
+        b.bs->mass = b.wblk ? ms * __longlong_as_double(static_cast<long long>(b.bs->wmax)) : ms;
         b.bs->next_tile = 0;
     }
     if (ADJ && threadIdx.x == 0) {
@@ -1068,8 +1068,7 @@ void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStr
         case kBlkAdj32: launch_mode<kBlkAdj32>(b, dpers, grid, s); break;
         case kBlkAdj64: launch_mode<kBlkAdj64>(b, dpers, grid, s); break;
         default:
-            hipLaunchKernelGGL((pr_sweep_blk<false, kBlkEv, false>), dim3(grid),
-                               dim3(BlkShape<kBlkEv>::kThreads), 0, s, b);
+            launch_mode<kBlkEv>(b, false, grid, s);
             break;
     }
 }
@@ -1098,8 +1097,7 @@ void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
         case kBlkAdj32: fold_mode<kBlkAdj32>(b, dpers, grid, s); break;
         case kBlkAdj64: fold_mode<kBlkAdj64>(b, dpers, grid, s); break;
         default:
-            hipLaunchKernelGGL((pr_fold_blk<false, kBlkEv, false>), dim3(grid), dim3(kBlock), 0, s,
-                               b);
+            fold_mode<kBlkEv>(b, false, grid, s);
             break;
     }
 }

@@ -266,10 +266,6 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     }
     GT_TRY(ensure_schedule(g, *pull, n));
     note_layout(g, *pull);
-    if (weighted) {
-        GT_TRY(stage_pull_weights<double>(g, weight, *pull, wsrc, wdt, ws.wcsc64));
-        wp = ws.wcsc64.as<double>();
-    }
     GT_TRY(prof_prepare(g));
     GT_TRY(ws.partials.reserve(sizeof(double) * kPartials));
     GT_TRY(ws.long_part.reserve(sizeof(double) * std::max<uint32_t>(1, pull->nslots)));
@@ -300,18 +296,23 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
     uint64_t T = 0;
     double *out = y[0];
     bool on_blocks = false;
-    if (!skip && R && !weighted && g->tune.block_tile_nnz > 0) {
-        // Block tiles (blocks.hpp): the unweighted sweep of a relabelled
-        // graph, its row sums in two-word fixed point.  The loop runs on the
-        // wave engine instead when a row is too long for the low words, or
-        // when a launch meets an iterate whose terms the fixed point cannot
-        // hold (its mass not finite or above kBlkMassMax).
+    if (!skip && R && g->tune.block_tile_nnz > 0) {
+        // Block tiles (blocks.hpp): the sweep of a relabelled graph, its row
+        // sums in two-word fixed point, weighted by the map in block order.
+        // The loop runs on the wave engine instead when a row is too long
+        // for the low words, a weight is negative or not finite, or a launch
+        // meets an iterate whose terms the fixed point cannot hold (its mass
+        // not finite or above kBlkMassMax).
         Blocks *B = nullptr;
         GT_TRY(ensure_blocks(g, *R, (uint32_t)g->tune.block_tile_nnz,
                              (uint32_t)g->tune.block_long_min, &B));
-        if (B->max_row <= kBlkF64MaxRow) {
+        bool usable = B->max_row <= kBlkF64MaxRow;
+        if (usable && weighted)
+            GT_TRY(blk_stage_weights(g, *B, weight, wsrc, wdt, true, ws.blk_w64, &usable));
+        if (usable) {
             GT_TRY(blk_reserve(g, *B));
             BlkArgs b = blk_layout_args(g, *B);
+            if (weighted) b.wblk = ws.blk_w64.ptr;
             b.st = reinterpret_cast<SweepState *>(st);
             b.ev = st;
             b.eps_abs = eps;
@@ -349,6 +350,11 @@ gt_status eigenvector_impl(gt_graph *g, const gt_view *view, const gt_prop *weig
                 GT_HIP(hipGetLastError());
             }
         }
+    }
+    if (!skip && !on_blocks && weighted) {
+        // The wave engine's weights, in its packed order.
+        GT_TRY(stage_pull_weights<double>(g, weight, *pull, wsrc, wdt, ws.wcsc64));
+        wp = ws.wcsc64.as<double>();
     }
     if (!skip && !on_blocks) {
         EvArgs a{};

@@ -421,9 +421,9 @@ typedef struct gt_tuning {
                                     or 0 / -1 for off (off) */
     int32_t sliced_share;        /* percent of the sliced sweep's workgroups given to the warm
                                     slices: 1..99 (0: from a cost model) */
-    int32_t block_tile_nnz;      /* PageRank and eigenvector sweeps on a relabelled, unfiltered
-                                    graph (PageRank: unweighted, or with finite non-negative
-                                    weights; eigenvector: unweighted): entries per block tile
+    int32_t block_tile_nnz;      /* PageRank, adjoint and eigenvector sweeps on a relabelled,
+                                    unfiltered graph, unweighted or with finite non-negative
+                                    weights: entries per block tile
                                     (rows sorted together by source, summed in 64-bit fixed
                                     point); -1 turns the block sweeps off and the wave engine
                                     runs: -1, 1024..1048576 (131072) */

@@ -574,3 +574,44 @@ def test_adjoint_negative_weight_runs_on_the_wave_engine(gpu):
     dev.set_tuning(block_tile_nnz=0)
     assert a[3] == b[3] and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert a[2] == b[2]
+
+
+def ev_w(g, eps, max_iter, weight):
+    xm = gt_hip.DenseProp.new(g.graph_id)
+    eig, it = gt_hip.eigenvector(g, weight, params(g.vertex_bound(), eps, max_iter), xm)
+    return np.asarray(xm.data).copy(), eig, it
+
+
+def test_weighted_eigenvector_on_block_tiles(gpu):
+    """The weighted eigenvector on block tiles against the oracle at the
+    drop-in's bar and against the wave engine; unit weights give the Unity
+    solve bit for bit; a negative weight runs on the wave engine."""
+    n, S, T = sym_rmat(13, 68)
+    rng = np.random.default_rng(17)
+    w = 0.1 + 0.9 * rng.random(len(S))
+    g = adj(n, S, T)
+    dev = g.device()
+    o = oracle.Adj(n, S, T)
+    for eps, mi in ((1e-30, 3), (1e-10, 1000)):
+        want, eo, ito = o.eigenvector(weight=w, epsilon=eps, max_iter=mi, threads=8)
+        x, eig, it = ev_w(g, eps, mi, gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
+        assert dev.layout()["block_tiles"] > 0
+        dev.set_tuning(block_tile_nnz=-1)
+        xw, eigw, itw = ev_w(g, eps, mi, gt_hip.DenseProp.from_vec(g.graph_id, w.copy()))
+        dev.set_tuning(block_tile_nnz=0)
+        parity(x, it, want, ito, eps)
+        assert abs(eig - eo) <= max(1e-12, eps) * eo
+        assert abs(it - itw) <= 1
+        if it == itw:
+            assert np.abs(x - xw).sum() <= 1e-12
+    a, ea, ia = ev(g, 1e-10, 1000)
+    b, eb, ib = ev_w(g, 1e-10, 1000, gt_hip.DenseProp.from_vec(g.graph_id, np.ones(len(S))))
+    assert ia == ib and ea == eb and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    wn = w.copy()
+    wn[11] = -0.5
+    x, eig, it = ev_w(g, 1e-10, 1000, gt_hip.DenseProp.from_vec(g.graph_id, wn.copy()))
+    assert dev.layout()["block_tiles"] == 0
+    dev.set_tuning(block_tile_nnz=-1)
+    xw, eigw, itw = ev_w(g, 1e-10, 1000, gt_hip.DenseProp.from_vec(g.graph_id, wn.copy()))
+    dev.set_tuning(block_tile_nnz=0)
+    assert it == itw and eig == eigw and np.array_equal(x.view(np.uint64), xw.view(np.uint64))
