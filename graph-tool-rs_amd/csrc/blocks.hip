@@ -115,16 +115,20 @@ template <> struct BlkTypes<kBlkEv> : BlkTypes<kBlkF64> {};
 template <> struct BlkTypes<kBlkAdj32> : BlkTypes<kBlkF32> {};
 template <> struct BlkTypes<kBlkAdj64> : BlkTypes<kBlkF64> {};
 
+// The sweep is compiled for four wavefronts per SIMD (at most 128 VGPRs):
+// two workgroups of 8 wavefronts per CU, or one of 16.
 template <int MODE, bool WGT = false> struct BlkShape {
     static constexpr int kWaves = BlkTypes<MODE>::kTwo ? 16 : kBlkWaves;
     static constexpr int kThreads = kWaves * kWave;
-    // Blocks per wavefront batch.  A weighted two-word sweep holds each
-    // lane's FP64 weights beside its terms: half the batch keeps it within
-    // the 128 VGPRs of 16 wavefronts.  (Its shape must be the unweighted
+    // Blocks per wavefront batch.  A weighted sweep holds each lane's
+    // weights beside its terms: half the batch keeps it within 128 VGPRs
+    // without spills (weighted FP32 sweep at R-MAT 22: 172.5 us, against
+    // 178.9 with the full batch and 36 bytes of spills, 179.5 at three
+    // wavefronts per SIMD).  Its thread count must be the unweighted
     // sweep's: the rows each thread finishes, and so the order of the
-    // partial sums, follow the thread count, and unit weights must give the
-    // unweighted result bit for bit.)
-    static constexpr int kBatch = BlkTypes<MODE>::kTwo && WGT ? 2 : kBlkBatch;
+    // partial sums, follow it, and unit weights must give the unweighted
+    // result bit for bit.
+    static constexpr int kBatch = WGT ? 2 : kBlkBatch;
     static constexpr int kRows = (kBlkRows + kThreads - 1) / kThreads;  // finisher rows a thread
 };
 
@@ -250,7 +254,8 @@ __device__ __forceinline__ unsigned long long sum16(unsigned long long v) {
 // dangling and mass partials go to its own entries of b.part, so the fold
 // sums them in tile order whichever workgroup took the tile.
 template <bool DPERS, int MODE, bool WGT>
-__global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(const BlkArgs b) {
+__global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) __attribute__((amdgpu_waves_per_eu(4)))
+void pr_sweep_blk(const BlkArgs b) {
     typedef BlkTypes<MODE> M;
     typedef typename M::Wt Wt;
     constexpr int NW = BlkShape<MODE, WGT>::kWaves;
@@ -323,6 +328,9 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
         double pr[NR];
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
+            // A wavefront whose 64 rows all lie past the tile loads nothing
+            // (the tiles of the hub rows hold a few dozen rows).
+            if (T.r0 + (tid & ~(kWave - 1u)) + k * NT >= T.r1) continue;
             const uint32_t i = min(T.r0 + tid + k * NT, b.n - 1);
             if constexpr (M::kTwo)
                 xr[k] = b.x_cur64[i];
@@ -356,6 +364,7 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
                 const Wt *wb = static_cast<const Wt *>(b.wblk);
 #pragma unroll
                 for (int j = 0; j < NB; ++j) {
+                    if (qi + j * NW >= nq) continue;
                     const uint32_t q = min(q0 + qi + j * NW, b.qlast);
                     const Wt *at4 = wb + ((size_t)q * kWave + lane) * kBlkQuad;
                     if constexpr (sizeof(Wt) == 4) {
@@ -377,6 +386,9 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
             }
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
+                // Blocks past the tile's last gather nothing (the last batch
+                // of a wavefront is often partial).
+                if (qi + j * NW >= nq) continue;
 #pragma unroll
                 for (int k = 0; k < kBlkQuad; ++k) {
                     const uint32_t word = W[j][k];
@@ -390,6 +402,8 @@ __global__ __launch_bounds__((BlkShape<MODE, WGT>::kThreads)) void pr_sweep_blk(
             }
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
+                // The next batch's block j, when the tile has it.
+                if (qi + (NB + j) * NW >= nq) continue;
                 const uint32_t q = min(q0 + qi + (NB + j) * NW, b.qlast);
                 W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
 #pragma unroll
