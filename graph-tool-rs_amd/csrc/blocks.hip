@@ -300,10 +300,10 @@ void pr_sweep_blk(const BlkArgs b) {
         const BlkTile T = blk_tile(b.tiles, t);
         // Groups come in blocks of kBlkQuad: lane l's 16-byte load of block q
         // holds its word of each of the block's groups.  Wavefront w takes
-        // blocks w, w + 8, ..., NB blocks per batch; the next batch's
+        // blocks w, w + NW, ..., NB blocks per batch; the next batch's
         // words and bases are loaded while this batch's gathers are in
-        // flight (a load past the tile's blocks reads a clamped block and is
-        // not added).  The words are read once per sweep: non-temporal loads
+        // flight.  A batch's blocks past the tile are neither loaded nor
+        // gathered.  The words are read once per sweep: non-temporal loads
         // keep them from evicting the gathered contributions.
         const uint32_t q0 = T.g0 / kBlkQuad;
         const uint32_t nq = (T.g1 - T.g0) / kBlkQuad;
@@ -313,6 +313,7 @@ void pr_sweep_blk(const BlkArgs b) {
         uint32_t base[NB * kBlkQuad];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
+            if (w + j * NW >= nq) continue;
             const uint32_t q = min(q0 + w + j * NW, b.qlast);
             W[j] = __builtin_nontemporal_load(wq + (size_t)q * kWave + lane);
 #pragma unroll
