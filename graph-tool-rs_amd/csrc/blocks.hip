@@ -147,16 +147,16 @@ __device__ __forceinline__ BlkTile blk_tile(const BlkTile *tiles, uint32_t t) {
 }
 
 // Row i's update from its sum `acc` and its inputs (iterate xc, out-degree
-// dg, personalisation pv).  The FP32 sweep follows PrFin's FP32 finisher
+// dg, personalisation pv, damping d).  The FP32 sweep follows PrFin's FP32 finisher
 // (the teleport update in FP64, the iterate and its contribution rounded to
 // FP32), the FP64 sweep PrFin's FP64 one; both add the L1 mass of the new
 // iterate.  The refinement follows RefineFin (the FP64 update is the
 // output; delta against the FP32 iterate; no contribution, no dangling mass).
 template <int MODE>
 __device__ __forceinline__ void blk_finish(const BlkArgs &b, uint32_t i, double acc, double xc,
-                                           double dg, double pv, double p_sink, double &dsum,
-                                           double &ssum, double &msum) {
-    const double nv = (1.0 - b.damping) * pv + b.damping * (p_sink * pv + acc);
+                                           double dg, double pv, double p_sink, double d,
+                                           double &dsum, double &ssum, double &msum) {
+    const double nv = (1.0 - d) * pv + d * (p_sink * pv + acc);
     if (MODE == kBlkRefine) {
         dsum += fabs(nv - xc);
         b.x_next64[i] = nv;
@@ -210,15 +210,15 @@ __device__ __forceinline__ void blk_finish_ev(const BlkArgs &b, uint32_t i, doub
 // next launch's mass).  The pass that computes mu writes mu alone.
 template <int MODE>
 __device__ __forceinline__ void blk_finish_adj(const BlkArgs &b, uint32_t i, double acc, double lc,
-                                               double dg, double pv, double pl, double &dsum,
-                                               double &psum, double &msum) {
+                                               double dg, double pv, double pl, double d,
+                                               double &dsum, double &psum, double &msum) {
     typedef typename BlkTypes<MODE>::X V;
     const double mu = dg != 0.0 ? acc / dg : 0.0;
     if (b.mu_out) {
         b.mu_out[i] = mu;
         return;
     }
-    const double v = b.cvec[i] + b.damping * mu + (dg == 0.0 ? b.damping * pl : 0.0);
+    const double v = b.cvec[i] + d * mu + (dg == 0.0 ? d * pl : 0.0);
     const V vv = static_cast<V>(v);
     dsum += fabs(static_cast<double>(vv) - lc);
     if constexpr (BlkTypes<MODE>::kTwo)
@@ -263,11 +263,17 @@ void pr_sweep_blk(const BlkArgs b) {
     constexpr int NR = BlkShape<MODE, WGT>::kRows;
     constexpr int NB = BlkShape<MODE, WGT>::kBatch;
     constexpr bool EV = MODE == kBlkEv;
-    constexpr bool ADJ = MODE == kBlkAdj32 || MODE == kBlkAdj64;
+    constexpr bool UNEV = MODE == kBlkUFwd || MODE == kBlkUAdj;
+    constexpr bool ADJ = MODE == kBlkAdj32 || MODE == kBlkAdj64 || MODE == kBlkUAdj;
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
     // done), and so does the adjoint's mu pass: they do not look at the flag.
-    if (MODE != kBlkRefine && !(ADJ && b.mu_out) && st->done) return;
+    // A batch of graphs runs while one is iterating and none has bailed.
+    if constexpr (UNEV) {
+        if (b.u.active[0] == 0 || b.u.active[1] != 0) return;
+    } else if (MODE != kBlkRefine && !(ADJ && b.mu_out) && st->done) {
+        return;
+    }
     __shared__ unsigned long long acc[kBlkLds];
     // The FP64 sweep's low words (a dummy otherwise).
     __shared__ unsigned long long acc_lo[M::kTwo ? kBlkLds : 1];
@@ -276,27 +282,52 @@ void pr_sweep_blk(const BlkArgs b) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (kWave - 1);
     const uint32_t w = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const double mass = b.bs->mass;
-    if (!(mass <= kBlkMassMax)) {
+    // A batch of graphs takes its scale, dangling mass, damping and
+    // personalisation per tile, from the tile's graph.
+    const double mass = UNEV ? 0.0 : b.bs->mass;
+    if (!UNEV && !(mass <= kBlkMassMax)) {
         if (blockIdx.x == 0 && tid == 0) {
             st->bail = 1u;
             st->done = 1u;
         }
         return;
     }
-    const int K = blk_scale(mass);
-    const double p_sink = EV ? 0.0 : st->p_sink;
+    int K = blk_scale(mass);
+    double p_sink = (EV || UNEV) ? 0.0 : st->p_sink;
+    double dmp = b.damping, pval = b.pval;
     const double sc_cur = EV ? b.ev->sc_cur : 1.0;
     const double sc_prev = EV ? b.ev->sc_prev : 1.0;
+    uint32_t *ctr = &b.bs->next_tile;
+    uint32_t nwork = b.ntiles;
+    if constexpr (UNEV) {
+        ctr = b.u.ctr;
+        nwork = __builtin_amdgcn_readfirstlane(*b.u.list_n);
+        if (blockIdx.x == 0 && tid == 0) {
+            *b.u.ctr_next = 0u;
+            *b.u.list_n_next = 0u;
+        }
+    }
     for (uint32_t i = tid; i < kBlkLds; i += NT) {
         acc[i] = 0ull;
         if (M::kTwo) acc_lo[i] = 0ull;
     }
-    if (tid == 0) s_tile[0] = atomicAdd(&b.bs->next_tile, 1u);
+    if (tid == 0) s_tile[0] = atomicAdd(ctr, 1u);
     __syncthreads();
     for (uint32_t it = 0;; ++it) {
-        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
-        if (t >= b.ntiles) break;
+        uint32_t t = __builtin_amdgcn_readfirstlane(s_tile[it & 1]);
+        if (t >= nwork) break;
+        if constexpr (UNEV) {
+            ConstU32 *lst = (ConstU32 *)b.u.list;
+            t = __builtin_amdgcn_readfirstlane(lst[t]);
+            const uint32_t gi = __builtin_amdgcn_readfirstlane(b.u.tile_graph[t]);
+            const double mg = b.u.mass[gi];
+            // A graph whose mass the fixed point cannot hold sums garbage
+            // here; its fold makes the batch bail.
+            K = mg <= kBlkMassMax ? blk_scale(mg) : 0;
+            p_sink = b.u.ps[gi];
+            dmp = b.u.damp[gi];
+            pval = 1.0 / static_cast<double>(b.u.voff[gi + 1] - b.u.voff[gi]);
+        }
         const BlkTile T = blk_tile(b.tiles, t);
         // Groups come in blocks of kBlkQuad: lane l's 16-byte load of block q
         // holds its word of each of the block's groups.  Wavefront w takes
@@ -346,7 +377,7 @@ void pr_sweep_blk(const BlkArgs b) {
             if constexpr (EV)
                 pr[k] = b.x_prev64[i];
             else
-                pr[k] = DPERS ? b.pers[i] : b.pval;
+                pr[k] = DPERS ? b.pers[i] : pval;
         }
         // The tile's long rows inside its row range are finished by the
         // fold: their row accumulators (no entry adds into them) are marked.
@@ -435,7 +466,7 @@ void pr_sweep_blk(const BlkArgs b) {
                 }
         }
         __syncthreads();
-        if (tid == 0) s_tile[(it + 1) & 1] = atomicAdd(&b.bs->next_tile, 1u);
+        if (tid == 0) s_tile[(it + 1) & 1] = atomicAdd(ctr, 1u);
         if (T.nlong) {
             // Long row j's 64 lane slots, 16 threads a row, 4 slots each.
             const uint32_t j = tid >> 4, part = tid & 15;
@@ -480,12 +511,12 @@ void pr_sweep_blk(const BlkArgs b) {
                                       msum);
                     else if constexpr (ADJ)
                         blk_finish_adj<MODE>(b, i, v, static_cast<double>(xr[k]),
-                                             static_cast<double>(dr[k]), pr[k], p_sink, dsum,
-                                             ssum, msum);
+                                             static_cast<double>(dr[k]), pr[k], p_sink, dmp,
+                                             dsum, ssum, msum);
                     else
                         blk_finish<MODE>(b, i, v, static_cast<double>(xr[k]),
-                                         static_cast<double>(dr[k]), pr[k], p_sink, dsum, ssum,
-                                         msum);
+                                         static_cast<double>(dr[k]), pr[k], p_sink, dmp, dsum,
+                                         ssum, msum);
                 }
             }
         }
@@ -563,15 +594,15 @@ __global__ __launch_bounds__(kBlock) void pr_fold_blk(const BlkArgs b) {
                 blk_finish_ev(b, r, v, b.x_cur64[r], b.x_prev64[r], b.odeg[r], sc_cur, sc_prev, d,
                               s, ms);
             else if constexpr (ADJ)
-                blk_finish_adj<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, d, s, ms);
+                blk_finish_adj<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, b.damping, d, s, ms);
             else
-                blk_finish<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, d, s, ms);
+                blk_finish<MODE>(b, r, v, b.x_cur64[r], dg, pv, p_sink, b.damping, d, s, ms);
         } else if constexpr (ADJ) {
             blk_finish_adj<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg, pv,
-                                 p_sink, d, s, ms);
+                                 p_sink, b.damping, d, s, ms);
         } else {
             blk_finish<MODE>(b, r, blk_value(hi, K), static_cast<double>(b.x_cur[r]), dg, pv,
-                             p_sink, d, s, ms);
+                             p_sink, b.damping, d, s, ms);
         }
     }
     // The mu pass has no partials and decides nothing.
@@ -672,6 +703,102 @@ __global__ __launch_bounds__(kBlock) void pr_start_blk(const BlkArgs b) {
     }
 }
 
+// The uneven batch's start: each graph's first mass, the sum over its
+// vertices of |x0| (times the times the vertex is gathered when `gathered`,
+// and then times the largest weight when weighted), one workgroup per graph
+// in a fixed order; the long-row sums cleared.
+__global__ __launch_bounds__(kBlock) void u_start_blk(const BlkArgs b, const float *x0,
+                                                      int gathered) {
+    __shared__ double red[4];
+    const BlkUneven &u = b.u;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < b.nlong; k += gridDim.x * kBlock)
+        b.slots[k] = 0ull;
+    const double wm = gathered && b.wblk
+                          ? __longlong_as_double(static_cast<long long>(b.bs->wmax))
+                          : 1.0;
+    for (uint32_t gi = blockIdx.x; gi < u.ngraphs; gi += gridDim.x) {
+        double m = 0.0;
+        for (uint32_t v = u.voff[gi] + threadIdx.x; v < u.voff[gi + 1]; v += kBlock)
+            m += fabs(static_cast<double>(x0[v])) *
+                 (gathered ? static_cast<double>(b.odeg[v]) : 1.0);
+        m = block_sum(m, red);
+        if (threadIdx.x == 0) u.mass[gi] = m * wm;
+    }
+}
+
+// The uneven batch's fold: one wavefront per graph still iterating.  It sums
+// the graph's tile partials in tile order (a strided share per lane, then the
+// wavefront's tree), finishes the graph's long rows, and decides the graph's
+// convergence as the wave engine's batch fold does: the forward stops when
+// delta < eps, the adjoint when delta < max(eps, eps_rel) ||dL/dx_g||_1, both
+// at max_iter sweeps when it is non-zero.  A graph that goes on lists its
+// tiles for the next sweep, and its mass is the new iterate's.  A mass the
+// fixed point cannot hold sets the bail word: every later launch returns,
+// and the batch runs again on the wave engine.
+template <int MODE, bool WGT>
+__global__ __launch_bounds__(kBlock) void u_fold_blk(const BlkArgs b) {
+    constexpr bool ADJ = MODE == kBlkUAdj;
+    const BlkUneven &u = b.u;
+    if (u.active[0] == 0 || u.active[1] != 0) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t gi =
+        __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
+    if (gi >= u.ngraphs || u.done[gi]) return;
+    const double mg = u.mass[gi];
+    const int K = mg <= kBlkMassMax ? blk_scale(mg) : 0;
+    const double dmp = u.damp[gi], ps = u.ps[gi];
+    const double pv = 1.0 / static_cast<double>(u.voff[gi + 1] - u.voff[gi]);
+    const uint32_t nt = b.ntiles;
+    double d = 0.0, s = 0.0, ms = 0.0;
+    for (uint32_t t = u.tb[gi] + lane; t < u.tb[gi + 1]; t += kWave) {
+        d += b.part[t];
+        s += b.part[nt + t];
+        ms += b.part[2 * nt + t];
+    }
+    for (uint32_t k = u.lb[gi] + lane; k < u.lb[gi + 1]; k += kWave) {
+        const uint32_t r = b.long_rows[k];
+        const double v = blk_value(b.slots[k], K);
+        b.slots[k] = 0ull;
+        const double dg = WGT   ? b.wdeg[r]
+                          : ADJ ? static_cast<double>(b.row_off[r + 1] - b.row_off[r])
+                                : static_cast<double>(b.odeg[r]);
+        if constexpr (ADJ)
+            blk_finish_adj<MODE>(b, r, v, static_cast<double>(b.x_cur[r]), dg, pv, ps, dmp, d, s,
+                                 ms);
+        else
+            blk_finish<MODE>(b, r, v, static_cast<double>(b.x_cur[r]), dg, pv, ps, dmp, d, s, ms);
+    }
+    d = wave_sum(d);
+    s = wave_sum(s);
+    ms = wave_sum(ms);
+    bool more = false;
+    if (lane == 0) {
+        const uint64_t it = u.iter[gi] + 1;
+        const double thr =
+            ADJ ? fmax(b.eps_abs, b.eps_rel) * (u.scale[gi] > 0.0 ? u.scale[gi] : 1.0) : b.eps_abs;
+        const double nm =
+            ADJ && WGT ? ms * __longlong_as_double(static_cast<long long>(b.bs->wmax)) : ms;
+        u.iter[gi] = it;
+        u.delta[gi] = d;
+        u.ps[gi] = s;
+        u.mass[gi] = nm;
+        if (!(nm <= kBlkMassMax)) atomicOr(u.active + 1, 1u);
+        if (!(d >= thr) || (b.max_iter > 0 && it >= b.max_iter)) {
+            u.done[gi] = 1u;
+            atomicSub(u.active, 1u);
+        } else {
+            more = true;
+        }
+    }
+    if (__ballot(more)) {
+        const uint32_t t0 = u.tb[gi], cnt = u.tb[gi + 1] - t0;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(u.list_n_next, cnt);
+        base = __shfl(base, 0, kWave);
+        for (uint32_t i = lane; i < cnt; i += kWave) u.list[base + i] = t0 + i;
+    }
+}
+
 // odeg[i] = off[i + 1] - off[i]: the out-degree in sweep labels, read by the
 // block sweeps' finishers in one load.
 __global__ __launch_bounds__(kBlock) void blk_out_degree(const uint32_t *__restrict__ off,
@@ -723,23 +850,45 @@ void parallel_for(uint32_t count, F f) {
     for (auto &th : pool) th.join();
 }
 
-}  // namespace
-
-gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
-                        Blocks **out) {
-    Blocks &B = R.blk;
-    *out = &B;
-    if (B.ready && B.tile_nnz == tile_nnz && B.long_min == long_min) return GT_OK;
+// The block tiles of orientation o into B; `gathered_off` (device, n + 1
+// offsets) gives the times each vertex is gathered as its row length there;
+// with `bounds`, no tile spans one of those rows.  Waits for the stream.
+gt_status build_blocks(gt_graph *g, const Orient &o, const uint32_t *gathered_off,
+                       uint32_t tile_nnz, uint32_t long_min, const std::vector<uint32_t> *bounds,
+                       Blocks &B) {
     B.ready = false;
     const uint32_t n = g->n;
-    const std::vector<uint32_t> &off = R.pull.h_off;
+    const std::vector<uint32_t> &off = o.h_off;
     const uint64_t m = off[n];
     std::vector<uint32_t> idx(m), eid(m);
     if (m) {
-        GT_HIP(hipMemcpy(idx.data(), R.pull.idx.ptr, sizeof(uint32_t) * m,
-                         hipMemcpyDeviceToHost));
-        GT_HIP(hipMemcpy(eid.data(), R.pull.eid.ptr, sizeof(uint32_t) * m,
-                         hipMemcpyDeviceToHost));
+        GT_HIP(hipMemcpy(idx.data(), o.idx.ptr, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+        GT_HIP(hipMemcpy(eid.data(), o.eid.ptr, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    }
+    // Each row's entries by source (stable: parallel edges keep their
+    // storage order), with their edge ids, where a row is not sorted yet (a
+    // relabelled orientation's rows are).
+    {
+        constexpr uint32_t kRowsPerTask = 4096;
+        parallel_for((n + kRowsPerTask - 1) / kRowsPerTask, [&](uint32_t k) {
+            std::vector<std::pair<uint32_t, uint32_t>> tmp;
+            const uint32_t r1 = std::min<uint32_t>(n, (k + 1) * kRowsPerTask);
+            for (uint32_t r = k * kRowsPerTask; r < r1; ++r) {
+                const uint64_t lo = off[r], hi = off[r + 1];
+                if (std::is_sorted(idx.begin() + lo, idx.begin() + hi)) continue;
+                tmp.resize(hi - lo);
+                for (uint64_t e = lo; e < hi; ++e) tmp[e - lo] = {idx[e], eid[e]};
+                std::stable_sort(tmp.begin(), tmp.end(),
+                                 [](const std::pair<uint32_t, uint32_t> &a,
+                                    const std::pair<uint32_t, uint32_t> &b) {
+                                     return a.first < b.first;
+                                 });
+                for (uint64_t e = lo; e < hi; ++e) {
+                    idx[e] = tmp[e - lo].first;
+                    eid[e] = tmp[e - lo].second;
+                }
+            }
+        });
     }
 
     // Tiles, in row order.
@@ -754,7 +903,13 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
         nl = 0;
         lfirst = (uint32_t)ltab.size();
     };
+    size_t bi = 0;
     for (uint32_t r = 0; r < n; ++r) {
+        if (bounds) {
+            bool cut = false;
+            for (; bi < bounds->size() && (*bounds)[bi] <= r; ++bi) cut |= (*bounds)[bi] == r;
+            if (cut) close(r);
+        }
         const uint64_t L = (uint64_t)off[r + 1] - off[r];
         if (L > tile_nnz) {
             close(r);
@@ -817,6 +972,10 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     const uint64_t ngroups = g0[ntiles];
     if (ngroups >= (1ull << 32) / kWave)
         return fail(GT_ERR_UNSUPPORTED, "block tiles: too many entry groups");
+    B.h_tile_row.resize(ntiles);
+    for (uint32_t t = 0; t < ntiles; ++t) B.h_tile_row[t] = ht[t].r0;
+    B.h_long_rows = long_rows;
+    B.bounds = bounds ? *bounds : std::vector<uint32_t>();
     // Padding groups (to whole blocks) hold null words over source 0.  A
     // graph without entries still gets one block: a sweep's first loads
     // read block 0 whether or not its tile has groups.
@@ -901,8 +1060,7 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     GT_TRY(B.odeg.reserve(sizeof(uint32_t) * std::max<uint32_t>(1, n)));
     if (n) {
         hipLaunchKernelGGL(blk_out_degree, dim3(std::min<uint32_t>(2048, (n + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, g->stream, R.push_off.as<uint32_t>(), n,
-                           B.odeg.as<uint32_t>());
+                           dim3(kBlock), 0, g->stream, gathered_off, n, B.odeg.as<uint32_t>());
         GT_HIP(hipGetLastError());
         GT_HIP(hipStreamSynchronize(g->stream));
     }
@@ -927,6 +1085,26 @@ gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t l
     B.long_min = long_min;
     B.ready = true;
     return GT_OK;
+}
+
+}  // namespace
+
+gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
+                        Blocks **out) {
+    Blocks &B = R.blk;
+    *out = &B;
+    if (B.ready && B.tile_nnz == tile_nnz && B.long_min == long_min) return GT_OK;
+    return build_blocks(g, R.pull, R.push_off.as<uint32_t>(), tile_nnz, long_min, nullptr, B);
+}
+
+gt_status ensure_blocks_cut(gt_graph *g, Orient &o, const uint32_t *gathered_off,
+                            uint32_t tile_nnz, uint32_t long_min,
+                            const std::vector<uint32_t> &bounds, Blocks **out) {
+    Blocks &B = o.blk;
+    *out = &B;
+    if (B.ready && B.tile_nnz == tile_nnz && B.long_min == long_min && B.bounds == bounds)
+        return GT_OK;
+    return build_blocks(g, o, gathered_off, tile_nnz, long_min, &bounds, B);
 }
 
 // A weighted sweep's weights in block order: out[k] = w(eids[k]) converted
@@ -1115,6 +1293,45 @@ void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s) {
             fold_mode<kBlkEv>(b, false, grid, s);
             break;
     }
+}
+
+void blk_launch_ustart(const BlkArgs &b, const float *x0, bool gathered, hipStream_t s) {
+    const int grid = fit_grid(2048, std::max(b.u.ngraphs, b.nlong / kBlock + 1));
+    hipLaunchKernelGGL(u_start_blk, dim3(grid), dim3(kBlock), 0, s, b, x0, gathered ? 1 : 0);
+}
+
+template <int MODE>
+void usweep_mode(const BlkArgs &b, int grid, hipStream_t s) {
+    if (b.wblk)
+        hipLaunchKernelGGL((pr_sweep_blk<false, MODE, true>), dim3(grid),
+                           dim3(BlkShape<MODE, true>::kThreads), 0, s, b);
+    else
+        hipLaunchKernelGGL((pr_sweep_blk<false, MODE, false>), dim3(grid),
+                           dim3(BlkShape<MODE, false>::kThreads), 0, s, b);
+}
+
+void blk_launch_usweep(const BlkArgs &b, int per_cu, int mode, hipStream_t s) {
+    const int grid = fit_grid(device_cus() * std::max(1, per_cu), b.ntiles);
+    if (mode == kBlkUAdj)
+        usweep_mode<kBlkUAdj>(b, grid, s);
+    else
+        usweep_mode<kBlkUFwd>(b, grid, s);
+}
+
+template <int MODE>
+void ufold_mode(const BlkArgs &b, hipStream_t s) {
+    const dim3 grid((b.u.ngraphs + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (b.wblk)
+        hipLaunchKernelGGL((u_fold_blk<MODE, true>), grid, dim3(kBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL((u_fold_blk<MODE, false>), grid, dim3(kBlock), 0, s, b);
+}
+
+void blk_launch_ufold(const BlkArgs &b, int mode, hipStream_t s) {
+    if (mode == kBlkUAdj)
+        ufold_mode<kBlkUAdj>(b, s);
+    else
+        ufold_mode<kBlkUFwd>(b, s);
 }
 
 }  // namespace gt

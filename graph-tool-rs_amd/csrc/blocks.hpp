@@ -28,9 +28,19 @@ namespace gt {
 // sweep (FP64 contributions, each term held in two 64-bit words: round(c
 // 2^K) and the rest at 2^(K + 38), so the sums keep about 100 bits of the
 // iterate's mass), the eigenvector sweep (the FP64 iterate gathered, in two
-// words), and the adjoint's transposed sweep in FP32 (one word) and FP64
-// (two words).
-enum : int { kBlkF32 = 0, kBlkRefine = 1, kBlkF64 = 2, kBlkEv = 3, kBlkAdj32 = 4, kBlkAdj64 = 5 };
+// words), the adjoint's transposed sweep in FP32 (one word) and FP64
+// (two words), and the uneven batch's forward and adjoint sweeps in FP32
+// (one word, per-graph damping, scale and loop state).
+enum : int {
+    kBlkF32 = 0,
+    kBlkRefine = 1,
+    kBlkF64 = 2,
+    kBlkEv = 3,
+    kBlkAdj32 = 4,
+    kBlkAdj64 = 5,
+    kBlkUFwd = 6,
+    kBlkUAdj = 7
+};
 // The two-word sweeps' low words stay below 2^62 for rows of at most this
 // many entries; a graph with a longer row runs them on the wave engine.
 constexpr uint32_t kBlkF64MaxRow = 1u << 23;
@@ -47,6 +57,30 @@ struct BlkState {
 // run again on the wave engine, whose FP32 sums follow the values to
 // overflow as the reference's would.
 constexpr double kBlkMassMax = 0x1p900;
+
+// A batch of graphs on tiles cut at the graphs' bounds (each tile in one
+// graph): per-graph parameters and loop state, and the list of the tiles of
+// the graphs still iterating.  Sweep k takes list entries through ctr and
+// clears ctr_next and list_n_next for sweep k + 1; the fold after it lists
+// the tiles of each graph that goes on.
+struct BlkUneven {
+    const uint32_t *tile_graph;  // [ntiles]
+    const uint32_t *voff;        // [G + 1]
+    const double *damp;          // [G]
+    double *mass;                // [G]: bound on the terms of the graph's next sweep
+    double *ps;                  // [G]: dangling mass (forward) / p . lambda (adjoint)
+    double *delta;               // [G]
+    const double *scale;         // [G]: adjoint, ||dL/dx_g||_1
+    uint64_t *iter;              // [G]
+    uint32_t *done;              // [G]
+    uint32_t *active;            // graphs still iterating; bail right after it
+    const uint32_t *tb, *lb;     // [G + 1]: first tile and first long row of each graph
+    uint32_t *list;
+    const uint32_t *list_n;
+    uint32_t *list_n_next;
+    uint32_t *ctr, *ctr_next;
+    uint32_t ngraphs;
+};
 
 struct BlkArgs {
     const BlkTile *tiles;
@@ -92,12 +126,21 @@ struct BlkArgs {
     const double *cvec;
     const uint32_t *row_off;
     double *mu_out;
+    BlkUneven u;                // the uneven batch's sweeps, else zero
 };
 
 // Build (once per tile size and long-row threshold) the block tiles of R's
 // pull orientation.  Waits for the stream.
 gt_status ensure_blocks(gt_graph *g, Relabeled &R, uint32_t tile_nnz, uint32_t long_min,
                         Blocks **out);
+// The block tiles of orientation o cut at `bounds` as well (no tile holds
+// rows on both sides of a bound), built once per tile size, threshold and
+// bounds into o.blk.  `gathered_off` (device, n + 1) gives the times each
+// vertex is gathered as its row length.  Rows need not be sorted.  Waits
+// for the stream.
+gt_status ensure_blocks_cut(gt_graph *g, Orient &o, const uint32_t *gathered_off,
+                            uint32_t tile_nnz, uint32_t long_min,
+                            const std::vector<uint32_t> &bounds, Blocks **out);
 // A weighted solve's weights in block order into `out` (FP64 when `f64`,
 // else FP32), from the map's device source (stage_edge_source).  Sets
 // *usable to whether every weight is finite and non-negative: the block
@@ -128,5 +171,12 @@ void blk_launch_start_ev(const BlkArgs &b, hipStream_t s);
 void blk_launch_sweep(const BlkArgs &b, bool dpers, int per_cu, int mode, hipStream_t s);
 // Completes a sweep: long rows, partials in tile order, convergence.
 void blk_launch_fold(const BlkArgs &b, bool dpers, int mode, hipStream_t s);
+// The uneven batch (mode kBlkUFwd or kBlkUAdj): each graph's first mass
+// from its start iterate x0 (sum of |x0| times the gathered count when
+// `gathered`, times the largest weight when b.wblk), long-row sums cleared;
+// then one sweep, and the per-graph fold (one wavefront a graph).
+void blk_launch_ustart(const BlkArgs &b, const float *x0, bool gathered, hipStream_t s);
+void blk_launch_usweep(const BlkArgs &b, int per_cu, int mode, hipStream_t s);
+void blk_launch_ufold(const BlkArgs &b, int mode, hipStream_t s);
 
 }  // namespace gt

@@ -123,6 +123,59 @@ struct BcSchedule {
     bool ready = false;
 };
 
+// Block tiles of an orientation (the block sweeps: a relabelled pull
+// orientation, or an orientation cut at the bounds of a batch of graphs):
+// each tile is a run of rows [r0, r1) with at most tile_nnz entries, its
+// entries sorted by source across all its rows and cut into groups of 64
+// (one gather instruction), padded to whole blocks of kBlkQuad groups; a
+// block's words are stored lane-major (lane l's words of its groups side by
+// side).  A word is ((source - gbase[group]) << 13) |
+// at, where `at` is the LDS accumulator the entry adds into: the row's,
+// r - r0 < kBlkRows, for a short row; for one of the tile's long rows (more
+// than long_min entries, at most kBlkLongMax per tile, the j-th) the slot of
+// the entry's lane in that row's lane slots, kBlkAcc + 64 j + (lane + j) %
+// 64; kBlkNull (a dummy) for a padding entry.  A row longer than tile_nnz is
+// cut into chunk tiles of its own (r0 == r1).  Long rows are finished by the
+// fold from int64 slots (long_rows[k], slot k; ltab lists each tile's).
+constexpr uint32_t kBlkCodeBits = 13;
+constexpr uint32_t kBlkRows = 4032;      // rows in a tile's range
+constexpr uint32_t kBlkAcc = 4096;       // row accumulators and dummies, then lane slots
+constexpr uint32_t kBlkNull = 4095;      // accumulator of the padding entries (a dummy)
+constexpr uint32_t kBlkLongMax = 32;     // long rows per tile
+constexpr uint32_t kBlkSpan = 1u << (32 - kBlkCodeBits);  // source span of a group
+constexpr uint32_t kBlkQuad = 4;         // groups per block (one 16-byte load a lane)
+constexpr uint32_t kBlkTileNnzDefault = 131072;
+constexpr uint32_t kBlkLongMinDefault = 2048;
+
+struct BlkTile {
+    uint32_t r0, r1;      // rows finished by the tile (long rows excepted)
+    uint32_t g0, g1;      // groups of 64 words
+    uint32_t lfirst;      // first entry of ltab
+    uint32_t nlong;       // long rows
+    uint32_t pad[2];
+};
+
+struct Blocks {
+    bool ready = false;
+    uint32_t tile_nnz = 0, long_min = 0;
+    DevBuf tiles;        // BlkTile[ntiles]
+    DevBuf words;        // uint32[ngroups * 64]
+    DevBuf gbase;        // uint32[ngroups]
+    DevBuf ltab;         // uint32[]: long-row slot of each tile's long code
+    DevBuf long_rows;    // uint32[nlong]
+    DevBuf odeg;         // uint32[n]: times each vertex is gathered (a pull
+                         // orientation's out-degree), in sweep labels
+    DevBuf eids;         // uint32[ngroups * 64]: edge id of each word (0 for padding)
+    uint32_t ntiles = 0, nlong = 0;
+    uint32_t max_row = 0;  // entries of the longest row
+    uint64_t ngroups = 0;
+    // Tiles cut at these row bounds as well (a batch of graphs), with the
+    // host copies the batch needs: each tile's first row (a chunk tile's
+    // row) and the long rows.
+    std::vector<uint32_t> bounds;
+    std::vector<uint32_t> h_tile_row, h_long_rows;
+};
+
 // One orientation of the adjacency: row r lists the entries a kernel pulls
 // from (in-halves for the CSC, out-halves for the CSR), in storage order.
 struct Orient {
@@ -147,6 +200,7 @@ struct Orient {
     std::vector<Tile> h_tiles;
     std::vector<uint32_t> h_long_rows;
     Schedule mm;             // SpMM schedule (batched personalisation)
+    Blocks blk;              // block tiles cut at graph bounds (uneven batch)
     // Rows for the wavefront (mid) and workgroup (heavy) degree sums.
     DevBuf heavy, mid;
     uint32_t nheavy = 0, nmid = 0;
@@ -202,51 +256,6 @@ struct Sliced {
     uint32_t heavy_wgs = 0;
 };
 
-// Block tiles of a relabelled pull orientation (the FP32 PageRank sweeps):
-// each tile is a run of rows [r0, r1) with at most tile_nnz entries, its
-// entries sorted by source across all its rows and cut into groups of 64
-// (one gather instruction), padded to whole blocks of kBlkQuad groups; a
-// block's words are stored lane-major (lane l's words of its groups side by
-// side).  A word is ((source - gbase[group]) << 13) |
-// at, where `at` is the LDS accumulator the entry adds into: the row's,
-// r - r0 < kBlkRows, for a short row; for one of the tile's long rows (more
-// than long_min entries, at most kBlkLongMax per tile, the j-th) the slot of
-// the entry's lane in that row's lane slots, kBlkAcc + 64 j + (lane + j) %
-// 64; kBlkNull (a dummy) for a padding entry.  A row longer than tile_nnz is
-// cut into chunk tiles of its own (r0 == r1).  Long rows are finished by the
-// fold from int64 slots (long_rows[k], slot k; ltab lists each tile's).
-constexpr uint32_t kBlkCodeBits = 13;
-constexpr uint32_t kBlkRows = 4032;      // rows in a tile's range
-constexpr uint32_t kBlkAcc = 4096;       // row accumulators and dummies, then lane slots
-constexpr uint32_t kBlkNull = 4095;      // accumulator of the padding entries (a dummy)
-constexpr uint32_t kBlkLongMax = 32;     // long rows per tile
-constexpr uint32_t kBlkSpan = 1u << (32 - kBlkCodeBits);  // source span of a group
-constexpr uint32_t kBlkQuad = 4;         // groups per block (one 16-byte load a lane)
-constexpr uint32_t kBlkTileNnzDefault = 131072;
-constexpr uint32_t kBlkLongMinDefault = 2048;
-
-struct BlkTile {
-    uint32_t r0, r1;      // rows finished by the tile (long rows excepted)
-    uint32_t g0, g1;      // groups of 64 words
-    uint32_t lfirst;      // first entry of ltab
-    uint32_t nlong;       // long rows
-    uint32_t pad[2];
-};
-
-struct Blocks {
-    bool ready = false;
-    uint32_t tile_nnz = 0, long_min = 0;
-    DevBuf tiles;        // BlkTile[ntiles]
-    DevBuf words;        // uint32[ngroups * 64]
-    DevBuf gbase;        // uint32[ngroups]
-    DevBuf ltab;         // uint32[]: long-row slot of each tile's long code
-    DevBuf long_rows;    // uint32[nlong]
-    DevBuf odeg;         // uint32[n]: out-degree in sweep labels
-    DevBuf eids;         // uint32[ngroups * 64]: edge id of each word (0 for padding)
-    uint32_t ntiles = 0, nlong = 0;
-    uint32_t max_row = 0;  // entries of the longest row
-    uint64_t ngroups = 0;
-};
 
 // A pull orientation in a locality-friendly labelling: vertices renumbered by
 // descending gather frequency (the row length of the opposite orientation:
@@ -298,7 +307,7 @@ struct Workspace {
     DevBuf blk_state;   // block sweeps: BlkState
     DevBuf blk_w, blk_w64;  // weighted block sweeps: weights in block order (f32, f64)
     DevBuf masks;       // staged vertex + edge masks
-    DevBuf ub[12];      // uneven batch: per-call graph, tile and state arrays
+    DevBuf ub[16];      // uneven batch: per-call graph, tile and state arrays
     DevBuf btw[10];     // betweenness: [n][64] rows, lane masks, accumulators, pieces
     DevBuf btwx[3][9];  // betweenness: the other streams' batch state and accumulators
     Orient fpull;       // filtered view: compacted pull orientation

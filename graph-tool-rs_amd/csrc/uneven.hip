@@ -31,8 +31,10 @@
 // the list's order changes no result.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
+#include "blocks.hpp"
 #include "device_common.hpp"
 #include "gt_internal.hpp"
 #include "props.hpp"
@@ -464,8 +466,25 @@ unsigned ugrid(uint64_t work) {
 
 // Workspace slots of the batch arrays.
 enum { kUVoff, kUDamp, kUVg, kUTg, kUTb, kULb, kUState, kUTpart, kUBad, kUSums, kUList, kUListN,
-       kUCount };
+       kUBTg, kUBTb, kUBLb, kUBMass, kUCount };
 static_assert(kUCount <= (int)(sizeof(Workspace::ub) / sizeof(DevBuf)), "workspace slots");
+
+// Per-graph loop state back to the start: every non-empty graph active, the
+// bail word (right after the active count) clear.  Waits for the stream.
+static_assert(offsetof(SweepState, launches) == offsetof(SweepState, counter) + 4,
+              "the active count and the bail word are read into adjacent fields");
+gt_status u_restart(gt_graph *g, uint32_t G, const uint32_t *voff, const UArgs &a) {
+    hipStream_t s = g->stream;
+    hipLaunchKernelGGL(u_reset, dim3(ugrid(G)), dim3(kBlock), 0, s, a);
+    uint32_t live = 0;
+    for (uint32_t k = 0; k < G; ++k) live += voff[k + 1] > voff[k];
+    g->h_state->counter = live;
+    g->h_state->launches = 0;
+    GT_HIP(hipMemcpyAsync(a.active, &g->h_state->counter, 2 * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    GT_HIP(hipStreamSynchronize(s));
+    return GT_OK;
+}
 
 // The device description of a batch over one orientation `o` (pull for the
 // forward, push for the adjoint): validation, per-vertex graph ids, the
@@ -584,13 +603,7 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
         if (bad) return fail(GT_ERR_INVALID_ARG, "an edge joins two graphs of the batch");
         g->batch_checked = bounds;
     }
-    hipLaunchKernelGGL(u_reset, dim3(ugrid(G)), dim3(kBlock), 0, s, a);
-    uint32_t live = 0;
-    for (uint32_t k = 0; k < G; ++k) live += voff[k + 1] > voff[k];
-    g->h_state->counter = live;
-    GT_HIP(hipMemcpyAsync(a.active, &g->h_state->counter, sizeof(uint32_t),
-                          hipMemcpyHostToDevice, s));
-    GT_HIP(hipStreamSynchronize(s));
+    GT_TRY(u_restart(g, G, voff, a));
     GT_HIP(hipGetLastError());
     return GT_OK;
 }
@@ -599,10 +612,12 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
 // a HIP graph on the graph's own stream (never the legacy default stream),
 // replayed.  x[par] / x[par ^ 1] alternate, so a graph's latest iterate is
 // x[iter_g & 1].  `sweep(args, k)` and `fold(args)` launch one pair on `cs`.
+// Ends early, with *bailed set, when a block sweep's fold raised the bail
+// word (the word after the active count).
 template <typename Sweep, typename Fold>
 gt_status u_iterate(gt_graph *g, UArgs a, const float *xs[2], float *xn[2], float *cn[2],
                     const float *srcs[2], uint64_t max_iter, Sweep sweep, Fold fold,
-                    uint64_t *replays_out) {
+                    uint64_t *replays_out, bool *bailed = nullptr) {
     hipStream_t cs = g->own_stream;
     if (g->stream != cs) GT_HIP(hipStreamSynchronize(g->stream));
     hipGraph_t graph = nullptr;
@@ -640,7 +655,7 @@ gt_status u_iterate(gt_graph *g, UArgs a, const float *xs[2], float *xn[2], floa
     for (;;) {
         e = hipGraphLaunch(exec, cs);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&g->h_state->counter, a.active, sizeof(uint32_t),
+            e = hipMemcpyAsync(&g->h_state->counter, a.active, 2 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, cs);
         if (e == hipSuccess) e = hipStreamSynchronize(cs);
         if (e != hipSuccess) {
@@ -648,6 +663,10 @@ gt_status u_iterate(gt_graph *g, UArgs a, const float *xs[2], float *xn[2], floa
             break;
         }
         ++r;
+        if (g->h_state->launches != 0) {
+            if (bailed) *bailed = true;
+            break;
+        }
         if (g->h_state->counter == 0) break;
         if (r >= limit) {
             st = fail(GT_ERR_INVALID_ARG, "uneven batch: no convergence within 2^23 sweeps");
@@ -660,19 +679,145 @@ gt_status u_iterate(gt_graph *g, UArgs a, const float *xs[2], float *xn[2], floa
     return st;
 }
 
-gt_status u_weights(gt_graph *g, const gt_prop *weight, Orient &pull, Orient &push, UArgs &a,
-                    bool *weighted, DevBuf &wbuf) {
-    *weighted = weight && weight->kind != GT_MAP_UNITY;
-    a.deg_off = push.off.as<uint32_t>();
-    if (!*weighted) return GT_OK;
+// The weight map's device source and the weighted out-degree (push: the
+// out-halves); the sweep's weights are staged for the layout that runs.
+struct UWeights {
+    bool on = false;
     const void *src = nullptr;
-    int dt = GT_F64;
-    GT_TRY(stage_edge_source(g, weight, &src, &dt));
-    GT_TRY(stage_degree(g, weight, push, src, dt, g->ws.deg));
-    GT_TRY(stage_pull_weights<float>(g, weight, pull, src, dt, wbuf));
+    int dtype = GT_F64;
+};
+
+gt_status u_weights(gt_graph *g, const gt_prop *weight, Orient &push, UArgs &a, UWeights *w) {
+    w->on = weight && weight->kind != GT_MAP_UNITY;
+    a.deg_off = push.off.as<uint32_t>();
+    if (!w->on) return GT_OK;
+    GT_TRY(stage_edge_source(g, weight, &w->src, &w->dtype));
+    GT_TRY(stage_degree(g, weight, push, w->src, w->dtype, g->ws.deg));
     a.deg = g->ws.deg.as<double>();
+    return GT_OK;
+}
+
+// The wave engine's weights: orientation o's in its packed order.
+gt_status u_wave_weights(gt_graph *g, const gt_prop *weight, Orient &o, const UWeights &w,
+                         UArgs &a, DevBuf &wbuf) {
+    if (!w.on || a.w) return GT_OK;
+    GT_TRY(stage_pull_weights<float>(g, weight, o, w.src, w.dtype, wbuf));
     a.w = wbuf.as<float>();
     return GT_OK;
+}
+
+// The batch on block tiles of orientation o cut at the graphs' bounds
+// (`gathered`: device offsets whose row lengths count how often each vertex
+// is gathered), unless the tuning turns them off: the per-tile graph, each
+// graph's first tile and first long row, the weights in block order, and
+// the sweep arguments.  *ok is false when the batch runs on the wave engine:
+// block tiles off, or a weight negative or not finite.
+gt_status u_blk_setup(gt_graph *g, uint32_t G, const uint32_t *voff, Orient &o,
+                      const uint32_t *gathered, const gt_prop *weight, const UWeights &w,
+                      UArgs &a, BlkArgs &bk, bool *ok) {
+    *ok = false;
+    if (g->tune.block_tile_nnz <= 0) return GT_OK;
+    Workspace &ws = g->ws;
+    hipStream_t s = g->stream;
+    const std::vector<uint32_t> bounds(voff, voff + G + 1);
+    Blocks *B = nullptr;
+    GT_TRY(ensure_blocks_cut(g, o, gathered, (uint32_t)g->tune.block_tile_nnz,
+                             (uint32_t)g->tune.block_long_min, bounds, &B));
+    if (w.on) {
+        bool usable = false;
+        GT_TRY(blk_stage_weights(g, *B, weight, w.src, w.dtype, false, ws.blk_w, &usable));
+        if (!usable) return GT_OK;
+    }
+    GT_TRY(blk_reserve(g, *B));
+    const uint32_t nt = B->ntiles;
+    std::vector<uint32_t> tg(std::max<uint32_t>(1, nt), 0), tb(G + 1), lb(G + 1);
+    for (uint32_t t = 0; t < nt; ++t)
+        tg[t] = (uint32_t)(std::upper_bound(voff, voff + G + 1, B->h_tile_row[t]) - voff) - 1;
+    const std::vector<uint32_t> &LR = B->h_long_rows;
+    for (uint32_t k = 0; k <= G; ++k) {
+        tb[k] = (uint32_t)(std::lower_bound(tg.begin(), tg.begin() + nt, k) - tg.begin());
+        lb[k] = (uint32_t)(std::lower_bound(LR.begin(), LR.end(), voff[k]) - LR.begin());
+    }
+    DevBuf *U = ws.ub;
+    GT_TRY(U[kUBTg].reserve(sizeof(uint32_t) * tg.size()));
+    GT_TRY(U[kUBTb].reserve(sizeof(uint32_t) * (G + 1)));
+    GT_TRY(U[kUBLb].reserve(sizeof(uint32_t) * (G + 1)));
+    GT_TRY(U[kUBMass].reserve(sizeof(double) * G + 2 * sizeof(uint32_t)));
+    if (U[kUList].bytes < sizeof(uint32_t) * std::max<uint32_t>(1, nt)) {
+        GT_TRY(U[kUList].reserve(sizeof(uint32_t) * std::max<uint32_t>(1, nt)));
+        a.list = U[kUList].as<uint32_t>();
+    }
+    GT_HIP(hipMemcpyAsync(U[kUBTg].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
+                          hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync(U[kUBTb].ptr, tb.data(), sizeof(uint32_t) * (G + 1),
+                          hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync(U[kUBLb].ptr, lb.data(), sizeof(uint32_t) * (G + 1),
+                          hipMemcpyHostToDevice, s));
+    bk = blk_layout_args(g, *B);
+    BlkUneven &u = bk.u;
+    u.tile_graph = U[kUBTg].as<uint32_t>();
+    u.voff = a.voff;
+    u.damp = a.damp;
+    u.mass = U[kUBMass].as<double>();
+    u.ps = a.st.p_sink;
+    u.delta = a.st.delta;
+    u.scale = a.st.scale;
+    u.iter = a.st.iter;
+    u.done = a.st.done;
+    u.active = a.active;
+    u.tb = U[kUBTb].as<uint32_t>();
+    u.lb = U[kUBLb].as<uint32_t>();
+    u.list = a.list;
+    u.ngraphs = G;
+    // The two tile counters, one per sweep parity, after the masses.
+    u.ctr = reinterpret_cast<uint32_t *>(u.mass + G);
+    bk.eps_abs = a.eps;
+    bk.eps_rel = a.eps_rel;
+    bk.max_iter = a.max_iter;
+    bk.cvec = a.cvec;
+    bk.row_off = a.deg_off;
+    if (w.on) {
+        bk.wblk = ws.blk_w.ptr;
+        bk.wdeg = a.deg;
+    }
+    *ok = true;
+    return GT_OK;
+}
+
+// Runs the batch's loop on block tiles (`mode` kBlkUFwd / kBlkUAdj) from the
+// start iterate the caller wrote into xs[0]; *bailed when a graph's mass
+// left the fixed point's range.
+gt_status u_blk_iterate(gt_graph *g, const UArgs &a, BlkArgs bk, int mode, const float *xs[2],
+                        float *xn[2], float *cn[2], const float *srcs[2], uint64_t max_iter,
+                        bool gathered, bool *bailed) {
+    hipStream_t s = g->stream;
+    blk_launch_ustart(bk, xs[0], gathered, s);
+    GT_HIP(hipMemsetAsync(bk.u.ctr, 0, 2 * sizeof(uint32_t), s));
+    GT_HIP(hipGetLastError());
+    UArgs ab = a;
+    ab.ntiles = bk.ntiles;
+    uint32_t *ctr = bk.u.ctr;
+    const int per_cu = g->tune.block_per_cu;
+    auto at = [&](const UArgs &b) {
+        BlkArgs k = bk;
+        const int par = (int)(b.list_n - a.list_count);
+        k.x_cur = b.x_cur;
+        k.x_next = b.x_next;
+        k.c_cur = b.src;
+        k.c_next = b.c_next;
+        k.u.list_n = b.list_n;
+        k.u.list_n_next = b.list_n_next;
+        k.u.ctr = ctr + par;
+        k.u.ctr_next = ctr + (par ^ 1);
+        return k;
+    };
+    uint64_t replays = 0;
+    *bailed = false;
+    return u_iterate(
+        g, ab, xs, xn, cn, srcs, max_iter,
+        [&](const UArgs &b, hipStream_t cs) { blk_launch_usweep(at(b), per_cu, mode, cs); },
+        [&](const UArgs &b, hipStream_t cs) { blk_launch_ufold(at(b), mode, cs); }, &replays,
+        bailed);
 }
 
 gt_status u_params(const gt_power_iteration *params, uint32_t n) {
@@ -702,8 +847,9 @@ gt_status pagerank_uneven_impl(gt_graph *g, uint32_t G, const uint32_t *voff,
     hipStream_t s = g->stream;
     UArgs a{};
     GT_TRY(u_setup(g, G, voff, damping, g->in, a));
-    bool weighted = false;
-    GT_TRY(u_weights(g, weight, g->in, g->out, a, &weighted, ws.wcsc));
+    UWeights wt;
+    GT_TRY(u_weights(g, weight, g->out, a, &wt));
+    const bool weighted = wt.on;
     // The FP32 change of a unit-mass iterate bottoms out near 2^-22.
     a.eps = std::max(params->epsilon, std::ldexp(1.0, -22));
     a.max_iter = params->max_iter;
@@ -714,7 +860,8 @@ gt_status pagerank_uneven_impl(gt_graph *g, uint32_t G, const uint32_t *voff,
     float *x[2] = {ws.x32[0].as<float>(), ws.x32[1].as<float>()};
     float *c[2] = {ws.c32[0].as<float>(), ws.c32[1].as<float>()};
     const uint32_t *vg = ws.ub[kUVg].as<uint32_t>();
-    {
+    // x0 = 1/n_g, c0 = x0 / deg, and each graph's first dangling mass.
+    auto init = [&]() -> gt_status {
         UArgs b = a;
         b.x_next = x[0];
         b.c_next = c[0];
@@ -727,28 +874,50 @@ gt_status pagerank_uneven_impl(gt_graph *g, uint32_t G, const uint32_t *voff,
                            FwdStartSinks{weighted ? a.deg : nullptr, a.deg_off, a.voff},
                            a.st.p_sink, a.st.scale);
         GT_HIP(hipGetLastError());
-    }
-    const int cap_w = occupancy_cap(
-        (const void *)u_sweep<UFwdFin<true>, true>, g->tune.sweep_blocks_per_cu);
-    const int cap_u = occupancy_cap(
-        (const void *)u_sweep<UFwdFin<false>, false>, g->tune.sweep_blocks_per_cu);
-    const dim3 sg(wave_grid(weighted ? cap_w : cap_u, a.ntiles)),
-        fg((unsigned)std::max<uint32_t>(1, (G + kWavesPerBlock - 1) / kWavesPerBlock)),
-        bd(kBlock);
+        return GT_OK;
+    };
     const float *xs[2] = {x[0], x[1]};
     const float *srcs[2] = {c[0], c[1]};
-    uint64_t replays = 0;
-    GT_TRY(u_iterate(
-        g, a, xs, x, c, srcs, params->max_iter,
-        [&](const UArgs &b, hipStream_t cs) {
-            if (weighted) hipLaunchKernelGGL((u_sweep<UFwdFin<true>, true>), sg, bd, 0, cs, b);
-            else hipLaunchKernelGGL((u_sweep<UFwdFin<false>, false>), sg, bd, 0, cs, b);
-        },
-        [&](const UArgs &b, hipStream_t cs) {
-            if (weighted) hipLaunchKernelGGL((u_fold<UFwdFin<true>, false>), fg, bd, 0, cs, b);
-            else hipLaunchKernelGGL((u_fold<UFwdFin<false>, false>), fg, bd, 0, cs, b);
-        },
-        &replays));
+    // Block tiles of the in-halves, cut at the graphs' bounds; the wave
+    // engine when they are off or cannot hold the batch.
+    bool on_blocks = false;
+    {
+        BlkArgs bk{};
+        bool ok = false;
+        GT_TRY(u_blk_setup(g, G, voff, g->in, g->out.off.as<uint32_t>(), weight, wt, a, bk, &ok));
+        if (ok) {
+            GT_TRY(init());
+            bool bailed = false;
+            GT_TRY(u_blk_iterate(g, a, bk, kBlkUFwd, xs, x, c, srcs, params->max_iter, false,
+                                 &bailed));
+            on_blocks = !bailed;
+            if (bailed) GT_TRY(u_restart(g, G, voff, a));
+        }
+    }
+    if (!on_blocks) {
+        note_layout(g, g->in);
+        GT_TRY(u_wave_weights(g, weight, g->in, wt, a, ws.wcsc));
+        GT_TRY(init());
+        const int cap_w = occupancy_cap(
+            (const void *)u_sweep<UFwdFin<true>, true>, g->tune.sweep_blocks_per_cu);
+        const int cap_u = occupancy_cap(
+            (const void *)u_sweep<UFwdFin<false>, false>, g->tune.sweep_blocks_per_cu);
+        const dim3 sg(wave_grid(weighted ? cap_w : cap_u, a.ntiles)),
+            fg((unsigned)std::max<uint32_t>(1, (G + kWavesPerBlock - 1) / kWavesPerBlock)),
+            bd(kBlock);
+        uint64_t replays = 0;
+        GT_TRY(u_iterate(
+            g, a, xs, x, c, srcs, params->max_iter,
+            [&](const UArgs &b, hipStream_t cs) {
+                if (weighted) hipLaunchKernelGGL((u_sweep<UFwdFin<true>, true>), sg, bd, 0, cs, b);
+                else hipLaunchKernelGGL((u_sweep<UFwdFin<false>, false>), sg, bd, 0, cs, b);
+            },
+            [&](const UArgs &b, hipStream_t cs) {
+                if (weighted) hipLaunchKernelGGL((u_fold<UFwdFin<true>, false>), fg, bd, 0, cs, b);
+                else hipLaunchKernelGGL((u_fold<UFwdFin<false>, false>), fg, bd, 0, cs, b);
+            },
+            &replays));
+    }
     hipStream_t cs = g->own_stream;
     double *out = rank;
     DevBuf host;
@@ -785,8 +954,9 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     hipStream_t s = g->stream;
     UArgs a{};
     GT_TRY(u_setup(g, G, voff, damping, g->out, a));
-    bool weighted = false;
-    GT_TRY(u_weights(g, weight, g->out, g->out, a, &weighted, ws.wcsr));
+    UWeights wt;
+    GT_TRY(u_weights(g, weight, g->out, a, &wt));
+    const bool weighted = wt.on;
     a.eps = params->epsilon;
     a.eps_rel = std::ldexp(1.0, -22);
     a.max_iter = params->max_iter;
@@ -805,11 +975,15 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     float *lam[2] = {ws.lam[0].as<float>(), ws.lam[1].as<float>()};
     const uint32_t *vg = ws.ub[kUVg].as<uint32_t>();
     double *sums = ws.ub[kUSums].as<double>();
-    hipLaunchKernelGGL(u_adj_init, dim3(ugrid(n)), dim3(kBlock), 0, s, a, n, lam[0]);
     const unsigned gg = std::min<uint32_t>(G, 65535);
-    hipLaunchKernelGGL(u_graph_sums<AdjStartSums>, dim3(gg), dim3(kBlock), 0, s, a.voff, G,
-                       AdjStartSums{cd, a.voff}, a.st.p_sink, a.st.scale);
-    GT_HIP(hipGetLastError());
+    // lambda_0 = dL/dx, and per graph p . lambda_0 and ||dL/dx_g||_1.
+    auto init = [&]() -> gt_status {
+        hipLaunchKernelGGL(u_adj_init, dim3(ugrid(n)), dim3(kBlock), 0, s, a, n, lam[0]);
+        hipLaunchKernelGGL(u_graph_sums<AdjStartSums>, dim3(gg), dim3(kBlock), 0, s, a.voff, G,
+                           AdjStartSums{cd, a.voff}, a.st.p_sink, a.st.scale);
+        GT_HIP(hipGetLastError());
+        return GT_OK;
+    };
     const int cap_w = occupancy_cap(
         (const void *)u_sweep<UAdjFin<true>, true>, g->tune.sweep_blocks_per_cu);
     const int cap_u = occupancy_cap(
@@ -818,18 +992,40 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
         fg((unsigned)std::max<uint32_t>(1, (G + kWavesPerBlock - 1) / kWavesPerBlock)),
         bd(kBlock);
     const float *ls[2] = {lam[0], lam[1]};
-    uint64_t replays = 0;
-    GT_TRY(u_iterate(
-        g, a, ls, lam, nullptr, ls, params->max_iter,
-        [&](const UArgs &b, hipStream_t cs) {
-            if (weighted) hipLaunchKernelGGL((u_sweep<UAdjFin<true>, true>), sg, bd, 0, cs, b);
-            else hipLaunchKernelGGL((u_sweep<UAdjFin<false>, false>), sg, bd, 0, cs, b);
-        },
-        [&](const UArgs &b, hipStream_t cs) {
-            if (weighted) hipLaunchKernelGGL((u_fold<UAdjFin<true>, true>), fg, bd, 0, cs, b);
-            else hipLaunchKernelGGL((u_fold<UAdjFin<false>, true>), fg, bd, 0, cs, b);
-        },
-        &replays));
+    // Block tiles of the out-halves, cut at the graphs' bounds (a vertex is
+    // gathered once per in-edge); the wave engine when they are off or
+    // cannot hold the batch.
+    bool on_blocks = false;
+    {
+        BlkArgs bk{};
+        bool ok = false;
+        GT_TRY(u_blk_setup(g, G, voff, g->out, g->in.off.as<uint32_t>(), weight, wt, a, bk, &ok));
+        if (ok) {
+            GT_TRY(init());
+            bool bailed = false;
+            GT_TRY(u_blk_iterate(g, a, bk, kBlkUAdj, ls, lam, nullptr, ls, params->max_iter, true,
+                                 &bailed));
+            on_blocks = !bailed;
+            if (bailed) GT_TRY(u_restart(g, G, voff, a));
+        }
+    }
+    if (!on_blocks) {
+        note_layout(g, g->out);
+        GT_TRY(u_wave_weights(g, weight, g->out, wt, a, ws.wcsr));
+        GT_TRY(init());
+        uint64_t replays = 0;
+        GT_TRY(u_iterate(
+            g, a, ls, lam, nullptr, ls, params->max_iter,
+            [&](const UArgs &b, hipStream_t cs) {
+                if (weighted) hipLaunchKernelGGL((u_sweep<UAdjFin<true>, true>), sg, bd, 0, cs, b);
+                else hipLaunchKernelGGL((u_sweep<UAdjFin<false>, false>), sg, bd, 0, cs, b);
+            },
+            [&](const UArgs &b, hipStream_t cs) {
+                if (weighted) hipLaunchKernelGGL((u_fold<UAdjFin<true>, true>), fg, bd, 0, cs, b);
+                else hipLaunchKernelGGL((u_fold<UAdjFin<false>, true>), fg, bd, 0, cs, b);
+            },
+            &replays));
+    }
     hipStream_t cs = g->own_stream;
     // Every graph's final lambda in one buffer (ws.y3 as f32 storage).
     GT_TRY(ws.y3.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
@@ -837,6 +1033,8 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     hipLaunchKernelGGL(u_latest<float>, dim3(ugrid(n)), dim3(kBlock), 0, cs, lam[0], lam[1], vg,
                        a.st.iter, n, lf);
     if (grad_w) {
+        // The mu pass runs on the wave engine, with its weights.
+        GT_TRY(u_wave_weights(g, weight, g->out, wt, a, ws.wcsr));
         GT_TRY(ensure_edge_list(g));
         GT_TRY(ws.mu.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
         GT_TRY(ws.arl[3].reserve(sizeof(double2) * std::max<uint32_t>(1, n)));
@@ -894,7 +1092,6 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     GT_HIP(hipStreamSynchronize(cs));
     if (grad_damping)
         for (uint32_t k = 0; k < G; ++k) grad_damping[k] = damping[k] != 0.0 ? hld[k] / damping[k] : 0.0;
-    (void)replays;
     return GT_OK;
 }
 

@@ -21,11 +21,28 @@ def _batch(G, seed, n_max=1500):
 
 
 def check_forward(o, got, it, weight, damping, eps=1e-6):
-    """Sweeps within one of the oracle's at `eps`; L1 <= 1e-6 against the
-    oracle's iterate after exactly `it` sweeps."""
+    """Sweeps within one of the oracle's at `eps`, or else a stop where the
+    oracle's own L1 change is within 10 % of eps (below 1.1 eps at sweep
+    `it`, at least 0.9 eps at sweep it - 1); L1 <= 1e-6 against the oracle's
+    iterate after exactly `it` sweeps.
+
+    The second form is for slowly converging graphs: an FP32 iterate's L1
+    change differs from the FP64 one by rounding.  On the damping-0.95 hub
+    graph of the edge-case batch the change falls by 5 % a sweep, and the
+    FP32 iterate -- emulated on the host with exact row sums -- stops at
+    sweep 281 where the FP64 iteration stops at 283 (its change at 281:
+    1.099e-6)."""
     _, ito = o.pagerank(weight=weight, damping=damping, epsilon=eps, grain=64)
-    assert abs(it - ito) <= 1, (it, ito)
     want, _ = o.pagerank(weight=weight, damping=damping, epsilon=1e-300, max_iter=it, grain=64)
+    if abs(it - ito) > 1:
+        assert it >= 2, (it, ito)
+        w1, _ = o.pagerank(weight=weight, damping=damping, epsilon=1e-300, max_iter=it - 1,
+                           grain=64)
+        w2, _ = o.pagerank(weight=weight, damping=damping, epsilon=1e-300, max_iter=it - 2,
+                           grain=64)
+        d_it = float(np.abs(want - w1).sum())
+        d_prev = float(np.abs(w1 - w2).sum())
+        assert d_it < 1.1 * eps and d_prev >= 0.9 * eps, (it, ito, d_it, d_prev)
     l1 = float(np.abs(got - want).sum())
     assert l1 <= 1e-6, (l1, it)
     return l1
@@ -128,13 +145,23 @@ def _edge_case_batch():
     return voff, eoff, d, s, t, w, n, gt_hip.DeviceGraph.from_edges(n, s, t)
 
 
-def test_uneven_list_edge_cases(gpu):
+# Block tiles cut at the graphs' bounds (the default), small tiles with many
+# long rows and chunk tiles, and the wave engine.
+TUNINGS = {"blocks": {}, "small_blocks": {"block_tile_nnz": 1024, "block_long_min": 64},
+           "wave": {"block_tile_nnz": -1}}
+
+
+@pytest.mark.parametrize("tune", list(TUNINGS))
+def test_uneven_list_edge_cases(gpu, tune):
     """Empty graphs, a split hub row and sweep counts from a few to hundreds,
     forward and adjoint, each non-empty graph against the oracle alone."""
     voff, eoff, d, s, t, w, n, dev = _edge_case_batch()
+    dev.set_tuning(**TUNINGS[tune])
     G = len(d)
     live = [k for k in range(G) if voff[k + 1] > voff[k]]
     x, it = gt_hip.pagerank_uneven(dev, voff, d, weight=w, epsilon=1e-6)
+    lay = dev.layout()
+    assert (lay["block_tiles"] > 0) == (tune != "wave"), lay
     rng = np.random.default_rng(3)
     c = rng.standard_normal(n)
     xs = np.zeros(n)
@@ -156,3 +183,40 @@ def test_uneven_list_edge_cases(gpu):
         assert np.abs(gw[lo:hi] - gw_o).max() <= 2e-4 * np.abs(gw_o).max(), k
         assert np.abs(gp[voff[k]:voff[k + 1]] - gp_o).max() <= 2e-4 * np.abs(gp_o).max(), k
         assert abs(gd[k] - gd_o) <= 2e-4 * max(1.0, abs(gd_o)), k
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_uneven_blocks_agree_with_wave_engine(gpu, weighted):
+    """The batch on block tiles and on the wave engine: each graph's forward
+    sweep count within one (adjoint: within 2 %), its ranks within L1 2e-6
+    (both are FP32 iterates within 1e-6 of the oracle's), and the adjoint's
+    gradients within 2e-4 of the largest."""
+    G = 20
+    voff, eoff, d, s, t, w, n, dev = _batch(G, 909)
+    wm = w if weighted else None
+    rng = np.random.default_rng(11)
+    c = rng.standard_normal(n)
+    out = {}
+    for tune in ("blocks", "wave"):
+        dev.set_tuning(**TUNINGS[tune])
+        x, it = gt_hip.pagerank_uneven(dev, voff, d, weight=wm, epsilon=1e-6)
+        assert (dev.layout()["block_tiles"] > 0) == (tune == "blocks")
+        gw, gp, gd, ita = gt_hip.pagerank_uneven_grad(dev, voff, d, x.astype(np.float64), c,
+                                                      weight=wm, epsilon=1e-9)
+        out[tune] = (x, it, gw, gp, gd, ita)
+    xb, itb, gwb, gpb, gdb, itab = out["blocks"]
+    xw, itw, gww, gpw, gdw, itaw = out["wave"]
+    assert np.all(np.abs(itb.astype(np.int64) - itw.astype(np.int64)) <= 1)
+    # The adjoint stops at 2^-22 of ||dL/dx_g||_1, where the FP32 iterate's
+    # change is within rounding of the threshold for a few sweeps: counts of
+    # 150-200 sweeps differ by up to 2 between the two layouts' roundings.
+    ita_b, ita_w = itab.astype(np.int64), itaw.astype(np.int64)
+    assert np.all(np.abs(ita_b - ita_w) <= np.maximum(1, ita_w // 50)), (ita_b, ita_w)
+    for k in range(G):
+        lo, hi = int(voff[k]), int(voff[k + 1])
+        assert np.abs(xb[lo:hi] - xw[lo:hi]).sum() <= 2e-6, k
+        elo, ehi = int(eoff[k]), int(eoff[k + 1])
+        if weighted:
+            assert np.abs(gwb[elo:ehi] - gww[elo:ehi]).max() <= 2e-4 * np.abs(gww[elo:ehi]).max(), k
+        assert np.abs(gpb[lo:hi] - gpw[lo:hi]).max() <= 2e-4 * np.abs(gpw[lo:hi]).max(), k
+        assert abs(gdb[k] - gdw[k]) <= 2e-4 * max(1.0, abs(gdw[k])), k
