@@ -1110,24 +1110,54 @@ gt_status ensure_blocks_cut(gt_graph *g, Orient &o, const uint32_t *gathered_off
 // A weighted sweep's weights in block order: out[k] = w(eids[k]) converted
 // to V, for every word slot.  A real entry (not padding) whose weight is
 // negative or not finite sets *bad: the block sweeps bound their sums by the
-// iterate's mass, which holds for non-negative weights only.
+// iterate's mass, which holds for non-negative weights only.  Each XCD takes
+// a contiguous eighth of the slots (workgroup b runs on XCD b % 8), so the
+// weights its slots gather -- those of the rows of a few tiles -- stay in
+// its L2; the slot and edge-id streams are read once and the output written
+// once, non-temporally, so they do not evict them.  Each thread keeps eight
+// slots a stride apart in flight.
 template <typename S, typename V>
 __global__ __launch_bounds__(kBlock) void blk_stage_w(const uint32_t *__restrict__ words,
                                                       const uint32_t *__restrict__ eids,
                                                       const void *w, int kind, double value,
                                                       uint64_t count, V *__restrict__ out,
                                                       uint32_t *bad, unsigned long long *wmax) {
+    uint64_t begin = 0, end = count, first = (uint64_t)blockIdx.x * kBlock + threadIdx.x,
+             stride = (uint64_t)gridDim.x * kBlock;
+    if (gridDim.x % kXcds == 0) {
+        const uint32_t x = blockIdx.x % kXcds;
+        begin = count * x / kXcds;
+        end = count * (x + 1) / kXcds;
+        first = begin + (uint64_t)(blockIdx.x / kXcds) * kBlock + threadIdx.x;
+        stride = (uint64_t)(gridDim.x / kXcds) * kBlock;
+    }
     bool neg = false;
     double top = 0.0;
-    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < count;
-         k += (uint64_t)gridDim.x * kBlock) {
-        const double x = kind == GT_MAP_CONSTANT
-                             ? value
-                             : static_cast<double>(static_cast<const S *>(w)[eids[k]]);
-        out[k] = static_cast<V>(x);
-        if ((words[k] & kBlkCodeMask) != kBlkNull) {
-            if (!(x >= 0.0 && x <= 1.7976931348623157e308)) neg = true;
-            top = fmax(top, static_cast<double>(static_cast<V>(x)));
+    for (uint64_t j0 = first; j0 < end; j0 += 8 * stride) {
+        uint32_t e[8], wd[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t j = j0 + u * stride;
+            e[u] = j < end ? __builtin_nontemporal_load(eids + j) : 0u;
+            wd[u] = j < end ? __builtin_nontemporal_load(words + j) : kBlkNull;
+        }
+        // Padding slots gather nothing (their weight is never used).
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            x[u] = j0 + u * stride >= end || (wd[u] & kBlkCodeMask) == kBlkNull ? 0.0
+                   : kind == GT_MAP_CONSTANT ? value
+                                             : static_cast<double>(static_cast<const S *>(w)[e[u]]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t j = j0 + u * stride;
+            if (j >= end) continue;
+            const V v = static_cast<V>(x[u]);
+            __builtin_nontemporal_store(v, out + j);
+            if ((wd[u] & kBlkCodeMask) != kBlkNull) {
+                if (!(x[u] >= 0.0 && x[u] <= 1.7976931348623157e308)) neg = true;
+                top = fmax(top, static_cast<double>(v));
+            }
         }
     }
     if (__any(neg) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(bad, 1u);
