@@ -578,6 +578,9 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
         backward()
         torch.cuda.synchronize()
         tb += time.perf_counter() - t
+    # Block tiles cut at the graphs' bounds unless they cannot hold the batch.
+    on_blocks = dev.layout()["block_tiles"] > 0
+    tkey = "pr_sweep_blk_forward_call" if on_blocks else "u_sweep_forward_call"
     mg = np.diff(eoff).astype(np.float64)
     ng = np.diff(voff.astype(np.int64)).astype(np.float64)
     itf, itb = it_f.astype(np.float64), it_b.astype(np.float64)
@@ -608,11 +611,12 @@ def bench_uneven(args, torch, gt_hip, _lib) -> int:
                            "mean": round(float(itb.mean()), 2)},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("uneven_chung_lu_4096", "u_sweep_forward_call")[0],
-                     "traffic_unit": "bytes per forward call",
-                     "traffic_source": profiled_traffic("uneven_chung_lu_4096",
-                                                        "u_sweep_forward_call")[1],
-                     "kernel": "u_sweep<UFwdFin<true>> (forward, wall time of the call)",
+                     "traffic": profiled_traffic("uneven_chung_lu_4096", tkey)[0],
+                     "traffic_unit": "bytes per forward call (its sweeps)",
+                     "traffic_source": profiled_traffic("uneven_chung_lu_4096", tkey)[1],
+                     "kernel": ("pr_sweep_blk<false, 6, true> (forward sweeps on block tiles cut at "
+                                "the graphs' bounds; wall time of the call)" if on_blocks else
+                                "u_sweep<UFwdFin<true>> (forward, wall time of the call)"),
                      "bytes_per_solve": int(bytes_f)},
     }
     if not args.no_cpu:

@@ -264,7 +264,10 @@ gt_status gt_pagerank_backward(gt_graph *g, const gt_view *view, const gt_prop *
  * iterate) or at max_iter sweeps, and is frozen after.  params->plan.len must
  * equal the vertex bound.  rank: [n] f64 in rank_loc memory (overwritten,
  * cold start); iters: [num_graphs] host.  The sweeps run as replays of a
- * captured HIP graph of 8 sweeps each. */
+ * captured HIP graph of 8 sweeps each, on block tiles cut at the graphs'
+ * bounds (each graph's row sums in 64-bit fixed point at its own scale)
+ * unless the tuning turns them off or a weight is negative or not finite,
+ * then on the wave engine. */
 gt_status gt_pagerank_uneven(gt_graph *g, uint32_t num_graphs, const uint32_t *vertex_offsets,
                              const double *damping, const gt_prop *weight,
                              const gt_power_iteration *params, double *rank, int32_t rank_loc,
@@ -422,8 +425,8 @@ typedef struct gt_tuning {
     int32_t sliced_share;        /* percent of the sliced sweep's workgroups given to the warm
                                     slices: 1..99 (0: from a cost model) */
     int32_t block_tile_nnz;      /* PageRank, adjoint and eigenvector sweeps on a relabelled,
-                                    unfiltered graph, unweighted or with finite non-negative
-                                    weights: entries per block tile
+                                    unfiltered graph, and the uneven batch's sweeps, unweighted
+                                    or with finite non-negative weights: entries per block tile
                                     (rows sorted together by source, summed in 64-bit fixed
                                     point); -1 turns the block sweeps off and the wave engine
                                     runs: -1, 1024..1048576 (131072) */
