@@ -857,6 +857,7 @@ gt_status build_blocks(gt_graph *g, const Orient &o, const uint32_t *gathered_of
                        uint32_t tile_nnz, uint32_t long_min, const std::vector<uint32_t> *bounds,
                        Blocks &B) {
     B.ready = false;
+    B.batch_bounds.clear();
     const uint32_t n = g->n;
     const std::vector<uint32_t> &off = o.h_off;
     const uint64_t m = off[n];

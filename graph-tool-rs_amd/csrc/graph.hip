@@ -250,6 +250,7 @@ gt_status ensure_schedule(gt_graph *g, Orient &o, uint32_t n,
                           const std::vector<uint32_t> *bounds) {
     const uint32_t max_rows = (uint32_t)g->tune.tile_rows;
     if (o.scheduled && o.sched_rows == max_rows && (!bounds || o.bounds == *bounds)) return GT_OK;
+    o.batch_bounds.clear();
     const std::vector<uint32_t> &off = o.h_off;
     std::vector<Tile> tiles;
     std::vector<uint32_t> long_rows, long_slot;

@@ -174,6 +174,10 @@ struct Blocks {
     // row) and the long rows.
     std::vector<uint32_t> bounds;
     std::vector<uint32_t> h_tile_row, h_long_rows;
+    // A batch's per-tile graph and per-graph first tile / first long row
+    // (uint32 each), uploaded for the bounds in batch_bounds.
+    DevBuf batch[3];
+    std::vector<uint32_t> batch_bounds;
 };
 
 // One orientation of the adjacency: row r lists the entries a kernel pulls
@@ -199,6 +203,11 @@ struct Orient {
     std::vector<uint32_t> bounds;
     std::vector<Tile> h_tiles;
     std::vector<uint32_t> h_long_rows;
+    // A batch's per-tile graph and per-graph first tile / first split row
+    // over the wave tiles (uint32 each), uploaded for the bounds in
+    // batch_bounds; cleared when the schedule is rebuilt.
+    DevBuf batch[3];
+    std::vector<uint32_t> batch_bounds;
     Schedule mm;             // SpMM schedule (batched personalisation)
     Blocks blk;              // block tiles cut at graph bounds (uneven batch)
     // Rows for the wavefront (mid) and workgroup (heavy) degree sums.

@@ -465,8 +465,8 @@ unsigned ugrid(uint64_t work) {
 }
 
 // Workspace slots of the batch arrays.
-enum { kUVoff, kUDamp, kUVg, kUTg, kUTb, kULb, kUState, kUTpart, kUBad, kUSums, kUList, kUListN,
-       kUBTg, kUBTb, kUBLb, kUBMass, kUCount };
+enum { kUVoff, kUDamp, kUVg, kUState, kUTpart, kUBad, kUSums, kUList, kUListN, kUBMass,
+       kUCount };
 static_assert(kUCount <= (int)(sizeof(Workspace::ub) / sizeof(DevBuf)), "workspace slots");
 
 // Per-graph loop state back to the start: every non-empty graph active, the
@@ -509,18 +509,20 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     GT_TRY(ensure_schedule(g, o, n, &bounds));
     note_layout(g, o);
     const std::vector<Tile> &T = o.h_tiles;
-    const std::vector<uint32_t> &off = o.h_off;
-    std::vector<uint32_t> tg(std::max<size_t>(1, T.size())), tb(G + 1, 0), lb(G + 1, 0);
-    uint32_t gi = 0;
-    for (size_t t = 0; t < T.size(); ++t) {
-        uint32_t row = T[t].row_begin;
-        if (T[t].row_end == kSplit)  // the row whose entries hold this chunk
-            row = (uint32_t)(std::upper_bound(off.begin(), off.end(), T[t].nnz_begin) -
-                             off.begin()) - 1;
-        while (gi + 1 < G && voff[gi + 1] <= row) ++gi;
-        tg[t] = gi;
-    }
-    {
+    // Per-tile graph and per-graph tile / split-row ranges, computed and
+    // uploaded once per schedule and set of bounds.
+    if (o.batch_bounds != bounds) {
+        const std::vector<uint32_t> &off = o.h_off;
+        std::vector<uint32_t> tg(std::max<size_t>(1, T.size())), tb(G + 1, 0), lb(G + 1, 0);
+        uint32_t gi = 0;
+        for (size_t t = 0; t < T.size(); ++t) {
+            uint32_t row = T[t].row_begin;
+            if (T[t].row_end == kSplit)  // the row whose entries hold this chunk
+                row = (uint32_t)(std::upper_bound(off.begin(), off.end(), T[t].nnz_begin) -
+                                 off.begin()) - 1;
+            while (gi + 1 < G && voff[gi + 1] <= row) ++gi;
+            tg[t] = gi;
+        }
         size_t t = 0;
         for (uint32_t k = 0; k <= G; ++k) {
             while (t < T.size() && tg[t] < k) ++t;
@@ -534,14 +536,21 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
             lb[k] = (uint32_t)l;
         }
         lb[G] = (uint32_t)LR.size();
+        GT_TRY(o.batch[0].reserve(sizeof(uint32_t) * tg.size()));
+        GT_TRY(o.batch[1].reserve(sizeof(uint32_t) * (G + 1)));
+        GT_TRY(o.batch[2].reserve(sizeof(uint32_t) * (G + 1)));
+        GT_HIP(hipMemcpy(o.batch[0].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
+                         hipMemcpyHostToDevice));
+        GT_HIP(hipMemcpy(o.batch[1].ptr, tb.data(), sizeof(uint32_t) * (G + 1),
+                         hipMemcpyHostToDevice));
+        GT_HIP(hipMemcpy(o.batch[2].ptr, lb.data(), sizeof(uint32_t) * (G + 1),
+                         hipMemcpyHostToDevice));
+        o.batch_bounds = bounds;
     }
     DevBuf *U = ws.ub;
     GT_TRY(U[kUVoff].reserve(sizeof(uint32_t) * (G + 1)));
     GT_TRY(U[kUDamp].reserve(sizeof(double) * G));
     GT_TRY(U[kUVg].reserve(sizeof(uint32_t) * std::max<uint32_t>(1, n)));
-    GT_TRY(U[kUTg].reserve(sizeof(uint32_t) * tg.size()));
-    GT_TRY(U[kUTb].reserve(sizeof(uint32_t) * (G + 1)));
-    GT_TRY(U[kULb].reserve(sizeof(uint32_t) * (G + 1)));
     // state: p_sink, delta, scale (f64), iter (u64), done, nsink (u32), + active
     GT_TRY(U[kUState].reserve((3 * sizeof(double) + sizeof(uint64_t) + 2 * sizeof(uint32_t)) *
                                   (size_t)G + 64));
@@ -552,12 +561,6 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     GT_TRY(U[kUListN].reserve(sizeof(uint32_t) * 2));
     GT_HIP(hipMemcpyAsync(U[kUVoff].ptr, voff, sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, s));
     GT_HIP(hipMemcpyAsync(U[kUDamp].ptr, damping, sizeof(double) * G, hipMemcpyHostToDevice, s));
-    GT_HIP(hipMemcpyAsync(U[kUTg].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
-                          hipMemcpyHostToDevice, s));
-    GT_HIP(hipMemcpyAsync(U[kUTb].ptr, tb.data(), sizeof(uint32_t) * (G + 1),
-                          hipMemcpyHostToDevice, s));
-    GT_HIP(hipMemcpyAsync(U[kULb].ptr, lb.data(), sizeof(uint32_t) * (G + 1),
-                          hipMemcpyHostToDevice, s));
     char *st = U[kUState].as<char>();
     a.st.p_sink = reinterpret_cast<double *>(st);
     a.st.delta = a.st.p_sink + G;
@@ -569,9 +572,9 @@ gt_status u_setup(gt_graph *g, uint32_t G, const uint32_t *voff, const double *d
     a.ngraphs = G;
     a.voff = U[kUVoff].as<uint32_t>();
     a.damp = U[kUDamp].as<double>();
-    a.tile_graph = U[kUTg].as<uint32_t>();
-    a.tb = U[kUTb].as<uint32_t>();
-    a.lb = U[kULb].as<uint32_t>();
+    a.tile_graph = o.batch[0].as<uint32_t>();
+    a.tb = o.batch[1].as<uint32_t>();
+    a.lb = o.batch[2].as<uint32_t>();
     a.tpart = U[kUTpart].as<double>();
     a.list = U[kUList].as<uint32_t>();
     a.list_count = U[kUListN].as<uint32_t>();
@@ -718,7 +721,6 @@ gt_status u_blk_setup(gt_graph *g, uint32_t G, const uint32_t *voff, Orient &o,
     *ok = false;
     if (g->tune.block_tile_nnz <= 0) return GT_OK;
     Workspace &ws = g->ws;
-    hipStream_t s = g->stream;
     const std::vector<uint32_t> bounds(voff, voff + G + 1);
     Blocks *B = nullptr;
     GT_TRY(ensure_blocks_cut(g, o, gathered, (uint32_t)g->tune.block_tile_nnz,
@@ -730,32 +732,35 @@ gt_status u_blk_setup(gt_graph *g, uint32_t G, const uint32_t *voff, Orient &o,
     }
     GT_TRY(blk_reserve(g, *B));
     const uint32_t nt = B->ntiles;
-    std::vector<uint32_t> tg(std::max<uint32_t>(1, nt), 0), tb(G + 1), lb(G + 1);
-    for (uint32_t t = 0; t < nt; ++t)
-        tg[t] = (uint32_t)(std::upper_bound(voff, voff + G + 1, B->h_tile_row[t]) - voff) - 1;
-    const std::vector<uint32_t> &LR = B->h_long_rows;
-    for (uint32_t k = 0; k <= G; ++k) {
-        tb[k] = (uint32_t)(std::lower_bound(tg.begin(), tg.begin() + nt, k) - tg.begin());
-        lb[k] = (uint32_t)(std::lower_bound(LR.begin(), LR.end(), voff[k]) - LR.begin());
+    if (B->batch_bounds != bounds) {
+        std::vector<uint32_t> tg(std::max<uint32_t>(1, nt), 0), tb(G + 1), lb(G + 1);
+        for (uint32_t t = 0; t < nt; ++t)
+            tg[t] = (uint32_t)(std::upper_bound(voff, voff + G + 1, B->h_tile_row[t]) - voff) - 1;
+        const std::vector<uint32_t> &LR = B->h_long_rows;
+        for (uint32_t k = 0; k <= G; ++k) {
+            tb[k] = (uint32_t)(std::lower_bound(tg.begin(), tg.begin() + nt, k) - tg.begin());
+            lb[k] = (uint32_t)(std::lower_bound(LR.begin(), LR.end(), voff[k]) - LR.begin());
+        }
+        GT_TRY(B->batch[0].reserve(sizeof(uint32_t) * tg.size()));
+        GT_TRY(B->batch[1].reserve(sizeof(uint32_t) * (G + 1)));
+        GT_TRY(B->batch[2].reserve(sizeof(uint32_t) * (G + 1)));
+        GT_HIP(hipMemcpy(B->batch[0].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
+                         hipMemcpyHostToDevice));
+        GT_HIP(hipMemcpy(B->batch[1].ptr, tb.data(), sizeof(uint32_t) * (G + 1),
+                         hipMemcpyHostToDevice));
+        GT_HIP(hipMemcpy(B->batch[2].ptr, lb.data(), sizeof(uint32_t) * (G + 1),
+                         hipMemcpyHostToDevice));
+        B->batch_bounds = bounds;
     }
     DevBuf *U = ws.ub;
-    GT_TRY(U[kUBTg].reserve(sizeof(uint32_t) * tg.size()));
-    GT_TRY(U[kUBTb].reserve(sizeof(uint32_t) * (G + 1)));
-    GT_TRY(U[kUBLb].reserve(sizeof(uint32_t) * (G + 1)));
     GT_TRY(U[kUBMass].reserve(sizeof(double) * G + 2 * sizeof(uint32_t)));
     if (U[kUList].bytes < sizeof(uint32_t) * std::max<uint32_t>(1, nt)) {
         GT_TRY(U[kUList].reserve(sizeof(uint32_t) * std::max<uint32_t>(1, nt)));
         a.list = U[kUList].as<uint32_t>();
     }
-    GT_HIP(hipMemcpyAsync(U[kUBTg].ptr, tg.data(), sizeof(uint32_t) * tg.size(),
-                          hipMemcpyHostToDevice, s));
-    GT_HIP(hipMemcpyAsync(U[kUBTb].ptr, tb.data(), sizeof(uint32_t) * (G + 1),
-                          hipMemcpyHostToDevice, s));
-    GT_HIP(hipMemcpyAsync(U[kUBLb].ptr, lb.data(), sizeof(uint32_t) * (G + 1),
-                          hipMemcpyHostToDevice, s));
     bk = blk_layout_args(g, *B);
     BlkUneven &u = bk.u;
-    u.tile_graph = U[kUBTg].as<uint32_t>();
+    u.tile_graph = B->batch[0].as<uint32_t>();
     u.voff = a.voff;
     u.damp = a.damp;
     u.mass = U[kUBMass].as<double>();
@@ -765,8 +770,8 @@ gt_status u_blk_setup(gt_graph *g, uint32_t G, const uint32_t *voff, Orient &o,
     u.iter = a.st.iter;
     u.done = a.st.done;
     u.active = a.active;
-    u.tb = U[kUBTb].as<uint32_t>();
-    u.lb = U[kUBLb].as<uint32_t>();
+    u.tb = B->batch[1].as<uint32_t>();
+    u.lb = B->batch[2].as<uint32_t>();
     u.list = a.list;
     u.ngraphs = G;
     // The two tile counters, one per sweep parity, after the masses.
