@@ -268,9 +268,10 @@ void pr_sweep_blk(const BlkArgs b) {
     SweepState *st = b.st;
     // The refinement runs once, after the FP32 loop finished (its state says
     // done), and so does the adjoint's mu pass: they do not look at the flag.
-    // A batch of graphs runs while one is iterating and none has bailed.
+    // A batch of graphs runs while one is iterating and none has bailed;
+    // its mu pass (after the loop) runs over every tile.
     if constexpr (UNEV) {
-        if (b.u.active[0] == 0 || b.u.active[1] != 0) return;
+        if (!b.mu_out && (b.u.active[0] == 0 || b.u.active[1] != 0)) return;
     } else if (MODE != kBlkRefine && !(ADJ && b.mu_out) && st->done) {
         return;
     }
@@ -301,10 +302,12 @@ void pr_sweep_blk(const BlkArgs b) {
     uint32_t nwork = b.ntiles;
     if constexpr (UNEV) {
         ctr = b.u.ctr;
-        nwork = __builtin_amdgcn_readfirstlane(*b.u.list_n);
-        if (blockIdx.x == 0 && tid == 0) {
-            *b.u.ctr_next = 0u;
-            *b.u.list_n_next = 0u;
+        if (!b.mu_out) {
+            nwork = __builtin_amdgcn_readfirstlane(*b.u.list_n);
+            if (blockIdx.x == 0 && tid == 0) {
+                *b.u.ctr_next = 0u;
+                *b.u.list_n_next = 0u;
+            }
         }
     }
     for (uint32_t i = tid; i < kBlkLds; i += NT) {
@@ -318,7 +321,7 @@ void pr_sweep_blk(const BlkArgs b) {
         if (t >= nwork) break;
         if constexpr (UNEV) {
             ConstU32 *lst = (ConstU32 *)b.u.list;
-            t = __builtin_amdgcn_readfirstlane(lst[t]);
+            if (!b.mu_out) t = __builtin_amdgcn_readfirstlane(lst[t]);
             const uint32_t gi = __builtin_amdgcn_readfirstlane(b.u.tile_graph[t]);
             const double mg = b.u.mass[gi];
             // A graph whose mass the fixed point cannot hold sums garbage
@@ -739,11 +742,13 @@ template <int MODE, bool WGT>
 __global__ __launch_bounds__(kBlock) void u_fold_blk(const BlkArgs b) {
     constexpr bool ADJ = MODE == kBlkUAdj;
     const BlkUneven &u = b.u;
-    if (u.active[0] == 0 || u.active[1] != 0) return;
+    // The adjoint's mu pass finishes every graph's long rows and decides
+    // nothing.
+    if (!b.mu_out && (u.active[0] == 0 || u.active[1] != 0)) return;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t gi =
         __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
-    if (gi >= u.ngraphs || u.done[gi]) return;
+    if (gi >= u.ngraphs || (!b.mu_out && u.done[gi])) return;
     const double mg = u.mass[gi];
     const int K = mg <= kBlkMassMax ? blk_scale(mg) : 0;
     const double dmp = u.damp[gi], ps = u.ps[gi];
@@ -768,6 +773,7 @@ __global__ __launch_bounds__(kBlock) void u_fold_blk(const BlkArgs b) {
         else
             blk_finish<MODE>(b, r, v, static_cast<double>(b.x_cur[r]), dg, pv, ps, dmp, d, s, ms);
     }
+    if (ADJ && b.mu_out) return;
     d = wave_sum(d);
     s = wave_sum(s);
     ms = wave_sum(ms);

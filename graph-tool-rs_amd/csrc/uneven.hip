@@ -1001,8 +1001,8 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     // gathered once per in-edge); the wave engine when they are off or
     // cannot hold the batch.
     bool on_blocks = false;
+    BlkArgs bk{};
     {
-        BlkArgs bk{};
         bool ok = false;
         GT_TRY(u_blk_setup(g, G, voff, g->out, g->in.off.as<uint32_t>(), weight, wt, a, bk, &ok));
         if (ok) {
@@ -1038,22 +1038,32 @@ gt_status pagerank_uneven_backward_impl(gt_graph *g, uint32_t G, const uint32_t 
     hipLaunchKernelGGL(u_latest<float>, dim3(ugrid(n)), dim3(kBlock), 0, cs, lam[0], lam[1], vg,
                        a.st.iter, n, lf);
     if (grad_w) {
-        // The mu pass runs on the wave engine, with its weights.
-        GT_TRY(u_wave_weights(g, weight, g->out, wt, a, ws.wcsr));
         GT_TRY(ensure_edge_list(g));
         GT_TRY(ws.mu.reserve(sizeof(double) * std::max<uint32_t>(1, n)));
         GT_TRY(ws.arl[3].reserve(sizeof(double2) * std::max<uint32_t>(1, n)));
         double *mu = ws.mu.as<double>();
         double2 *pair = ws.arl[3].as<double2>();
-        UArgs b = a;
-        b.src = lf;
-        const dim3 lg(fold_grid(g->out.nlong));
-        if (weighted) {
-            hipLaunchKernelGGL(u_mu<true>, sg, bd, 0, cs, b, mu);
-            hipLaunchKernelGGL(u_mu_long<true>, lg, bd, 0, cs, b, g->out.nlong, mu);
+        if (on_blocks) {
+            // mu = M^T lambda on the loop's block tiles: one more sweep over
+            // every tile at each graph's last scale, rows written as acc / deg.
+            BlkArgs k = bk;
+            k.mu_out = mu;
+            k.x_cur = lf;
+            k.c_cur = lf;
+            GT_HIP(hipMemsetAsync(k.u.ctr, 0, sizeof(uint32_t), cs));
+            blk_launch_usweep(k, g->tune.block_per_cu, kBlkUAdj, cs);
+            blk_launch_ufold(k, kBlkUAdj, cs);
         } else {
-            hipLaunchKernelGGL(u_mu<false>, sg, bd, 0, cs, b, mu);
-            hipLaunchKernelGGL(u_mu_long<false>, lg, bd, 0, cs, b, g->out.nlong, mu);
+            UArgs b = a;
+            b.src = lf;
+            const dim3 lg(fold_grid(g->out.nlong));
+            if (weighted) {
+                hipLaunchKernelGGL(u_mu<true>, sg, bd, 0, cs, b, mu);
+                hipLaunchKernelGGL(u_mu_long<true>, lg, bd, 0, cs, b, g->out.nlong, mu);
+            } else {
+                hipLaunchKernelGGL(u_mu<false>, sg, bd, 0, cs, b, mu);
+                hipLaunchKernelGGL(u_mu_long<false>, lg, bd, 0, cs, b, g->out.nlong, mu);
+            }
         }
         const uint64_t eb = g->edge_bound;
         float *gdst = grad_w;
