@@ -417,14 +417,17 @@ __global__ __launch_bounds__(kBlock) void u_grad_edges(const uint32_t *es, const
         first = begin + (uint64_t)(blockIdx.x / kXcds) * kBlock + threadIdx.x;
         stride = (uint64_t)(gridDim.x / kXcds) * kBlock;
     }
+    // The edge list is read once and the gradients written once, past L2, so
+    // they do not evict the pairs and iterates the edges gather.
     for (uint64_t e = first; e < end; e += stride) {
-        const uint32_t sv = es[e];
+        const uint32_t sv = __builtin_nontemporal_load(es + e);
+        const uint32_t tv = __builtin_nontemporal_load(et + e);
         float out = 0.f;
         if (sv != kNoEdge) {
             const double2 p = pair[sv];
-            out = static_cast<float>(p.x * (static_cast<double>(lam[et[e]]) - p.y));
+            out = static_cast<float>(p.x * (static_cast<double>(lam[tv]) - p.y));
         }
-        grad[e] = out;
+        __builtin_nontemporal_store(out, grad + e);
     }
 }
 
