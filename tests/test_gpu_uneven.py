@@ -220,3 +220,18 @@ def test_uneven_blocks_agree_with_wave_engine(gpu, weighted):
             assert np.abs(gwb[elo:ehi] - gww[elo:ehi]).max() <= 2e-4 * np.abs(gww[elo:ehi]).max(), k
         assert np.abs(gpb[lo:hi] - gpw[lo:hi]).max() <= 2e-4 * np.abs(gpw[lo:hi]).max(), k
         assert abs(gdb[k] - gdw[k]) <= 2e-4 * max(1.0, abs(gdw[k])), k
+
+
+def test_uneven_negative_weight_runs_on_the_wave_engine(gpu):
+    """A negative weight breaks the block sweeps' bound on their fixed-point
+    sums (a source's terms sum to at most its |x|): the batch runs on the
+    wave engine, and every graph still matches the oracle alone."""
+    G = 6
+    voff, eoff, d, s, t, w, n, dev = _batch(G, 515, n_max=600)
+    w = w.copy()
+    w[int(eoff[2]) + 3] = -0.25
+    x, it = gt_hip.pagerank_uneven(dev, voff, d, weight=w, epsilon=1e-6)
+    assert dev.layout()["block_tiles"] == 0
+    for k in range(G):
+        nk, sk, tk, wk = _graph(k, voff, eoff, s, t, w)
+        check_forward(oracle.Adj(nk, sk, tk), x[voff[k]:voff[k + 1]], int(it[k]), wk, float(d[k]))
